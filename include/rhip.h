@@ -1,0 +1,158 @@
+/*
+ * rhip.h -- C-ABI of librhip, the MI355X (gfx950) rasteriser behind
+ * SoftGLRender's abstract Renderer interface.
+ *
+ * Each entry point replaces one call of the reference's interface
+ * (src/Render/Renderer.h:24-59 and the resource classes it returns); the
+ * citation is on each declaration. Plain pointers, sizes and POD structs only.
+ *
+ * Conventions
+ *  - Every function returns RHIP_OK (0) or a negative RHIP_E_* code; the
+ *    message of the last failure on the calling thread is rhip_last_error().
+ *    (The reference logs LOGE and returns nullptr/false: RendererSoft.cpp:28-34,
+ *    TextureSoft.h:116-124.)
+ *  - Handles are positive ints, valid within the device that made them.
+ *  - Work is queued on one HIP stream per device. Host buffers passed in are
+ *    copied before the call returns (the reference copies vertex, index and
+ *    uniform bytes too: VertexSoft.h:16-27, UniformSoft.h:31-37).
+ *  - rhip_draw snapshots the uniform blocks it is given, so callers may
+ *    rewrite them before the next draw (Viewer.cpp:684-716 does).
+ *  - rhip_texture_read and rhip_wait_idle wait for queued work.
+ */
+#ifndef RHIP_H
+#define RHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "rhip_stream.h" /* RhsStates, shading / define / slot enums */
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  RHIP_OK = 0,
+  RHIP_E_INVALID = -1,     /* bad argument or handle */
+  RHIP_E_HIP = -2,         /* a HIP runtime call failed */
+  RHIP_E_UNSUPPORTED = -3, /* a valid request this build does not implement */
+  RHIP_E_OOM = -4,         /* device allocation failed */
+  RHIP_E_DEVICE = -5,      /* a kernel reported an internal overflow */
+};
+
+/* Texel storage order of a sampled texture. The reference picks one for every
+ * buffer at compile time (SOFTGL_TEXTURE_TILED / _MORTON, Buffer.h:215-227);
+ * here it is per texture. Sampled values do not depend on it. */
+enum { RHIP_LAYOUT_LINEAR = 0, RHIP_LAYOUT_TILED = 1, RHIP_LAYOUT_MORTON = 2 };
+
+typedef struct rhip_device_s *rhip_device;
+
+/* TextureDesc + SamplerDesc (Texture.h:17-46), enum values as the reference's. */
+typedef struct rhip_texture_desc {
+  int width, height;
+  int type;        /* TextureType: 0 = 2D, 1 = CUBE */
+  int format;      /* TextureFormat: 0 = RGBA8, 1 = FLOAT32 */
+  uint32_t usage;  /* TextureUsage bits */
+  int use_mipmaps; /* bool */
+  int multi_sample;/* bool: 4 samples per texel */
+  int layout;      /* RHIP_LAYOUT_* */
+} rhip_texture_desc;
+
+typedef struct rhip_sampler_desc {
+  int filter_min, filter_mag; /* FilterMode */
+  int wrap_s, wrap_t, wrap_r; /* WrapMode */
+  int border_color;           /* BorderColor */
+} rhip_sampler_desc;
+
+/* ClearStates (RenderStates.h:65-70). */
+typedef struct rhip_clear_states {
+  int depth_flag, color_flag;
+  float clear_color[4];
+  float clear_depth;
+} rhip_clear_states;
+
+/* One draw: everything RendererSoft::draw reads from the bound state
+ * (vao, program, resources, pipeline states; RendererSoft.cpp:115-164). */
+typedef struct rhip_draw_desc {
+  int vao;
+  int program;
+  RhsStates states;                        /* RenderStates (RenderStates.h:52-63) */
+  const void *uniform_model;               /* UniformsModel, 256 B (Material.h:70-76) */
+  const void *uniform_scene;               /* UniformsScene, 64 B */
+  const void *uniform_material;            /* UniformsMaterial, 48 B */
+  int sampler_tex[RHS_SLOT_COUNT];         /* texture per RhsSlot, 0 = unbound */
+} rhip_draw_desc;
+
+/* Per-frame counters (for Gfrag/s and diagnostics), summed since the last reset. */
+typedef struct rhip_stats {
+  long long fs_invocations; /* fragment-shader invocations (pixel granularity) */
+  long long samples_written;
+  long long triangles_binned;
+  long long tile_pairs;
+  long long clipped_triangles;
+  long long draws;
+} rhip_stats;
+
+/* ---------------------------------------------------------------- device (Renderer::create/destroy) */
+int rhip_device_create(int hip_device, rhip_device *out);              /* Renderer::create  (Renderer.h:27) */
+int rhip_device_destroy(rhip_device dev);                              /* Renderer::destroy (Renderer.h:28) */
+const char *rhip_last_error(void);
+
+/* ---------------------------------------------------------------- textures */
+int rhip_texture_create(rhip_device dev, const rhip_texture_desc *desc, int *out_tex);       /* createTexture (Renderer.h:34) */
+int rhip_texture_set_sampler(rhip_device dev, int tex, const rhip_sampler_desc *s);          /* Texture::setSamplerDesc (Texture.h:97) */
+int rhip_texture_init(rhip_device dev, int tex);                                             /* Texture::initImageData (Texture.h:98) */
+/* Texture::setImageData (Texture.h:99-100): level 0 of each layer, row-major linear,
+ * 4 bytes per texel; mip levels are generated when the texture uses mipmaps. */
+int rhip_texture_upload(rhip_device dev, int tex, int layers, const void *data, size_t bytes);
+/* Writes one mip level of one layer (row-major linear); no reference counterpart:
+ * the reference fills IBL mip chains by rendering into them (Environment.cpp:84-103). */
+int rhip_texture_upload_level(rhip_device dev, int tex, int layer, int level, const void *data, size_t bytes);
+/* Reads one level back, row-major linear. which = 0: the single-sample / resolved
+ * image; which = 1: the raw 4x samples (4 consecutive values per pixel).
+ * Texture::dumpImage (Texture.h:101) and ViewerSoftware::swapBuffer's read (ViewerSoftware.h:30-44). */
+int rhip_texture_read(rhip_device dev, int tex, int layer, int level, int which, void *out, size_t bytes);
+int rhip_texture_destroy(rhip_device dev, int tex);
+
+/* ---------------------------------------------------------------- vertex arrays */
+/* createVertexArrayObject (Renderer.h:37): float attributes, `stride` bytes per
+ * vertex, int32 triangle indices. */
+int rhip_vao_create(rhip_device dev, const void *vertices, int stride, int nverts, const int32_t *indices,
+                    int nindices, int *out_vao);
+int rhip_vao_update(rhip_device dev, int vao, const void *vertices, size_t bytes); /* VertexArrayObject::updateVertexData (Vertex.h:18) */
+int rhip_vao_destroy(rhip_device dev, int vao);
+
+/* ---------------------------------------------------------------- framebuffers (Framebuffer.h:20-91) */
+int rhip_fbo_create(rhip_device dev, int *out_fbo);                                       /* createFrameBuffer (Renderer.h:31) */
+int rhip_fbo_attach_color(rhip_device dev, int fbo, int tex, int layer, int level);       /* setColorAttachment (Framebuffer.h:27-39) */
+int rhip_fbo_attach_depth(rhip_device dev, int fbo, int tex);                             /* setDepthAttachment (Framebuffer.h:41-46) */
+int rhip_fbo_destroy(rhip_device dev, int fbo);
+
+/* ---------------------------------------------------------------- programs */
+/* createShaderProgram + addDefine(s) (Renderer.h:40, ShaderProgram.h:19-25): a program
+ * is a shading model (RhsShading) plus a define bitmask (RhsDefine). */
+int rhip_program_create(rhip_device dev, int shading, uint32_t defines, int *out_program);
+
+/* ---------------------------------------------------------------- pipeline */
+int rhip_begin_render_pass(rhip_device dev, int fbo, const rhip_clear_states *clear); /* beginRenderPass (Renderer.h:50) */
+int rhip_set_viewport(rhip_device dev, int x, int y, int width, int height);          /* setViewPort (Renderer.h:51) */
+int rhip_draw(rhip_device dev, const rhip_draw_desc *draw);                           /* set*(...) + draw (Renderer.h:52-56) */
+int rhip_end_render_pass(rhip_device dev);                                            /* endRenderPass (Renderer.h:57) */
+int rhip_wait_idle(rhip_device dev);                                                  /* waitIdle (Renderer.h:58) */
+
+/* ---------------------------------------------------------------- measurement */
+int rhip_stats_get(rhip_device dev, rhip_stats *out); /* waits for queued work */
+int rhip_stats_reset(rhip_device dev);
+/* The HIP stream the device queues on (hipStream_t), for event timing. */
+void *rhip_device_stream(rhip_device dev);
+/* Per-kernel timing with HIP events recorded on the device's stream around every
+ * launch while enabled; enabling (or disabling) clears what was recorded. */
+int rhip_profile_enable(rhip_device dev, int on);
+/* Entry `index` of the per-kernel totals since profiling was enabled: kernel
+ * name, summed milliseconds and launch count; RHIP_E_INVALID past the end. */
+int rhip_profile_query(rhip_device dev, int index, char *name, int name_cap, double *total_ms, long long *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RHIP_H */

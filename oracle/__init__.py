@@ -64,13 +64,18 @@ def ref_replay(stream_bytes: bytes, out_nbytes: int, repeats: int = 1, threaded:
     return _replay(lib.ref_replay, stream_bytes, out_nbytes, repeats)
 
 
-def cpu_ref_replay(stream_bytes: bytes, out_nbytes: int, repeats: int = 1, threads: int = 1, stats=None):
+def cpu_ref_replay(stream_bytes: bytes, out_nbytes: int, repeats: int = 1, threads: int = 1, stats=None,
+                   exact_math: bool = False):
     """Runs the stream through the restatement. Returns (output bytes, per-frame ms).
+
+    exact_math=False calls the host libm's powf/log2f/exp2f like the reference;
+    exact_math=True uses correctly rounded versions, the arithmetic of the GPU path.
 
     `threads` > 1 splits fragment work over rows of 32-pixel blocks (same result).
     `stats`, if a dict, receives counters of the last frame (fragments shaded etc.)."""
     lib = _load(os.path.join(BUILD_DIR, "libcpu_ref.so"))
     lib.cpu_ref_set_threads(ctypes.c_int(threads))
+    lib.cpu_ref_set_math(ctypes.c_int(1 if exact_math else 0))
     out = _replay(lib.cpu_ref_replay, stream_bytes, out_nbytes, repeats)
     if stats is not None:
         arr = (ctypes.c_longlong * 8)()

@@ -18,6 +18,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <thread>
@@ -28,6 +30,16 @@
 namespace cref {
 
 static int g_threads = 1;
+static int g_trace_x = -1, g_trace_y = -1; /* debugging aid: CPU_REF_TRACE=x,y */
+
+/* The reference calls the host libm's powf / log2f / exp2f (glibc: within ~0.8 ulp,
+ * not always correctly rounded). With g_cr_math set, the checker evaluates them in
+ * double precision and rounds once, i.e. correctly rounded in all but vanishingly
+ * rare cases; that is the mode the GPU path must match bit for bit. */
+static bool g_cr_math = false;
+static inline float m_powf(float x, float y) { return g_cr_math ? (float)pow((double)x, (double)y) : powf(x, y); }
+static inline float m_log2f(float x) { return g_cr_math ? (float)log2((double)x) : log2f(x); }
+static inline float m_exp2f(float x) { return g_cr_math ? (float)exp2((double)x) : exp2f(x); }
 static long long g_stats[8];
 
 /* ------------------------------------------------------------------ small vector helpers */
@@ -447,7 +459,7 @@ static float lodFor(const DrawCtx &c, int slot, const FragIn &f) {
   float ddx = fmaf(dxy, dxy, dxx * dxx);
   float ddy = fmaf(dyy, dyy, dyx * dyx);
   float d = ddy < ddx ? ddx : ddy; /* vmaxss */
-  float lod = 0.5f * log2f(d);
+  float lod = 0.5f * m_log2f(d);
   return 0.f < lod ? lod : 0.f;
 }
 
@@ -514,7 +526,7 @@ static V4 fsBlinnPhong(const DrawCtx &c, const FragIn &f, bool directional) {
     V3 C = normalize3({v[12], v[13], v[14]});
     V3 H = normalize3(add3(L, C));
     float sa = maxf(dot3(N, H), 0.0f);
-    float sp = u.kSpecular * powf(sa, 128.f);
+    float sp = u.kSpecular * m_powf(sa, 128.f);
     spe = {sp, sp, sp};
     if (u.enableShadow) {
       float s = 1.0f - shadowCalc(c, {v[20], v[21], v[22], v[23]}, N, ldv);
@@ -546,7 +558,7 @@ static float schlickGGX(float nv, float r) {
 }
 static V3 envBRDFApprox(V3 spec, float r, float nv) {
   float rx = r * -1.f + 1.f, ry = r * -0.0275f + 0.0425f, rz = r * -0.572f + 1.04f, rw = r * 0.022f + -0.04f;
-  float a004 = minf(rx * rx, exp2f(-9.28f * nv)) * rx + ry;
+  float a004 = minf(rx * rx, m_exp2f(-9.28f * nv)) * rx + ry;
   float ax = -1.04f * a004 + rz, ay = 1.04f * a004 + rw;
   ay *= maxf(0.f, minf(1.f, 50.0f * spec.y));
   return {spec.x * ax + ay, spec.y * ax + ay, spec.z * ax + ay};
@@ -559,7 +571,7 @@ static V4 fsPbr(const DrawCtx &c, const FragIn &f) {
   V2 uv{v[0], v[1]};
   V4 albedoRGBA = (c.defines & RHS_DEF_ALBEDO_MAP) ? tex2D(c, RHS_SLOT_ALBEDO, uv, lodFor(c, RHS_SLOT_ALBEDO, f))
                                                    : u.baseColor;
-  V3 albedo{powf(albedoRGBA.x, 2.2f), powf(albedoRGBA.y, 2.2f), powf(albedoRGBA.z, 2.2f)};
+  V3 albedo{m_powf(albedoRGBA.x, 2.2f), m_powf(albedoRGBA.y, 2.2f), m_powf(albedoRGBA.z, 2.2f)};
   float metallic = 0.0f, rough = 1.0f;
   if (c.defines & RHS_DEF_METALROUGHNESS_MAP) {
     /* no LOD callback is ever installed on this sampler (PbrSoft.h:138-151): lod 0 */
@@ -587,7 +599,7 @@ static V4 fsPbr(const DrawCtx &c, const FragIn &f) {
     float nv = maxf(dot3(N, V), 0.0f), nl = maxf(dot3(N, L), 0.0f);
     float G = schlickGGX(nl, rough) * schlickGGX(nv, rough);
     float ct = maxf(dot3(H, V), 0.0f);
-    float p5 = powf(clampf(1.0f - ct, 0.0f, 1.0f), 5.0f);
+    float p5 = m_powf(clampf(1.0f - ct, 0.0f, 1.0f), 5.0f);
     V3 F{F0.x + (1.0f - F0.x) * p5, F0.y + (1.0f - F0.y) * p5, F0.z + (1.0f - F0.z) * p5};
     float ng = NDF * G;
     V3 num = scale3(F, ng);
@@ -600,7 +612,7 @@ static V4 fsPbr(const DrawCtx &c, const FragIn &f) {
   V3 ambient;
   if (u.enableIBL) {
     float nv = maxf(dot3(N, V), 0.0f);
-    float p5 = powf(clampf(1.0f - nv, 0.0f, 1.0f), 5.0f);
+    float p5 = m_powf(clampf(1.0f - nv, 0.0f, 1.0f), 5.0f);
     float omr = 1.0f - rough;
     V3 F{F0.x + (maxf(omr, F0.x) - F0.x) * p5, F0.y + (maxf(omr, F0.y) - F0.y) * p5,
          F0.z + (maxf(omr, F0.z) - F0.z) * p5};
@@ -615,7 +627,7 @@ static V4 fsPbr(const DrawCtx &c, const FragIn &f) {
     ambient = scale3(mul3(u.ambient, albedo), ao);
   }
   V3 col = add3(ambient, Lo);
-  col = {powf(col.x, 1.0f / 2.2f), powf(col.y, 1.0f / 2.2f), powf(col.z, 1.0f / 2.2f)};
+  col = {m_powf(col.x, 1.0f / 2.2f), m_powf(col.y, 1.0f / 2.2f), m_powf(col.z, 1.0f / 2.2f)};
   if (c.defines & RHS_DEF_EMISSIVE_MAP) {
     V4 e = tex2D(c, RHS_SLOT_EMISSIVE, uv, lodFor(c, RHS_SLOT_EMISSIVE, f));
     col = add3(col, {e.x, e.y, e.z});
@@ -1023,6 +1035,12 @@ struct Renderer {
               float *o = px[k].vary.data();
               for (int i = 0; i < ctx.nvary; i++) o[i] = fmaf(vv[2][i], b.z, fmaf(vv[1][i], b.y, vv[0][i] * b.x));
             }
+            if (g_trace_x >= 0)
+              for (int k = 0; k < 4; k++)
+                if (x + (k & 1) == g_trace_x && y + (k >> 1) == g_trace_y)
+                  fprintf(stderr, "trace tri v(%zu,%zu,%zu) quad (%d,%d) block (%d,%d) bb %.9g %.9g %.9g %.9g bc %d %d k=%d inside=%d shade=%d s_in=%d%d%d%d\n",
+                          tri.i[0], tri.i[1], tri.i[2], x, y, bx, by, minX, minY, maxX, maxY, bcx, bcy, k, (int)px[k].inside, px[k].shading,
+                          (int)px[k].s[0].inside, (int)px[k].s[1].inside, (int)px[k].s[2].inside, (int)px[k].s[3].inside);
             FragIn fi;
             for (int k = 0; k < 4; k++) fi.v[k] = px[k].vary.data();
             fi.front = tri.front;
@@ -1327,7 +1345,13 @@ struct Renderer {
 
 extern "C" {
 
-void cpu_ref_set_threads(int n) { cref::g_threads = n < 1 ? 1 : n; }
+void cpu_ref_set_math(int correctly_rounded) { cref::g_cr_math = correctly_rounded != 0; }
+
+void cpu_ref_set_threads(int n) {
+  cref::g_threads = n < 1 ? 1 : n;
+  const char *t = getenv("CPU_REF_TRACE");
+  if (t) sscanf(t, "%d,%d", &cref::g_trace_x, &cref::g_trace_y);
+}
 
 void cpu_ref_stats(long long *out8) { memcpy(out8, cref::g_stats, sizeof(cref::g_stats)); }
 

@@ -15,6 +15,10 @@ extern "C" {
  * stream or unsupported command, -2 when out_cap is too small. */
 long long cpu_ref_replay(const void *stream, size_t bytes, void *out, size_t out_cap, int repeats, double *frame_ms);
 
+/* 0: powf / log2f / exp2f from the host libm, as the reference calls them.
+ * 1: the same functions correctly rounded (double precision, one rounding). */
+void cpu_ref_set_math(int correctly_rounded);
+
 /* Worker threads for the fragment stage (1 = serial). Results do not depend on it. */
 void cpu_ref_set_threads(int n);
 

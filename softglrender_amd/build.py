@@ -1,0 +1,49 @@
+"""Builds librhip (gfx950) in-tree: softglrender_amd/lib/librhip.so.
+
+Every HIP translation unit is compiled with -ffp-contract=off (fused
+multiply-adds are written out where the reference's arithmetic has them) and
+with packed-FP32 instructions disabled.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB_DIR = os.path.join(HERE, "lib")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+HIP_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-I", os.path.join(ROOT, "include"), "-I", CSRC]
+
+TARGETS = {
+    "librhip.so": ["rhip_api.hip"],
+}
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"build failed: {' '.join(cmd[:3])} ...")
+
+
+def build(verbose: bool = False) -> None:
+    os.makedirs(LIB_DIR, exist_ok=True)
+    for lib, srcs in TARGETS.items():
+        out = os.path.join(LIB_DIR, lib)
+        tmp = out + ".tmp"
+        cmd = [HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *[os.path.join(CSRC, s) for s in srcs]]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        _run(cmd)
+        os.replace(tmp, out)
+
+
+if __name__ == "__main__":
+    build(verbose=True)

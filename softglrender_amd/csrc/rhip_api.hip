@@ -1,0 +1,943 @@
+/*
+ * rhip_api.hip -- librhip: the C-ABI declared in include/rhip.h.
+ *
+ * Host orchestration of RendererHIP: resources live in device memory, every
+ * call queues work on the device's HIP stream, and one draw runs the kernel
+ * pipeline of rhip_kernels.h.
+ */
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "rhip.h"
+#include "rhip_kernels.h"
+
+using namespace rhip;
+
+/* Every kernel launch goes through this macro (one place to change how kernels are started). */
+#ifndef RHIP_LAUNCH
+#define RHIP_LAUNCH(kern, grid, block, stream, ...) kern<<<(grid), (block), 0, (stream)>>>(__VA_ARGS__)
+#endif
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) return fail(RHIP_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+/* A grow-only device buffer. */
+struct DevBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes) {
+    if (bytes <= cap) return RHIP_OK;
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max(bytes, (size_t)4096);
+    want = want + want / 4;
+    if (hipMalloc(&p, want) != hipSuccess) return fail(RHIP_E_OOM, "device allocation of %zu bytes failed", want);
+    cap = want;
+    return RHIP_OK;
+  }
+  template <typename T>
+  T *as() const {
+    return (T *)p;
+  }
+  ~DevBuf() {
+    if (p) hipFree(p);
+  }
+};
+
+struct Texture {
+  rhip_texture_desc desc{};
+  rhip_sampler_desc samp{};
+  TexDesc hd{};               /* host copy of the descriptor */
+  TexDesc *dd = nullptr;      /* device copy */
+  uint8_t *data = nullptr;    /* all levels of all layers */
+  size_t bytes = 0;
+  uint8_t *resolved = nullptr; /* resolve target of a multisampled colour texture */
+  bool initialised = false;
+  ~Texture() {
+    if (dd) hipFree(dd);
+    if (data) hipFree(data);
+    if (resolved) hipFree(resolved);
+  }
+};
+
+struct Vao {
+  float *verts = nullptr;
+  int32_t *idx = nullptr;
+  int stride = 0, nverts = 0, nidx = 0;
+  ~Vao() {
+    if (verts) hipFree(verts);
+    if (idx) hipFree(idx);
+  }
+};
+
+struct Fbo {
+  int color = 0, color_layer = 0, color_level = 0, depth = 0;
+};
+
+struct Program {
+  int shading = 0;
+  uint32_t defines = 0;
+};
+
+}  // namespace
+
+struct rhip_device_s {
+  int hip_device = 0;
+  hipStream_t stream = nullptr;
+  int next_id = 1;
+  std::map<int, std::unique_ptr<Texture>> tex;
+  std::map<int, std::unique_ptr<Vao>> vao;
+  std::map<int, Fbo> fbo;
+  std::map<int, Program> prog;
+  int cur_fbo = 0;
+  int vp_x = 0, vp_y = 0, vp_w = 0, vp_h = 0;
+  /* per-draw arenas */
+  DevBuf clip, frag, mask, vary, clip_raw, clip_flag, clip_list, clip_poly, fan_count, fan_offset, prims,
+      tile_count, tile_offset, pair_key, pair_val, pair_key_s, pair_val_s, tile_start, cub_tmp, scalars;
+  unsigned long long *counters = nullptr; /* device */
+  /* optional per-kernel timing: an event pair around every launch */
+  bool profiling = false;
+  struct Timed {
+    const char *name;
+    hipEvent_t a, b;
+  };
+  std::vector<Timed> timed;
+  std::vector<hipEvent_t> event_pool;
+  int *host_scalars = nullptr;            /* pinned */
+  rhip_stats host_stats{};
+};
+
+namespace {
+
+int checkDev(rhip_device d) {
+  if (!d) return fail(RHIP_E_INVALID, "null device");
+  hipError_t e = hipSetDevice(d->hip_device);
+  if (e != hipSuccess) return fail(RHIP_E_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+  return RHIP_OK;
+}
+
+Texture *getTex(rhip_device d, int id) {
+  auto it = d->tex.find(id);
+  return it == d->tex.end() ? nullptr : it->second.get();
+}
+
+/* Texel index in a level's storage order (host side of texelIndex). */
+size_t hostTexelIndex(const TexLevel &l, int layout, int x, int y) {
+  if (layout == LAYOUT_TILED)
+    return ((size_t)((uint32_t)(y >> 2) * l.tiles_x + (uint32_t)(x >> 2)) << 4) + ((y & 3) << 2) + (x & 3);
+  if (layout == LAYOUT_MORTON) {
+    uint32_t r = (uint32_t)(x & 31) | ((uint32_t)(y & 31) << 16);
+    r = (r | (r << 4)) & 0x0f0f0f0fu;
+    r = (r | (r << 2)) & 0x33333333u;
+    r = (r | (r << 1)) & 0x55555555u;
+    r = (r | (r >> 15)) & 0xffffu;
+    return ((size_t)((uint32_t)(y >> 5) * l.tiles_x + (uint32_t)(x >> 5)) << 10) + r;
+  }
+  return (size_t)y * l.w + x;
+}
+
+int levelCount(int w, int h) { return (int)std::floor(std::log2((double)std::max(w, h))) + 1; }
+
+/* Allocates storage for every level of every layer in the texture's layout. */
+int allocTexture(rhip_device d, Texture &t) {
+  const rhip_texture_desc &ds = t.desc;
+  TexDesc &h = t.hd;
+  memset(&h, 0, sizeof h);
+  h.type = ds.type;
+  h.format = ds.format;
+  h.width = ds.width;
+  h.height = ds.height;
+  h.layers = ds.type == 1 ? 6 : 1;
+  h.msaa = ds.multi_sample ? 1 : 0;
+  /* attachments and multisampled images are always stored linearly */
+  h.layout = (ds.multi_sample || (ds.usage & 12)) ? LAYOUT_LINEAR : ds.layout;
+  h.levels = ds.use_mipmaps ? levelCount(ds.width, ds.height) : 1;
+  if (h.levels > kMaxLevels) return fail(RHIP_E_UNSUPPORTED, "texture too large for %d mip levels", kMaxLevels);
+  size_t texel = (size_t)4 * (h.msaa ? 4 : 1);
+  size_t off = 0;
+  for (int l = 0; l < h.layers; l++)
+    for (int v = 0; v < h.levels; v++) {
+      TexLevel &lv = h.lv[l][v];
+      lv.w = std::max(1, ds.width >> v);
+      lv.h = std::max(1, ds.height >> v);
+      int ts = h.layout == LAYOUT_TILED ? 4 : (h.layout == LAYOUT_MORTON ? 32 : 1);
+      lv.tiles_x = (lv.w + ts - 1) / ts;
+      lv.inner_w = lv.tiles_x * ts;
+      lv.inner_h = ((lv.h + ts - 1) / ts) * ts;
+      lv.offset = off;
+      off += (size_t)lv.inner_w * lv.inner_h * texel;
+      off = (off + 255) & ~(size_t)255;
+    }
+  t.bytes = off;
+  if (hipMalloc(&t.data, off) != hipSuccess) return fail(RHIP_E_OOM, "texture allocation of %zu bytes failed", off);
+  HIPCHK(hipMemsetAsync(t.data, 0, off, d->stream));
+  if (h.msaa && ds.format == 0) {
+    size_t rb = (size_t)ds.width * ds.height * 4;
+    if (hipMalloc(&t.resolved, rb) != hipSuccess) return fail(RHIP_E_OOM, "resolve target allocation failed");
+    HIPCHK(hipMemsetAsync(t.resolved, 0, rb, d->stream));
+  }
+  h.base = t.data;
+  h.resolved = t.resolved;
+  return RHIP_OK;
+}
+
+int syncDesc(rhip_device d, Texture &t) {
+  t.hd.filter_min = t.samp.filter_min;
+  t.hd.filter_mag = t.samp.filter_mag;
+  t.hd.wrap = t.samp.wrap_s; /* the reference samples with wrapS on both axes (SamplerSoft.h:392) */
+  t.hd.border = t.samp.border_color;
+  HIPCHK(hipMemcpyAsync(t.dd, &t.hd, sizeof(TexDesc), hipMemcpyHostToDevice, d->stream));
+  return RHIP_OK;
+}
+
+int fillLevel(rhip_device d, Texture &t, int layer, int level, const uint8_t *src) {
+  const TexLevel &lv = t.hd.lv[layer][level];
+  size_t n = (size_t)lv.inner_w * lv.inner_h;
+  std::vector<uint32_t> tmp(n, 0);
+  const uint32_t *s = (const uint32_t *)src;
+  for (int y = 0; y < lv.h; y++)
+    for (int x = 0; x < lv.w; x++) tmp[hostTexelIndex(lv, t.hd.layout, x, y)] = s[(size_t)y * lv.w + x];
+  HIPCHK(hipMemcpyAsync(t.data + lv.offset, tmp.data(), n * 4, hipMemcpyHostToDevice, d->stream));
+  HIPCHK(hipStreamSynchronize(d->stream));
+  return RHIP_OK;
+}
+
+/* decode the 256 / 64 / 48-byte uniform blocks (Material.h:63-95) */
+void decodeUniforms(const rhip_draw_desc &dd, Uniforms &u) {
+  const uint8_t *m = (const uint8_t *)dd.uniform_model, *s = (const uint8_t *)dd.uniform_scene,
+                *t = (const uint8_t *)dd.uniform_material;
+  memset(&u, 0, sizeof u);
+  if (m) {
+    memcpy(&u.reverse_z, m, 4);
+    memcpy(u.model, m + 16, 64);
+    memcpy(u.mvp, m + 80, 64);
+    memcpy(u.inv_t, m + 144, 48);
+    memcpy(u.shadow_mvp, m + 192, 64);
+  }
+  if (s) {
+    memcpy(u.ambient, s, 12);
+    memcpy(u.cam_pos, s + 16, 12);
+    memcpy(u.light_pos, s + 32, 12);
+    memcpy(u.light_color, s + 48, 12);
+  }
+  if (t) {
+    memcpy(&u.enable_light, t, 4);
+    memcpy(&u.enable_ibl, t + 4, 4);
+    memcpy(&u.enable_shadow, t + 8, 4);
+    memcpy(&u.point_size, t + 12, 4);
+    memcpy(&u.k_specular, t + 16, 4);
+    memcpy(u.base_color, t + 32, 16);
+  }
+}
+
+unsigned blocks(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+hipEvent_t takeEvent(rhip_device d) {
+  if (!d->event_pool.empty()) {
+    hipEvent_t e = d->event_pool.back();
+    d->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  hipEventCreate(&e);
+  return e;
+}
+
+/* Brackets one kernel launch with events when profiling is on. */
+struct Timer {
+  rhip_device d;
+  const char *name;
+  hipEvent_t a = nullptr;
+  Timer(rhip_device dev, const char *n) : d(dev), name(n) {
+    if (d->profiling) {
+      a = takeEvent(d);
+      hipEventRecord(a, d->stream);
+    }
+  }
+  ~Timer() {
+    if (a) {
+      hipEvent_t b = takeEvent(d);
+      hipEventRecord(b, d->stream);
+      d->timed.push_back({name, a, b});
+    }
+  }
+};
+
+template <int S>
+void launchRaster(const DrawParams &P, int ntiles, int need_deriv, hipStream_t st) {
+  switch (P.shading) {
+    case RHS_SHADING_BASIC:
+      {
+#define K_RASTER k_raster<S, RHS_SHADING_BASIC>
+      RHIP_LAUNCH(K_RASTER, ntiles, 256, st, P, need_deriv);
+#undef K_RASTER
+      }
+      break;
+    case RHS_SHADING_BLINN_PHONG:
+      {
+#define K_RASTER k_raster<S, RHS_SHADING_BLINN_PHONG>
+      RHIP_LAUNCH(K_RASTER, ntiles, 256, st, P, need_deriv);
+#undef K_RASTER
+      }
+      break;
+    case RHS_SHADING_BLINN_PHONG_DIR:
+      {
+#define K_RASTER k_raster<S, RHS_SHADING_BLINN_PHONG_DIR>
+      RHIP_LAUNCH(K_RASTER, ntiles, 256, st, P, need_deriv);
+#undef K_RASTER
+      }
+      break;
+    case RHS_SHADING_PBR:
+      {
+#define K_RASTER k_raster<S, RHS_SHADING_PBR>
+      RHIP_LAUNCH(K_RASTER, ntiles, 256, st, P, need_deriv);
+#undef K_RASTER
+      }
+      break;
+    default:
+      {
+#define K_RASTER k_raster<S, RHS_SHADING_VERTEX_COLOR>
+      RHIP_LAUNCH(K_RASTER, ntiles, 256, st, P, need_deriv);
+#undef K_RASTER
+      }
+      break;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *rhip_last_error(void) { return g_err.c_str(); }
+
+int rhip_device_create(int hip_device, rhip_device *out) {
+  if (!out) return fail(RHIP_E_INVALID, "null output");
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (hip_device < 0 || hip_device >= n) return fail(RHIP_E_INVALID, "no HIP device %d (%d present)", hip_device, n);
+  HIPCHK(hipSetDevice(hip_device));
+  std::unique_ptr<rhip_device_s> d(new rhip_device_s());
+  d->hip_device = hip_device;
+  HIPCHK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+  HIPCHK(hipMalloc(&d->counters, 8 * sizeof(unsigned long long)));
+  HIPCHK(hipMemset(d->counters, 0, 8 * sizeof(unsigned long long)));
+  HIPCHK(hipHostMalloc((void **)&d->host_scalars, 64, hipHostMallocDefault));
+  *out = d.release();
+  return RHIP_OK;
+}
+
+int rhip_device_destroy(rhip_device d) {
+  if (!d) return RHIP_OK;
+  hipSetDevice(d->hip_device);
+  hipStreamSynchronize(d->stream);
+  d->tex.clear();
+  d->vao.clear();
+  if (d->counters) hipFree(d->counters);
+  if (d->host_scalars) hipHostFree(d->host_scalars);
+  hipStreamDestroy(d->stream);
+  delete d;
+  return RHIP_OK;
+}
+
+void *rhip_device_stream(rhip_device d) { return d ? (void *)d->stream : nullptr; }
+
+/* ------------------------------------------------------------------ textures */
+
+int rhip_texture_create(rhip_device d, const rhip_texture_desc *desc, int *out) {
+  int rc = checkDev(d);
+  if (rc) return rc;
+  if (!desc || !out) return fail(RHIP_E_INVALID, "null argument");
+  if (desc->width <= 0 || desc->height <= 0 || desc->width > 16384 || desc->height > 16384)
+    return fail(RHIP_E_INVALID, "texture size %dx%d out of range", desc->width, desc->height);
+  if (desc->type != 0 && desc->type != 1) return fail(RHIP_E_INVALID, "texture type %d", desc->type);
+  if (desc->format != 0 && desc->format != 1) return fail(RHIP_E_INVALID, "texture format %d", desc->format);
+  if (desc->layout < 0 || desc->layout > 2) return fail(RHIP_E_INVALID, "texture layout %d", desc->layout);
+  std::unique_ptr<Texture> t(new Texture());
+  t->desc = *desc;
+  t->samp.filter_min = t->samp.filter_mag = 0;
+  t->samp.wrap_s = t->samp.wrap_t = t->samp.wrap_r = 2;
+  if ((rc = allocTexture(d, *t))) return rc;
+  HIPCHK(hipMalloc(&t->dd, sizeof(TexDesc)));
+  if ((rc = syncDesc(d, *t))) return rc;
+  int id = d->next_id++;
+  d->tex[id] = std::move(t);
+  *out = id;
+  return RHIP_OK;
+}
+
+int rhip_texture_set_sampler(rhip_device d, int id, const rhip_sampler_desc *s) {
+  int rc = checkDev(d);
+  if (rc) return rc;
+  Texture *t = getTex(d, id);
+  if (!t || !s) return fail(RHIP_E_INVALID, "invalid texture %d", id);
+  if (s->filter_min < 0 || s->filter_min > 5 || s->wrap_s < 0 || s->wrap_s > 3)
+    return fail(RHIP_E_INVALID, "invalid sampler description");
+  t->samp = *s;
+  return syncDesc(d, *t);
+}
+
+int rhip_texture_init(rhip_device d, int id) {
+  int rc = checkDev(d);
+  if (rc) return rc;
+  Texture *t = getTex(d, id);
+  if (!t) return fail(RHIP_E_INVALID, "invalid texture %d", id);
+  t->initialised = true; /* storage is allocated and zeroed at creation */
+  return RHIP_OK;
+}
+
+int rhip_texture_upload(rhip_device d, int id, int layers, const void *data, size_t bytes) {
+  int rc = checkDev(d);
+  if (rc) return rc;
+  Texture *t = getTex(d, id);
+  if (!t || !data) return fail(RHIP_E_INVALID, "invalid texture %d", id);
+  if (t->desc.multi_sample) return fail(RHIP_E_INVALID, "setImageData not supported on a multisampled texture");
+  if (layers != t->hd.layers) return fail(RHIP_E_INVALID, "texture has %d layers, got %d", t->hd.layers, layers);
+  size_t per = (size_t)t->desc.width * t->desc.height * 4;
+  if (bytes < per * layers) return fail(RHIP_E_INVALID, "image data too small: %zu < %zu", bytes, per * layers);
+  for (int l = 0; l < layers; l++)
+    if ((rc = fillLevel(d, *t, l, 0, (const uint8_t *)data + per * l))) return rc;
+  for (int l = 0; l < layers; l++)
+    for (int v = 1; v < t->hd.levels; v++) {
+      const TexLevel &lv = t->hd.lv[l][v];
+      dim3 b(16, 16), g(blocks(lv.w, 16), blocks(lv.h, 16));
+      if (t->desc.format == 0)
+        {
+#define K_MIP k_mipgen<uint32_t>
+        RHIP_LAUNCH(K_MIP, g, b, d->stream, t->dd, l, v);
+#undef K_MIP
+      }
+      else
+        {
+#define K_MIP k_mipgen<float>
+        RHIP_LAUNCH(K_MIP, g, b, d->stream, t->dd, l, v);
+#undef K_MIP
+      }
+      HIPCHK(hipGetLastError());
+    }
+  t->initialised = true;
+  return RHIP_OK;
+}
+
+int rhip_texture_upload_level(rhip_device d, int id, int layer, int level, const void *data, size_t bytes) {
+  int rc = checkDev(d);
+  if (rc) return rc;
+  Texture *t = getTex(d, id);
+  if (!t || !data) return fail(RHIP_E_INVALID, "invalid texture %d", id);
+  if (t->desc.multi_sample) return fail(RHIP_E_INVALID, "multisampled texture");
+  if (layer < 0 || layer >= t->hd.layers || level < 0 || level >= t->hd.levels)
+    return fail(RHIP_E_INVALID, "layer %d / level %d out of range", layer, level);
+  const TexLevel &lv = t->hd.lv[layer][level];
+  if (bytes < (size_t)lv.w * lv.h * 4) return fail(RHIP_E_INVALID, "level data too small");
+  return fillLevel(d, *t, layer, level, (const uint8_t *)data);
+}
+
+int rhip_texture_read(rhip_device d, int id, int layer, int level, int which, void *out, size_t bytes) {
+  int rc = checkDev(d);
+  if (rc) return rc;
+  Texture *t = getTex(d, id);
+  if (!t || !out) return fail(RHIP_E_INVALID, "invalid texture %d", id);
+  if (layer < 0 || layer >= t->hd.layers || level < 0 || level >= t->hd.levels)
+    return fail(RHIP_E_INVALID, "layer %d / level %d out of range", layer, level);
+  const TexLevel &lv = t->hd.lv[layer][level];
+  if (which == 1) {
+    if (!t->desc.multi_sample) return fail(RHIP_E_INVALID, "texture is not multisampled");
+    size_t n = (size_t)lv.w * lv.h * 16;
+    if (bytes < n) return fail(RHIP_E_INVALID, "output too small");
+    HIPCHK(hipMemcpyAsync(out, t->data + lv.offset, n, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    return RHIP_OK;
+  }
+  size_t n = (size_t)lv.w * lv.h * 4;
+  if (bytes < n) return fail(RHIP_E_INVALID, "output too small");
+  if (t->desc.multi_sample) {
+    if (!t->resolved) return fail(RHIP_E_INVALID, "a multisampled depth texture has no single-sample image");
+    HIPCHK(hipMemcpyAsync(out, t->resolved, n, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    return RHIP_OK;
+  }
+  std::vector<uint32_t> tmp((size_t)lv.inner_w * lv.inner_h);
+  HIPCHK(hipMemcpyAsync(tmp.data(), t->data + lv.offset, tmp.size() * 4, hipMemcpyDeviceToHost, d->stream));
+  HIPCHK(hipStreamSynchronize(d->stream));
+  uint32_t *o = (uint32_t *)out;
+  for (int y = 0; y < lv.h; y++)
+    for (int x = 0; x < lv.w; x++) o[(size_t)y * lv.w + x] = tmp[hostTexelIndex(lv, t->hd.layout, x, y)];
+  return RHIP_OK;
+}
+
+int rhip_texture_destroy(rhip_device d, int id) {
+  int rc = checkDev(d);
+  if (rc) return rc;
+  if (!d->tex.count(id)) return fail(RHIP_E_INVALID, "invalid texture %d", id);
+  HIPCHK(hipStreamSynchronize(d->stream));
+  d->tex.erase(id);
+  return RHIP_OK;
+}
+
+/* ------------------------------------------------------------------ vertex arrays */
+
+int rhip_vao_create(rhip_device d, const void *vertices, int stride, int nverts, const int32_t *indices,
+                    int nindices, int *out) {
+  int rc = checkDev(d);
+  if (rc) return rc;
+  if (!out || !vertices || !indices) return fail(RHIP_E_INVALID, "null argument");
+  if (stride <= 0 || stride % 4 || stride > 64) return fail(RHIP_E_UNSUPPORTED, "vertex stride %d (need 4..64, x4)", stride);
+  if (nverts <= 0 || nindices < 0 || nindices % 3) return fail(RHIP_E_INVALID, "bad vertex/index count");
+  for (int i = 0; i < nindices; i++)
+    if (indices[i] < 0 || indices[i] >= nverts) return fail(RHIP_E_INVALID, "index %d out of range at %d", indices[i], i);
+  std::unique_ptr<Vao> v(new Vao());
+  v->stride = stride;
+  v->nverts = nverts;
+  v->nidx = nindices;
+  if (hipMalloc(&v->verts, (size_t)stride * nverts) != hipSuccess) return fail(RHIP_E_OOM, "vertex buffer");
+  if (hipMalloc(&v->idx, (size_t)std::max(nindices, 1) * 4) != hipSuccess) return fail(RHIP_E_OOM, "index buffer");
+  HIPCHK(hipMemcpyAsync(v->verts, vertices, (size_t)stride * nverts, hipMemcpyHostToDevice, d->stream));
+  if (nindices) HIPCHK(hipMemcpyAsync(v->idx, indices, (size_t)nindices * 4, hipMemcpyHostToDevice, d->stream));
+  HIPCHK(hipStreamSynchronize(d->stream));
+  int id = d->next_id++;
+  d->vao[id] = std::move(v);
+  *out = id;
+  return RHIP_OK;
+}
+
+int rhip_vao_update(rhip_device d, int id, const void *vertices, size_t bytes) {
+  int rc = checkDev(d);
+  if (rc) return rc;
+  auto it = d->vao.find(id);
+  if (it == d->vao.end() || !vertices) return fail(RHIP_E_INVALID, "invalid vao %d", id);
+  size_t n = std::min(bytes, (size_t)it->second->stride * it->second->nverts);
+  HIPCHK(hipMemcpyAsync(it->second->verts, vertices, n, hipMemcpyHostToDevice, d->stream));
+  HIPCHK(hipStreamSynchronize(d->stream));
+  return RHIP_OK;
+}
+
+int rhip_vao_destroy(rhip_device d, int id) {
+  int rc = checkDev(d);
+  if (rc) return rc;
+  if (!d->vao.count(id)) return fail(RHIP_E_INVALID, "invalid vao %d", id);
+  HIPCHK(hipStreamSynchronize(d->stream));
+  d->vao.erase(id);
+  return RHIP_OK;
+}
+
+/* ------------------------------------------------------------------ framebuffers / programs */
+
+int rhip_fbo_create(rhip_device d, int *out) {
+  int rc = checkDev(d);
+  if (rc) return rc;
+  if (!out) return fail(RHIP_E_INVALID, "null output");
+  int id = d->next_id++;
+  d->fbo[id] = Fbo();
+  *out = id;
+  return RHIP_OK;
+}
+
+int rhip_fbo_attach_color(rhip_device d, int id, int tex, int layer, int level) {
+  auto it = d ? d->fbo.find(id) : decltype(d->fbo.end())();
+  if (!d || it == d->fbo.end()) return fail(RHIP_E_INVALID, "invalid framebuffer %d", id);
+  Texture *t = getTex(d, tex);
+  if (!t || t->desc.format != 0) return fail(RHIP_E_INVALID, "colour attachment must be an RGBA8 texture");
+  if (layer < 0 || layer >= t->hd.layers || level < 0 || level >= t->hd.levels)
+    return fail(RHIP_E_INVALID, "attachment layer/level out of range");
+  it->second.color = tex;
+  it->second.color_layer = layer;
+  it->second.color_level = level;
+  return RHIP_OK;
+}
+
+int rhip_fbo_attach_depth(rhip_device d, int id, int tex) {
+  auto it = d ? d->fbo.find(id) : decltype(d->fbo.end())();
+  if (!d || it == d->fbo.end()) return fail(RHIP_E_INVALID, "invalid framebuffer %d", id);
+  Texture *t = getTex(d, tex);
+  if (!t || t->desc.format != 1) return fail(RHIP_E_INVALID, "depth attachment must be a FLOAT32 texture");
+  it->second.depth = tex;
+  return RHIP_OK;
+}
+
+int rhip_fbo_destroy(rhip_device d, int id) {
+  if (!d || !d->fbo.erase(id)) return fail(RHIP_E_INVALID, "invalid framebuffer %d", id);
+  if (d->cur_fbo == id) d->cur_fbo = 0;
+  return RHIP_OK;
+}
+
+int rhip_program_create(rhip_device d, int shading, uint32_t defines, int *out) {
+  if (!d || !out) return fail(RHIP_E_INVALID, "null argument");
+  if (shading < RHS_SHADING_BASIC || shading > RHS_SHADING_VERTEX_COLOR)
+    return fail(RHIP_E_UNSUPPORTED, "shading model %d", shading);
+  if (defines & ~31u) return fail(RHIP_E_INVALID, "unknown define bits 0x%x", defines);
+  int id = d->next_id++;
+  d->prog[id] = Program{shading, defines};
+  *out = id;
+  return RHIP_OK;
+}
+
+/* ------------------------------------------------------------------ pipeline */
+
+int rhip_begin_render_pass(rhip_device d, int fbo, const rhip_clear_states *cs) {
+  int rc = checkDev(d);
+  if (rc) return rc;
+  auto it = d->fbo.find(fbo);
+  if (it == d->fbo.end()) {
+    d->cur_fbo = 0; /* the reference ignores draws without a framebuffer (RendererSoft.cpp:62-66) */
+    return fail(RHIP_E_INVALID, "invalid framebuffer %d", fbo);
+  }
+  d->cur_fbo = fbo;
+  if (!cs) return RHIP_OK;
+  const Fbo &f = it->second;
+  /* RendererSoft::beginRenderPass (RendererSoft.cpp:61-90): u8 truncation of clear * 255 */
+  if (cs->color_flag && f.color) {
+    Texture *t = getTex(d, f.color);
+    if (!t) return fail(RHIP_E_INVALID, "colour attachment was destroyed");
+    auto u8 = [](float v) { return (uint32_t)(uint8_t)(int32_t)(v * 255); };
+    uint32_t px = u8(cs->clear_color[0]) | (u8(cs->clear_color[1]) << 8) | (u8(cs->clear_color[2]) << 16) |
+                  (u8(cs->clear_color[3]) << 24);
+    const TexLevel &lv = t->hd.lv[f.color_layer][f.color_level];
+    size_t n = (size_t)lv.w * lv.h * (t->hd.msaa ? 4 : 1);
+    RHIP_LAUNCH(k_fill_u32, std::min(blocks(n, 256), 4096u), 256, d->stream, (uint32_t *)(t->data + lv.offset), n, px);
+    HIPCHK(hipGetLastError());
+  }
+  if (cs->depth_flag && f.depth) {
+    Texture *t = getTex(d, f.depth);
+    if (!t) return fail(RHIP_E_INVALID, "depth attachment was destroyed");
+    const TexLevel &lv = t->hd.lv[0][0];
+    size_t n = (size_t)lv.w * lv.h * (t->hd.msaa ? 4 : 1);
+    uint32_t bits;
+    memcpy(&bits, &cs->clear_depth, 4);
+    RHIP_LAUNCH(k_fill_u32, std::min(blocks(n, 256), 4096u), 256, d->stream, (uint32_t *)(t->data + lv.offset), n, bits);
+    HIPCHK(hipGetLastError());
+  }
+  return RHIP_OK;
+}
+
+int rhip_set_viewport(rhip_device d, int x, int y, int w, int h) {
+  if (!d) return fail(RHIP_E_INVALID, "null device");
+  d->vp_x = x;
+  d->vp_y = y;
+  d->vp_w = w;
+  d->vp_h = h;
+  return RHIP_OK;
+}
+
+int rhip_end_render_pass(rhip_device d) { return d ? RHIP_OK : fail(RHIP_E_INVALID, "null device"); }
+
+int rhip_wait_idle(rhip_device d) {
+  int rc = checkDev(d);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(d->stream));
+  return RHIP_OK;
+}
+
+int rhip_draw(rhip_device d, const rhip_draw_desc *dd) {
+  int rc = checkDev(d);
+  if (rc) return rc;
+  if (!dd) return fail(RHIP_E_INVALID, "null draw");
+  if (!d->cur_fbo) return RHIP_OK; /* RendererSoft::draw returns when no framebuffer is bound (:137-139) */
+  auto vit = d->vao.find(dd->vao);
+  auto pit = d->prog.find(dd->program);
+  if (vit == d->vao.end()) return fail(RHIP_E_INVALID, "invalid vao %d", dd->vao);
+  if (pit == d->prog.end()) return fail(RHIP_E_INVALID, "invalid program %d", dd->program);
+  const Vao &va = *vit->second;
+  const Program &pg = pit->second;
+  const RhsStates &st = dd->states;
+  if (st.primitiveType != 2 || st.polygonMode != 2)
+    return fail(RHIP_E_UNSUPPORTED, "only filled triangles are implemented (primitive %d, polygon mode %d)",
+                st.primitiveType, st.polygonMode);
+  if (st.depthFunc < 0 || st.depthFunc > 7 || st.blendFuncRgb < 0 || st.blendFuncRgb > 4)
+    return fail(RHIP_E_INVALID, "invalid render states");
+  const Fbo &f = d->fbo[d->cur_fbo];
+  Texture *ct = f.color ? getTex(d, f.color) : nullptr;
+  Texture *dt = f.depth ? getTex(d, f.depth) : nullptr;
+  if (!ct && !dt) return RHIP_OK;
+  if (d->vp_w <= 0 || d->vp_h <= 0) return fail(RHIP_E_INVALID, "viewport not set");
+
+  DrawParams P;
+  memset(&P, 0, sizeof P);
+  decodeUniforms(*dd, P.u);
+  P.st = st;
+  P.shading = pg.shading;
+  P.defines = pg.defines;
+  bool need_deriv = false;
+  for (int s = 0; s < RHS_SLOT_COUNT; s++) {
+    int id = dd->sampler_tex[s];
+    if (!id) continue;
+    Texture *t = getTex(d, id);
+    if (!t) return fail(RHIP_E_INVALID, "sampler slot %d: invalid texture %d", s, id);
+    if (!t->initialised) return fail(RHIP_E_INVALID, "sampler slot %d: texture %d has no image", s, id);
+    P.tex[s] = t->dd;
+    if (t->samp.filter_min > 1 && s != RHS_SLOT_METAL_ROUGHNESS && s <= RHS_SLOT_AO) need_deriv = true;
+  }
+  P.vp_w = (float)d->vp_w;
+  P.vp_h = (float)d->vp_h;
+  P.o_x = (float)d->vp_x + P.vp_w / 2.f;
+  P.o_y = (float)d->vp_y + P.vp_h / 2.f;
+  P.p_x = P.vp_w / 2.f;
+  P.p_y = P.vp_h / 2.f;
+  int fbw = 0, fbh = 0, samples = 1;
+  if (ct) {
+    const TexLevel &lv = ct->hd.lv[f.color_layer][f.color_level];
+    P.color = (uint32_t *)(ct->data + lv.offset);
+    P.color_resolved = (uint32_t *)ct->resolved;
+    fbw = lv.w;
+    fbh = lv.h;
+    samples = ct->hd.msaa ? 4 : 1;
+  }
+  if (dt) {
+    const TexLevel &lv = dt->hd.lv[0][0];
+    P.depth = (float *)(dt->data + lv.offset);
+    if (!ct) {
+      fbw = lv.w;
+      fbh = lv.h;
+      samples = dt->hd.msaa ? 4 : 1;
+    } else if (lv.w != fbw || lv.h != fbh || (dt->hd.msaa ? 4 : 1) != samples) {
+      return fail(RHIP_E_UNSUPPORTED, "colour and depth attachments differ in size or sample count");
+    }
+  }
+  P.fb_w = fbw;
+  P.fb_h = fbh;
+  P.samples = samples;
+  P.tiles_x = (fbw + kTile - 1) / kTile;
+  P.tiles_y = (fbh + kTile - 1) / kTile;
+  const int ntiles = P.tiles_x * P.tiles_y;
+  P.verts = va.verts;
+  P.stride_f = va.stride / 4;
+  P.nverts = va.nverts;
+  P.idx = va.idx;
+  P.ntris = va.nidx / 3;
+  P.counters = d->counters;
+  hipStream_t s = d->stream;
+  const size_t V = (size_t)va.nverts, T = (size_t)P.ntris;
+
+  /* vertex stage; room for the vertices clipping may add */
+  const size_t reserveClip = std::min<size_t>(T, 4096);
+  size_t vcap = V + reserveClip * kMaxClipVerts;
+  if ((rc = d->clip.ensure(vcap * 16)) || (rc = d->frag.ensure(vcap * 16)) || (rc = d->mask.ensure(vcap * 4)) ||
+      (rc = d->vary.ensure(vcap * kVaryStride * 4)))
+    return rc;
+  P.clip = d->clip.as<float4>();
+  P.frag = d->frag.as<float4>();
+  P.mask = d->mask.as<int>();
+  P.vary = d->vary.as<float>();
+  if (T == 0) return RHIP_OK;
+  { Timer tm(d, "k_vertex"); RHIP_LAUNCH(k_vertex, blocks(V, 256), 256, s, P); }
+  HIPCHK(hipGetLastError());
+
+  /* setup */
+  size_t pcap = T + reserveClip * 6;
+  if ((rc = d->prims.ensure(pcap * sizeof(PrimSetup))) || (rc = d->tile_count.ensure((pcap + 1) * 4)) ||
+      (rc = d->tile_offset.ensure((pcap + 1) * 4)) || (rc = d->clip_flag.ensure(T * 4)) ||
+      (rc = d->scalars.ensure(64)))
+    return rc;
+  P.prims = d->prims.as<PrimSetup>();
+  P.tile_count = d->tile_count.as<int>();
+  P.tile_offset = d->tile_offset.as<int>();
+  int *dsc = d->scalars.as<int>();
+  RHIP_LAUNCH(k_fill_u32, 1, 64, s, (uint32_t *)dsc, 16, 0u);
+  { Timer tm(d, "k_setup"); RHIP_LAUNCH(k_setup, blocks(T, 256), 256, s, P, d->clip_flag.as<int>(), dsc); }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(d->host_scalars, dsc, 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  const int nclip = d->host_scalars[0];
+  int nappend = 0;
+  if (nclip > 0) {
+    /* grow the vertex and primitive arenas for the clipped triangles */
+    size_t need_v = V + (size_t)nclip * kMaxClipVerts, need_p = T + (size_t)nclip * 6;
+    if (need_v > vcap) {
+      DevBuf *bufs[4] = {&d->clip, &d->frag, &d->mask, &d->vary};
+      size_t esz[4] = {16, 16, 4, kVaryStride * 4};
+      for (int b = 0; b < 4; b++) {
+        void *np = nullptr;
+        if (hipMalloc(&np, need_v * esz[b]) != hipSuccess) return fail(RHIP_E_OOM, "clip vertex arena");
+        HIPCHK(hipMemcpyAsync(np, bufs[b]->p, V * esz[b], hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        hipFree(bufs[b]->p);
+        bufs[b]->p = np;
+        bufs[b]->cap = need_v * esz[b];
+      }
+      P.clip = d->clip.as<float4>();
+      P.frag = d->frag.as<float4>();
+      P.mask = d->mask.as<int>();
+      P.vary = d->vary.as<float>();
+    }
+    if (need_p > pcap) {
+      DevBuf *bufs[3] = {&d->prims, &d->tile_count, &d->tile_offset};
+      size_t esz[3] = {sizeof(PrimSetup), 4, 4};
+      for (int b = 0; b < 3; b++) {
+        void *np = nullptr;
+        if (hipMalloc(&np, (need_p + 1) * esz[b]) != hipSuccess) return fail(RHIP_E_OOM, "clip primitive arena");
+        HIPCHK(hipMemcpyAsync(np, bufs[b]->p, T * esz[b], hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        hipFree(bufs[b]->p);
+        bufs[b]->p = np;
+        bufs[b]->cap = (need_p + 1) * esz[b];
+      }
+      P.prims = d->prims.as<PrimSetup>();
+      P.tile_count = d->tile_count.as<int>();
+      P.tile_offset = d->tile_offset.as<int>();
+    }
+    if ((rc = d->clip_raw.ensure((size_t)nclip * kMaxClipVerts * va.stride)) ||
+        (rc = d->clip_list.ensure((size_t)nclip * 4 + 4)) ||
+        (rc = d->clip_poly.ensure((size_t)nclip * (kMaxPoly + 1) * 4)) ||
+        (rc = d->fan_count.ensure((size_t)nclip * 4 + 4)) || (rc = d->fan_offset.ensure((size_t)nclip * 4 + 4)))
+      return rc;
+    P.clip_raw = d->clip_raw.as<float>();
+    P.clip_list = d->clip_list.as<int>();
+    P.clip_poly = d->clip_poly.as<int>();
+    P.fan_offset = d->fan_offset.as<int>();
+    P.nclip = nclip;
+    /* compact the triangle ids that need clipping, in order */
+    hipcub::CountingInputIterator<int> ids(0);
+    size_t tmp = 0;
+    HIPCHK(hipcub::DeviceSelect::Flagged(nullptr, tmp, ids, d->clip_flag.as<int>(), P.clip_list, dsc + 1, (int)T, s));
+    if ((rc = d->cub_tmp.ensure(tmp))) return rc;
+    HIPCHK(hipcub::DeviceSelect::Flagged(d->cub_tmp.p, tmp, ids, d->clip_flag.as<int>(), P.clip_list, dsc + 1, (int)T,
+                                         s));
+    RHIP_LAUNCH(k_clip, blocks(nclip, 64), 64, s, P);
+    HIPCHK(hipGetLastError());
+    RHIP_LAUNCH(k_fan_count, blocks(nclip, 256), 256, s, P, d->fan_count.as<int>());
+    tmp = 0;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d->fan_count.as<int>(), P.fan_offset, nclip, s));
+    if ((rc = d->cub_tmp.ensure(tmp))) return rc;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(d->cub_tmp.p, tmp, d->fan_count.as<int>(), P.fan_offset, nclip, s));
+    HIPCHK(hipMemcpyAsync(d->host_scalars + 2, P.fan_offset + (nclip - 1), 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(d->host_scalars + 3, d->fan_count.as<int>() + (nclip - 1), 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    nappend = d->host_scalars[2] + d->host_scalars[3];
+    RHIP_LAUNCH(k_clip_fans, blocks(nclip, 64), 64, s, P);
+    HIPCHK(hipGetLastError());
+    d->host_stats.clipped_triangles += nclip;
+  }
+  P.nprims = (int)T + nappend;
+
+  /* binning: per-primitive tile counts -> offsets -> (tile, prim) pairs -> stable sort by tile */
+  size_t tmp = 0;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, P.tile_count, P.tile_offset, P.nprims + 1, s));
+  if ((rc = d->cub_tmp.ensure(tmp))) return rc;
+  RHIP_LAUNCH(k_fill_u32, 1, 64, s, (uint32_t *)(P.tile_count + P.nprims), 1, 0u);
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(d->cub_tmp.p, tmp, P.tile_count, P.tile_offset, P.nprims + 1, s));
+  HIPCHK(hipMemcpyAsync(d->host_scalars + 4, P.tile_offset + P.nprims, 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  const int npairs = d->host_scalars[4];
+  if ((rc = d->pair_key.ensure((size_t)npairs * 4 + 4)) || (rc = d->pair_val.ensure((size_t)npairs * 4 + 4)) ||
+      (rc = d->pair_key_s.ensure((size_t)npairs * 4 + 4)) || (rc = d->pair_val_s.ensure((size_t)npairs * 4 + 4)) ||
+      (rc = d->tile_start.ensure((size_t)(ntiles + 1) * 4)))
+    return rc;
+  P.pair_key = d->pair_key.as<uint32_t>();
+  P.pair_val = d->pair_val.as<int>();
+  P.pair_key_sorted = d->pair_key_s.as<uint32_t>();
+  P.pair_val_sorted = d->pair_val_s.as<int>();
+  P.npairs = npairs;
+  P.tile_start = d->tile_start.as<int>();
+  if (npairs > 0) {
+    { Timer tm(d, "k_emit"); RHIP_LAUNCH(k_emit, blocks(P.nprims, 256), 256, s, P); }
+    HIPCHK(hipGetLastError());
+    int bits = 1;
+    while ((1 << bits) < ntiles) bits++;
+    tmp = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, P.pair_key, P.pair_key_sorted, P.pair_val,
+                                              P.pair_val_sorted, npairs, 0, bits, s));
+    if ((rc = d->cub_tmp.ensure(tmp))) return rc;
+    {
+      Timer tm(d, "radix_sort_pairs");
+      HIPCHK(hipcub::DeviceRadixSort::SortPairs(d->cub_tmp.p, tmp, P.pair_key, P.pair_key_sorted, P.pair_val,
+                                                P.pair_val_sorted, npairs, 0, bits, s));
+    }
+  }
+  RHIP_LAUNCH(k_tile_ranges, blocks(ntiles + 1, 256), 256, s, P);
+  HIPCHK(hipGetLastError());
+
+  {
+    Timer tm(d, samples == 4 ? "k_raster_msaa4" : "k_raster");
+    if (samples == 4)
+      launchRaster<4>(P, ntiles, need_deriv ? 1 : 0, s);
+    else
+      launchRaster<1>(P, ntiles, need_deriv ? 1 : 0, s);
+  }
+  HIPCHK(hipGetLastError());
+  d->host_stats.draws++;
+  d->host_stats.tile_pairs += npairs;
+  d->host_stats.triangles_binned += P.nprims;
+  return RHIP_OK;
+}
+
+int rhip_profile_enable(rhip_device d, int on) {
+  int rc = checkDev(d);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(d->stream));
+  for (auto &t : d->timed) {
+    d->event_pool.push_back(t.a);
+    d->event_pool.push_back(t.b);
+  }
+  d->timed.clear();
+  d->profiling = on != 0;
+  return RHIP_OK;
+}
+
+int rhip_profile_query(rhip_device d, int index, char *name, int name_cap, double *total_ms, long long *count) {
+  int rc = checkDev(d);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(d->stream));
+  std::vector<std::pair<std::string, std::pair<double, long long>>> agg;
+  for (auto &t : d->timed) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, t.a, t.b));
+    auto it = std::find_if(agg.begin(), agg.end(), [&](const auto &e) { return e.first == t.name; });
+    if (it == agg.end()) agg.push_back({t.name, {ms, 1}});
+    else it->second.first += ms, it->second.second++;
+  }
+  if (index < 0 || index >= (int)agg.size()) return fail(RHIP_E_INVALID, "profile entry %d of %zu", index, agg.size());
+  if (name && name_cap > 0) snprintf(name, name_cap, "%s", agg[index].first.c_str());
+  if (total_ms) *total_ms = agg[index].second.first;
+  if (count) *count = agg[index].second.second;
+  return RHIP_OK;
+}
+
+int rhip_stats_get(rhip_device d, rhip_stats *out) {
+  int rc = checkDev(d);
+  if (rc) return rc;
+  if (!out) return fail(RHIP_E_INVALID, "null output");
+  unsigned long long c[8];
+  HIPCHK(hipMemcpyAsync(c, d->counters, sizeof c, hipMemcpyDeviceToHost, d->stream));
+  HIPCHK(hipStreamSynchronize(d->stream));
+  *out = d->host_stats;
+  out->fs_invocations = (long long)c[0];
+  out->samples_written = (long long)c[1];
+  return RHIP_OK;
+}
+
+int rhip_stats_reset(rhip_device d) {
+  int rc = checkDev(d);
+  if (rc) return rc;
+  HIPCHK(hipMemsetAsync(d->counters, 0, 8 * sizeof(unsigned long long), d->stream));
+  d->host_stats = rhip_stats();
+  return RHIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,110 @@
+/*
+ * rhip_internal.h -- POD records shared by librhip's host code and its gfx950
+ * kernels. Nothing here is part of the public C-ABI.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rhip_stream.h"
+
+namespace rhip {
+
+constexpr int kMaxLevels = 16;
+constexpr int kMaxLayers = 6;
+constexpr int kTile = 16;         /* screen tile edge in pixels (one workgroup) */
+constexpr int kSub = 8;           /* 8x8 sub-tile per wavefront */
+constexpr int kVaryStride = 32;   /* floats per vertex in the varyings buffer */
+constexpr int kMaxClipVerts = 12; /* new vertices one clipped triangle can create */
+constexpr int kMaxPoly = 9;       /* polygon vertices after six planes */
+
+enum Layout { LAYOUT_LINEAR = 0, LAYOUT_TILED = 1, LAYOUT_MORTON = 2 };
+
+/* One mip level of one layer. `inner_w` is the padded row length in texels
+ * (whole 4x4 or 32x32 tiles for the tiled layouts), `tiles_x` the tile count per row. */
+struct TexLevel {
+  uint64_t offset; /* bytes from TexDesc::base */
+  int w, h;
+  int inner_w, inner_h;
+  int tiles_x;
+  int pad;
+};
+
+/* Device-resident texture descriptor (read with scalar loads by the kernels). */
+struct TexDesc {
+  uint8_t *base;
+  uint8_t *resolved; /* single-sample image of a multisampled colour target */
+  int type, format, width, height;
+  int layers, levels, layout, msaa;
+  int filter_min, filter_mag, wrap, border;
+  TexLevel lv[kMaxLayers][kMaxLevels];
+};
+
+/* Uniform blocks decoded to floats (Material.h:63-95 layouts). */
+struct Uniforms {
+  int reverse_z;
+  float model[16], mvp[16], inv_t[12], shadow_mvp[16];
+  float ambient[3], cam_pos[3], light_pos[3], light_color[3];
+  int enable_light, enable_ibl, enable_shadow;
+  float point_size, k_specular;
+  float base_color[4];
+};
+
+/* Per-primitive setup record (after clipping / culling). */
+struct PrimSetup {
+  int v[3];            /* vertex ids */
+  int flags;           /* bit0 live, bit1 front facing */
+  float min_x, min_y;  /* triangleBoundingBox min - 1 (RendererSoft.cpp:724-725) */
+  float max_x, max_y;  /* triangleBoundingBox max */
+  int bcx, bcy;        /* 32-px block counts */
+  int qx0, qy0, qx1, qy1; /* pixel range touched by the triangle's quads (inclusive), empty if qx1 < qx0 */
+};
+
+struct DrawParams {
+  Uniforms u;
+  const TexDesc *tex[RHS_SLOT_COUNT];
+  RhsStates st;
+  int shading;
+  uint32_t defines;
+  int nvary;
+  /* viewport (RendererSoft.cpp:92-113) */
+  float vp_w, vp_h, o_x, o_y, p_x, p_y;
+  /* framebuffer */
+  uint32_t *color;          /* RGBA8, `samples` per pixel */
+  uint32_t *color_resolved; /* RGBA8 single sample, when samples == 4 */
+  float *depth;             /* f32, `samples` per pixel */
+  int fb_w, fb_h;           /* attachment size */
+  int samples;              /* 1 or 4 */
+  int tiles_x, tiles_y;
+  /* geometry */
+  const float *verts;
+  const float *clip_raw;
+  int stride_f; /* floats per vertex */
+  int nverts;
+  const int *idx;
+  int ntris;
+  /* vertex stage outputs */
+  float4 *clip;
+  float4 *frag;
+  int *mask;
+  float *vary;
+  /* clipping */
+  int *clip_list; /* compacted ids of triangles that need clipping */
+  int nclip;
+  int *clip_poly; /* kMaxPoly vertex ids + count per clipped triangle */
+  int *fan_offset;
+  /* primitives */
+  PrimSetup *prims;
+  int nprims;
+  int *tile_count;
+  int *tile_offset;
+  uint32_t *pair_key;
+  int *pair_val;
+  uint32_t *pair_key_sorted;
+  int *pair_val_sorted;
+  int npairs;
+  int *tile_start; /* [tiles+1] */
+  unsigned long long *counters; /* [0] fs invocations, [1] samples written, [2] overflow flag */
+};
+
+}  // namespace rhip

@@ -1,0 +1,564 @@
+/*
+ * rhip_shading.h -- device-side texture sampling, vertex shaders and fragment
+ * shaders of librhip (gfx950).
+ *
+ * The arithmetic follows SoftGLRender's software pipeline operation by
+ * operation, including the fused multiply-adds its x86 build (-O3 -mavx2 -mfma)
+ * emits, so that texel values, depths and coverage agree bit for bit where the
+ * reference is bit-deterministic. Every kernel TU is compiled with
+ * -ffp-contract=off: each fusion below is an explicit fmaf.
+ */
+#pragma once
+#include "rhip_internal.h"
+
+namespace rhip {
+
+struct f2 {
+  float x, y;
+};
+struct f3 {
+  float x, y, z;
+};
+
+__device__ __forceinline__ f3 sub3(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 add3(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 mul3(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ f3 scl3(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+/* glm::dot(vec3) as compiled by the reference build: fma(z, z', fma(x, x', y*y')) */
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return fmaf(a.z, b.z, fmaf(a.x, b.x, a.y * b.y)); }
+__device__ __forceinline__ f3 nrm3(f3 v) { return scl3(v, 1.0f / sqrtf(dot3(v, v))); }
+__device__ __forceinline__ f3 cross3(f3 a, f3 b) {
+  return {fmaf(a.y, b.z, -(a.z * b.y)), fmaf(a.z, b.x, -(a.x * b.z)), fmaf(a.x, b.y, -(a.y * b.x))};
+}
+/* powf / log2f / exp2f as glibc's single-precision versions round them: those
+ * work in double precision internally and are within ~0.5 ulp of the exact
+ * value, so the double-precision result rounded to float reproduces them. */
+__device__ __forceinline__ float rpowf(float x, float y) { return (float)pow((double)x, (double)y); }
+__device__ __forceinline__ float rlog2f(float x) { return (float)log2((double)x); }
+__device__ __forceinline__ float rexp2f(float x) { return (float)exp2((double)x); }
+__device__ __forceinline__ float gmax(float a, float b) { return a < b ? b : a; } /* glm::max */
+__device__ __forceinline__ float gmin(float a, float b) { return b < a ? b : a; } /* glm::min */
+__device__ __forceinline__ float gclamp(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+
+/* mat4 * vec4(p, 1): (fma(c0, x, c1*y)) + (fma(c2, z, c3))  -- glm type_mat4x4.inl:536-575 */
+__device__ __forceinline__ float4 m4p(const float *m, float x, float y, float z) {
+  float r[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) r[i] = fmaf(m[i], x, m[4 + i] * y) + fmaf(m[8 + i], z, m[12 + i]);
+  return make_float4(r[0], r[1], r[2], r[3]);
+}
+/* mat3 * vec3: fma(c2, z, fma(c0, x, c1*y)); columns `cs` floats apart */
+__device__ __forceinline__ f3 m3v(const float *m, int cs, f3 v) {
+  return {fmaf(m[2 * cs + 0], v.z, fmaf(m[0], v.x, m[cs + 0] * v.y)),
+          fmaf(m[2 * cs + 1], v.z, fmaf(m[1], v.x, m[cs + 1] * v.y)),
+          fmaf(m[2 * cs + 2], v.z, fmaf(m[2], v.x, m[cs + 2] * v.y))};
+}
+
+/* --------------------------------------------------------------------------- texel addressing
+ * Buffer.h:31-33 (linear), :151-158 (4x4 tiles), :185-202 (32x32 tiles, 16-bit Morton inside). */
+__device__ __forceinline__ uint32_t morton16(uint32_t x, uint32_t y) {
+  uint32_t r = x | (y << 16);
+  r = (r | (r << 4)) & 0x0f0f0f0fu;
+  r = (r | (r << 2)) & 0x33333333u;
+  r = (r | (r << 1)) & 0x55555555u;
+  return (r | (r >> 15)) & 0xffffu;
+}
+
+__device__ __forceinline__ size_t texelIndex(const TexLevel &l, int layout, int x, int y) {
+  if (layout == LAYOUT_TILED) {
+    uint32_t tx = (uint32_t)x >> 2, ty = (uint32_t)y >> 2;
+    return ((size_t)(ty * (uint32_t)l.tiles_x + tx) << 4) + ((uint32_t)(y & 3) << 2) + (uint32_t)(x & 3);
+  }
+  if (layout == LAYOUT_MORTON) {
+    uint32_t tx = (uint32_t)x >> 5, ty = (uint32_t)y >> 5;
+    return ((size_t)(ty * (uint32_t)l.tiles_x + tx) << 10) + morton16(x & 31, y & 31);
+  }
+  return (size_t)y * (uint32_t)l.w + (uint32_t)x;
+}
+
+__device__ __forceinline__ int chan(uint32_t c, int i) { return (int)((c >> (8 * i)) & 255u); }
+__device__ __forceinline__ uint32_t pack4(int r, int g, int b, int a) {
+  return (uint32_t)(r & 255) | ((uint32_t)(g & 255) << 8) | ((uint32_t)(b & 255) << 16) | ((uint32_t)(a & 255) << 24);
+}
+/* float -> u8 the way the reference's conversion compiles: truncate to int32, keep the low byte. */
+__device__ __forceinline__ int u8t(float v) { return (int)(uint8_t)(int32_t)v; }
+
+/* glm::mix on u8vec4 as compiled: u8(fma(y, a, x * (1 - a))) per channel (SamplerSoft.h:164, :265) */
+__device__ __forceinline__ uint32_t mixRGBA(uint32_t x, uint32_t y, float a) {
+  float na = 1.0f - a;
+  return pack4(u8t(fmaf((float)chan(y, 0), a, (float)chan(x, 0) * na)),
+               u8t(fmaf((float)chan(y, 1), a, (float)chan(x, 1) * na)),
+               u8t(fmaf((float)chan(y, 2), a, (float)chan(x, 2) * na)),
+               u8t(fmaf((float)chan(y, 3), a, (float)chan(x, 3) * na)));
+}
+__device__ __forceinline__ float mixF(float x, float y, float a) { return fmaf(y, a, x * (1.0f - a)); }
+
+template <typename T>
+struct Texel;
+template <>
+struct Texel<uint32_t> {
+  __device__ static uint32_t mix(uint32_t a, uint32_t b, float t) { return mixRGBA(a, b, t); }
+  __device__ static uint32_t border(const TexDesc &d) { return d.border == 1 ? 0xffffffffu : 0u; }
+};
+template <>
+struct Texel<float> {
+  __device__ static float mix(float a, float b, float t) { return mixF(a, b, t); }
+  __device__ static float border(const TexDesc &d) { return d.border == 1 ? 1.f : 0.f; }
+};
+
+/* pixelWithWrapMode (SamplerSoft.h:170-213). REPEAT masks with (n-1): the
+ * reference's CoordMod macro parses as i & (2n-1) & (n-1). */
+template <typename T>
+__device__ __forceinline__ T fetchWrap(const TexDesc &d, const TexLevel &l, int x, int y, int wrap) {
+  int w = l.w, h = l.h;
+  if (wrap == 0) {
+    x &= w - 1;
+    y &= h - 1;
+  } else if (wrap == 1) {
+    x &= 2 * w - 1;
+    y &= 2 * h - 1;
+    x -= w;
+    y -= h;
+    x = x >= 0 ? x : (-1 - x);
+    y = y >= 0 ? y : (-1 - y);
+    x = w - 1 - x;
+    y = h - 1 - y;
+  } else if (wrap == 2) {
+    x = x < 0 ? 0 : (x >= w ? w - 1 : x);
+    y = y < 0 ? 0 : (y >= h ? h - 1 : y);
+  } else {
+    if (x < 0 || x >= w || y < 0 || y >= h) return Texel<T>::border(d);
+  }
+  if ((unsigned)x >= (unsigned)w || (unsigned)y >= (unsigned)h) return T(0);
+  const T *p = (const T *)(d.base + l.offset);
+  return p[texelIndex(l, d.layout, x, y)];
+}
+
+/* samplePixelBilinear (SamplerSoft.h:254-266) at texel-space (ux, uy) */
+template <typename T>
+__device__ __forceinline__ T bilinear(const TexDesc &d, const TexLevel &l, float ux, float uy, int wrap) {
+  float fx0 = ux - 0.5f, fy0 = uy - 0.5f;
+  float flx = floorf(fx0), fly = floorf(fy0);
+  int x = (int)flx, y = (int)fly;
+  T s1 = fetchWrap<T>(d, l, x, y, wrap);
+  T s2 = fetchWrap<T>(d, l, x + 1, y, wrap);
+  T s3 = fetchWrap<T>(d, l, x, y + 1, wrap);
+  T s4 = fetchWrap<T>(d, l, x + 1, y + 1, wrap);
+  float fx = fx0 - flx, fy = fy0 - fly;
+  return Texel<T>::mix(Texel<T>::mix(s1, s2, fx), Texel<T>::mix(s3, s4, fx), fy);
+}
+
+template <typename T>
+__device__ __forceinline__ T sampleLevel(const TexDesc &d, int layer, int level, f2 uv, bool linear) {
+  const TexLevel &l = d.lv[layer][level];
+  float tx = uv.x * (float)l.w, ty = uv.y * (float)l.h;
+  if (linear) return bilinear<T>(d, l, tx, ty, d.wrap);
+  return fetchWrap<T>(d, l, (int)floorf(tx), (int)floorf(ty), d.wrap);
+}
+
+/* BaseSampler::textureImpl (SamplerSoft.h:117-168) */
+template <typename T>
+__device__ T textureImpl(const TexDesc &d, int layer, f2 uv, float lod) {
+  int fm = d.filter_min;
+  if (fm == 0) return sampleLevel<T>(d, layer, 0, uv, false);
+  if (fm == 1) return sampleLevel<T>(d, layer, 0, uv, true);
+  int maxLevel = d.levels - 1;
+  if (fm == 2 || fm == 3) {
+    int level = min(max((int)ceilf(lod + 0.5f) - 1, 0), maxLevel);
+    return sampleLevel<T>(d, layer, level, uv, fm == 3);
+  }
+  int hi = min(max((int)floorf(lod), 0), maxLevel);
+  int lo = min(max(hi + 1, 0), maxLevel);
+  T th = sampleLevel<T>(d, layer, hi, uv, fm == 5);
+  if (hi == lo) return th;
+  T tl = sampleLevel<T>(d, layer, lo, uv, fm == 5);
+  return Texel<T>::mix(th, tl, lod - floorf(lod));
+}
+
+/* convertXYZ2UV (SamplerSoft.h:311-373); later faces win ties */
+__device__ __forceinline__ void cubeFace(f3 c, int *face, f2 *uv) {
+  float ax = fabsf(c.x), ay = fabsf(c.y), az = fabsf(c.z);
+  bool px = c.x > 0, py = c.y > 0, pz = c.z > 0;
+  float ma = 0.f, uc = 0.f, vc = 0.f;
+  int idx = 0;
+  if (px && ax >= ay && ax >= az) { ma = ax; uc = -c.z; vc = c.y; idx = 0; }
+  if (!px && ax >= ay && ax >= az) { ma = ax; uc = c.z; vc = c.y; idx = 1; }
+  if (py && ay >= ax && ay >= az) { ma = ay; uc = c.x; vc = -c.z; idx = 2; }
+  if (!py && ay >= ax && ay >= az) { ma = ay; uc = c.x; vc = c.z; idx = 3; }
+  if (pz && az >= ax && az >= ay) { ma = az; uc = c.x; vc = c.y; idx = 4; }
+  if (!pz && az >= ax && az >= ay) { ma = az; uc = -c.x; vc = c.y; idx = 5; }
+  vc = -vc;
+  *face = idx;
+  uv->x = 0.5f * (uc / ma + 1.0f);
+  uv->y = 0.5f * (vc / ma + 1.0f);
+}
+
+struct RGBAf {
+  float r, g, b, a;
+};
+__device__ __forceinline__ RGBAf unpack01(uint32_t p) {
+  return {(float)chan(p, 0) / 255.f, (float)chan(p, 1) / 255.f, (float)chan(p, 2) / 255.f, (float)chan(p, 3) / 255.f};
+}
+
+/* --------------------------------------------------------------------------- vertex shaders */
+
+__device__ __forceinline__ int varyingCount(int shading) {
+  return (shading == RHS_SHADING_BLINN_PHONG || shading == RHS_SHADING_BLINN_PHONG_DIR) ? 32
+         : (shading == RHS_SHADING_PBR || shading == RHS_SHADING_NEURAL_PBR)            ? 28
+         : shading == RHS_SHADING_VERTEX_COLOR                                          ? 4
+                                                                                        : 0;
+}
+
+/* BasicSoft.h:66-69, BlinnPhongSoft.h:103-121, PbrSoft.h:110-127, plus the
+ * directional-light Blinn-Phong (constant light vector) and per-vertex colour. */
+__device__ __forceinline__ float4 runVS(const DrawParams &P, const float *a, float *v) {
+  const Uniforms &u = P.u;
+  float4 clip = m4p(u.mvp, a[0], a[1], a[2]);
+  int sh = P.shading;
+  if (sh == RHS_SHADING_BLINN_PHONG || sh == RHS_SHADING_BLINN_PHONG_DIR || sh == RHS_SHADING_PBR ||
+      sh == RHS_SHADING_NEURAL_PBR) {
+    bool bp = sh == RHS_SHADING_BLINN_PHONG || sh == RHS_SHADING_BLINN_PHONG_DIR;
+    v[0] = a[4];
+    v[1] = a[5];
+    float4 wp4 = m4p(u.model, a[0], a[1], a[2]);
+    f3 wp{wp4.x, wp4.y, wp4.z};
+    f3 nrm{a[8], a[9], a[10]};
+    f3 nv = m3v(u.model, 4, nrm);
+    v[4] = nv.x, v[5] = nv.y, v[6] = nv.z;
+    v[8] = wp.x, v[9] = wp.y, v[10] = wp.z;
+    f3 cam{u.cam_pos[0], u.cam_pos[1], u.cam_pos[2]};
+    f3 lp{u.light_pos[0], u.light_pos[1], u.light_pos[2]};
+    f3 cd = sub3(cam, wp);
+    v[12] = cd.x, v[13] = cd.y, v[14] = cd.z;
+    f3 ld = sh == RHS_SHADING_BLINN_PHONG_DIR ? lp : sub3(lp, wp);
+    v[16] = ld.x, v[17] = ld.y, v[18] = ld.z;
+    if (bp) {
+      float4 s = m4p(u.shadow_mvp, a[0], a[1], a[2]);
+      v[20] = s.x, v[21] = s.y, v[22] = s.z, v[23] = s.w;
+    }
+    if (P.defines & RHS_DEF_NORMAL_MAP) {
+      f3 tg{a[12], a[13], a[14]};
+      f3 N = nrm3(m3v(u.inv_t, 4, nrm));
+      f3 T = nrm3(m3v(u.inv_t, 4, tg));
+      f3 Tn = nrm3(sub3(T, scl3(N, dot3(T, N))));
+      int o = bp ? 24 : 20;
+      v[o] = N.x, v[o + 1] = N.y, v[o + 2] = N.z;
+      v[o + 4] = Tn.x, v[o + 5] = Tn.y, v[o + 6] = Tn.z;
+    }
+  } else if (sh == RHS_SHADING_VERTEX_COLOR) {
+    v[0] = a[4], v[1] = a[5], v[2] = a[6], v[3] = a[7];
+  }
+  return clip;
+}
+
+/* countFrustumClipMask, RendererSoft.cpp:994-1003 */
+__device__ __forceinline__ int clipMask(float4 c) {
+  int m = 0;
+  if (c.w < c.x) m |= 1;
+  if (c.w < -c.x) m |= 2;
+  if (c.w < c.y) m |= 4;
+  if (c.w < -c.y) m |= 8;
+  if (c.w < c.z) m |= 16;
+  if (c.w < -c.z) m |= 32;
+  return m;
+}
+
+/* perspectiveDivideImpl + viewportTransformImpl, RendererSoft.cpp:981-992:
+ * multiply and add are separate roundings there. */
+__device__ __forceinline__ float4 toScreen(const DrawParams &P, float4 c) {
+  float iw = 1.f / c.w;
+  float x = c.x * iw, y = c.y * iw, z = c.z * iw;
+  return make_float4(x * P.p_x + P.o_x, y * P.p_y + P.o_y, z * 1.f + 0.f, iw * 1.f + 0.f);
+}
+
+/* --------------------------------------------------------------------------- fragment shaders */
+
+/* The quad context a fragment shader needs: its own varyings and the texture
+ * coordinates of quad pixels 0, 1, 2 for the coarse derivative. */
+struct Frag {
+  const float *v; /* own interpolated varyings (indexed like the VS output) */
+  f2 tc[3];       /* v_texCoord of quad pixels p0, p1, p2 */
+};
+
+/* ShaderSoft::getSampler2DLod (ShaderSoft.h:117-133), used only when the sampler filters with mipmaps */
+__device__ __forceinline__ float lodFor(const TexDesc *t, const Frag &f) {
+  if (!t || t->filter_min <= 1) return 0.f;
+  float w = (float)t->width, h = (float)t->height;
+  float dxx = (f.tc[1].x - f.tc[0].x) * w, dyx = (f.tc[2].x - f.tc[0].x) * w;
+  float dxy = (f.tc[1].y - f.tc[0].y) * h, dyy = (f.tc[2].y - f.tc[0].y) * h;
+  float ddx = fmaf(dxy, dxy, dxx * dxx);
+  float ddy = fmaf(dyy, dyy, dyx * dyx);
+  float d = ddy < ddx ? ddx : ddy;
+  float lod = 0.5f * rlog2f(d);
+  return 0.f < lod ? lod : 0.f;
+}
+
+__device__ __forceinline__ RGBAf tex2D(const TexDesc *t, f2 uv, float lod) {
+  if (!t) return {0, 0, 0, 0};
+  return unpack01(textureImpl<uint32_t>(*t, 0, uv, lod));
+}
+
+__device__ __forceinline__ RGBAf texCube(const TexDesc *t, f3 dir, float lod) {
+  if (!t) return {0, 0, 0, 0};
+  int face;
+  f2 uv;
+  cubeFace(dir, &face, &uv);
+  return unpack01(textureImpl<uint32_t>(*t, face, uv, lod));
+}
+
+__device__ __forceinline__ f3 normalFromMap(const DrawParams &P, const Frag &f, int offN, int offT) {
+  const float *v = f.v;
+  if (P.defines & RHS_DEF_NORMAL_MAP) {
+    f3 N = nrm3({v[offN], v[offN + 1], v[offN + 2]});
+    f3 T = nrm3({v[offT], v[offT + 1], v[offT + 2]});
+    T = nrm3(sub3(T, scl3(N, dot3(T, N))));
+    f3 B = cross3(T, N);
+    const TexDesc *nt = P.tex[RHS_SLOT_NORMAL];
+    RGBAf t4 = tex2D(nt, {v[0], v[1]}, lodFor(nt, f));
+    f3 tn{t4.r * 2.0f - 1.0f, t4.g * 2.0f - 1.0f, t4.b * 2.0f - 1.0f};
+    f3 r{fmaf(N.x, tn.z, fmaf(T.x, tn.x, B.x * tn.y)), fmaf(N.y, tn.z, fmaf(T.y, tn.x, B.y * tn.y)),
+         fmaf(N.z, tn.z, fmaf(T.z, tn.x, B.z * tn.y))};
+    return nrm3(r);
+  }
+  return nrm3({v[4], v[5], v[6]});
+}
+
+/* BlinnPhongSoft.h:165-189 */
+__device__ float shadowCalc(const DrawParams &P, float4 sp, f3 n, f3 lightDir) {
+  f3 pc{sp.x / sp.w, sp.y / sp.w, sp.z / sp.w};
+  float cur = pc.z;
+  if (cur < 0.f || cur > 1.f) return 0.0f;
+  float bias = gmax(0.00025f * (1.0f - dot3(n, nrm3(lightDir))), 0.00005f);
+  const TexDesc *t = P.tex[RHS_SLOT_SHADOW];
+  if (!t) return 0.f;
+  float ox = 1.0f / (float)t->width, oy = 1.0f / (float)t->height;
+  float s = 0.f;
+  for (int x = -1; x <= 1; ++x)
+    for (int y = -1; y <= 1; ++y) {
+      float d = textureImpl<float>(*t, 0, {pc.x + (float)x * ox, pc.y + (float)y * oy}, 0.f);
+      if (P.u.reverse_z)
+        s += cur + bias < d ? 1.0f : 0.0f;
+      else
+        s += cur - bias > d ? 1.0f : 0.0f;
+    }
+  return s / 9.0f;
+}
+
+/* BlinnPhongSoft.h:191-242; `directional` = constant light vector, no range attenuation */
+__device__ RGBAf fsBlinnPhong(const DrawParams &P, const Frag &f, bool directional) {
+  const float *v = f.v;
+  const Uniforms &u = P.u;
+  f2 uv{v[0], v[1]};
+  const TexDesc *ta = P.tex[RHS_SLOT_ALBEDO];
+  RGBAf base = (P.defines & RHS_DEF_ALBEDO_MAP)
+                   ? tex2D(ta, uv, lodFor(ta, f))
+                   : RGBAf{u.base_color[0], u.base_color[1], u.base_color[2], u.base_color[3]};
+  f3 N = normalFromMap(P, f, 24, 28);
+  const TexDesc *tao = P.tex[RHS_SLOT_AO];
+  float ao = (P.defines & RHS_DEF_AO_MAP) ? tex2D(tao, uv, lodFor(tao, f)).r : 1.f;
+  f3 b3{base.r, base.g, base.b};
+  f3 amb = scl3(mul3(b3, {u.ambient[0], u.ambient[1], u.ambient[2]}), ao);
+  f3 dif{0, 0, 0}, spe{0, 0, 0}, emi{0, 0, 0};
+  if (u.enable_light) {
+    f3 ldv{v[16], v[17], v[18]};
+    f3 L = nrm3(ldv);
+    float diffuse = gmax(dot3(N, L), 0.0f);
+    f3 lc{u.light_color[0], u.light_color[1], u.light_color[2]};
+    if (directional) {
+      dif = scl3(mul3(lc, b3), diffuse);
+    } else {
+      f3 l = scl3(ldv, 1.0f / 5.f);
+      float att = gclamp(1.0f - dot3(l, l), 0.0f, 1.0f);
+      dif = scl3(scl3(mul3(lc, b3), diffuse), att);
+    }
+    f3 C = nrm3({v[12], v[13], v[14]});
+    f3 H = nrm3(add3(L, C));
+    float sa = gmax(dot3(N, H), 0.0f);
+    float sp = u.k_specular * rpowf(sa, 128.f);
+    spe = {sp, sp, sp};
+    if (u.enable_shadow) {
+      float s = 1.0f - shadowCalc(P, make_float4(v[20], v[21], v[22], v[23]), N, ldv);
+      dif = scl3(dif, s);
+      spe = scl3(spe, s);
+    }
+  }
+  if (P.defines & RHS_DEF_EMISSIVE_MAP) {
+    const TexDesc *te = P.tex[RHS_SLOT_EMISSIVE];
+    RGBAf e = tex2D(te, uv, lodFor(te, f));
+    emi = {e.r, e.g, e.b};
+  }
+  f3 col = add3(add3(add3(amb, dif), spe), emi);
+  return {col.x, col.y, col.z, base.a};
+}
+
+/* PbrSoft.h:168-223 */
+constexpr float kPI = 3.14159265359f;
+__device__ __forceinline__ float distGGX(f3 N, f3 H, float r) {
+  float a = r * r, a2 = a * a;
+  float nh = gmax(dot3(N, H), 0.0f);
+  float nh2 = nh * nh;
+  float den = nh2 * (a2 - 1.0f) + 1.0f;
+  den = kPI * den * den;
+  return a2 / den;
+}
+__device__ __forceinline__ float schlickGGX(float nv, float r) {
+  float k = ((r + 1.0f) * (r + 1.0f)) / 8.0f;
+  return nv / (nv * (1.0f - k) + k);
+}
+__device__ __forceinline__ f3 envBRDFApprox(f3 spec, float r, float nv) {
+  float rx = r * -1.f + 1.f, ry = r * -0.0275f + 0.0425f, rz = r * -0.572f + 1.04f, rw = r * 0.022f + -0.04f;
+  float a004 = gmin(rx * rx, rexp2f(-9.28f * nv)) * rx + ry;
+  float ax = -1.04f * a004 + rz, ay = 1.04f * a004 + rw;
+  ay *= gmax(0.f, gmin(1.f, 50.0f * spec.y));
+  return {spec.x * ax + ay, spec.y * ax + ay, spec.z * ax + ay};
+}
+
+/* Material inputs of the PBR shader, shared with the learned shader. */
+struct PbrInputs {
+  RGBAf albedo_rgba;
+  f3 albedo;
+  float metallic, rough, ao;
+  f3 N, V, R;
+};
+
+__device__ __forceinline__ PbrInputs pbrInputs(const DrawParams &P, const Frag &f) {
+  const float *v = f.v;
+  const Uniforms &u = P.u;
+  PbrInputs in;
+  f2 uv{v[0], v[1]};
+  const TexDesc *ta = P.tex[RHS_SLOT_ALBEDO];
+  in.albedo_rgba = (P.defines & RHS_DEF_ALBEDO_MAP)
+                       ? tex2D(ta, uv, lodFor(ta, f))
+                       : RGBAf{u.base_color[0], u.base_color[1], u.base_color[2], u.base_color[3]};
+  in.albedo = {rpowf(in.albedo_rgba.r, 2.2f), rpowf(in.albedo_rgba.g, 2.2f), rpowf(in.albedo_rgba.b, 2.2f)};
+  in.metallic = 0.0f;
+  in.rough = 1.0f;
+  if (P.defines & RHS_DEF_METALROUGHNESS_MAP) {
+    /* this sampler never gets a LOD callback (PbrSoft.h:138-151): level-0 lookups */
+    RGBAf mr = tex2D(P.tex[RHS_SLOT_METAL_ROUGHNESS], uv, 0.f);
+    in.metallic = mr.b;
+    in.rough = mr.g;
+  }
+  const TexDesc *tao = P.tex[RHS_SLOT_AO];
+  in.ao = (P.defines & RHS_DEF_AO_MAP) ? tex2D(tao, uv, lodFor(tao, f)).r : 1.f;
+  in.N = normalFromMap(P, f, 20, 24);
+  in.V = nrm3({v[12], v[13], v[14]});
+  f3 I{-in.V.x, -in.V.y, -in.V.z};
+  float dNI = dot3(in.N, I);
+  in.R = sub3(I, scl3(in.N, dNI * 2.0f));
+  return in;
+}
+
+/* PbrSoft.h:225-322 */
+__device__ RGBAf fsPbr(const DrawParams &P, const Frag &f) {
+  const float *v = f.v;
+  const Uniforms &u = P.u;
+  PbrInputs in = pbrInputs(P, f);
+  f3 albedo = in.albedo, N = in.N, V = in.V;
+  float metallic = in.metallic, rough = in.rough;
+  f3 F0{mixF(0.04f, albedo.x, metallic), mixF(0.04f, albedo.y, metallic), mixF(0.04f, albedo.z, metallic)};
+  f3 Lo{0, 0, 0};
+  if (u.enable_light) {
+    f3 ldv{v[16], v[17], v[18]};
+    f3 L = nrm3(ldv);
+    f3 H = nrm3(add3(V, L));
+    f3 ld = scl3(ldv, 1.0f / 5.f);
+    float att = gclamp(1.0f - dot3(ld, ld), 0.0f, 1.0f);
+    f3 radiance = scl3({u.light_color[0], u.light_color[1], u.light_color[2]}, att);
+    float NDF = distGGX(N, H, rough);
+    float nv = gmax(dot3(N, V), 0.0f), nl = gmax(dot3(N, L), 0.0f);
+    float G = schlickGGX(nl, rough) * schlickGGX(nv, rough);
+    float ct = gmax(dot3(H, V), 0.0f);
+    float p5 = rpowf(gclamp(1.0f - ct, 0.0f, 1.0f), 5.0f);
+    f3 F{F0.x + (1.0f - F0.x) * p5, F0.y + (1.0f - F0.y) * p5, F0.z + (1.0f - F0.z) * p5};
+    float ng = NDF * G;
+    f3 num = scl3(F, ng);
+    float den = 4.0f * nv * nl + 0.0001f;
+    f3 spec{num.x / den, num.y / den, num.z / den};
+    f3 kD{(1.0f - F.x) * (1.0f - metallic), (1.0f - F.y) * (1.0f - metallic), (1.0f - F.z) * (1.0f - metallic)};
+    f3 t{kD.x * albedo.x / kPI + spec.x, kD.y * albedo.y / kPI + spec.y, kD.z * albedo.z / kPI + spec.z};
+    Lo = scl3(mul3(t, radiance), nl);
+  }
+  f3 ambient;
+  if (u.enable_ibl) {
+    float nv = gmax(dot3(N, V), 0.0f);
+    float p5 = rpowf(gclamp(1.0f - nv, 0.0f, 1.0f), 5.0f);
+    float omr = 1.0f - rough;
+    f3 F{F0.x + (gmax(omr, F0.x) - F0.x) * p5, F0.y + (gmax(omr, F0.y) - F0.y) * p5,
+         F0.z + (gmax(omr, F0.z) - F0.z) * p5};
+    f3 kD{(1.0f - F.x) * (1.0f - metallic), (1.0f - F.y) * (1.0f - metallic), (1.0f - F.z) * (1.0f - metallic)};
+    RGBAf irr = texCube(P.tex[RHS_SLOT_IRRADIANCE], N, 0.f);
+    f3 diffuse{irr.r * albedo.x, irr.g * albedo.y, irr.b * albedo.z};
+    RGBAf pf = texCube(P.tex[RHS_SLOT_PREFILTER], in.R, rough * 4.0f);
+    f3 brdf = envBRDFApprox(F, rough, nv);
+    f3 spec{pf.r * brdf.x, pf.g * brdf.y, pf.b * brdf.z};
+    ambient = scl3(add3(mul3(kD, diffuse), spec), in.ao);
+  } else {
+    ambient = scl3(mul3({u.ambient[0], u.ambient[1], u.ambient[2]}, albedo), in.ao);
+  }
+  f3 col = add3(ambient, Lo);
+  col = {rpowf(col.x, 1.0f / 2.2f), rpowf(col.y, 1.0f / 2.2f), rpowf(col.z, 1.0f / 2.2f)};
+  if (P.defines & RHS_DEF_EMISSIVE_MAP) {
+    const TexDesc *te = P.tex[RHS_SLOT_EMISSIVE];
+    RGBAf e = tex2D(te, {v[0], v[1]}, lodFor(te, f));
+    col = add3(col, {e.r, e.g, e.b});
+  }
+  return {col.x, col.y, col.z, in.albedo_rgba.a};
+}
+
+/* --------------------------------------------------------------------------- per-sample operations */
+
+/* DepthSoft.h:13-25 */
+__device__ __forceinline__ bool depthPass(float a, float b, int fn) {
+  const float eps = 1.19209290e-07f;
+  switch (fn) {
+    case 0: return false;
+    case 1: return a < b;
+    case 2: return fabsf(a - b) <= eps;
+    case 3: return a <= b;
+    case 4: return a > b;
+    case 5: return fabsf(a - b) > eps;
+    case 6: return a >= b;
+    default: return true;
+  }
+}
+
+/* BlendSoft.h:13-56 */
+__device__ __forceinline__ float blendFactor(int f, float s, float sa, float d, float da) {
+  switch (f) {
+    case 0: return 0.f;
+    case 1: return 1.f;
+    case 2: return s;
+    case 3: return sa;
+    case 4: return d;
+    case 5: return da;
+    case 6: return 1.f - s;
+    case 7: return 1.f - sa;
+    case 8: return 1.f - d;
+    case 9: return 1.f - da;
+  }
+  return 0.f;
+}
+__device__ __forceinline__ float blendFunc(int fn, float s, float d) {
+  switch (fn) {
+    case 1: return s - d;
+    case 2: return d - s;
+    case 3: return gmin(s, d);
+    case 4: return gmax(s, d);
+    default: return s + d;
+  }
+}
+__device__ __forceinline__ RGBAf blendColor(RGBAf s, RGBAf d, const RhsStates &st) {
+  float sc[3] = {s.r, s.g, s.b}, dc[3] = {d.r, d.g, d.b}, r[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    float fs = blendFactor(st.blendSrcRgb, sc[i], s.a, dc[i], d.a);
+    float fd = blendFactor(st.blendDstRgb, sc[i], s.a, dc[i], d.a);
+    r[i] = blendFunc(st.blendFuncRgb, sc[i] * fs, dc[i] * fd);
+  }
+  float fsa = blendFactor(st.blendSrcAlpha, s.a, s.a, d.a, d.a);
+  float fda = blendFactor(st.blendDstAlpha, s.a, s.a, d.a, d.a);
+  return {r[0], r[1], r[2], blendFunc(st.blendFuncAlpha, s.a * fsa, d.a * fda)};
+}
+
+}  // namespace rhip

@@ -267,3 +267,363 @@ def scene_c1(width=640, height=480, seed=1, msaa=False, lon=64, lat=32, read_msa
     if msaa and read_msaa:
         st.read(2, which=1)
     return Scene("c1", st, width, height, msaa, dict(triangles=int(idx.shape[0]), vertices=int(verts.shape[0])))
+
+
+# ----------------------------------------------------------------------------- IBL inputs
+
+def _sh_basis(d):
+    x, y, z = d[..., 0], d[..., 1], d[..., 2]
+    return np.stack([np.full_like(x, 0.282095), 0.488603 * y, 0.488603 * z, 0.488603 * x, 1.092548 * x * y,
+                     1.092548 * y * z, 0.315392 * (3 * z * z - 1), 1.092548 * x * z, 0.546274 * (x * x - y * y)], -1)
+
+
+def _cube_dirs(size):
+    """World directions of texel centres of the six faces, in the face order and
+    orientation the reference's cube sampler uses (SamplerSoft.h:311-373)."""
+    c = (np.arange(size) + 0.5) / size * 2 - 1
+    vc, uc = np.meshgrid(c, c, indexing="ij")
+    vc = -vc  # the sampler flips v
+    one = np.ones_like(uc)
+    faces = [np.stack([one, vc, -uc], -1), np.stack([-one, vc, uc], -1), np.stack([uc, one, -vc], -1),
+             np.stack([uc, -one, vc], -1), np.stack([uc, vc, one], -1), np.stack([-uc, vc, -one], -1)]
+    return [f / np.linalg.norm(f, axis=-1, keepdims=True) for f in faces]
+
+
+def ibl_coefficients(seed=7):
+    """Seeded smooth low-frequency radiance: order-2 spherical harmonics per RGB channel."""
+    rng = np.random.default_rng(seed)
+    coef = rng.normal(0, 0.25, size=(9, 3))
+    coef[0] = rng.uniform(1.2, 1.8, size=3)
+    return coef
+
+
+def ibl_maps(seed=7, irr_size=32, pref_size=128, pref_levels=5):
+    """RGBA8 irradiance cube and prefiltered specular mip chain (the reference's IBL map
+    formats, Environment.h:20-22) for an SH radiance field. Irradiance uses the exact
+    cosine-lobe convolution of the SH bands; level k of the prefiltered chain
+    attenuates band l by exp(-r^2 l(l+1)) with r = k / (levels-1)."""
+    coef = ibl_coefficients(seed)
+    band = np.array([0, 1, 1, 1, 2, 2, 2, 2, 2])
+    irr_w = np.array([np.pi, 2 * np.pi / 3, np.pi / 4])[band] / np.pi
+    irr = [_to_u8(np.clip(_sh_basis(d) @ (coef * irr_w[:, None]), 0, 1) * 0.8) for d in _cube_dirs(irr_size)]
+    irr = [np.concatenate([f, np.full(f.shape[:2] + (1,), 255, np.uint8)], -1) for f in irr]
+    levels = []
+    nlev = int(np.floor(np.log2(pref_size))) + 1
+    for k in range(nlev):
+        s = max(1, pref_size >> k)
+        r = min(k, pref_levels - 1) / (pref_levels - 1)
+        w = np.exp(-(r * r) * band * (band + 1))
+        faces = [_to_u8(np.clip(_sh_basis(d) @ (coef * w[:, None]), 0, 1) * 0.8) for d in _cube_dirs(s)]
+        levels.append([np.concatenate([f, np.full(f.shape[:2] + (1,), 255, np.uint8)], -1) for f in faces])
+    return irr, levels
+
+
+def pbr_textures(size, seed):
+    albedo = noise_texture(size, seed, lambda n: np.concatenate(
+        [_to_u8(0.15 + 0.8 * n), np.full(n.shape[:2] + (1,), 255, np.uint8)], -1))
+
+    def nmap(n):
+        t = (n[..., :2] - 0.5) * 0.8
+        v = np.concatenate([t, np.ones(t.shape[:2] + (1,))], -1)
+        v /= np.linalg.norm(v, axis=-1, keepdims=True)
+        return np.concatenate([_to_u8(v * 0.5 + 0.5), np.full(n.shape[:2] + (1,), 255, np.uint8)], -1)
+
+    normal = noise_texture(size, seed + 1, nmap)
+    mr = noise_texture(size, seed + 2, lambda n: np.stack(
+        [np.zeros(n.shape[:2], np.uint8), _to_u8(0.2 + 0.7 * n[..., 0]), _to_u8((n[..., 1] > 0.55) * 0.9 + 0.05),
+         np.full(n.shape[:2], 255, np.uint8)], -1))
+    return albedo, normal, mr
+
+
+def _translate(t):
+    m = np.eye(4)
+    m[:3, 3] = t
+    return m
+
+
+def scene_c2(width=1920, height=1080, freq=224, tex_size=2048, shadow_size=2048, seed=3, msaa=True,
+             layout="morton", read_shadow=False, read_samples=False, translucent=True, shadow=True):
+    """C2: 1920x1080 MSAA-4x, reversed Z; displaced icosphere (freq 224 -> 1,003,520 triangles, 3-octave value
+    noise, amplitude 0.1); PBR metallic-roughness with 2048^2 albedo/normal/metal-rough maps (full mips, Morton
+    layout); IBL irradiance (32^2) and prefiltered (128^2, 5 used levels) RGBA8 cubes from a seeded SH radiance;
+    a translucent sphere (alpha 0.5) blended over it; a 2048^2 shadow-map depth pass first."""
+    st = S.Stream()
+    verts, idx = icosphere(freq, 1.0, displace=0.1, seed=seed)
+    tverts, tidx = icosphere(32, 0.35)
+    albedo, normal, mr = pbr_textures(tex_size, seed)
+    irr, pref = ibl_maps(seed + 4)
+    lay = {"linear": 0, "tiled": 1, "morton": 2}[layout]
+    mip = dict(mipmaps=True, filter_min=S.FILTER_LINEAR_MIPMAP_LINEAR, filter_mag=S.FILTER_LINEAR,
+               wrap=S.WRAP_REPEAT, usage=S.USAGE_SAMPLER | S.USAGE_UPLOAD)
+    for tid, img in ((1, albedo), (2, normal), (3, mr)):
+        st.texture(tid, tex_size, tex_size, **mip)
+        st.tex_data(tid, [img])
+    st.texture(4, 32, 32, type=S.TEX_CUBE, filter_min=S.FILTER_LINEAR, filter_mag=S.FILTER_LINEAR,
+               usage=S.USAGE_SAMPLER | S.USAGE_UPLOAD)
+    st.tex_data(4, irr)
+    st.texture(5, 128, 128, type=S.TEX_CUBE, mipmaps=True, filter_min=S.FILTER_LINEAR_MIPMAP_LINEAR,
+               filter_mag=S.FILTER_LINEAR, usage=S.USAGE_SAMPLER | S.USAGE_UPLOAD)
+    st.tex_data(5, pref[0])
+    for lev in range(1, len(pref)):
+        for face in range(6):
+            st.tex_level(5, face, lev, pref[lev][face])
+    st.vao(1, verts, idx)
+    st.vao(2, tverts, tidx)
+    st.texture(6, width, height, usage=S.USAGE_COLOR | S.USAGE_SAMPLER, msaa=msaa, filter_min=S.FILTER_LINEAR,
+               filter_mag=S.FILTER_LINEAR)
+    st.tex_init(6)
+    st.texture(7, width, height, format=S.FMT_FLOAT32, usage=S.USAGE_DEPTH | S.USAGE_SAMPLER, msaa=msaa)
+    st.tex_init(7)
+    st.fbo(1, color=6, depth=7)
+    if shadow:
+        st.texture(8, shadow_size, shadow_size, format=S.FMT_FLOAT32, usage=S.USAGE_DEPTH | S.USAGE_SAMPLER,
+                   wrap=S.WRAP_CLAMP_TO_BORDER)
+        st.tex_init(8)
+        st.fbo(2, depth=8)
+    st.frame()
+    light = (1.6, 2.0, 2.4)
+    eye = (0.0, 0.0, 2.6)
+    model = np.eye(4)
+    if shadow:
+        lview = look_at(light, (0, 0, 0), (0, 1, 0))
+        lproj = projection(math.radians(60.0), 1.0, 0.1, reverse_z=True)
+        rs = _common_states(S.DEPTH_GREATER, cull=True)
+        st.begin(2, False, True, (0, 0, 0, 0), 0.0)
+        st.viewport(0, 0, shadow_size, shadow_size)
+        st.draw(1, S.SHADING_BASIC, 0, rs, S.uniforms_model(model, lproj @ lview @ model, reverse_z=True),
+                bytes(S.SCENE_BYTES), S.uniforms_material(0, 0, 0), samplers=[])
+        st.end()
+    view = look_at(eye, (0, 0, 0), (0, 1, 0))
+    proj = projection(math.radians(60.0), width / height, 0.1, reverse_z=True)
+    scene = S.uniforms_scene((0.1, 0.1, 0.1), eye, light, (2.0, 2.0, 2.0))
+    st.begin(1, True, True, (0.02, 0.02, 0.05, 1.0), 0.0)
+    st.viewport(0, 0, width, height)
+    st.draw(1, S.SHADING_PBR, S.DEF_ALBEDO | S.DEF_NORMAL | S.DEF_METALROUGH,
+            _common_states(S.DEPTH_GREATER, cull=True),
+            S.uniforms_model(model, proj @ view @ model, reverse_z=True), scene,
+            S.uniforms_material(1, 1, 0, 1.0, 0.5, (1, 1, 1, 1)),
+            samplers=[(S.SLOT_ALBEDO, 1), (S.SLOT_NORMAL, 2), (S.SLOT_METALROUGH, 3), (S.SLOT_IRRADIANCE, 4),
+                      (S.SLOT_PREFILTER, 5)])
+    if translucent:
+        tm = _translate((0.45, -0.3, 1.25))
+        st.draw(2, S.SHADING_PBR, 0, _common_states(S.DEPTH_GREATER, cull=True, blend=True),
+                S.uniforms_model(tm, proj @ view @ tm, reverse_z=True), scene,
+                S.uniforms_material(1, 1, 0, 1.0, 0.5, (0.2, 0.6, 1.0, 0.5)),
+                samplers=[(S.SLOT_IRRADIANCE, 4), (S.SLOT_PREFILTER, 5)])
+    st.end()
+    st.read(6, which=0)
+    st.read(7, which=1 if msaa else 0)
+    if read_samples and msaa:
+        st.read(6, which=1)
+    if read_shadow and shadow:
+        st.read(8, which=0)
+    return Scene("c2", st, width, height, msaa,
+                 dict(triangles=int(idx.shape[0]), vertices=int(verts.shape[0]),
+                      translucent_triangles=int(tidx.shape[0]), tex_size=tex_size, shadow_size=shadow_size,
+                      layouts={1: lay, 2: lay, 3: lay}))
+
+
+# ----------------------------------------------------------------------------- small state variants
+
+def random_triangles(n, width, height, seed, area_lo=0.25, area_hi=16.0, z_lo=0.0, z_hi=1.0, big=0):
+    """Screen-space random triangles in NDC (w = 1) with log-uniform pixel area, uniform
+    centres and NDC depth, flat RGBA32F colours per vertex; plus `big` screen-covering
+    triangles at random constant depths. Vertex layout {vec3 a_position; vec4 a_color}
+    (8 floats). Returns vertices (3 per triangle, unshared) and indices."""
+    rng = np.random.default_rng(seed)
+    area = np.exp(rng.uniform(np.log(area_lo), np.log(area_hi), n))
+    cx, cy = rng.uniform(0, width, n), rng.uniform(0, height, n)
+    ang = rng.uniform(0, 2 * np.pi, (n, 1)) + np.array([0, 2.1, 4.2]) + rng.uniform(-0.4, 0.4, (n, 3))
+    rad = np.sqrt(area / (0.5 * 3 * np.sin(2.1)))[:, None] * rng.uniform(0.8, 1.2, (n, 3))
+    px = cx[:, None] + rad * np.cos(ang)
+    py = cy[:, None] + rad * np.sin(ang)
+    z = rng.uniform(z_lo, z_hi, n)
+    pos = np.stack([px / width * 2 - 1, py / height * 2 - 1, np.repeat(z[:, None], 3, 1)], -1)
+    col = rng.uniform(0, 1, (n, 3, 4)).astype(np.float32)
+    col[..., 3] = 1.0
+    if big:
+        bz = rng.uniform(z_lo, z_hi, big)
+        bp = np.stack([np.tile([[-1.0, -1.0], [3.0, -1.0], [-1.0, 3.0]], (big, 1, 1))[..., 0],
+                       np.tile([[-1.0, -1.0], [3.0, -1.0], [-1.0, 3.0]], (big, 1, 1))[..., 1],
+                       np.repeat(bz[:, None], 3, 1)], -1)
+        bc = rng.uniform(0, 1, (big, 3, 4)).astype(np.float32)
+        bc[..., 3] = 1.0
+        pos = np.concatenate([pos, bp])
+        col = np.concatenate([col, bc])
+    m = pos.shape[0]
+    v = np.zeros((m * 3, 8), np.float32)
+    v[:, 0:3] = pos.reshape(-1, 3)
+    v[:, 4:8] = col.reshape(-1, 4)
+    return v, np.arange(m * 3, dtype=np.int32).reshape(-1, 3)
+
+
+VARIANTS = ["blend_alpha", "blend_add_one", "blend_minmax", "blend_sub", "depth_lequal", "depth_gequal_rz",
+            "depth_always_nomask", "depth_equal", "depth_notequal", "depth_never", "cull_off", "near_clip",
+            "side_clip", "filter_nearest", "filter_linear", "filter_nn", "filter_ln", "filter_nl", "wrap_mirror",
+            "wrap_clamp_edge", "wrap_border_white", "shadow_blinn", "point_light", "vertex_color",
+            "basic_nodepth", "viewport_offset", "msaa_blend", "msaa_clip", "msaa_near", "layout_tiled",
+            "layout_morton", "depth_only_msaa", "ao_emissive"]
+
+
+def scene_variant(name, width=160, height=120, seed=11):
+    """One small frame exercising one pipeline state (for fixtures and parity tests)."""
+    st = S.Stream()
+    msaa = name.startswith("msaa") or name == "depth_only_msaa"
+    layouts = {}
+    verts, idx = uv_sphere(24, 12)
+    tex = checker_texture(64, 8, seed)
+    filt = {"filter_nearest": S.FILTER_NEAREST, "filter_linear": S.FILTER_LINEAR,
+            "filter_nn": S.FILTER_NEAREST_MIPMAP_NEAREST, "filter_ln": S.FILTER_LINEAR_MIPMAP_NEAREST,
+            "filter_nl": S.FILTER_NEAREST_MIPMAP_LINEAR}.get(name, S.FILTER_LINEAR_MIPMAP_LINEAR)
+    wrap = {"wrap_mirror": S.WRAP_MIRRORED_REPEAT, "wrap_clamp_edge": S.WRAP_CLAMP_TO_EDGE,
+            "wrap_border_white": S.WRAP_CLAMP_TO_BORDER}.get(name, S.WRAP_REPEAT)
+    st.texture(1, 64, 64, mipmaps=filt > S.FILTER_LINEAR, filter_min=filt, filter_mag=S.FILTER_LINEAR, wrap=wrap,
+               border=S.BORDER_WHITE if name == "wrap_border_white" else S.BORDER_BLACK,
+               usage=S.USAGE_SAMPLER | S.USAGE_UPLOAD)
+    st.tex_data(1, [tex])
+    if name == "layout_tiled":
+        layouts[1] = 1
+    if name == "layout_morton":
+        layouts[1] = 2
+    if name.startswith("wrap_"):
+        # texture coordinates outside [0, 1]
+        verts = verts.copy()
+        verts[:, 4:6] = verts[:, 4:6] * 3.0 - 1.0
+    st.vao(1, verts, idx)
+    st.texture(2, width, height, usage=S.USAGE_COLOR, msaa=msaa)
+    st.tex_init(2)
+    st.texture(3, width, height, format=S.FMT_FLOAT32, usage=S.USAGE_DEPTH, msaa=msaa)
+    st.tex_init(3)
+    st.fbo(1, color=-1 if name == "depth_only_msaa" else 2, depth=3)
+    eye = (0.0, 0.0, 3.0)
+    rz = name == "depth_gequal_rz"
+    fov = math.radians(60.0)
+    center = (0.0, 0.0, 0.0)
+    if name == "near_clip":
+        eye = (0.0, 0.3, 1.02)
+    if name == "msaa_near":
+        eye = (0.0, 0.3, 1.048)  # fragments between n/2 and n get z < 0: dropped at 4x, clamped at 1x
+    if name in ("side_clip", "msaa_clip"):
+        eye, center = (0.0, 0.0, 2.2), (0.9, 0.5, 0.0)
+    view = look_at(eye, center, (0, 1, 0))
+    proj = projection(fov, width / height, 0.1, reverse_z=rz)
+    model = np.eye(4)
+    mvp = proj @ view @ model
+    scene_u = S.uniforms_scene((0.2, 0.2, 0.2), eye, (1.0, 1.2, 1.5), (0.9, 0.9, 0.9))
+    if name in ("shadow_blinn", "point_light"):
+        scene_u = S.uniforms_scene((0.15, 0.15, 0.15), eye, (1.2, 1.6, 2.0), (1.0, 1.0, 1.0))
+    shadow_mvp = None
+    if name == "shadow_blinn":
+        # a depth pass from the light first, sampled by the Blinn-Phong shadow lookup
+        st.texture(4, 128, 128, format=S.FMT_FLOAT32, usage=S.USAGE_DEPTH | S.USAGE_SAMPLER,
+                   wrap=S.WRAP_CLAMP_TO_BORDER)
+        st.tex_init(4)
+        st.fbo(2, depth=4)
+        verts2, idx2 = uv_sphere(16, 8, 0.45)
+        verts2[:, 0:3] += np.array([0.5, 0.9, 1.0], np.float32)
+        st.vao(2, verts2, idx2)
+    st.frame()
+    clear_depth = 0.0 if rz else 1.0
+    if name == "shadow_blinn":
+        lview = look_at((1.2, 1.6, 2.0), (0, 0, 0), (0, 1, 0))
+        lproj = projection(math.radians(70.0), 1.0, 0.1)
+        bias = np.array([[0.5, 0, 0, 0.5], [0, 0.5, 0, 0.5], [0, 0, 1, 0], [0, 0, 0, 1]])
+        shadow_mvp = bias @ lproj @ lview
+        st.begin(2, False, True, (0, 0, 0, 0), 1.0)
+        st.viewport(0, 0, 128, 128)
+        rsd = _common_states(S.DEPTH_LESS, cull=True)
+        for vid in (1, 2):
+            st.draw(vid, S.SHADING_BASIC, 0, rsd, S.uniforms_model(model, lproj @ lview), bytes(S.SCENE_BYTES),
+                    S.uniforms_material(0, 0, 0))
+        st.end()
+    st.begin(1, True, True, (0.1, 0.2, 0.3, 1.0), clear_depth)
+    if name == "viewport_offset":
+        st.viewport(20, 10, width - 40, height - 30)
+    else:
+        st.viewport(0, 0, width, height)
+    rs = _common_states(S.DEPTH_LESS, cull=True)
+    shading, defines = S.SHADING_BLINN_PHONG_DIR, S.DEF_ALBEDO
+    samplers = [(S.SLOT_ALBEDO, 1)]
+    mat = S.uniforms_material(1, 0, 0, 1.0, 0.5, (1, 1, 1, 1))
+    second = None
+    if name.startswith("blend") or name == "msaa_blend":
+        rs = _common_states(S.DEPTH_LESS, cull=False, blend=True)
+        if name == "blend_add_one":
+            rs.set_blend_factor(S.BF_ONE, S.BF_ONE)
+        if name == "blend_minmax":
+            rs.blend_func_rgb, rs.blend_func_alpha = S.BLEND_MIN, S.BLEND_MAX
+            rs.blend_src_rgb, rs.blend_dst_rgb = S.BF_SRC_COLOR, S.BF_ONE_MINUS_DST_COLOR
+        if name == "blend_sub":
+            rs.blend_func_rgb, rs.blend_func_alpha = S.BLEND_SUBTRACT, S.BLEND_REVERSE_SUBTRACT
+            rs.blend_src_rgb, rs.blend_dst_rgb = S.BF_DST_ALPHA, S.BF_ONE_MINUS_DST_ALPHA
+            rs.blend_src_alpha, rs.blend_dst_alpha = S.BF_SRC_ALPHA, S.BF_DST_COLOR
+        mat = S.uniforms_material(1, 0, 0, 1.0, 0.5, (1, 1, 1, 0.6))
+        defines = 0
+        samplers = []
+    elif name in ("depth_lequal", "depth_gequal_rz", "depth_always_nomask", "depth_equal", "depth_notequal",
+                  "depth_never"):
+        fn = {"depth_lequal": S.DEPTH_LEQUAL, "depth_gequal_rz": S.DEPTH_GEQUAL, "depth_always_nomask": S.DEPTH_ALWAYS,
+              "depth_equal": S.DEPTH_EQUAL, "depth_notequal": S.DEPTH_NOTEQUAL, "depth_never": S.DEPTH_NEVER}[name]
+        rs = _common_states(S.DEPTH_LESS, cull=False)
+        # first draw with LESS (or GREATER for reverse Z), second with the function under test
+        second = _common_states(fn, cull=False)
+        if name == "depth_always_nomask":
+            second.depth_mask = False
+        if rz:
+            rs.depth_func = S.DEPTH_GREATER
+    elif name == "cull_off":
+        rs = _common_states(S.DEPTH_LESS, cull=False)
+    elif name in ("shadow_blinn", "point_light"):
+        shading = S.SHADING_BLINN_PHONG
+        mat = S.uniforms_material(1, 0, 1 if name == "shadow_blinn" else 0, 1.0, 0.8, (1, 1, 1, 1))
+        if name == "shadow_blinn":
+            samplers = [(S.SLOT_ALBEDO, 1), (S.SLOT_SHADOW, 4)]
+    elif name == "basic_nodepth":
+        shading, defines, samplers = S.SHADING_BASIC, 0, []
+        rs = S.RenderStates()
+        mat = S.uniforms_material(0, 0, 0, 1.0, 0.0, (0.9, 0.4, 0.2, 1.0))
+    elif name == "ao_emissive":
+        shading = S.SHADING_BLINN_PHONG
+        defines = S.DEF_ALBEDO | S.DEF_AO | S.DEF_EMISSIVE
+        samplers = [(S.SLOT_ALBEDO, 1), (S.SLOT_AO, 1), (S.SLOT_EMISSIVE, 1)]
+    elif name == "depth_only_msaa":
+        shading, defines, samplers = S.SHADING_BASIC, 0, []
+    model_u = S.uniforms_model(model, mvp, shadow_mvp, reverse_z=rz)
+    if name == "vertex_color":
+        v, i = random_triangles(300, width, height, seed, 1.0, 200.0, 0.0, 1.0, big=2)
+        st2 = st  # appended VAO must precede the frame: rebuild the stream with it
+        st = S.Stream()
+        st.texture(2, width, height, usage=S.USAGE_COLOR)
+        st.tex_init(2)
+        st.texture(3, width, height, format=S.FMT_FLOAT32, usage=S.USAGE_DEPTH)
+        st.tex_init(3)
+        st.fbo(1, color=2, depth=3)
+        st.vao(3, v, i)
+        st.frame()
+        st.begin(1, True, True, (0, 0, 0, 0), 1.0)
+        st.viewport(0, 0, width, height)
+        st.draw(3, S.SHADING_VERTEX_COLOR, 0, _common_states(S.DEPTH_LESS, cull=False),
+                S.uniforms_model(np.eye(4), np.eye(4)), bytes(S.SCENE_BYTES), S.uniforms_material(0, 0, 0))
+        st.end()
+        st.read(2)
+        st.read(3)
+        del st2
+        return Scene(name, st, width, height, False, dict(layouts={}))
+    st.draw(1, shading, defines, rs, model_u, scene_u, mat, samplers=samplers)
+    if name.startswith("blend") or name == "msaa_blend":
+        # a second, offset sphere blended over the first
+        tm = _translate((0.35, 0.2, 0.4))
+        st.draw(1, shading, defines, rs, S.uniforms_model(tm, proj @ view @ tm), scene_u,
+                S.uniforms_material(1, 0, 0, 1.0, 0.5, (0.3, 0.9, 0.5, 0.5)), samplers=samplers)
+    if second is not None:
+        sm = np.diag([1.0, 1.0, 1.0, 1.0])
+        sm[0, 3] = 0.15
+        st.draw(1, S.SHADING_BASIC, 0, second, S.uniforms_model(sm, proj @ view @ sm, reverse_z=rz), scene_u,
+                S.uniforms_material(0, 0, 0, 1.0, 0.0, (0.9, 0.2, 0.1, 1.0)))
+    st.end()
+    if name != "depth_only_msaa":
+        st.read(2, which=0)
+        if msaa:
+            st.read(2, which=1)
+    st.read(3, which=1 if msaa else 0)
+    return Scene(name, st, width, height, msaa, dict(layouts=layouts))

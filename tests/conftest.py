@@ -1,0 +1,21 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) and the built librhip")
+
+
+@pytest.fixture(scope="session")
+def renderer():
+    from softglrender_amd.renderer import RendererHIP
+    r = RendererHIP(0)
+    r.create()
+    yield r
+    r.destroy()
