@@ -292,6 +292,52 @@ struct Timer {
   }
 };
 
+template <int S, int SH>
+void launchVisMode(const DrawParams &P, int ntiles, int need_deriv, hipStream_t st) {
+  switch (P.st.depthFunc) {
+    case 1: {
+#define K_VIS k_raster_vis<S, SH, VIS_LESS>
+      RHIP_LAUNCH(K_VIS, ntiles, 256, st, P, need_deriv);
+#undef K_VIS
+      break;
+    }
+    case 3: {
+#define K_VIS k_raster_vis<S, SH, VIS_LEQUAL>
+      RHIP_LAUNCH(K_VIS, ntiles, 256, st, P, need_deriv);
+#undef K_VIS
+      break;
+    }
+    case 4: {
+#define K_VIS k_raster_vis<S, SH, VIS_GREATER>
+      RHIP_LAUNCH(K_VIS, ntiles, 256, st, P, need_deriv);
+#undef K_VIS
+      break;
+    }
+    default: {
+#define K_VIS k_raster_vis<S, SH, VIS_GEQUAL>
+      RHIP_LAUNCH(K_VIS, ntiles, 256, st, P, need_deriv);
+#undef K_VIS
+      break;
+    }
+  }
+}
+
+/* Two-phase raster (visibility by LDS atomics, then one shading per winner). */
+template <int S>
+void launchVis(const DrawParams &P, int ntiles, int need_deriv, hipStream_t st) {
+  if (!P.color) {
+    launchVisMode<S, RHS_SHADING_BASIC>(P, ntiles, need_deriv, st);
+    return;
+  }
+  switch (P.shading) {
+    case RHS_SHADING_BASIC: launchVisMode<S, RHS_SHADING_BASIC>(P, ntiles, need_deriv, st); break;
+    case RHS_SHADING_BLINN_PHONG: launchVisMode<S, RHS_SHADING_BLINN_PHONG>(P, ntiles, need_deriv, st); break;
+    case RHS_SHADING_BLINN_PHONG_DIR: launchVisMode<S, RHS_SHADING_BLINN_PHONG_DIR>(P, ntiles, need_deriv, st); break;
+    case RHS_SHADING_PBR: launchVisMode<S, RHS_SHADING_PBR>(P, ntiles, need_deriv, st); break;
+    default: launchVisMode<S, RHS_SHADING_VERTEX_COLOR>(P, ntiles, need_deriv, st); break;
+  }
+}
+
 template <int S>
 void launchRaster(const DrawParams &P, int ntiles, int need_deriv, hipStream_t st) {
   switch (P.shading) {
@@ -873,7 +919,16 @@ int rhip_draw(rhip_device d, const rhip_draw_desc *dd) {
   RHIP_LAUNCH(k_tile_ranges, blocks(ntiles + 1, 256), 256, s, P);
   HIPCHK(hipGetLastError());
 
-  {
+  /* draws whose result follows from depths and order alone take the two-phase raster */
+  const bool twoPhase = !st.blend && P.depth && st.depthTest && st.depthMask &&
+                        (st.depthFunc == 1 || st.depthFunc == 3 || st.depthFunc == 4 || st.depthFunc == 6);
+  if (twoPhase) {
+    Timer tm(d, samples == 4 ? "k_raster_vis_msaa4" : "k_raster_vis");
+    if (samples == 4)
+      launchVis<4>(P, ntiles, need_deriv ? 1 : 0, s);
+    else
+      launchVis<1>(P, ntiles, need_deriv ? 1 : 0, s);
+  } else {
     Timer tm(d, samples == 4 ? "k_raster_msaa4" : "k_raster");
     if (samples == 4)
       launchRaster<4>(P, ntiles, need_deriv ? 1 : 0, s);

@@ -595,6 +595,249 @@ __global__ __launch_bounds__(256) void k_raster(DrawParams P, int need_deriv) {
   if (nwr) atomicAdd(&P.counters[1], nwr);
 }
 
+/* --------------------------------------------------------------------------- two-phase raster
+ *
+ * For draws without blending whose depth test writes depth with LESS, LEQUAL,
+ * GREATER or GEQUAL, the colour a sample ends with is the fragment shader
+ * output of the last fragment that passed its depth test, and which fragment
+ * that is follows from depths and API order alone: the minimum (or maximum)
+ * depth, ties going to the earliest fragment for the strict tests and the
+ * latest for the others. Phase 1 finds that winner per sample with 64-bit LDS
+ * atomics on key = (monotonic depth bits << 32) | order word, one lane per
+ * triangle of the tile; phase 2 shades each pixel's winners once, one lane per
+ * pixel. The fragment shader's inputs depend only on the triangle, the pixel
+ * and the 2x2 quad it is shaded in, so the result equals the sequential one. */
+enum VisMode { VIS_LESS = 0, VIS_LEQUAL = 1, VIS_GREATER = 2, VIS_GEQUAL = 3 };
+
+template <int MODE>
+__device__ __forceinline__ constexpr bool visMax() { return MODE == VIS_GREATER || MODE == VIS_GEQUAL; }
+/* strict tests keep the first of equal depths; LEQUAL / GEQUAL the last */
+template <int MODE>
+__device__ __forceinline__ constexpr bool visFirst() { return MODE == VIS_LESS || MODE == VIS_GREATER; }
+/* the low word grows with the order for LESS and GEQUAL and shrinks for the others */
+template <int MODE>
+__device__ __forceinline__ uint32_t visLow(uint32_t order) {
+  return (MODE == VIS_LESS || MODE == VIS_GEQUAL) ? order : ~order;
+}
+
+/* Unsigned key that orders like the float (both zeros equal). */
+__device__ __forceinline__ uint32_t depthKey(float z) {
+  uint32_t b = __float_as_uint(z);
+  if ((b << 1) == 0) b = 0;
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+template <int MODE>
+__device__ __forceinline__ unsigned long long visInit(float dep, bool open) {
+  /* a NaN in the buffer fails every comparison: no fragment may win there */
+  if (!open || dep != dep) return ((unsigned long long)(visMax<MODE>() ? 0xffffffffu : 0u) << 32) | visLow<MODE>(0);
+  return ((unsigned long long)depthKey(dep) << 32) | visLow<MODE>(0);
+}
+
+__device__ __forceinline__ Tri makeTri(const float4 &f0, const float4 &f1, const float4 &f2) {
+  Tri T;
+  T.v0x = f0.x;
+  T.v0y = f0.y;
+  T.a0 = f2.x - f0.x;
+  T.a1 = f1.x - f0.x;
+  T.b0 = f2.y - f0.y;
+  T.b1 = f1.y - f0.y;
+  T.uz = fmaf(T.a0, T.b1, -(T.a1 * T.b0));
+  T.z[0] = f0.z, T.z[1] = f1.z, T.z[2] = f2.z;
+  T.w[0] = f0.w, T.w[1] = f1.w, T.w[2] = f2.w;
+  return T;
+}
+
+template <int S, int SHADING, int MODE>
+__global__ __launch_bounds__(256) void k_raster_vis(DrawParams P, int need_deriv) {
+  constexpr int NPIX = kTile * kTile;
+  __shared__ unsigned long long keys[NPIX * S];
+  const int tile = blockIdx.x;
+  const int tx0 = (tile % P.tiles_x) * kTile, ty0 = (tile / P.tiles_x) * kTile;
+  const int lp = threadIdx.x;
+  const int px = tx0 + (lp & (kTile - 1)), py = ty0 + lp / kTile;
+  const bool inFb = px < P.fb_w && py < P.fb_h;
+  const size_t pix = (size_t)py * P.fb_w + px;
+  const bool hasC = P.color != nullptr;
+  const int start = P.tile_start[tile], end = P.tile_start[tile + 1];
+  const float lx[4] = {0.375f, 0.875f, 0.125f, 0.625f};
+  const float ly[4] = {0.875f, 0.625f, 0.375f, 0.125f};
+
+  float dep[S];
+  uint32_t col[S];
+#pragma unroll
+  for (int s = 0; s < S; s++) {
+    dep[s] = 0.f;
+    col[s] = 0u;
+  }
+  if (inFb) {
+    for (int s = 0; s < S; s++) dep[s] = P.depth[pix * S + s];
+    if (hasC)
+      for (int s = 0; s < S; s++) col[s] = P.color[pix * S + s];
+  }
+  if (start == end) {
+    if (inFb && hasC && S == 4) {
+      int r[4];
+#pragma unroll
+      for (int ch = 0; ch < 4; ch++) r[ch] = (chan(col[0], ch) + chan(col[1], ch) + chan(col[2], ch) + chan(col[3], ch)) >> 2;
+      P.color_resolved[pix] = pack4(r[0], r[1], r[2], r[3]);
+    }
+    return;
+  }
+#pragma unroll
+  for (int s = 0; s < S; s++) keys[lp * S + s] = visInit<MODE>(dep[s], inFb);
+  __syncthreads();
+
+  /* phase 1: visibility, one lane per triangle of the tile */
+  for (int it = start + lp; it < end; it += blockDim.x) {
+    const int id = P.pair_val_sorted[it];
+    const PrimSetup ps = P.prims[id];
+    const int x0 = max(ps.qx0, tx0), x1 = min(min(ps.qx1, tx0 + kTile - 1), P.fb_w - 1);
+    const int y0 = max(ps.qy0, ty0), y1 = min(min(ps.qy1, ty0 + kTile - 1), P.fb_h - 1);
+    if (x0 > x1 || y0 > y1) continue;
+    const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
+    for (int y = y0; y <= y1; y++) {
+      int qys[2];
+      const int nqy = axisQuads(ps.min_y, ps.max_y, ps.bcy, y, qys);
+      if (!nqy) continue;
+      for (int x = x0; x <= x1; x++) {
+        int qxs[2];
+        const int nqx = axisQuads(ps.min_x, ps.max_x, ps.bcx, x, qxs);
+        if (!nqx) continue;
+        /* of several quads of one triangle holding this pixel, the test keeps the first or the last */
+        const uint32_t inst = visFirst<MODE>() ? 0u : (uint32_t)((nqy - 1) * 2 + (nqx - 1));
+        const uint32_t low = visLow<MODE>((((uint32_t)id << 2) | inst) + 1u);
+        unsigned long long *kp = &keys[((y - ty0) * kTile + (x - tx0)) * S];
+        const float fx = (float)x, fy = (float)y;
+        if (S == 4) {
+#pragma unroll
+          for (int s = 0; s < 4; s++) {
+            float4 bc;
+            if (!bary(T, lx[s] + fx, ly[s] + fy, bc)) continue;
+            const float z = dpps3(bc, T.z);
+            if (z < 0.f || z > 1.f) continue; /* multisampled depth clip (RendererSoft.cpp:796-799) */
+            const unsigned long long key = ((unsigned long long)depthKey(z) << 32) | low;
+            if (visMax<MODE>())
+              atomicMax(kp + s, key);
+            else
+              atomicMin(kp + s, key);
+          }
+        } else {
+          float4 bc;
+          if (!bary(T, fx + 0.5f, fy + 0.5f, bc)) continue;
+          const float z = gclamp(dpps3(bc, T.z), 0.f, 1.f);
+          if (z != z) continue;
+          const unsigned long long key = ((unsigned long long)depthKey(z) << 32) | low;
+          if (visMax<MODE>())
+            atomicMax(kp, key);
+          else
+            atomicMin(kp, key);
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  /* phase 2: shade each pixel's winners, one lane per pixel */
+  if (!inFb) return;
+  uint32_t win[S];
+#pragma unroll
+  for (int s = 0; s < S; s++) {
+    const uint32_t low = (uint32_t)keys[lp * S + s];
+    const uint32_t ord = (MODE == VIS_LESS || MODE == VIS_GEQUAL) ? low : ~low;
+    win[s] = ord ? ord - 1u : 0xffffffffu;
+  }
+  unsigned long long nfs = 0, nwr = 0;
+#pragma unroll
+  for (int s = 0; s < S; s++) {
+    const uint32_t w = win[s];
+    if (w == 0xffffffffu) continue;
+    bool seen = false;
+#pragma unroll
+    for (int t = 0; t < s; t++) seen = seen || win[t] == w;
+    if (seen) continue;
+    const int id = (int)(w >> 2), inst = (int)(w & 3u);
+    const PrimSetup ps = P.prims[id];
+    const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
+    const float fx = (float)px, fy = (float)py;
+    /* raw barycentrics of the samples and the shading sample (RendererSoft.cpp:776-802) */
+    float4 sbc[5];
+    bool sin[5];
+    int shade;
+    if (S == 4) {
+#pragma unroll
+      for (int t = 0; t < 4; t++) sin[t] = bary(T, lx[t] + fx, ly[t] + fy, sbc[t]);
+      sin[4] = bary(T, fx + 0.5f, fy + 0.5f, sbc[4]);
+      shade = 4;
+      if (!sin[4]) {
+#pragma unroll
+        for (int t = 3; t >= 0; t--)
+          if (sin[t]) shade = t;
+      }
+    } else {
+      sin[0] = bary(T, fx + 0.5f, fy + 0.5f, sbc[0]);
+      shade = 0;
+    }
+    RGBAf c = {0.f, 0.f, 0.f, 0.f};
+    if (hasC) {
+      float4 b = sbc[shade];
+      const float w = dpps3(b, T.w);
+      const float iw = 1.f / w;
+      b = make_float4(b.x * (iw * T.w[0]), b.y * (iw * T.w[1]), b.z * (iw * T.w[2]), b.w * (iw * 1.f));
+      float vv[kVaryStride];
+      const float *va = P.vary + (size_t)ps.v[0] * kVaryStride;
+      const float *vb = P.vary + (size_t)ps.v[1] * kVaryStride;
+      const float *vc = P.vary + (size_t)ps.v[2] * kVaryStride;
+#pragma unroll
+      for (int j = 0; j < kVaryStride; j++) vv[j] = varyUsed<SHADING>(P, j) ? interp(b, va[j], vb[j], vc[j]) : 0.f;
+      Frag fr;
+      fr.v = vv;
+      fr.tc[0] = fr.tc[1] = fr.tc[2] = {0.f, 0.f};
+      if (need_deriv) {
+        int qxs[2], qys[2];
+        axisQuads(ps.min_x, ps.max_x, ps.bcx, px, qxs);
+        axisQuads(ps.min_y, ps.max_y, ps.bcy, py, qys);
+        const int qx = qxs[inst & 1], qy = qys[inst >> 1];
+        const int k = (px - qx) + 2 * (py - qy);
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          if (j == k) {
+            fr.tc[j] = {vv[0], vv[1]};
+          } else {
+            const float4 pb = shadingBc<S>(T, (float)(qx + (j & 1)), (float)(qy + (j >> 1)));
+            fr.tc[j] = {interp(pb, va[0], vb[0], vc[0]), interp(pb, va[1], vb[1], vc[1])};
+          }
+        }
+      }
+      c = runFS<SHADING>(P, fr);
+      c = {gclamp(c.r, 0.f, 1.f), gclamp(c.g, 0.f, 1.f), gclamp(c.b, 0.f, 1.f), gclamp(c.a, 0.f, 1.f)};
+      nfs++;
+    }
+    const uint32_t packed = pack4(u8t(c.r * 255.f), u8t(c.g * 255.f), u8t(c.b * 255.f), u8t(c.a * 255.f));
+#pragma unroll
+    for (int t = 0; t < S; t++) {
+      if (win[t] != w) continue;
+      dep[t] = gclamp(dpps3(sbc[S == 4 ? t : 0], T.z), 0.f, 1.f);
+      if (hasC) {
+        col[t] = packed;
+        nwr++;
+      }
+    }
+  }
+  for (int s = 0; s < S; s++) P.depth[pix * S + s] = dep[s];
+  if (hasC) {
+    for (int s = 0; s < S; s++) P.color[pix * S + s] = col[s];
+    if (S == 4) {
+      int r[4];
+#pragma unroll
+      for (int ch = 0; ch < 4; ch++) r[ch] = (chan(col[0], ch) + chan(col[1], ch) + chan(col[2], ch) + chan(col[3], ch)) >> 2;
+      P.color_resolved[pix] = pack4(r[0], r[1], r[2], r[3]);
+    }
+  }
+  if (nfs) atomicAdd(&P.counters[0], nfs);
+  if (nwr) atomicAdd(&P.counters[1], nwr);
+}
+
 /* --------------------------------------------------------------------------- textures */
 
 /* generateMipmaps / sampleBufferBilinear (SamplerSoft.h:89-110, 240-252): level i is a

@@ -38,7 +38,8 @@ __device__ __forceinline__ float rlog2f(float x) { return (float)log2((double)x)
 __device__ __forceinline__ float rexp2f(float x) { return (float)exp2((double)x); }
 __device__ __forceinline__ float gmax(float a, float b) { return a < b ? b : a; } /* glm::max */
 __device__ __forceinline__ float gmin(float a, float b) { return b < a ? b : a; } /* glm::min */
-__device__ __forceinline__ float gclamp(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+/* glm::clamp = min(max(x, lo), hi) with glm's comparisons (keeps -0 and NaN like the reference) */
+__device__ __forceinline__ float gclamp(float v, float lo, float hi) { return gmin(gmax(v, lo), hi); }
 
 /* mat4 * vec4(p, 1): (fma(c0, x, c1*y)) + (fma(c2, z, c3))  -- glm type_mat4x4.inl:536-575 */
 __device__ __forceinline__ float4 m4p(const float *m, float x, float y, float z) {
