@@ -25,6 +25,11 @@ TARGETS = {
     "librhip.so": ["rhip_api.hip"],
 }
 
+HOST = os.path.join(HERE, "host")
+# RendererHIP (the C++ Renderer implementation) and its stream driver, on librhip
+HOST_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-Wno-unused-function",
+              "-I", os.path.join(HOST, "compat"), "-I", HOST, "-I", os.path.join(ROOT, "include")]
+
 
 def _run(cmd):
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -43,6 +48,13 @@ def build(verbose: bool = False) -> None:
             print(" ".join(cmd), flush=True)
         _run(cmd)
         os.replace(tmp, out)
+    out = os.path.join(LIB_DIR, "librenderer_hip.so")
+    cmd = ["g++", *HOST_FLAGS, "-o", out + ".tmp", os.path.join(HOST, "RendererHIP.cpp"), os.path.join(HOST, "replay.cpp"),
+           "-L", LIB_DIR, "-lrhip", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    _run(cmd)
+    os.replace(out + ".tmp", out)
 
 
 if __name__ == "__main__":
