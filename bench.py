@@ -39,15 +39,64 @@ def _cpu_model():
     return platform.processor()
 
 
-def raster_algorithmic_bytes(w, h, samples, nprims, npairs, nvary_floats):
-    """Compulsory HBM bytes of one fine-raster launch: colour and depth samples read
-    and written once, the resolved image written once, each tile pair and
-    primitive setup record read once, and the screen position and the varyings
-    the shader reads for each vertex read once (3 vertices per primitive is the
-    upper bound; shared vertices are counted once per primitive here)."""
-    fb = w * h * samples * (4 + 4) * 2 + (w * h * 4 if samples == 4 else 0)
-    geom = npairs * 4 + nprims * 64 + nprims * 3 * (16 + nvary_floats * 4) // 6
-    return fb + geom
+# varying floats the fragment shader reads per vertex, by shading model and normal-map define
+def _vary_floats(shading, defines):
+    from softglrender_amd import stream as S
+    nm = bool(defines & S.DEF_NORMAL)
+    if shading == S.SHADING_PBR:
+        return 14 if nm else 11
+    if shading in (S.SHADING_BLINN_PHONG, S.SHADING_BLINN_PHONG_DIR):
+        return 14 if nm else 11
+    if shading == S.SHADING_VERTEX_COLOR:
+        return 4
+    return 0
+
+
+def frame_draws(data):
+    """Per draw of the frame part of a stream: triangles, vertices, framebuffer size and samples, the varying
+    floats its shader reads, whether it has a colour target, and the raster kernel it runs."""
+    import struct
+    from softglrender_amd import stream as S
+    from softglrender_amd.replay import parse
+    tex, vao, fbo, draws = {}, {}, {}, []
+    cur = None
+    in_frame = False
+    for op, p in parse(data):
+        if op == S.OP_TEXTURE:
+            tid, typ, fmt, w, h, usage, mips, msaa = struct.unpack_from("<8i", p, 0)
+            tex[tid] = (w, h, 4 if msaa else 1)
+        elif op == S.OP_VAO:
+            vid, stride, nv, ni = struct.unpack_from("<4i", p, 0)
+            vao[vid] = (ni // 3, nv)
+        elif op == S.OP_FBO:
+            fid, color, layer, level, depth = struct.unpack_from("<5i", p, 0)
+            fbo[fid] = (color, depth)
+        elif op == S.OP_FRAME:
+            in_frame = True
+        elif op == S.OP_BEGIN:
+            cur = struct.unpack_from("<i", p, 0)[0]
+        elif op == S.OP_DRAW and in_frame:
+            vid, shading, defines = struct.unpack_from("<iiI", p, 0)
+            st = struct.unpack_from("<13i", p, 12)
+            color, depth = fbo[cur]
+            w, h, smp = tex[color if color >= 0 else depth]
+            two_phase = (not st[0]) and depth >= 0 and st[7] and st[8] and st[9] in (1, 3, 4, 6)
+            kern = ("k_raster_vis" if two_phase else "k_raster_ord") + ("_msaa4" if smp == 4 else "")
+            t, v = vao[vid]
+            draws.append(dict(kernel=kern, tris=t, verts=v, w=w, h=h, samples=smp, color=color >= 0,
+                              vary=_vary_floats(shading, defines) if color >= 0 else 0))
+    return draws
+
+
+def raster_algorithmic_bytes(d):
+    """Compulsory HBM bytes of one fine-raster launch: every colour and depth sample read and written once, the
+    resolved image written once, 12 bytes of indices per triangle, and per vertex its screen position and the
+    varyings the shader reads, once (texel reads are not counted)."""
+    px = d["w"] * d["h"]
+    fb = px * d["samples"] * 4 * 2 + (px * d["samples"] * 4 * 2 if d["color"] else 0)
+    if d["color"] and d["samples"] == 4:
+        fb += px * 4
+    return fb + 12 * d["tris"] + d["verts"] * (16 + 4 * d["vary"])
 
 
 def main():
@@ -55,7 +104,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=3, help="timed reference frames (after one warm-up)")
     args = ap.parse_args()
@@ -73,6 +122,9 @@ def main():
     if args.config == "c2":
         sc = scenes.scene_c2()
         workload = "c2_1080p_msaa4x_reverseZ_pbr_ibl_1M_tris_plus_translucent_plus_2048_shadow"
+    elif args.config == "c3":
+        sc = scenes.scene_c3()
+        workload = "c3_2160p_msaa4x_2M_random_tris_plus_2k_screen_covering_vertex_colour_less"
     else:
         sc = scenes.scene_c1()
         workload = "c1_640x480_uvsphere_4k_tris_blinnphong_trilinear"
@@ -106,6 +158,12 @@ def main():
     for _ in range(args.warmup):
         run_frame()
     r.waitIdle()
+    # frame time without per-kernel events
+    t_a = time.perf_counter()
+    for _ in range(args.steps):
+        run_frame()
+    r.waitIdle()
+    ms_no_events = (time.perf_counter() - t_a) * 1e3 / args.steps
     r.resetStats()
     r.profile(True)
     r.waitIdle()
@@ -121,22 +179,17 @@ def main():
     fs_per_frame = stats["fs_invocations"] / args.steps
     print(f"[bench] {args.steps} frames: {ms_per_frame:.3f} ms/frame; kernels {prof}", file=sys.stderr, flush=True)
 
-    # roofline of the dominant kernel (largest summed time over the timed frames)
+    # roofline of the dominant kernel (largest summed time over the timed frames); its algorithmic bytes are
+    # those of the draws it ran for, averaged over its launches
+    draws = frame_draws(data)
     dom = max(prof.items(), key=lambda kv: kv[1][0]) if prof else None
     roofline = None
     if dom:
         name, (tot_ms, launches) = dom
         avg_ms = tot_ms / launches
-        w, h = sc.width, sc.height
-        samples = 4 if sc.msaa else 1
-        if name.startswith("k_raster") and name.endswith("msaa4"):
-            # launches alternate between passes; count the bytes of the average launch
-            pairs = stats["tile_pairs"] / max(stats["draws"], 1)
-            prims = stats["triangles_binned"] / max(stats["draws"], 1)
-            algo = raster_algorithmic_bytes(w, h, samples, prims, pairs, 11)
-        else:
-            algo = None
-        if algo:
+        mine = [d for d in draws if d["kernel"] == name]
+        if mine:
+            algo = sum(raster_algorithmic_bytes(d) for d in mine) / len(mine)
             achieved = algo / (avg_ms * 1e-3) / 1e9
             roofline = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
@@ -144,7 +197,28 @@ def main():
 
     cpu = None
     ref_frags = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.config == "c3":
+        # the reference needs about an hour for a whole c3 frame: time it on a 240x135 crop (1/256 of the
+        # area) holding the same triangle density and all 2000 screen-covering triangles, and scale
+        try:
+            import oracle
+            crop = scenes.scene_c3(240, 135, 2_000_000 // 256, 2000, seed=5)
+            st = {}
+            oracle.cpu_ref_replay(crop.stream.bytes(), crop.stream.output_bytes(), stats=st,
+                                  threads=min(16, os.cpu_count() or 1))
+            ref_frags = st["fs_invocations"] * 256
+            if oracle.have_reference(threaded=False):
+                _, times = oracle.ref_replay(crop.stream.bytes(), crop.stream.output_bytes(), repeats=1)
+                est = times[0] * 256
+                cpu = {"value": round(est, 1), "unit": "ms/frame", "cores": 1, "kind": "reference",
+                       "sample": f"one 240x135 frame with {2_000_000 // 256} random triangles and the 2000 "
+                                 f"screen-covering ones ({times[0]:.0f} ms) of SoftGLRender RendererSoft built "
+                                 f"serially from its sources, times 256 (the area ratio); host CPU "
+                                 f"'{_cpu_model()}', nproc {os.cpu_count()}",
+                       "gpu_speedup": round(est / ms_per_frame, 1)}
+        except Exception as e:  # report, do not hide
+            cpu = {"error": repr(e)}
+    if not args.no_cpu_baseline and args.config != "c3":
         try:
             import oracle
             # fragments the reference shades per frame (its early-Z survivors), counted by the
@@ -171,6 +245,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_per_frame, 4),
+        "ms_per_frame_without_kernel_events": round(ms_no_events, 4),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
@@ -179,6 +254,8 @@ def main():
         "config": {"workload": workload, "width": sc.width, "height": sc.height, "msaa": 4 if sc.msaa else 1,
                    "triangles": sc.meta.get("triangles"), "parallelism": "single_gpu"},
         "gfrag_per_s": round(ref_frags / (ms_per_frame * 1e-3) / 1e9, 4) if ref_frags else None,
+        "reference_fragments_note": ("counted by the checker on the 1/256 crop, times 256"
+                                     if args.config == "c3" and ref_frags else None),
         "reference_fragments_per_frame": ref_frags,
         "gpu_fs_invocations_per_frame": int(fs_per_frame),
         "kernels_ms_per_frame": {k: round(v[0] / args.steps, 4) for k, v in prof.items()},

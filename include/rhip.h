@@ -91,6 +91,7 @@ typedef struct rhip_stats {
   long long tile_pairs;
   long long clipped_triangles;
   long long draws;
+  long long hiz_culled; /* (tile, triangle) pairs skipped by the per-tile hierarchical-Z test */
 } rhip_stats;
 
 /* ---------------------------------------------------------------- device (Renderer::create/destroy) */

@@ -53,7 +53,8 @@ class DrawDesc(ctypes.Structure):
 
 class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_longlong) for n in ("fs_invocations", "samples_written", "triangles_binned",
-                                                  "tile_pairs", "clipped_triangles", "draws")]
+                                                  "tile_pairs", "clipped_triangles", "draws",
+                                                  "hiz_culled")]
 
 
 # every exported symbol of rhip.h, with its argument types

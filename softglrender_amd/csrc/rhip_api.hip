@@ -6,7 +6,6 @@
  * pipeline of rhip_kernels.h.
  */
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cstdarg>
@@ -24,6 +23,9 @@
 using namespace rhip;
 
 /* Every kernel launch goes through this macro (one place to change how kernels are started). */
+/* a comma inside a macro argument (template argument lists) */
+#define RHIP_COMMA ,
+
 #ifndef RHIP_LAUNCH
 #define RHIP_LAUNCH(kern, grid, block, stream, ...) kern<<<(grid), (block), 0, (stream)>>>(__VA_ARGS__)
 #endif
@@ -121,8 +123,7 @@ struct rhip_device_s {
   int cur_fbo = 0;
   int vp_x = 0, vp_y = 0, vp_w = 0, vp_h = 0;
   /* per-draw arenas */
-  DevBuf clip, frag, mask, vary, clip_raw, clip_flag, clip_list, clip_poly, fan_count, fan_offset, prims,
-      tile_count, tile_offset, pair_key, pair_val, pair_key_s, pair_val_s, tile_start, cub_tmp, scalars;
+  DevBuf clip, frag, mask, vary, clip_raw, clip_poly, blk, prims, bins, bin_items, scalars, jobs, hiz;
   unsigned long long *counters = nullptr; /* device */
   /* optional per-kernel timing: an event pair around every launch */
   bool profiling = false;
@@ -132,7 +133,6 @@ struct rhip_device_s {
   };
   std::vector<Timed> timed;
   std::vector<hipEvent_t> event_pool;
-  int *host_scalars = nullptr;            /* pinned */
   rhip_stats host_stats{};
 };
 
@@ -292,90 +292,73 @@ struct Timer {
   }
 };
 
-template <int S, int SH>
-void launchVisMode(const DrawParams &P, int ntiles, int need_deriv, hipStream_t st) {
-  switch (P.st.depthFunc) {
-    case 1: {
-#define K_VIS k_raster_vis<S, SH, VIS_LESS>
-      RHIP_LAUNCH(K_VIS, ntiles, 256, st, P, need_deriv);
-#undef K_VIS
+/* Two-phase raster: coarse hierarchical Z, then visibility and depth per tile. */
+template <int S>
+void launchVis(const DrawParams &P, int ntiles, int ncoarse, hipStream_t st) {
+  const int f = P.st.depthFunc;
+  if (f == 4 || f == 6) {
+    RHIP_LAUNCH(k_hiz_coarse<S RHIP_COMMA true>, ncoarse, 256, st, P);
+  } else {
+    RHIP_LAUNCH(k_hiz_coarse<S RHIP_COMMA false>, ncoarse, 256, st, P);
+  }
+  switch (f) {
+    case 1: RHIP_LAUNCH(k_raster_vis<S RHIP_COMMA VIS_LESS>, ntiles, 256, st, P); break;
+    case 3: RHIP_LAUNCH(k_raster_vis<S RHIP_COMMA VIS_LEQUAL>, ntiles, 256, st, P); break;
+    case 4: RHIP_LAUNCH(k_raster_vis<S RHIP_COMMA VIS_GREATER>, ntiles, 256, st, P); break;
+    default: RHIP_LAUNCH(k_raster_vis<S RHIP_COMMA VIS_GEQUAL>, ntiles, 256, st, P); break;
+  }
+}
+
+/* Shading of the fragments the two-phase raster queued. */
+template <int S>
+void launchShade(const DrawParams &P, int grid, int need_deriv, hipStream_t st) {
+  switch (P.shading) {
+    case RHS_SHADING_BASIC: RHIP_LAUNCH(k_shade<S RHIP_COMMA RHS_SHADING_BASIC>, grid, 256, st, P, need_deriv); break;
+    case RHS_SHADING_BLINN_PHONG:
+      RHIP_LAUNCH(k_shade<S RHIP_COMMA RHS_SHADING_BLINN_PHONG>, grid, 256, st, P, need_deriv);
+      break;
+    case RHS_SHADING_BLINN_PHONG_DIR:
+      RHIP_LAUNCH(k_shade<S RHIP_COMMA RHS_SHADING_BLINN_PHONG_DIR>, grid, 256, st, P, need_deriv);
+      break;
+    case RHS_SHADING_PBR: RHIP_LAUNCH(k_shade<S RHIP_COMMA RHS_SHADING_PBR>, grid, 256, st, P, need_deriv); break;
+    default: RHIP_LAUNCH(k_shade<S RHIP_COMMA RHS_SHADING_VERTEX_COLOR>, grid, 256, st, P, need_deriv); break;
+  }
+}
+
+/* Ordered raster (every state the two-phase raster does not take). */
+template <int S>
+void launchOrdered(const DrawParams &P, int ntiles, int need_deriv, int flush_each, hipStream_t st) {
+  switch (P.shading) {
+    case RHS_SHADING_BASIC: {
+#define K_ORD k_raster_ord<S, RHS_SHADING_BASIC>
+      RHIP_LAUNCH(K_ORD, ntiles, 256, st, P, need_deriv, flush_each);
+#undef K_ORD
       break;
     }
-    case 3: {
-#define K_VIS k_raster_vis<S, SH, VIS_LEQUAL>
-      RHIP_LAUNCH(K_VIS, ntiles, 256, st, P, need_deriv);
-#undef K_VIS
+    case RHS_SHADING_BLINN_PHONG: {
+#define K_ORD k_raster_ord<S, RHS_SHADING_BLINN_PHONG>
+      RHIP_LAUNCH(K_ORD, ntiles, 256, st, P, need_deriv, flush_each);
+#undef K_ORD
       break;
     }
-    case 4: {
-#define K_VIS k_raster_vis<S, SH, VIS_GREATER>
-      RHIP_LAUNCH(K_VIS, ntiles, 256, st, P, need_deriv);
-#undef K_VIS
+    case RHS_SHADING_BLINN_PHONG_DIR: {
+#define K_ORD k_raster_ord<S, RHS_SHADING_BLINN_PHONG_DIR>
+      RHIP_LAUNCH(K_ORD, ntiles, 256, st, P, need_deriv, flush_each);
+#undef K_ORD
+      break;
+    }
+    case RHS_SHADING_PBR: {
+#define K_ORD k_raster_ord<S, RHS_SHADING_PBR>
+      RHIP_LAUNCH(K_ORD, ntiles, 256, st, P, need_deriv, flush_each);
+#undef K_ORD
       break;
     }
     default: {
-#define K_VIS k_raster_vis<S, SH, VIS_GEQUAL>
-      RHIP_LAUNCH(K_VIS, ntiles, 256, st, P, need_deriv);
-#undef K_VIS
+#define K_ORD k_raster_ord<S, RHS_SHADING_VERTEX_COLOR>
+      RHIP_LAUNCH(K_ORD, ntiles, 256, st, P, need_deriv, flush_each);
+#undef K_ORD
       break;
     }
-  }
-}
-
-/* Two-phase raster (visibility by LDS atomics, then one shading per winner). */
-template <int S>
-void launchVis(const DrawParams &P, int ntiles, int need_deriv, hipStream_t st) {
-  if (!P.color) {
-    launchVisMode<S, RHS_SHADING_BASIC>(P, ntiles, need_deriv, st);
-    return;
-  }
-  switch (P.shading) {
-    case RHS_SHADING_BASIC: launchVisMode<S, RHS_SHADING_BASIC>(P, ntiles, need_deriv, st); break;
-    case RHS_SHADING_BLINN_PHONG: launchVisMode<S, RHS_SHADING_BLINN_PHONG>(P, ntiles, need_deriv, st); break;
-    case RHS_SHADING_BLINN_PHONG_DIR: launchVisMode<S, RHS_SHADING_BLINN_PHONG_DIR>(P, ntiles, need_deriv, st); break;
-    case RHS_SHADING_PBR: launchVisMode<S, RHS_SHADING_PBR>(P, ntiles, need_deriv, st); break;
-    default: launchVisMode<S, RHS_SHADING_VERTEX_COLOR>(P, ntiles, need_deriv, st); break;
-  }
-}
-
-template <int S>
-void launchRaster(const DrawParams &P, int ntiles, int need_deriv, hipStream_t st) {
-  switch (P.shading) {
-    case RHS_SHADING_BASIC:
-      {
-#define K_RASTER k_raster<S, RHS_SHADING_BASIC>
-      RHIP_LAUNCH(K_RASTER, ntiles, 256, st, P, need_deriv);
-#undef K_RASTER
-      }
-      break;
-    case RHS_SHADING_BLINN_PHONG:
-      {
-#define K_RASTER k_raster<S, RHS_SHADING_BLINN_PHONG>
-      RHIP_LAUNCH(K_RASTER, ntiles, 256, st, P, need_deriv);
-#undef K_RASTER
-      }
-      break;
-    case RHS_SHADING_BLINN_PHONG_DIR:
-      {
-#define K_RASTER k_raster<S, RHS_SHADING_BLINN_PHONG_DIR>
-      RHIP_LAUNCH(K_RASTER, ntiles, 256, st, P, need_deriv);
-#undef K_RASTER
-      }
-      break;
-    case RHS_SHADING_PBR:
-      {
-#define K_RASTER k_raster<S, RHS_SHADING_PBR>
-      RHIP_LAUNCH(K_RASTER, ntiles, 256, st, P, need_deriv);
-#undef K_RASTER
-      }
-      break;
-    default:
-      {
-#define K_RASTER k_raster<S, RHS_SHADING_VERTEX_COLOR>
-      RHIP_LAUNCH(K_RASTER, ntiles, 256, st, P, need_deriv);
-#undef K_RASTER
-      }
-      break;
   }
 }
 
@@ -396,7 +379,6 @@ int rhip_device_create(int hip_device, rhip_device *out) {
   HIPCHK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
   HIPCHK(hipMalloc(&d->counters, 8 * sizeof(unsigned long long)));
   HIPCHK(hipMemset(d->counters, 0, 8 * sizeof(unsigned long long)));
-  HIPCHK(hipHostMalloc((void **)&d->host_scalars, 64, hipHostMallocDefault));
   *out = d.release();
   return RHIP_OK;
 }
@@ -408,7 +390,6 @@ int rhip_device_destroy(rhip_device d) {
   d->tex.clear();
   d->vao.clear();
   if (d->counters) hipFree(d->counters);
-  if (d->host_scalars) hipHostFree(d->host_scalars);
   hipStreamDestroy(d->stream);
   delete d;
   return RHIP_OK;
@@ -781,164 +762,95 @@ int rhip_draw(rhip_device d, const rhip_draw_desc *dd) {
   hipStream_t s = d->stream;
   const size_t V = (size_t)va.nverts, T = (size_t)P.ntris;
 
-  /* vertex stage; room for the vertices clipping may add */
-  const size_t reserveClip = std::min<size_t>(T, 4096);
-  size_t vcap = V + reserveClip * kMaxClipVerts;
+  /* Arenas sized for the worst case, so that no count has to come back to the
+   * host: every triangle may clip (kMaxClipVerts new vertices, kMaxFans extra
+   * triangles) and every primitive is listed in at most kMaxBinItems bins. */
+  const size_t vcap = V + T * kMaxClipVerts;
+  const size_t pcap = T * (1 + kMaxFans);
+  if (pcap * kMaxBinItems >= (size_t)INT32_MAX || vcap >= (size_t)INT32_MAX)
+    return fail(RHIP_E_UNSUPPORTED, "draw too large (%zu triangles)", T);
+  const int nblk = (int)blocks(T, 256);
+  P.ntiles = ntiles;
+  P.coarse_x = (P.tiles_x + kCoarse - 1) / kCoarse;
+  P.coarse_y = (P.tiles_y + kCoarse - 1) / kCoarse;
+  P.nbins = ntiles + P.coarse_x * P.coarse_y + 1;
   if ((rc = d->clip.ensure(vcap * 16)) || (rc = d->frag.ensure(vcap * 16)) || (rc = d->mask.ensure(vcap * 4)) ||
-      (rc = d->vary.ensure(vcap * kVaryStride * 4)))
+      (rc = d->vary.ensure(vcap * kVaryStride * 4)) || (rc = d->clip_raw.ensure(T * kMaxClipVerts * va.stride)) ||
+      (rc = d->clip_poly.ensure(T * (kMaxPoly + 1) * 4)) || (rc = d->blk.ensure((size_t)nblk * 3 * 4)) ||
+      (rc = d->prims.ensure(pcap * sizeof(PrimSetup))) || (rc = d->bins.ensure(((size_t)P.nbins * 3 + 1) * 4)) ||
+      (rc = d->bin_items.ensure(pcap * kMaxBinItems * 4)) || (rc = d->scalars.ensure(64)))
     return rc;
   P.clip = d->clip.as<float4>();
   P.frag = d->frag.as<float4>();
   P.mask = d->mask.as<int>();
   P.vary = d->vary.as<float>();
-  if (T == 0) return RHIP_OK;
-  { Timer tm(d, "k_vertex"); RHIP_LAUNCH(k_vertex, blocks(V, 256), 256, s, P); }
-  HIPCHK(hipGetLastError());
-
-  /* setup */
-  size_t pcap = T + reserveClip * 6;
-  if ((rc = d->prims.ensure(pcap * sizeof(PrimSetup))) || (rc = d->tile_count.ensure((pcap + 1) * 4)) ||
-      (rc = d->tile_offset.ensure((pcap + 1) * 4)) || (rc = d->clip_flag.ensure(T * 4)) ||
-      (rc = d->scalars.ensure(64)))
-    return rc;
+  P.clip_raw = d->clip_raw.as<float>();
+  P.clip_poly = d->clip_poly.as<int>();
+  P.nblk = nblk;
+  P.blk_clip = d->blk.as<int>();
+  P.blk_fans = P.blk_clip + nblk;
+  P.blk_fan_off = P.blk_fans + nblk;
+  P.dcount = d->scalars.as<int>();
   P.prims = d->prims.as<PrimSetup>();
-  P.tile_count = d->tile_count.as<int>();
-  P.tile_offset = d->tile_offset.as<int>();
-  int *dsc = d->scalars.as<int>();
-  RHIP_LAUNCH(k_fill_u32, 1, 64, s, (uint32_t *)dsc, 16, 0u);
-  { Timer tm(d, "k_setup"); RHIP_LAUNCH(k_setup, blocks(T, 256), 256, s, P, d->clip_flag.as<int>(), dsc); }
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(d->host_scalars, dsc, 4, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  const int nclip = d->host_scalars[0];
-  int nappend = 0;
-  if (nclip > 0) {
-    /* grow the vertex and primitive arenas for the clipped triangles */
-    size_t need_v = V + (size_t)nclip * kMaxClipVerts, need_p = T + (size_t)nclip * 6;
-    if (need_v > vcap) {
-      DevBuf *bufs[4] = {&d->clip, &d->frag, &d->mask, &d->vary};
-      size_t esz[4] = {16, 16, 4, kVaryStride * 4};
-      for (int b = 0; b < 4; b++) {
-        void *np = nullptr;
-        if (hipMalloc(&np, need_v * esz[b]) != hipSuccess) return fail(RHIP_E_OOM, "clip vertex arena");
-        HIPCHK(hipMemcpyAsync(np, bufs[b]->p, V * esz[b], hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipStreamSynchronize(s));
-        hipFree(bufs[b]->p);
-        bufs[b]->p = np;
-        bufs[b]->cap = need_v * esz[b];
-      }
-      P.clip = d->clip.as<float4>();
-      P.frag = d->frag.as<float4>();
-      P.mask = d->mask.as<int>();
-      P.vary = d->vary.as<float>();
-    }
-    if (need_p > pcap) {
-      DevBuf *bufs[3] = {&d->prims, &d->tile_count, &d->tile_offset};
-      size_t esz[3] = {sizeof(PrimSetup), 4, 4};
-      for (int b = 0; b < 3; b++) {
-        void *np = nullptr;
-        if (hipMalloc(&np, (need_p + 1) * esz[b]) != hipSuccess) return fail(RHIP_E_OOM, "clip primitive arena");
-        HIPCHK(hipMemcpyAsync(np, bufs[b]->p, T * esz[b], hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipStreamSynchronize(s));
-        hipFree(bufs[b]->p);
-        bufs[b]->p = np;
-        bufs[b]->cap = (need_p + 1) * esz[b];
-      }
-      P.prims = d->prims.as<PrimSetup>();
-      P.tile_count = d->tile_count.as<int>();
-      P.tile_offset = d->tile_offset.as<int>();
-    }
-    if ((rc = d->clip_raw.ensure((size_t)nclip * kMaxClipVerts * va.stride)) ||
-        (rc = d->clip_list.ensure((size_t)nclip * 4 + 4)) ||
-        (rc = d->clip_poly.ensure((size_t)nclip * (kMaxPoly + 1) * 4)) ||
-        (rc = d->fan_count.ensure((size_t)nclip * 4 + 4)) || (rc = d->fan_offset.ensure((size_t)nclip * 4 + 4)))
-      return rc;
-    P.clip_raw = d->clip_raw.as<float>();
-    P.clip_list = d->clip_list.as<int>();
-    P.clip_poly = d->clip_poly.as<int>();
-    P.fan_offset = d->fan_offset.as<int>();
-    P.nclip = nclip;
-    /* compact the triangle ids that need clipping, in order */
-    hipcub::CountingInputIterator<int> ids(0);
-    size_t tmp = 0;
-    HIPCHK(hipcub::DeviceSelect::Flagged(nullptr, tmp, ids, d->clip_flag.as<int>(), P.clip_list, dsc + 1, (int)T, s));
-    if ((rc = d->cub_tmp.ensure(tmp))) return rc;
-    HIPCHK(hipcub::DeviceSelect::Flagged(d->cub_tmp.p, tmp, ids, d->clip_flag.as<int>(), P.clip_list, dsc + 1, (int)T,
-                                         s));
-    RHIP_LAUNCH(k_clip, blocks(nclip, 64), 64, s, P);
-    HIPCHK(hipGetLastError());
-    RHIP_LAUNCH(k_fan_count, blocks(nclip, 256), 256, s, P, d->fan_count.as<int>());
-    tmp = 0;
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d->fan_count.as<int>(), P.fan_offset, nclip, s));
-    if ((rc = d->cub_tmp.ensure(tmp))) return rc;
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(d->cub_tmp.p, tmp, d->fan_count.as<int>(), P.fan_offset, nclip, s));
-    HIPCHK(hipMemcpyAsync(d->host_scalars + 2, P.fan_offset + (nclip - 1), 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(d->host_scalars + 3, d->fan_count.as<int>() + (nclip - 1), 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    nappend = d->host_scalars[2] + d->host_scalars[3];
-    RHIP_LAUNCH(k_clip_fans, blocks(nclip, 64), 64, s, P);
-    HIPCHK(hipGetLastError());
-    d->host_stats.clipped_triangles += nclip;
-  }
-  P.nprims = (int)T + nappend;
+  P.bin_count = d->bins.as<int>();
+  P.bin_start = P.bin_count + P.nbins;
+  P.bin_cursor = P.bin_start + P.nbins + 1;
+  P.bin_items = d->bin_items.as<int>();
+  if (T == 0) return RHIP_OK;
 
-  /* binning: per-primitive tile counts -> offsets -> (tile, prim) pairs -> stable sort by tile */
-  size_t tmp = 0;
-  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, P.tile_count, P.tile_offset, P.nprims + 1, s));
-  if ((rc = d->cub_tmp.ensure(tmp))) return rc;
-  RHIP_LAUNCH(k_fill_u32, 1, 64, s, (uint32_t *)(P.tile_count + P.nprims), 1, 0u);
-  HIPCHK(hipcub::DeviceScan::ExclusiveSum(d->cub_tmp.p, tmp, P.tile_count, P.tile_offset, P.nprims + 1, s));
-  HIPCHK(hipMemcpyAsync(d->host_scalars + 4, P.tile_offset + P.nprims, 4, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  const int npairs = d->host_scalars[4];
-  if ((rc = d->pair_key.ensure((size_t)npairs * 4 + 4)) || (rc = d->pair_val.ensure((size_t)npairs * 4 + 4)) ||
-      (rc = d->pair_key_s.ensure((size_t)npairs * 4 + 4)) || (rc = d->pair_val_s.ensure((size_t)npairs * 4 + 4)) ||
-      (rc = d->tile_start.ensure((size_t)(ntiles + 1) * 4)))
-    return rc;
-  P.pair_key = d->pair_key.as<uint32_t>();
-  P.pair_val = d->pair_val.as<int>();
-  P.pair_key_sorted = d->pair_key_s.as<uint32_t>();
-  P.pair_val_sorted = d->pair_val_s.as<int>();
-  P.npairs = npairs;
-  P.tile_start = d->tile_start.as<int>();
-  if (npairs > 0) {
-    { Timer tm(d, "k_emit"); RHIP_LAUNCH(k_emit, blocks(P.nprims, 256), 256, s, P); }
-    HIPCHK(hipGetLastError());
-    int bits = 1;
-    while ((1 << bits) < ntiles) bits++;
-    tmp = 0;
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, P.pair_key, P.pair_key_sorted, P.pair_val,
-                                              P.pair_val_sorted, npairs, 0, bits, s));
-    if ((rc = d->cub_tmp.ensure(tmp))) return rc;
-    {
-      Timer tm(d, "radix_sort_pairs");
-      HIPCHK(hipcub::DeviceRadixSort::SortPairs(d->cub_tmp.p, tmp, P.pair_key, P.pair_key_sorted, P.pair_val,
-                                                P.pair_val_sorted, npairs, 0, bits, s));
-    }
-  }
-  RHIP_LAUNCH(k_tile_ranges, blocks(ntiles + 1, 256), 256, s, P);
+  RHIP_LAUNCH(k_fill_u32, blocks(P.nbins, 256), 256, s, (uint32_t *)P.bin_count, (size_t)P.nbins, 0u);
+  { Timer tm(d, "k_vertex"); RHIP_LAUNCH(k_vertex, blocks(V, 256), 256, s, P); }
+  { Timer tm(d, "k_setup"); RHIP_LAUNCH(k_setup, nblk, 256, s, P); }
+  { Timer tm(d, "k_clip"); RHIP_LAUNCH(k_clip, nblk, 256, s, P); }
+  RHIP_LAUNCH(k_scan_fans, 1, 1024, s, P);
+  RHIP_LAUNCH(k_clip_fans, nblk, 256, s, P);
+  { Timer tm(d, "k_bin_scan"); RHIP_LAUNCH(k_bin_scan, 1, 1024, s, P); }
+  { Timer tm(d, "k_bin_emit"); RHIP_LAUNCH(k_bin_emit, std::min(nblk, 8192), 256, s, P); }
   HIPCHK(hipGetLastError());
 
   /* draws whose result follows from depths and order alone take the two-phase raster */
   const bool twoPhase = !st.blend && P.depth && st.depthTest && st.depthMask &&
                         (st.depthFunc == 1 || st.depthFunc == 3 || st.depthFunc == 4 || st.depthFunc == 6);
   if (twoPhase) {
-    Timer tm(d, samples == 4 ? "k_raster_vis_msaa4" : "k_raster_vis");
-    if (samples == 4)
-      launchVis<4>(P, ntiles, need_deriv ? 1 : 0, s);
-    else
-      launchVis<1>(P, ntiles, need_deriv ? 1 : 0, s);
+    const int ncoarse = P.coarse_x * P.coarse_y;
+    const size_t njobs = (size_t)fbw * fbh * samples;
+    if ((rc = d->jobs.ensure(njobs * 8)) || (rc = d->hiz.ensure((size_t)ncoarse * (2 + kHizCap) * 4))) return rc;
+    P.jobs = d->jobs.as<uint2>();
+    P.hiz_bound = d->hiz.as<uint32_t>();
+    P.hiz_count = (int *)P.hiz_bound + ncoarse;
+    P.hiz_surv = P.hiz_count + ncoarse;
+    {
+      Timer tm(d, samples == 4 ? "k_raster_vis_msaa4" : "k_raster_vis");
+      if (samples == 4)
+        launchVis<4>(P, ntiles, ncoarse, s);
+      else
+        launchVis<1>(P, ntiles, ncoarse, s);
+    }
+    if (P.color) {
+      const int grid = (int)std::min<size_t>(blocks(njobs, 256), 4096);
+      {
+        Timer tm(d, samples == 4 ? "k_shade_msaa4" : "k_shade");
+        if (samples == 4)
+          launchShade<4>(P, grid, need_deriv ? 1 : 0, s);
+        else
+          launchShade<1>(P, grid, need_deriv ? 1 : 0, s);
+      }
+      if (samples == 4) {
+        Timer tm(d, "k_resolve");
+        RHIP_LAUNCH(k_resolve, (unsigned)std::min<size_t>(blocks((size_t)fbw * fbh, 256), 4096), 256, s, P);
+      }
+    }
   } else {
-    Timer tm(d, samples == 4 ? "k_raster_msaa4" : "k_raster");
+    /* NOTEQUAL with depth writes: early-Z must see every earlier write */
+    const int flush_each = (P.depth && st.depthTest && st.depthMask && st.depthFunc == 5) ? 1 : 0;
+    Timer tm(d, samples == 4 ? "k_raster_ord_msaa4" : "k_raster_ord");
     if (samples == 4)
-      launchRaster<4>(P, ntiles, need_deriv ? 1 : 0, s);
+      launchOrdered<4>(P, ntiles, need_deriv ? 1 : 0, flush_each, s);
     else
-      launchRaster<1>(P, ntiles, need_deriv ? 1 : 0, s);
+      launchOrdered<1>(P, ntiles, need_deriv ? 1 : 0, flush_each, s);
   }
   HIPCHK(hipGetLastError());
   d->host_stats.draws++;
-  d->host_stats.tile_pairs += npairs;
-  d->host_stats.triangles_binned += P.nprims;
   return RHIP_OK;
 }
 
@@ -984,6 +896,10 @@ int rhip_stats_get(rhip_device d, rhip_stats *out) {
   *out = d->host_stats;
   out->fs_invocations = (long long)c[0];
   out->samples_written = (long long)c[1];
+  out->tile_pairs = (long long)c[3];
+  out->triangles_binned = (long long)c[4];
+  out->clipped_triangles = (long long)c[5];
+  out->hiz_culled = (long long)c[6];
   return RHIP_OK;
 }
 

@@ -17,6 +17,15 @@ constexpr int kSub = 8;           /* 8x8 sub-tile per wavefront */
 constexpr int kVaryStride = 32;   /* floats per vertex in the varyings buffer */
 constexpr int kMaxClipVerts = 12; /* new vertices one clipped triangle can create */
 constexpr int kMaxPoly = 9;       /* polygon vertices after six planes */
+constexpr int kMaxFans = kMaxPoly - 3; /* extra triangles one clipped triangle can append */
+/* Binning: a primitive whose quads touch at most kBinFine tiles is listed in
+ * each of those tiles; a larger one in each kCoarse x kCoarse-tile coarse bin
+ * it touches if those are at most kBinCoarse, else in the one global bin. So
+ * no primitive is listed more than kMaxBinItems times. */
+constexpr int kBinFine = 16;
+constexpr int kCoarse = 8;
+constexpr int kBinCoarse = 16;
+constexpr int kMaxBinItems = 16;
 
 enum Layout { LAYOUT_LINEAR = 0, LAYOUT_TILED = 1, LAYOUT_MORTON = 2 };
 
@@ -58,7 +67,11 @@ struct PrimSetup {
   float max_x, max_y;  /* triangleBoundingBox max */
   int bcx, bcy;        /* 32-px block counts */
   int qx0, qy0, qx1, qy1; /* pixel range touched by the triangle's quads (inclusive), empty if qx1 < qx0 */
+  float zlo, zhi;         /* bounds of every depth the triangle's samples can take */
 };
+
+/* PrimSetup.flags */
+constexpr int kPrimLive = 1, kPrimFront = 2, kPrimCoarse = 4, kPrimGlobal = 8;
 
 struct DrawParams {
   Uniforms u;
@@ -88,23 +101,29 @@ struct DrawParams {
   float4 *frag;
   int *mask;
   float *vary;
-  /* clipping */
-  int *clip_list; /* compacted ids of triangles that need clipping */
-  int nclip;
-  int *clip_poly; /* kMaxPoly vertex ids + count per clipped triangle */
-  int *fan_offset;
+  /* clipping: triangle t owns new-vertex slots nverts + t * kMaxClipVerts ... */
+  int *clip_poly; /* kMaxPoly vertex ids + count per triangle of a block that clips */
+  int *blk_clip;  /* per 256-triangle block: triangles that need clipping */
+  int *blk_fans;  /* per block: fan triangles its clipped triangles append */
+  int *blk_fan_off;
+  int nblk;
+  int *dcount; /* device scalars: [0] appended fan triangles, [1] primitive count, [2] queued fragments */
   /* primitives */
   PrimSetup *prims;
-  int nprims;
-  int *tile_count;
-  int *tile_offset;
-  uint32_t *pair_key;
-  int *pair_val;
-  uint32_t *pair_key_sorted;
-  int *pair_val_sorted;
-  int npairs;
-  int *tile_start; /* [tiles+1] */
-  unsigned long long *counters; /* [0] fs invocations, [1] samples written, [2] overflow flag */
+  /* bins: ntiles fine bins, then coarse_x * coarse_y coarse bins, then the global bin */
+  int ntiles, coarse_x, coarse_y, nbins;
+  int *bin_count;  /* [nbins] */
+  int *bin_start;  /* [nbins + 1] */
+  int *bin_cursor; /* [nbins] */
+  int *bin_items;  /* primitive ids, unordered within a bin */
+  /* two-phase raster */
+  uint2 *jobs;          /* (pixel | sample mask << 28, winner) per visible fragment; count in dcount[2] */
+  uint32_t *hiz_bound; /* per coarse bin: depth bound from big triangles covering it */
+  int *hiz_count;      /* per coarse bin: big triangles that bound leaves, -1 if over kHizCap */
+  int *hiz_surv;       /* per coarse bin: those triangles (kHizCap slots) */
+  unsigned long long *counters; /* [0] fs invocations, [1] samples written, [2] clip overflows,
+                                   [3] bin items, [4] primitives binned, [5] clipped triangles,
+                                   [6] (tile, triangle) pairs skipped by the hierarchical-Z test */
 };
 
 }  // namespace rhip

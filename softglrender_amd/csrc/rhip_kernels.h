@@ -1,16 +1,20 @@
 /*
  * rhip_kernels.h -- the gfx950 kernels of one RendererHIP draw.
  *
- * Pipeline (one draw; RendererSoft::draw, RendererSoft.cpp:136-164):
+ * Pipeline (one draw; RendererSoft::draw, RendererSoft.cpp:136-164), queued on
+ * one stream with no host round trip:
  *   k_vertex      vertex shader, clip mask, perspective divide, viewport   (:170-190, :971-992)
- *   k_setup       assembly, clip test, face culling, bounding box, tile count (:192-204, :277-299, :1005-1019)
+ *   k_setup       assembly, clip test, face culling, bounding box, quad range, bin counts
+ *                 (:192-204, :277-299, :1005-1019)
  *   k_clip        Sutherland-Hodgman against the six planes, re-shading new vertices (:489-559, :956-970)
- *   k_clip_fans   the fan triangles a clipped polygon appends after all originals (:550-558, :227)
- *   k_emit        (tile, primitive) pairs; sorted stably by tile so every tile keeps API order
- *   k_tile_ranges start of each tile's run in the sorted pairs
- *   k_raster      one workgroup per 16x16 tile, one lane per pixel: quads anchored per
- *                 triangle bounding box, coverage, z/w, early-Z, interpolation, fragment
- *                 shading, depth test, blending, writes; then the MSAA resolve (:720-912)
+ *   k_scan_fans   where the fan triangles of clipped polygons go
+ *   k_clip_fans   those triangles, after all originals in clipped-triangle order (:550-558, :227)
+ *   k_bin_scan    start of every bin
+ *   k_bin_emit    primitive ids into their bins
+ *   k_raster_vis  draws whose result follows from depths and order: visibility, then one
+ *                 shading per visible fragment
+ *   k_raster_ord  every other draw: fragments in API order per pixel, shading deferred and
+ *                 batched per wavefront, per-sample operations in order (:720-912)
  */
 #pragma once
 #include "rhip_shading.h"
@@ -45,6 +49,142 @@ __global__ __launch_bounds__(256) void k_vertex(DrawParams P) {
   for (int k = 0; k < kVaryStride / 4; k++) o[k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
 }
 
+/* --------------------------------------------------------------------------- block helpers */
+
+/* Inclusive prefix sum over the N threads of a block (N = blockDim.x). */
+template <int N>
+__device__ __forceinline__ int blockInclusiveScan(int v, int *part) {
+  part[threadIdx.x] = v;
+  __syncthreads();
+#pragma unroll 1
+  for (int off = 1; off < N; off <<= 1) {
+    const int t = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += t;
+    __syncthreads();
+  }
+  const int r = part[threadIdx.x];
+  __syncthreads();
+  return r;
+}
+
+/* Inclusive prefix sum over the 64 lanes of a wavefront. */
+__device__ __forceinline__ int waveInclusiveScan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(v, off, 64);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint32_t waveMaxU(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  return v;
+}
+__device__ __forceinline__ uint32_t waveMinU(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  return v;
+}
+
+/* Orders LDS accesses of the lanes of one wavefront around this point. */
+__device__ __forceinline__ void waveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* atomicAdd(&a[b], 1) for every lane with `active`, merged for lanes of the
+ * wavefront that share b (up to four distinct b per call; the rest add alone).
+ * Returns the lane's old value. Every lane of the wavefront must call it. */
+__device__ __forceinline__ int waveAtomicInc(int *a, int b, bool active) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long pending = __ballot(active);
+  int res = 0;
+  for (int r = 0; r < 4 && pending; r++) {
+    const int leader = __ffsll(pending) - 1;
+    const int bl = __shfl(b, leader, 64);
+    const bool mine = ((pending >> lane) & 1ull) && b == bl;
+    const unsigned long long same = __ballot(mine);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&a[bl], __popcll(same));
+    base = __shfl(base, leader, 64);
+    if (mine) res = base + __popcll(same & ((1ull << lane) - 1ull));
+    pending &= ~same;
+  }
+  if ((pending >> lane) & 1ull) res = atomicAdd(&a[b], 1);
+  return res;
+}
+
+/* Exclusive prefix sum of in[0..n) into out[0..n) by one block of 1024 threads,
+ * 8192 entries at a time with coalesced loads and stores; also clears clear[0..n)
+ * when given. Returns the total (to every thread). */
+__device__ int blockScanArray(const int *in, int *out, int n, int *clear) {
+  __shared__ int buf[8192];
+  __shared__ int wsum[16];
+  __shared__ int s_carry;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  if (t == 0) s_carry = 0;
+  __syncthreads();
+  for (int c0 = 0; c0 < n; c0 += 8192) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int i = c0 + k * 1024 + t;
+      buf[k * 1024 + t] = i < n ? in[i] : 0;
+    }
+    __syncthreads();
+    int v[8], sum = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      v[k] = buf[t * 8 + k];
+      sum += v[k];
+    }
+    const int inc = waveInclusiveScan(sum);
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    int run = s_carry + inc - sum;
+    for (int w = 0; w < wave; w++) run += wsum[w];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      buf[t * 8 + k] = run;
+      run += v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int i = c0 + k * 1024 + t;
+      if (i < n) {
+        out[i] = buf[k * 1024 + t];
+        if (clear) clear[i] = 0;
+      }
+    }
+    if (t == 1023) s_carry = run;
+    __syncthreads();
+  }
+  return s_carry;
+}
+
+/* Ascending bitonic sort of n (a power of two) words in LDS by the whole block. */
+__device__ void blockSortU32(uint32_t *a, int n) {
+  for (int k = 2; k <= n; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int l = i ^ j;
+        if (l > i) {
+          const uint32_t x = a[i], y = a[l];
+          if ((x > y) == ((i & k) == 0)) {
+            a[i] = y;
+            a[l] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+}
+
 /* --------------------------------------------------------------------------- triangle setup */
 
 /* One axis of the reference's block loop (RendererSoft.cpp:727-763): blocks of
@@ -73,9 +213,53 @@ __device__ __forceinline__ void axisRange(float mn, float mx, int bc, int *first
   }
 }
 
-/* Fills the setup record of primitive `id` from three screen-space vertices and
- * returns the number of screen tiles its quads touch (0 if culled or empty). */
-__device__ int setupPrim(const DrawParams &P, int id, int i0, int i1, int i2) {
+/* Tiles touched by a live primitive's quads, clipped to the framebuffer. */
+__device__ __forceinline__ void tileRect(const DrawParams &P, const PrimSetup &s, int &tx0, int &ty0, int &tx1,
+                                         int &ty1) {
+  tx0 = max(s.qx0, 0) / kTile;
+  tx1 = min(s.qx1, P.fb_w - 1) / kTile;
+  ty0 = max(s.qy0, 0) / kTile;
+  ty1 = min(s.qy1, P.fb_h - 1) / kTile;
+}
+
+/* Which bins list a primitive: its tiles (0), its coarse bins, or the global bin. */
+__device__ __forceinline__ int binClass(int tx0, int ty0, int tx1, int ty1) {
+  if ((tx1 - tx0 + 1) * (ty1 - ty0 + 1) <= kBinFine) return 0;
+  if ((tx1 / kCoarse - tx0 / kCoarse + 1) * (ty1 / kCoarse - ty0 / kCoarse + 1) <= kBinCoarse) return kPrimCoarse;
+  return kPrimGlobal;
+}
+
+/* Counts (EMIT false) or lists (EMIT true) primitive `id` in each of its bins. */
+template <bool EMIT>
+__device__ __forceinline__ void binVisit(const DrawParams &P, int id, int cls, int tx0, int ty0, int tx1, int ty1) {
+  auto put = [&](int b) {
+    if (EMIT) {
+      const int slot = atomicAdd(&P.bin_cursor[b], 1);
+      P.bin_items[P.bin_start[b] + slot] = id;
+    } else {
+      atomicAdd(&P.bin_count[b], 1);
+    }
+  };
+  if (cls == kPrimGlobal) {
+    put(P.nbins - 1);
+  } else if (cls == kPrimCoarse) {
+    for (int cy = ty0 / kCoarse; cy <= ty1 / kCoarse; cy++)
+      for (int cx = tx0 / kCoarse; cx <= tx1 / kCoarse; cx++) put(P.ntiles + cy * P.coarse_x + cx);
+  } else {
+    for (int ty = ty0; ty <= ty1; ty++)
+      for (int tx = tx0; tx <= tx1; tx++) put(ty * P.tiles_x + tx);
+  }
+}
+
+/* The bins of a live primitive: class (-1 none, 0 tiles, kPrimCoarse,
+ * kPrimGlobal) and its tile rectangle. */
+struct BinRect {
+  int cls, tx0, ty0, tx1, ty1;
+};
+
+/* Fills the setup record of primitive `id` from three screen-space vertices;
+ * returns the bins it goes to (the caller counts it there). */
+__device__ BinRect setupPrim(const DrawParams &P, int id, int i0, int i1, int i2) {
   PrimSetup s;
   s.v[0] = i0;
   s.v[1] = i1;
@@ -111,32 +295,81 @@ __device__ int setupPrim(const DrawParams &P, int id, int i0, int i1, int i2) {
   s.qx1 = lx + 1;
   s.qy0 = fy;
   s.qy1 = ly + 1;
-  int count = 0;
-  if (live && fx <= lx && fy <= ly) {
-    int x0 = max(fx, 0), x1 = min(lx + 1, P.fb_w - 1);
-    int y0 = max(fy, 0), y1 = min(ly + 1, P.fb_h - 1);
-    if (x0 <= x1 && y0 <= y1) count = (x1 / kTile - x0 / kTile + 1) * (y1 / kTile - y0 / kTile + 1);
+  /* every depth a sample can take: a convex combination of the vertex depths,
+   * widened by the rounding of the barycentrics and of the dot product */
+  if (a.z != a.z || b.z != b.z || c.z != c.z) {
+    s.zlo = -INFINITY;
+    s.zhi = INFINITY;
+  } else {
+    const float sl = 16.f * 1.19209290e-07f * (fabsf(a.z) + fabsf(b.z) + fabsf(c.z)) + 1e-7f;
+    s.zlo = fminf(fminf(a.z, b.z), c.z) - sl;
+    s.zhi = fmaxf(fmaxf(a.z, b.z), c.z) + sl;
   }
-  s.flags = (count > 0 ? 1 : 0) | (front ? 2 : 0);
+  int flags = front ? kPrimFront : 0;
+  BinRect r = {-1, 0, 0, 0, 0};
+  if (live && fx <= lx && fy <= ly) {
+    const int x0 = max(fx, 0), x1 = min(lx + 1, P.fb_w - 1);
+    const int y0 = max(fy, 0), y1 = min(ly + 1, P.fb_h - 1);
+    if (x0 <= x1 && y0 <= y1) {
+      r = {0, x0 / kTile, y0 / kTile, x1 / kTile, y1 / kTile};
+      r.cls = binClass(r.tx0, r.ty0, r.tx1, r.ty1);
+      flags |= kPrimLive | r.cls;
+    }
+  }
+  s.flags = flags;
   P.prims[id] = s;
-  P.tile_count[id] = count;
-  return count;
+  return r;
 }
 
-__global__ __launch_bounds__(256) void k_setup(DrawParams P, int *clip_flag, int *clip_total) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= P.ntris) return;
-  int i0 = P.idx[3 * t], i1 = P.idx[3 * t + 1], i2 = P.idx[3 * t + 2];
-  int m = P.mask[i0] | P.mask[i1] | P.mask[i2];
-  if (m) {
-    clip_flag[t] = 1;
-    atomicAdd(clip_total, 1);
-    P.prims[t].flags = 0;
-    P.tile_count[t] = 0;
-    return;
+/* Counts a primitive in its bins with plain atomics (for divergent callers). */
+__device__ __forceinline__ void binCount(const DrawParams &P, const BinRect &r) {
+  if (r.cls >= 0) binVisit<false>(P, 0, r.cls, r.tx0, r.ty0, r.tx1, r.ty1);
+}
+
+/* Visits the bins of every lane's primitive; fine bins with wavefront-merged
+ * atomics. Every lane of the wavefront must call it (r.cls < 0: nothing). */
+template <bool EMIT>
+__device__ __forceinline__ void binWave(const DrawParams &P, int id, const BinRect &r) {
+  if (r.cls > 0) binVisit<EMIT>(P, id, r.cls, r.tx0, r.ty0, r.tx1, r.ty1);
+  const int w = r.tx1 - r.tx0 + 1;
+  const int nt = r.cls == 0 ? w * (r.ty1 - r.ty0 + 1) : 0;
+  const int most = (int)waveMaxU((uint32_t)nt);
+  for (int k = 0; k < most; k++) {
+    const bool act = k < nt;
+    const int b = act ? (r.ty0 + k / w) * P.tiles_x + r.tx0 + k % w : 0;
+    if (EMIT) {
+      const int slot = waveAtomicInc(P.bin_cursor, b, act);
+      if (act) P.bin_items[P.bin_start[b] + slot] = id;
+    } else {
+      waveAtomicInc(P.bin_count, b, act);
+    }
   }
-  clip_flag[t] = 0;
-  setupPrim(P, t, i0, i1, i2);
+}
+
+__device__ __forceinline__ bool validTri(const DrawParams &P, int i0, int i1, int i2) {
+  return (unsigned)i0 < (unsigned)P.nverts && (unsigned)i1 < (unsigned)P.nverts && (unsigned)i2 < (unsigned)P.nverts;
+}
+
+/* processPolygonAssembly (RendererSoft.cpp:428-437) + the clip test: triangles
+ * with a vertex outside the frustum are set up later by k_clip. */
+__global__ __launch_bounds__(256) void k_setup(DrawParams P) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int needs = 0;
+  BinRect r = {-1, 0, 0, 0, 0};
+  if (t < P.ntris) {
+    const int i0 = P.idx[3 * t], i1 = P.idx[3 * t + 1], i2 = P.idx[3 * t + 2];
+    if (!validTri(P, i0, i1, i2)) {
+      P.prims[t].flags = 0;
+    } else if (P.mask[i0] | P.mask[i1] | P.mask[i2]) {
+      needs = 1;
+      P.prims[t].flags = 0;
+    } else {
+      r = setupPrim(P, t, i0, i1, i2);
+    }
+  }
+  binWave<false>(P, t, r);
+  const int n = __syncthreads_count(needs);
+  if (threadIdx.x == 0) P.blk_clip[blockIdx.x] = n;
 }
 
 /* --------------------------------------------------------------------------- clipping */
@@ -153,13 +386,18 @@ __device__ __forceinline__ const float *rawOf(const DrawParams &P, int v) {
   return v < P.nverts ? P.verts + (size_t)v * P.stride_f : P.clip_raw + (size_t)(v - P.nverts) * P.stride_f;
 }
 
-/* clippingTriangle (RendererSoft.cpp:489-559) for the k-th triangle that needs it. */
-__global__ void k_clip(DrawParams P) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= P.nclip) return;
-  int t = P.clip_list[k];
-  int i0 = P.idx[3 * t], i1 = P.idx[3 * t + 1], i2 = P.idx[3 * t + 2];
-  int mask = P.mask[i0] | P.mask[i1] | P.mask[i2];
+/* clippingTriangle (RendererSoft.cpp:489-559) for triangle t; its new vertices
+ * take slots nverts + t * kMaxClipVerts onward. Returns the fan triangles it appends. */
+__device__ int clipTriangle(const DrawParams &P, int t) {
+  int *poly = P.clip_poly + (size_t)t * (kMaxPoly + 1);
+  const int i0 = P.idx[3 * t], i1 = P.idx[3 * t + 1], i2 = P.idx[3 * t + 2];
+  const int mask = validTri(P, i0, i1, i2) ? (P.mask[i0] | P.mask[i1] | P.mask[i2]) : 0;
+  if (!mask) {
+    poly[kMaxPoly] = 0;
+    return 0;
+  }
+  atomicAdd(&P.counters[5], 1ull);
+  const int k = t;
   int in[kMaxPoly + 1], out[kMaxPoly + 1];
   int nin = 3, nnew = 0;
   in[0] = i0;
@@ -227,68 +465,114 @@ __global__ void k_clip(DrawParams P) {
     for (int q = 0; q < nout; q++) in[q] = out[q];
     nin = nout;
   }
-  int *poly = P.clip_poly + k * (kMaxPoly + 1);
   if (full || nin < 3) { /* fewer than 3 left: the reference reads past its index list here */
     poly[kMaxPoly] = 0;
     P.prims[t].flags = 0;
-    P.tile_count[t] = 0;
-    return;
+    return 0;
   }
   for (int q = 0; q < nin; q++) poly[q] = in[q];
   poly[kMaxPoly] = nin;
-  setupPrim(P, t, in[0], in[1], in[2]);
+  binCount(P, setupPrim(P, t, in[0], in[1], in[2]));
+  return nin - 3;
+}
+
+/* One block per 256-triangle block of k_setup; blocks with nothing to clip return at once. */
+__global__ __launch_bounds__(256) void k_clip(DrawParams P) {
+  __shared__ int fans;
+  const int b = blockIdx.x;
+  if (P.blk_clip[b] == 0) {
+    if (threadIdx.x == 0) P.blk_fans[b] = 0;
+    return;
+  }
+  if (threadIdx.x == 0) fans = 0;
+  __syncthreads();
+  const int t = b * blockDim.x + threadIdx.x;
+  const int n = t < P.ntris ? clipTriangle(P, t) : 0;
+  if (n) atomicAdd(&fans, n);
+  __syncthreads();
+  if (threadIdx.x == 0) P.blk_fans[b] = fans;
+}
+
+/* One block of 1024: exclusive offsets of every block's fan triangles, and the primitive count. */
+__global__ __launch_bounds__(1024) void k_scan_fans(DrawParams P) {
+  const int total = blockScanArray(P.blk_fans, P.blk_fan_off, P.nblk, nullptr);
+  if (threadIdx.x == 0) {
+    P.dcount[0] = total;
+    P.dcount[1] = P.ntris + total;
+    P.dcount[2] = 0;
+  }
 }
 
 /* fans appended after all original primitives, in clipped-triangle order (:550-558) */
-__global__ void k_fan_count(DrawParams P, int *fan_count) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= P.nclip) return;
-  int n = P.clip_poly[k * (kMaxPoly + 1) + kMaxPoly];
-  fan_count[k] = n > 3 ? n - 3 : 0;
-}
-
-__global__ void k_clip_fans(DrawParams P) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= P.nclip) return;
-  const int *poly = P.clip_poly + k * (kMaxPoly + 1);
-  int n = poly[kMaxPoly];
-  for (int i = 3; i < n; i++) setupPrim(P, P.ntris + P.fan_offset[k] + (i - 3), poly[0], poly[i - 1], poly[i]);
+__global__ __launch_bounds__(256) void k_clip_fans(DrawParams P) {
+  __shared__ int part[256];
+  const int b = blockIdx.x;
+  if (P.blk_fans[b] == 0) return;
+  const int t = b * blockDim.x + threadIdx.x;
+  const int *poly = P.clip_poly + (size_t)t * (kMaxPoly + 1);
+  const int n = t < P.ntris ? poly[kMaxPoly] : 0;
+  const int nf = n > 3 ? n - 3 : 0;
+  const int inc = blockInclusiveScan<256>(nf, part);
+  const int base = P.ntris + P.blk_fan_off[b] + inc - nf;
+  for (int i = 3; i < n; i++) binCount(P, setupPrim(P, base + i - 3, poly[0], poly[i - 1], poly[i]));
 }
 
 /* --------------------------------------------------------------------------- binning */
 
-__global__ __launch_bounds__(256) void k_emit(DrawParams P) {
-  int id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= P.nprims) return;
-  int n = P.tile_count[id];
-  if (n == 0) return;
-  const PrimSetup &s = P.prims[id];
-  int x0 = max(s.qx0, 0) / kTile, x1 = min(s.qx1, P.fb_w - 1) / kTile;
-  int y0 = max(s.qy0, 0) / kTile, y1 = min(s.qy1, P.fb_h - 1) / kTile;
-  int o = P.tile_offset[id];
-  for (int ty = y0; ty <= y1; ty++)
-    for (int tx = x0; tx <= x1; tx++) {
-      P.pair_key[o] = (uint32_t)(ty * P.tiles_x + tx);
-      P.pair_val[o] = id;
-      o++;
-    }
-}
-
-__global__ void k_tile_ranges(DrawParams P) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  int ntiles = P.tiles_x * P.tiles_y;
-  if (t > ntiles) return;
-  /* lower bound of tile t in the sorted keys */
-  int lo = 0, hi = P.npairs;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (P.pair_key_sorted[mid] < (uint32_t)t) lo = mid + 1;
-    else hi = mid;
+/* One block of 1024: bin starts from the counts setup made; cursors cleared. */
+__global__ __launch_bounds__(1024) void k_bin_scan(DrawParams P) {
+  const int total = blockScanArray(P.bin_count, P.bin_start, P.nbins, P.bin_cursor);
+  if (threadIdx.x == 0) {
+    P.bin_start[P.nbins] = total;
+    atomicAdd(&P.counters[3], (unsigned long long)total);
+    atomicAdd(&P.counters[4], (unsigned long long)P.dcount[1]);
   }
-  P.tile_start[t] = lo;
 }
 
-/* --------------------------------------------------------------------------- fine raster */
+/* Lists every live primitive in its bins (wavefronts walk the ids together). */
+__global__ __launch_bounds__(256) void k_bin_emit(DrawParams P) {
+  const int n = P.dcount[1];
+  for (int b0 = blockIdx.x * blockDim.x; b0 < n; b0 += gridDim.x * blockDim.x) {
+    const int id = b0 + threadIdx.x;
+    BinRect r = {-1, 0, 0, 0, 0};
+    if (id < n) {
+      const PrimSetup &ps = P.prims[id];
+      if (ps.flags & kPrimLive) {
+        tileRect(P, ps, r.tx0, r.ty0, r.tx1, r.ty1);
+        r.cls = ps.flags & (kPrimCoarse | kPrimGlobal);
+      }
+    }
+    binWave<true>(P, id, r);
+  }
+}
+
+/* The entries of one tile: its own bin, its coarse bin, the global bin. */
+struct TileBins {
+  int s[3], n[3];
+  __device__ __forceinline__ int total() const { return n[0] + n[1] + n[2]; }
+  __device__ __forceinline__ int at(const DrawParams &P, int i) const {
+    if (i < n[0]) return P.bin_items[s[0] + i];
+    i -= n[0];
+    if (i < n[1]) return P.bin_items[s[1] + i];
+    return P.bin_items[s[2] + i - n[1]];
+  }
+};
+
+__device__ __forceinline__ TileBins tileBins(const DrawParams &P, int tile) {
+  const int tx = tile % P.tiles_x, ty = tile / P.tiles_x;
+  const int c = P.ntiles + (ty / kCoarse) * P.coarse_x + tx / kCoarse;
+  const int g = P.nbins - 1;
+  TileBins b;
+  b.s[0] = P.bin_start[tile];
+  b.n[0] = P.bin_start[tile + 1] - b.s[0];
+  b.s[1] = P.bin_start[c];
+  b.n[1] = P.bin_start[c + 1] - b.s[1];
+  b.s[2] = P.bin_start[g];
+  b.n[2] = P.bin_start[g + 1] - b.s[2];
+  return b;
+}
+
+/* --------------------------------------------------------------------------- fine raster helpers */
 
 /* barycentric (RendererSoft.cpp:1021-1056) as its SSE path computes it:
  * u = vmulps/vfmsub cross product, divided by u.z, bc = (1-(ux+uy), uy, ux).
@@ -309,15 +593,20 @@ __device__ __forceinline__ bool bary(const Tri &T, float px, float py, float4 &b
   return !(bc.x < 0 || bc.y < 0 || bc.z < 0);
 }
 
+/* Position of MSAA-4x sample s in its pixel (RendererInternal.h:59-186) */
+__device__ __forceinline__ float sampleX(int s) { return s == 0 ? 0.375f : s == 1 ? 0.875f : s == 2 ? 0.125f : 0.625f; }
+__device__ __forceinline__ float sampleY(int s) { return s == 0 ? 0.875f : s == 1 ? 0.625f : s == 2 ? 0.375f : 0.125f; }
+
 /* dpps with imm 0xff over (b0, b1, b2, 0) . (v0, v1, v2, 0) */
 __device__ __forceinline__ float dpps3(float4 b, const float *v) {
   return (b.x * v[0] + b.y * v[1]) + (b.z * v[2] + b.w * 0.f);
 }
 
 /* Quad origins that contain pixel coordinate p on one axis, in the order the
- * reference visits them (increasing block). Returns 0..2. */
-__device__ __forceinline__ int axisQuads(float mn, float mx, int bc, int p, int *q) {
+ * reference visits them (increasing block): q0, then q1. Returns 0..2. */
+__device__ __forceinline__ int axisQuads(float mn, float mx, int bc, int p, int &q0, int &q1) {
   int n = 0;
+  q0 = q1 = 0;
   int fl = (int)floorf(mx);
   int s0 = (int)mn;
 #pragma unroll
@@ -340,14 +629,13 @@ __device__ __forceinline__ int axisQuads(float mn, float mx, int bc, int p, int 
         }
       }
     }
-    if (found && ((x - sb - 1) & 1) == 0) q[n++] = x;
+    const bool hit = found && ((x - sb - 1) & 1) == 0;
+    q1 = (hit && n == 1) ? x : q1;
+    q0 = (hit && n == 0) ? x : q0;
+    n += hit ? 1 : 0;
   }
   return n;
 }
-
-struct FragOut {
-  bool any;
-};
 
 /* Shading-sample barycentrics of one pixel of a quad, as the quad context
  * leaves them for interpolation (RendererSoft.cpp:776-820): the centre sample
@@ -415,184 +703,48 @@ __device__ __forceinline__ bool varyUsed(const DrawParams &P, int j) {
   return nm && ((j >= 24 && j <= 26) || (j >= 28 && j <= 30));
 }
 
+/* Shading of one fragment: `b` is the raw (screen-space) barycentric of its shading
+ * sample, `inst` which of the triangle's quads holding the pixel it was produced in
+ * (low bit: x, high bit: y, in the reference's visiting order). RendererSoft.cpp:800-851. */
 template <int S, int SHADING>
-__global__ __launch_bounds__(256) void k_raster(DrawParams P, int need_deriv) {
-  const int tile = blockIdx.x;
-  const int tx = tile % P.tiles_x, ty = tile / P.tiles_x;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int wx0 = tx * kTile + (wave & 1) * kSub, wy0 = ty * kTile + (wave >> 1) * kSub;
-  const int px = wx0 + (lane & 7), py = wy0 + (lane >> 3);
-  const bool inFb = px < P.fb_w && py < P.fb_h;
-  const size_t pix = (size_t)py * P.fb_w + px;
-  const bool hasC = P.color != nullptr, hasD = P.depth != nullptr;
-
-  float dep[S];
-  uint32_t col[S];
+__device__ __forceinline__ RGBAf shadeAt(const DrawParams &P, const PrimSetup &ps, const Tri &T, int px, int py,
+                                         float4 b, int inst, int need_deriv) {
+  const float w = dpps3(b, T.w);
+  const float iw = 1.f / w;
+  b = make_float4(b.x * (iw * T.w[0]), b.y * (iw * T.w[1]), b.z * (iw * T.w[2]), b.w * (iw * 1.f));
+  const float *va = P.vary + (size_t)ps.v[0] * kVaryStride;
+  const float *vb = P.vary + (size_t)ps.v[1] * kVaryStride;
+  const float *vc = P.vary + (size_t)ps.v[2] * kVaryStride;
+  Frag fr;
 #pragma unroll
-  for (int s = 0; s < S; s++) {
-    dep[s] = 0.f;
-    col[s] = 0u;
-  }
-  if (inFb) {
-    if (hasD)
-      for (int s = 0; s < S; s++) dep[s] = P.depth[pix * S + s];
-    if (hasC)
-      for (int s = 0; s < S; s++) col[s] = P.color[pix * S + s];
-  }
-
-  const int start = P.tile_start[tile], end = P.tile_start[tile + 1];
-  const float lx[4] = {0.375f, 0.875f, 0.125f, 0.625f};
-  const float ly[4] = {0.875f, 0.625f, 0.375f, 0.125f};
-  unsigned long long nfs = 0, nwr = 0;
-  const bool dtest = P.st.depthTest && hasD;
-
-  for (int it = start; it < end; it++) {
-    const int id = P.pair_val_sorted[it];
-    const PrimSetup &ps = P.prims[id];
-    if (ps.qx1 < wx0 || ps.qx0 > wx0 + kSub - 1 || ps.qy1 < wy0 || ps.qy0 > wy0 + kSub - 1) continue;
-    if (!inFb) continue;
-    int qxs[2], qys[2];
-    int nqx = axisQuads(ps.min_x, ps.max_x, ps.bcx, px, qxs);
-    if (nqx == 0) continue;
-    int nqy = axisQuads(ps.min_y, ps.max_y, ps.bcy, py, qys);
-    if (nqy == 0) continue;
-
-    const float4 f0 = P.frag[ps.v[0]], f1 = P.frag[ps.v[1]], f2 = P.frag[ps.v[2]];
-    Tri T;
-    T.v0x = f0.x;
-    T.v0y = f0.y;
-    T.a0 = f2.x - f0.x;
-    T.a1 = f1.x - f0.x;
-    T.b0 = f2.y - f0.y;
-    T.b1 = f1.y - f0.y;
-    T.uz = fmaf(T.a0, T.b1, -(T.a1 * T.b0));
-    T.z[0] = f0.z, T.z[1] = f1.z, T.z[2] = f2.z;
-    T.w[0] = f0.w, T.w[1] = f1.w, T.w[2] = f2.w;
-
-    for (int iy = 0; iy < nqy; iy++)
-      for (int ix = 0; ix < nqx; ix++) {
-        const int qx = qxs[ix], qy = qys[iy];
-        const float fx = (float)px, fy = (float)py;
-        /* own pixel: samples 0..3 (+ centre) for 4x, the centre for 1x */
-        bool sin[5];
-        float4 sbc[5];
-        float sz[5];
-        int shade;
-        bool pin;
-        if (S == 4) {
+  for (int j = 0; j < kVaryStride; j++) fr.v[j] = varyUsed<SHADING>(P, j) ? interp(b, va[j], vb[j], vc[j]) : 0.f;
+  fr.tc[0] = fr.tc[1] = fr.tc[2] = {0.f, 0.f};
+  if (need_deriv) {
+    /* coarse derivatives from quad pixels 0, 1, 2 (RendererSoft.cpp:329-334) */
+    int qx0, qx1, qy0, qy1;
+    axisQuads(ps.min_x, ps.max_x, ps.bcx, px, qx0, qx1);
+    axisQuads(ps.min_y, ps.max_y, ps.bcy, py, qy0, qy1);
+    const int qx = (inst & 1) ? qx1 : qx0, qy = (inst >> 1) ? qy1 : qy0;
+    const int k = (px - qx) + 2 * (py - qy);
 #pragma unroll
-          for (int s = 0; s < 4; s++) sin[s] = bary(T, lx[s] + fx, ly[s] + fy, sbc[s]);
-          sin[4] = bary(T, fx + 0.5f, fy + 0.5f, sbc[4]);
-          pin = sin[0] || sin[1] || sin[2] || sin[3];
-          shade = 4;
-          if (!sin[4]) {
-#pragma unroll
-            for (int s = 3; s >= 0; s--)
-              if (sin[s]) shade = s;
-          }
-        } else {
-          sin[0] = bary(T, fx + 0.5f, fy + 0.5f, sbc[0]);
-          pin = sin[0];
-          shade = 0;
-        }
-        if (!pin) continue;
-        const int NS = S == 4 ? 5 : 1;
-#pragma unroll
-        for (int s = 0; s < NS; s++) {
-          sz[s] = 0.f;
-          if (!sin[s]) continue;
-          float z = dpps3(sbc[s], T.z);
-          float w = dpps3(sbc[s], T.w);
-          sz[s] = z;
-          if (z < 0.f || z > 1.f) sin[s] = false;
-          float iw = 1.f / w;
-          sbc[s] = make_float4(sbc[s].x * (iw * T.w[0]), sbc[s].y * (iw * T.w[1]), sbc[s].z * (iw * T.w[2]),
-                               sbc[s].w * (iw * 1.f));
-        }
-        /* early-Z (RendererSoft.cpp:853-878): test without writing */
-        if (P.st.depthTest) {
-          if (S == 4) {
-            bool any = false;
-#pragma unroll
-            for (int s = 0; s < 4; s++) {
-              if (!sin[s]) continue;
-              if (hasD) sin[s] = depthPass(gclamp(sz[s], 0.f, 1.f), dep[s], P.st.depthFunc);
-              any = any || sin[s];
-            }
-            pin = any;
-          } else {
-            bool ok = hasD ? depthPass(gclamp(sz[0], 0.f, 1.f), dep[0], P.st.depthFunc) : true;
-            sin[0] = ok;
-            pin = ok;
-          }
-          if (!pin) continue;
-        }
-        RGBAf c = {0.f, 0.f, 0.f, 0.f};
-        if (hasC) {
-          /* interpolate the varyings the shader reads at the shading sample */
-          float vv[kVaryStride];
-          const float *va = P.vary + (size_t)ps.v[0] * kVaryStride;
-          const float *vb = P.vary + (size_t)ps.v[1] * kVaryStride;
-          const float *vc = P.vary + (size_t)ps.v[2] * kVaryStride;
-          const float4 b = sbc[shade];
-#pragma unroll
-          for (int j = 0; j < kVaryStride; j++) vv[j] = varyUsed<SHADING>(P, j) ? interp(b, va[j], vb[j], vc[j]) : 0.f;
-          Frag fr;
-          fr.v = vv;
-          if (need_deriv) {
-            const int k = (px - qx) + 2 * (py - qy);
-#pragma unroll
-            for (int j = 0; j < 3; j++) {
-              const int jx = qx + (j & 1), jy = qy + (j >> 1);
-              if (j == k) {
-                fr.tc[j] = {vv[0], vv[1]};
-              } else {
-                float4 pb = shadingBc<S>(T, (float)jx, (float)jy);
-                fr.tc[j] = {interp(pb, va[0], vb[0], vc[0]), interp(pb, va[1], vb[1], vc[1])};
-              }
-            }
-          } else {
-            fr.tc[0] = fr.tc[1] = fr.tc[2] = {0.f, 0.f};
-          }
-          c = runFS<SHADING>(P, fr);
-          nfs++;
-        }
-        /* per-sample operations (RendererSoft.cpp:358-407) */
-#pragma unroll
-        for (int s = 0; s < S; s++) {
-          const int si = S == 4 ? s : shade;
-          if (S == 4 && !sin[s]) continue;
-          if (dtest) {
-            float z = gclamp(sz[si], 0.f, 1.f);
-            if (!depthPass(z, dep[s], P.st.depthFunc)) continue;
-            if (P.st.depthMask) dep[s] = z;
-          }
-          if (!hasC) continue;
-          RGBAf o = {gclamp(c.r, 0.f, 1.f), gclamp(c.g, 0.f, 1.f), gclamp(c.b, 0.f, 1.f), gclamp(c.a, 0.f, 1.f)};
-          if (P.st.blend) o = blendColor(o, unpack01(col[s]), P.st);
-          col[s] = pack4(u8t(o.r * 255.f), u8t(o.g * 255.f), u8t(o.b * 255.f), u8t(o.a * 255.f));
-          nwr++;
-        }
-      }
-  }
-
-  if (inFb) {
-    if (hasD && start < end)
-      for (int s = 0; s < S; s++) P.depth[pix * S + s] = dep[s];
-    if (hasC) {
-      if (start < end)
-        for (int s = 0; s < S; s++) P.color[pix * S + s] = col[s];
-      if (S == 4) {
-        /* multiSampleResolve (RendererSoft.cpp:880-912): u8(sum / 4) */
-        int r[4];
-#pragma unroll
-        for (int ch = 0; ch < 4; ch++) r[ch] = (chan(col[0], ch) + chan(col[1], ch) + chan(col[2], ch) + chan(col[3], ch)) >> 2;
-        P.color_resolved[pix] = pack4(r[0], r[1], r[2], r[3]);
+    for (int j = 0; j < 3; j++) {
+      if (j == k) {
+        fr.tc[j] = {fr.v[0], fr.v[1]};
+      } else {
+        const float4 pb = shadingBc<S>(T, (float)(qx + (j & 1)), (float)(qy + (j >> 1)));
+        fr.tc[j] = {interp(pb, va[0], vb[0], vc[0]), interp(pb, va[1], vb[1], vc[1])};
       }
     }
   }
-  if (nfs) atomicAdd(&P.counters[0], nfs);
-  if (nwr) atomicAdd(&P.counters[1], nwr);
+  return runFS<SHADING>(P, fr);
+}
+
+__device__ __forceinline__ uint32_t resolve4(const uint32_t *c) {
+  /* multiSampleResolve (RendererSoft.cpp:880-912): u8(sum / 4) */
+  int r[4];
+#pragma unroll
+  for (int ch = 0; ch < 4; ch++) r[ch] = (chan(c[0], ch) + chan(c[1], ch) + chan(c[2], ch) + chan(c[3], ch)) >> 2;
+  return pack4(r[0], r[1], r[2], r[3]);
 }
 
 /* --------------------------------------------------------------------------- two-phase raster
@@ -603,10 +755,12 @@ __global__ __launch_bounds__(256) void k_raster(DrawParams P, int need_deriv) {
  * that is follows from depths and API order alone: the minimum (or maximum)
  * depth, ties going to the earliest fragment for the strict tests and the
  * latest for the others. Phase 1 finds that winner per sample with 64-bit LDS
- * atomics on key = (monotonic depth bits << 32) | order word, one lane per
- * triangle of the tile; phase 2 shades each pixel's winners once, one lane per
- * pixel. The fragment shader's inputs depend only on the triangle, the pixel
- * and the 2x2 quad it is shaded in, so the result equals the sequential one. */
+ * atomics on key = (monotonic depth bits << 32) | order word: big triangles one
+ * at a time with a lane per pixel, the others with a lane per triangle, all
+ * after the hierarchical-Z cull. Phase 2 lists each pixel's distinct winners
+ * and shades the list with every lane busy. The fragment shader's inputs depend
+ * only on the triangle, the pixel and the 2x2 quad it is shaded in, so the
+ * result equals the sequential one. */
 enum VisMode { VIS_LESS = 0, VIS_LEQUAL = 1, VIS_GREATER = 2, VIS_GEQUAL = 3 };
 
 template <int MODE>
@@ -648,18 +802,459 @@ __device__ __forceinline__ Tri makeTri(const float4 &f0, const float4 &f1, const
   return T;
 }
 
-template <int S, int SHADING, int MODE>
-__global__ __launch_bounds__(256) void k_raster_vis(DrawParams P, int need_deriv) {
+/* --------------------------------------------------------------------------- hierarchical Z
+ *
+ * Per tile, one depth bound in depthKey order. For LESS / LEQUAL it is the
+ * deepest depth any sample of the tile can hold at the end of the draw: the
+ * deepest non-NaN depth already there, lowered to the far depth of every big
+ * triangle that covers the whole tile (such a triangle puts a fragment at every
+ * sample no deeper than that). A triangle whose nearest possible depth lies
+ * beyond the bound cannot win or pass at any sample and is skipped; results are
+ * unchanged. GREATER / GEQUAL mirror it. */
+
+/* True if the triangle covers every sample of pixels [x0, x1] x [y0, y1] with at
+ * least a pixel of margin, inside its quads and, multisampled, inside the depth
+ * range; [zmin, zmax] then bounds its depth over them. Evaluated in double. */
+template <int S>
+__device__ bool coversRect(const PrimSetup &ps, const float4 &a, const float4 &b, const float4 &c, int x0, int y0,
+                           int x1, int y1, float &zmin, float &zmax) {
+  if (ps.qx0 > x0 || ps.qx1 < x1 || ps.qy0 > y0 || ps.qy1 < y1) return false;
+  const double ax = a.x, ay = a.y, bx = b.x, by = b.y, cx = c.x, cy = c.y;
+  const double area = (bx - ax) * (cy - ay) - (by - ay) * (cx - ax);
+  if (!(fabs(area) > 1.0)) return false;
+  const double sg = area > 0 ? 1.0 : -1.0;
+  const double dzdx = ((double)(b.z - a.z) * (cy - ay) - (double)(c.z - a.z) * (by - ay)) / area;
+  const double dzdy = ((double)(c.z - a.z) * (bx - ax) - (double)(b.z - a.z) * (cx - ax)) / area;
+  double lo = INFINITY, hi = -INFINITY;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const double X = (k & 1) ? x1 + 2.0 : x0 - 1.0, Y = (k & 2) ? y1 + 2.0 : y0 - 1.0;
+    const double e0 = ((bx - ax) * (Y - ay) - (by - ay) * (X - ax)) * sg;
+    const double e1 = ((cx - bx) * (Y - by) - (cy - by) * (X - bx)) * sg;
+    const double e2 = ((ax - cx) * (Y - cy) - (ay - cy) * (X - cx)) * sg;
+    if (!(e0 > 0 && e1 > 0 && e2 > 0)) return false;
+    const double z = (double)a.z + dzdx * (X - ax) + dzdy * (Y - ay);
+    lo = fmin(lo, z);
+    hi = fmax(hi, z);
+  }
+  const double sl = 1e-5 + 16.0 * 1.19209290e-07 * (fabs((double)a.z) + fabs((double)b.z) + fabs((double)c.z));
+  lo -= sl;
+  hi += sl;
+  if (!(lo == lo) || !(hi == hi)) return false;
+  if (S == 4 && (lo < 0.0 || hi > 1.0)) return false;
+  zmin = (float)lo;
+  zmax = (float)hi;
+  return true;
+}
+
+__device__ __forceinline__ bool zCulled(const PrimSetup &ps, uint32_t bound, bool mx) {
+  return mx ? depthKey(gclamp(ps.zhi, 0.f, 1.f)) < bound : depthKey(gclamp(ps.zlo, 0.f, 1.f)) > bound;
+}
+
+/* Bound from the depths a lane holds; combined over the block with atomicMax (LESS) / atomicMin. */
+template <int S>
+__device__ __forceinline__ void hizFromDepth(const float *dep, bool mx, uint32_t *bound) {
+  uint32_t kb = mx ? 0xffffffffu : 0u;
+  bool any = false;
+#pragma unroll
+  for (int s = 0; s < S; s++) {
+    if (dep[s] != dep[s]) continue;
+    const uint32_t k = depthKey(dep[s]);
+    kb = mx ? min(kb, k) : max(kb, k);
+    any = true;
+  }
+  if (!any) return;
+  if (mx)
+    atomicMin(bound, kb);
+  else
+    atomicMax(bound, kb);
+}
+
+/* Phase-1 visibility of triangle `id` at pixel (x, y): every quad of the triangle
+ * holding the pixel is one fragment; of several, the test keeps the first or the last. */
+template <int S, int MODE>
+__device__ __forceinline__ void visPixel(const PrimSetup &ps, const Tri &T, int id, int x, int y,
+                                         unsigned long long *kp) {
+  const float lx[4] = {0.375f, 0.875f, 0.125f, 0.625f};
+  const float ly[4] = {0.875f, 0.625f, 0.375f, 0.125f};
+  int q0, q1;
+  const int nqy = axisQuads(ps.min_y, ps.max_y, ps.bcy, y, q0, q1);
+  if (!nqy) return;
+  const int nqx = axisQuads(ps.min_x, ps.max_x, ps.bcx, x, q0, q1);
+  if (!nqx) return;
+  const uint32_t inst = visFirst<MODE>() ? 0u : (uint32_t)((nqy - 1) * 2 + (nqx - 1));
+  const uint32_t low = visLow<MODE>((((uint32_t)id << 2) | inst) + 1u);
+  const float fx = (float)x, fy = (float)y;
+  if (S == 4) {
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      float4 bc;
+      if (!bary(T, lx[s] + fx, ly[s] + fy, bc)) continue;
+      const float z = dpps3(bc, T.z);
+      if (z < 0.f || z > 1.f) continue; /* multisampled depth clip (RendererSoft.cpp:796-799) */
+      const unsigned long long key = ((unsigned long long)depthKey(z) << 32) | low;
+      if (visMax<MODE>())
+        atomicMax(kp + s, key);
+      else
+        atomicMin(kp + s, key);
+    }
+  } else {
+    float4 bc;
+    if (!bary(T, fx + 0.5f, fy + 0.5f, bc)) return;
+    const float z = gclamp(dpps3(bc, T.z), 0.f, 1.f);
+    if (z != z) return;
+    const unsigned long long key = ((unsigned long long)depthKey(z) << 32) | low;
+    if (visMax<MODE>())
+      atomicMax(kp, key);
+    else
+      atomicMin(kp, key);
+  }
+}
+
+
+/* Big triangles (coarse and global bins) per 128x128 coarse bin, before the
+ * tiles: the depth bound that covering ones give the whole bin, and the list
+ * of those that bound does not rule out (or -1 when longer than kHizCap: the
+ * tiles then read the full bins). One block per coarse bin. */
+constexpr int kHizCap = 1024;
+
+template <int S, bool MX>
+__global__ __launch_bounds__(256) void k_hiz_coarse(DrawParams P) {
+  __shared__ uint32_t s_bound;
+  __shared__ int s_cnt;
+  const int c = blockIdx.x;
+  const int cx = c % P.coarse_x, cy = c / P.coarse_x;
+  const int x0 = cx * kCoarse * kTile, y0 = cy * kCoarse * kTile;
+  const int x1 = min(x0 + kCoarse * kTile, P.fb_w) - 1, y1 = min(y0 + kCoarse * kTile, P.fb_h) - 1;
+  const int gb = P.nbins - 1, cb = P.ntiles + c;
+  const int cs = P.bin_start[cb], cn = P.bin_start[cb + 1] - cs;
+  const int gs = P.bin_start[gb], gn = P.bin_start[gb + 1] - gs;
+  const int n = cn + gn;
+  const int lp = threadIdx.x;
+  if (lp == 0) {
+    s_bound = MX ? 0u : 0xffffffffu;
+    s_cnt = 0;
+  }
+  __syncthreads();
+  for (int i = lp; i < n; i += blockDim.x) {
+    const int id = i < cn ? P.bin_items[cs + i] : P.bin_items[gs + i - cn];
+    const PrimSetup ps = P.prims[id];
+    float zmin, zmax;
+    if (coversRect<S>(ps, P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]], x0, y0, x1, y1, zmin, zmax)) {
+      if (MX)
+        atomicMax(&s_bound, depthKey(gclamp(zmin, 0.f, 1.f)));
+      else
+        atomicMin(&s_bound, depthKey(gclamp(zmax, 0.f, 1.f)));
+    }
+  }
+  __syncthreads();
+  const uint32_t bound = s_bound;
+  for (int i = lp; i < n; i += blockDim.x) {
+    const int id = i < cn ? P.bin_items[cs + i] : P.bin_items[gs + i - cn];
+    const PrimSetup &ps = P.prims[id];
+    if (ps.qx0 > x1 || ps.qx1 < x0 || ps.qy0 > y1 || ps.qy1 < y0 || zCulled(ps, bound, MX)) continue;
+    const int k = atomicAdd(&s_cnt, 1);
+    if (k < kHizCap) P.hiz_surv[c * kHizCap + k] = id;
+  }
+  __syncthreads();
+  if (lp == 0) {
+    P.hiz_bound[c] = bound;
+    P.hiz_count[c] = s_cnt <= kHizCap ? s_cnt : -1;
+  }
+}
+
+template <int S, int MODE>
+__global__ __launch_bounds__(256) void k_raster_vis(DrawParams P) {
   constexpr int NPIX = kTile * kTile;
+  constexpr bool MX = visMax<MODE>();
   __shared__ unsigned long long keys[NPIX * S];
+  __shared__ int big[NPIX];
+  __shared__ uint32_t big_z[NPIX];
+  __shared__ uint32_t s_bound, s_bound2;
+  __shared__ int s_n, s_base;
   const int tile = blockIdx.x;
   const int tx0 = (tile % P.tiles_x) * kTile, ty0 = (tile / P.tiles_x) * kTile;
+  const int X1 = min(tx0 + kTile, P.fb_w) - 1, Y1 = min(ty0 + kTile, P.fb_h) - 1;
   const int lp = threadIdx.x;
   const int px = tx0 + (lp & (kTile - 1)), py = ty0 + lp / kTile;
   const bool inFb = px < P.fb_w && py < P.fb_h;
   const size_t pix = (size_t)py * P.fb_w + px;
   const bool hasC = P.color != nullptr;
-  const int start = P.tile_start[tile], end = P.tile_start[tile + 1];
+  const TileBins tb = tileBins(P, tile);
+  const int coarse = (ty0 / (kCoarse * kTile)) * P.coarse_x + tx0 / (kCoarse * kTile);
+  const float lx[4] = {0.375f, 0.875f, 0.125f, 0.625f};
+  const float ly[4] = {0.875f, 0.625f, 0.375f, 0.125f};
+
+  if (tb.total() == 0) return;
+  float dep[S];
+#pragma unroll
+  for (int s = 0; s < S; s++) dep[s] = inFb ? P.depth[pix * S + s] : 0.f;
+  /* big triangles: the coarse bin's survivors, or all of its and the global bin's entries */
+  const int hc = P.hiz_count[coarse];
+  const int nbig = hc >= 0 ? hc : tb.n[1] + tb.n[2];
+  auto bigAt = [&](int i) { return hc >= 0 ? P.hiz_surv[coarse * kHizCap + i] : tb.at(P, tb.n[0] + i); };
+  if (lp == 0) {
+    s_bound = P.hiz_bound[coarse];
+    s_bound2 = MX ? 0xffffffffu : 0u;
+    s_n = 0;
+  }
+#pragma unroll
+  for (int s = 0; s < S; s++) keys[lp * S + s] = visInit<MODE>(dep[s], inFb);
+  __syncthreads();
+  /* the tile's own depths: atomicMin (LESS) of their deepest, combined below */
+  if (inFb) hizFromDepth<S>(dep, MX, &s_bound2);
+  __syncthreads();
+  if (lp == 0) s_bound = MX ? max(s_bound, s_bound2) : min(s_bound, s_bound2);
+  __syncthreads();
+
+  /* big triangles that cover the tile tighten the bound */
+  for (int i = lp; i < nbig; i += NPIX) {
+    const int id = bigAt(i);
+    const PrimSetup ps = P.prims[id];
+    float zmin, zmax;
+    if (coversRect<S>(ps, P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]], tx0, ty0, X1, Y1, zmin, zmax)) {
+      if (MX)
+        atomicMax(&s_bound, depthKey(gclamp(zmin, 0.f, 1.f)));
+      else
+        atomicMin(&s_bound, depthKey(gclamp(zmax, 0.f, 1.f)));
+    }
+  }
+  __syncthreads();
+  const uint32_t bound = s_bound;
+  unsigned long long nculled = 0;
+
+  /* phase 1a: surviving big triangles one at a time, one lane per pixel; a
+   * wavefront skips a triangle no pixel of it can take at its current depths */
+  uint32_t mine = 0; /* this lane's deepest (LESS) / shallowest (GREATER) current depth key */
+#pragma unroll
+  for (int s = 0; s < S; s++) {
+    const uint32_t k = (uint32_t)(keys[lp * S + s] >> 32);
+    mine = s == 0 ? k : (MX ? min(mine, k) : max(mine, k));
+  }
+  uint32_t wave_lim = MX ? waveMinU(mine) : waveMaxU(mine);
+  for (int c0 = 0; c0 < nbig; c0 += NPIX) {
+    if (c0 + lp < nbig) {
+      const int id = bigAt(c0 + lp);
+      const PrimSetup &ps = P.prims[id];
+      if (ps.qx0 <= X1 && ps.qx1 >= tx0 && ps.qy0 <= Y1 && ps.qy1 >= ty0) {
+        if (!zCulled(ps, bound, MX)) {
+          const int k = atomicAdd(&s_n, 1);
+          big[k] = id;
+          big_z[k] = MX ? depthKey(gclamp(ps.zhi, 0.f, 1.f)) : depthKey(gclamp(ps.zlo, 0.f, 1.f));
+        } else {
+          nculled++;
+        }
+      }
+    }
+    __syncthreads();
+    const int nb = s_n;
+    for (int j = 0; j < nb; j++) {
+      const uint32_t zk = big_z[j];
+      if (MX ? zk < wave_lim : zk > wave_lim) continue; /* wave-uniform */
+      const int id = big[j];
+      const PrimSetup ps = P.prims[id];
+      if (inFb && px >= ps.qx0 && px <= ps.qx1 && py >= ps.qy0 && py <= ps.qy1) {
+        const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
+        visPixel<S, MODE>(ps, T, id, px, py, &keys[lp * S]);
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+          const uint32_t k = (uint32_t)(keys[lp * S + s] >> 32);
+          mine = s == 0 ? k : (MX ? min(mine, k) : max(mine, k));
+        }
+      }
+      wave_lim = MX ? waveMinU(mine) : waveMaxU(mine);
+    }
+    __syncthreads();
+    if (lp == 0) s_n = 0;
+    __syncthreads();
+  }
+  /* the depths after the big triangles bound the rest */
+  if (lp == 0) s_bound2 = MX ? 0xffffffffu : 0u;
+  __syncthreads();
+  if (MX)
+    atomicMin(&s_bound2, mine);
+  else
+    atomicMax(&s_bound2, mine);
+  __syncthreads();
+  const uint32_t bound2 = MX ? max(bound, s_bound2) : min(bound, s_bound2);
+
+  /* phase 1b: the tile's own triangles, one lane per triangle */
+  for (int i = lp; i < tb.n[0]; i += NPIX) {
+    const int id = P.bin_items[tb.s[0] + i];
+    const PrimSetup ps = P.prims[id];
+    if (zCulled(ps, bound2, MX)) {
+      nculled++;
+      continue;
+    }
+    const int x0 = max(ps.qx0, tx0), x1 = min(ps.qx1, X1);
+    const int y0 = max(ps.qy0, ty0), y1 = min(ps.qy1, Y1);
+    if (x0 > x1 || y0 > y1) continue;
+    const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
+    for (int y = y0; y <= y1; y++)
+      for (int x = x0; x <= x1; x++)
+        visPixel<S, MODE>(ps, T, id, x, y, &keys[((y - ty0) * kTile + (x - tx0)) * S]);
+  }
+  __syncthreads();
+
+  /* phase 2: each pixel's distinct winners write their depth here; colour is
+   * queued, one job per (pixel, winner), for k_shade */
+  uint32_t win[S];
+#pragma unroll
+  for (int s = 0; s < S; s++) {
+    const uint32_t low = (uint32_t)keys[lp * S + s];
+    const uint32_t ord = (MODE == VIS_LESS || MODE == VIS_GEQUAL) ? low : ~low;
+    win[s] = (inFb && ord) ? ord - 1u : 0xffffffffu;
+  }
+  int nj = 0;
+#pragma unroll
+  for (int s = 0; s < S; s++) {
+    bool seen = win[s] == 0xffffffffu;
+#pragma unroll
+    for (int t = 0; t < s; t++) seen = seen || win[t] == win[s];
+    nj += seen ? 0 : 1;
+  }
+  int base = 0;
+  if (hasC) {
+    if (lp == 0) s_n = 0;
+    __syncthreads();
+    const int off = nj ? atomicAdd(&s_n, nj) : 0;
+    __syncthreads();
+    if (lp == 0 && s_n) s_base = atomicAdd(&P.dcount[2], s_n);
+    __syncthreads();
+    base = s_base + off;
+  }
+  if (inFb) {
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      const uint32_t w = win[s];
+      if (w == 0xffffffffu) continue;
+      bool seen = false;
+#pragma unroll
+      for (int t = 0; t < s; t++) seen = seen || win[t] == w;
+      if (seen) continue;
+      const int id = (int)(w >> 2);
+      const PrimSetup ps = P.prims[id];
+      const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
+      uint32_t m = 0;
+#pragma unroll
+      for (int t = s; t < S; t++) {
+        if (win[t] != w) continue;
+        m |= 1u << t;
+        float4 bc;
+        if (S == 4)
+          bary(T, lx[t] + (float)px, ly[t] + (float)py, bc);
+        else
+          bary(T, (float)px + 0.5f, (float)py + 0.5f, bc);
+        dep[t] = gclamp(dpps3(bc, T.z), 0.f, 1.f);
+      }
+      if (hasC) P.jobs[base++] = make_uint2((uint32_t)pix | (m << 28), w);
+    }
+#pragma unroll
+    for (int s = 0; s < S; s++) P.depth[pix * S + s] = dep[s];
+  }
+  if (nculled) atomicAdd(&P.counters[6], nculled);
+}
+
+/* Colour of the fragments the two-phase raster queued: one lane per (pixel,
+ * winning triangle), every sample the triangle won gets its shader output. */
+template <int S, int SHADING>
+__global__ __launch_bounds__(256) void k_shade(DrawParams P, int need_deriv) {
+  __shared__ unsigned long long s_fs, s_wr;
+  if (threadIdx.x == 0) s_fs = s_wr = 0;
+  __syncthreads();
+  const float lx[4] = {0.375f, 0.875f, 0.125f, 0.625f};
+  const float ly[4] = {0.875f, 0.625f, 0.375f, 0.125f};
+  const int n = P.dcount[2];
+  unsigned long long nfs = 0, nwr = 0;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const uint2 jb = P.jobs[j];
+    const uint32_t pix = jb.x & 0x0fffffffu, m = jb.x >> 28, w = jb.y;
+    const int px = (int)(pix % (uint32_t)P.fb_w), py = (int)(pix / (uint32_t)P.fb_w);
+    const int id = (int)(w >> 2), inst = (int)(w & 3u);
+    const PrimSetup ps = P.prims[id];
+    const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
+    const float fx = (float)px, fy = (float)py;
+    /* the shading sample: the centre if inside, else the first inside sample (RendererSoft.cpp:776-802) */
+    float4 bs;
+    if (S == 4) {
+      if (!bary(T, fx + 0.5f, fy + 0.5f, bs)) {
+#pragma unroll
+        for (int t = 3; t >= 0; t--) {
+          float4 b;
+          if (bary(T, lx[t] + fx, ly[t] + fy, b)) bs = b;
+        }
+      }
+    } else {
+      bary(T, fx + 0.5f, fy + 0.5f, bs);
+    }
+    RGBAf c = shadeAt<S, SHADING>(P, ps, T, px, py, bs, inst, need_deriv);
+    c = {gclamp(c.r, 0.f, 1.f), gclamp(c.g, 0.f, 1.f), gclamp(c.b, 0.f, 1.f), gclamp(c.a, 0.f, 1.f)};
+    const uint32_t packed = pack4(u8t(c.r * 255.f), u8t(c.g * 255.f), u8t(c.b * 255.f), u8t(c.a * 255.f));
+    nfs++;
+#pragma unroll
+    for (int t = 0; t < S; t++)
+      if ((m >> t) & 1u) {
+        P.color[(size_t)pix * S + t] = packed;
+        nwr++;
+      }
+  }
+  if (nfs) atomicAdd(&s_fs, nfs);
+  if (nwr) atomicAdd(&s_wr, nwr);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (s_fs) atomicAdd(&P.counters[0], s_fs);
+    if (s_wr) atomicAdd(&P.counters[1], s_wr);
+  }
+}
+
+/* multiSampleResolve (RendererSoft.cpp:880-912) of a whole 4x colour target */
+__global__ __launch_bounds__(256) void k_resolve(DrawParams P) {
+  const size_t n = (size_t)P.fb_w * P.fb_h;
+  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
+    const uint4 c = ((const uint4 *)P.color)[p];
+    const uint32_t cs[4] = {c.x, c.y, c.z, c.w};
+    P.color_resolved[p] = resolve4(cs);
+  }
+}
+
+/* --------------------------------------------------------------------------- ordered raster
+ *
+ * One workgroup per 16x16 tile, one wavefront per 8x8 quarter, one lane per
+ * pixel. The tile's triangles are sorted by primitive id in LDS (in windows of
+ * kOrdWin ids when there are more). Walking them in order, each lane finds the
+ * fragments of its pixel (one per quad of the triangle holding it), tests
+ * coverage and early-Z, and appends them to its wavefront's queue; when the
+ * queue fills, the wavefront shades the queued fragments with all lanes busy,
+ * then every lane applies its own fragments' per-sample operations in order
+ * (late depth test, depth write, blending). Early-Z against a depth that
+ * fragments still in the queue may lower is conservative for the monotonic
+ * tests: such a fragment also fails the late test. For NOTEQUAL with depth
+ * writes the queue is drained after every triangle. */
+constexpr int kOrdWin = 2048;
+constexpr int kOrdJobsW = 256; /* queue entries per wavefront (64 lanes x 4 quads at most per triangle) */
+
+template <int S, int SHADING>
+__global__ __launch_bounds__(256) void k_raster_ord(DrawParams P, int need_deriv, int flush_each) {
+  __shared__ uint32_t win[kOrdWin];
+  __shared__ uint32_t job_id[4 * kOrdJobsW];
+  __shared__ uint32_t job_meta[4 * kOrdJobsW];
+  __shared__ short job_next[4 * kOrdJobsW];
+  __shared__ float4 job_col[4 * kOrdJobsW];
+  __shared__ float job_z[4 * kOrdJobsW * S];
+  __shared__ uint32_t s_bound;
+  __shared__ int s_n, s_cnt;
+  const int tile = blockIdx.x;
+  const int tx0 = (tile % P.tiles_x) * kTile, ty0 = (tile / P.tiles_x) * kTile;
+  const int X1 = min(tx0 + kTile, P.fb_w) - 1, Y1 = min(ty0 + kTile, P.fb_h) - 1;
+  const int lp = threadIdx.x, wave = lp >> 6, lane = lp & 63;
+  const int wx0 = tx0 + (wave & 1) * kSub, wy0 = ty0 + (wave >> 1) * kSub;
+  const int px = wx0 + (lane & 7), py = wy0 + (lane >> 3);
+  const int pl = (py - ty0) * kTile + (px - tx0);
+  const bool inFb = px < P.fb_w && py < P.fb_h;
+  const size_t pix = (size_t)py * P.fb_w + px;
+  const bool hasC = P.color != nullptr, hasD = P.depth != nullptr;
+  const TileBins tb = tileBins(P, tile);
+  const int n = tb.total();
   const float lx[4] = {0.375f, 0.875f, 0.125f, 0.625f};
   const float ly[4] = {0.875f, 0.625f, 0.375f, 0.125f};
 
@@ -671,171 +1266,263 @@ __global__ __launch_bounds__(256) void k_raster_vis(DrawParams P, int need_deriv
     col[s] = 0u;
   }
   if (inFb) {
-    for (int s = 0; s < S; s++) dep[s] = P.depth[pix * S + s];
+    if (hasD)
+      for (int s = 0; s < S; s++) dep[s] = P.depth[pix * S + s];
     if (hasC)
       for (int s = 0; s < S; s++) col[s] = P.color[pix * S + s];
   }
-  if (start == end) {
-    if (inFb && hasC && S == 4) {
-      int r[4];
-#pragma unroll
-      for (int ch = 0; ch < 4; ch++) r[ch] = (chan(col[0], ch) + chan(col[1], ch) + chan(col[2], ch) + chan(col[3], ch)) >> 2;
-      P.color_resolved[pix] = pack4(r[0], r[1], r[2], r[3]);
-    }
+  if (n == 0) {
+    if (inFb && hasC && S == 4) P.color_resolved[pix] = resolve4(col);
     return;
   }
-#pragma unroll
-  for (int s = 0; s < S; s++) keys[lp * S + s] = visInit<MODE>(dep[s], inFb);
-  __syncthreads();
-
-  /* phase 1: visibility, one lane per triangle of the tile */
-  for (int it = start + lp; it < end; it += blockDim.x) {
-    const int id = P.pair_val_sorted[it];
-    const PrimSetup ps = P.prims[id];
-    const int x0 = max(ps.qx0, tx0), x1 = min(min(ps.qx1, tx0 + kTile - 1), P.fb_w - 1);
-    const int y0 = max(ps.qy0, ty0), y1 = min(min(ps.qy1, ty0 + kTile - 1), P.fb_h - 1);
-    if (x0 > x1 || y0 > y1) continue;
-    const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
-    for (int y = y0; y <= y1; y++) {
-      int qys[2];
-      const int nqy = axisQuads(ps.min_y, ps.max_y, ps.bcy, y, qys);
-      if (!nqy) continue;
-      for (int x = x0; x <= x1; x++) {
-        int qxs[2];
-        const int nqx = axisQuads(ps.min_x, ps.max_x, ps.bcx, x, qxs);
-        if (!nqx) continue;
-        /* of several quads of one triangle holding this pixel, the test keeps the first or the last */
-        const uint32_t inst = visFirst<MODE>() ? 0u : (uint32_t)((nqy - 1) * 2 + (nqx - 1));
-        const uint32_t low = visLow<MODE>((((uint32_t)id << 2) | inst) + 1u);
-        unsigned long long *kp = &keys[((y - ty0) * kTile + (x - tx0)) * S];
-        const float fx = (float)x, fy = (float)y;
-        if (S == 4) {
-#pragma unroll
-          for (int s = 0; s < 4; s++) {
-            float4 bc;
-            if (!bary(T, lx[s] + fx, ly[s] + fy, bc)) continue;
-            const float z = dpps3(bc, T.z);
-            if (z < 0.f || z > 1.f) continue; /* multisampled depth clip (RendererSoft.cpp:796-799) */
-            const unsigned long long key = ((unsigned long long)depthKey(z) << 32) | low;
-            if (visMax<MODE>())
-              atomicMax(kp + s, key);
-            else
-              atomicMin(kp + s, key);
-          }
-        } else {
-          float4 bc;
-          if (!bary(T, fx + 0.5f, fy + 0.5f, bc)) continue;
-          const float z = gclamp(dpps3(bc, T.z), 0.f, 1.f);
-          if (z != z) continue;
-          const unsigned long long key = ((unsigned long long)depthKey(z) << 32) | low;
-          if (visMax<MODE>())
-            atomicMax(kp, key);
-          else
-            atomicMin(kp, key);
-        }
-      }
-    }
+  const int dfunc = P.st.depthFunc;
+  const bool dtest = P.st.depthTest && hasD;
+  const bool hiz = dtest && (dfunc == 1 || dfunc == 3 || dfunc == 4 || dfunc == 6);
+  const bool mx = dfunc == 4 || dfunc == 6;
+  if (lp == 0) {
+    s_bound = mx ? 0xffffffffu : 0u;
+    s_n = 0;
   }
   __syncthreads();
+  if (hiz && inFb) hizFromDepth<S>(dep, mx, &s_bound);
+  __syncthreads();
+  const uint32_t bound = s_bound;
 
-  /* phase 2: shade each pixel's winners, one lane per pixel */
-  if (!inFb) return;
-  uint32_t win[S];
-#pragma unroll
-  for (int s = 0; s < S; s++) {
-    const uint32_t low = (uint32_t)keys[lp * S + s];
-    const uint32_t ord = (MODE == VIS_LESS || MODE == VIS_GEQUAL) ? low : ~low;
-    win[s] = ord ? ord - 1u : 0xffffffffu;
-  }
+  const int maxid = P.dcount[1] - 1;
+  const int jb = wave * kOrdJobsW;
+  unsigned long long nculled = 0;
+  int head = -1, tail = -1, wcnt = 0;
   unsigned long long nfs = 0, nwr = 0;
-#pragma unroll
-  for (int s = 0; s < S; s++) {
-    const uint32_t w = win[s];
-    if (w == 0xffffffffu) continue;
-    bool seen = false;
-#pragma unroll
-    for (int t = 0; t < s; t++) seen = seen || win[t] == w;
-    if (seen) continue;
-    const int id = (int)(w >> 2), inst = (int)(w & 3u);
-    const PrimSetup ps = P.prims[id];
-    const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
-    const float fx = (float)px, fy = (float)py;
-    /* raw barycentrics of the samples and the shading sample (RendererSoft.cpp:776-802) */
-    float4 sbc[5];
-    bool sin[5];
-    int shade;
-    if (S == 4) {
-#pragma unroll
-      for (int t = 0; t < 4; t++) sin[t] = bary(T, lx[t] + fx, ly[t] + fy, sbc[t]);
-      sin[4] = bary(T, fx + 0.5f, fy + 0.5f, sbc[4]);
-      shade = 4;
-      if (!sin[4]) {
-#pragma unroll
-        for (int t = 3; t >= 0; t--)
-          if (sin[t]) shade = t;
+  int lo = 0;
+  while (lo <= maxid) {
+    int hi = maxid;
+    if (n > kOrdWin) {
+      /* the largest hi with at most kOrdWin entries in [lo, hi] */
+      int L = lo, R = maxid;
+      while (L < R) {
+        const int mid = L + (R - L + 1) / 2;
+        if (lp == 0) s_cnt = 0;
+        __syncthreads();
+        int cnt = 0;
+        for (int i = lp; i < n; i += 256) {
+          const int id = tb.at(P, i);
+          cnt += (id >= lo && id <= mid) ? 1 : 0;
+        }
+        if (cnt) atomicAdd(&s_cnt, cnt);
+        __syncthreads();
+        const int c = s_cnt;
+        __syncthreads();
+        if (c <= kOrdWin)
+          L = mid;
+        else
+          R = mid - 1;
       }
-    } else {
-      sin[0] = bary(T, fx + 0.5f, fy + 0.5f, sbc[0]);
-      shade = 0;
+      hi = L;
     }
-    RGBAf c = {0.f, 0.f, 0.f, 0.f};
-    if (hasC) {
-      float4 b = sbc[shade];
-      const float w = dpps3(b, T.w);
-      const float iw = 1.f / w;
-      b = make_float4(b.x * (iw * T.w[0]), b.y * (iw * T.w[1]), b.z * (iw * T.w[2]), b.w * (iw * 1.f));
-      float vv[kVaryStride];
-      const float *va = P.vary + (size_t)ps.v[0] * kVaryStride;
-      const float *vb = P.vary + (size_t)ps.v[1] * kVaryStride;
-      const float *vc = P.vary + (size_t)ps.v[2] * kVaryStride;
+    for (int i = lp; i < n; i += 256) {
+      const int id = tb.at(P, i);
+      if (id < lo || id > hi) continue;
+      const PrimSetup &ps = P.prims[id];
+      if (i >= tb.n[0] && (ps.qx0 > X1 || ps.qx1 < tx0 || ps.qy0 > Y1 || ps.qy1 < ty0)) continue;
+      if (hiz && zCulled(ps, bound, mx)) {
+        nculled++;
+        continue;
+      }
+      win[atomicAdd(&s_n, 1)] = (uint32_t)id;
+    }
+    __syncthreads();
+    const int m = s_n;
+    int mp = 1;
+    while (mp < m) mp <<= 1;
+    for (int i = m + lp; i < mp; i += 256) win[i] = 0xffffffffu;
+    __syncthreads();
+    blockSortU32(win, mp);
+
+    /* walk the window in order; the queue is drained at one place */
+    int k = 0, tot = 0, nI = 0, pend_id = 0;
+    uint32_t meta = 0;
+    int qn = 0;
+    bool have = false;
+    float zs[S];
 #pragma unroll
-      for (int j = 0; j < kVaryStride; j++) vv[j] = varyUsed<SHADING>(P, j) ? interp(b, va[j], vb[j], vc[j]) : 0.f;
-      Frag fr;
-      fr.v = vv;
-      fr.tc[0] = fr.tc[1] = fr.tc[2] = {0.f, 0.f};
-      if (need_deriv) {
-        int qxs[2], qys[2];
-        axisQuads(ps.min_x, ps.max_x, ps.bcx, px, qxs);
-        axisQuads(ps.min_y, ps.max_y, ps.bcy, py, qys);
-        const int qx = qxs[inst & 1], qy = qys[inst >> 1];
-        const int k = (px - qx) + 2 * (py - qy);
+    for (int s = 0; s < S; s++) zs[s] = 0.f;
+    for (;;) {
+      const bool drain = wcnt > 0 && (have ? (wcnt + tot > kOrdJobsW) : (k == m || flush_each));
+      if (drain) {
+        waveSync();
+        for (int j = lane; j < wcnt; j += 64) {
+          const int slot = jb + j;
+          if (!hasC) break;
+          const int id = (int)job_id[slot];
+          const uint32_t mt = job_meta[slot];
+          const int jp = (int)(mt & 255u), inst = (int)((mt >> 8) & 3u), shade = (int)((mt >> 14) & 7u);
+          const int jx = tx0 + (jp & (kTile - 1)), jy = ty0 + jp / kTile;
+          const PrimSetup ps = P.prims[id];
+          const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
+          const float fx = (float)jx, fy = (float)jy;
+          float4 b;
+          if (S == 4 && shade < 4)
+            bary(T, sampleX(shade) + fx, sampleY(shade) + fy, b);
+          else
+            bary(T, fx + 0.5f, fy + 0.5f, b);
+          const RGBAf c = shadeAt<S, SHADING>(P, ps, T, jx, jy, b, inst, need_deriv);
+          job_col[slot] = make_float4(gclamp(c.r, 0.f, 1.f), gclamp(c.g, 0.f, 1.f), gclamp(c.b, 0.f, 1.f),
+                                      gclamp(c.a, 0.f, 1.f));
+          nfs++;
+        }
+        waveSync();
+        /* per-sample operations of this lane's fragments, in order (RendererSoft.cpp:358-407) */
+        for (int j = head; j >= 0; j = job_next[j]) {
+          const int smask = (int)((job_meta[j] >> 10) & 15u);
+          const float4 c = job_col[j];
 #pragma unroll
-        for (int j = 0; j < 3; j++) {
-          if (j == k) {
-            fr.tc[j] = {vv[0], vv[1]};
+          for (int s = 0; s < S; s++) {
+            if (!((smask >> s) & 1)) continue;
+            if (dtest) {
+              const float z = job_z[j * S + s];
+              if (!depthPass(z, dep[s], dfunc)) continue;
+              if (P.st.depthMask) dep[s] = z;
+            }
+            if (!hasC) continue;
+            RGBAf o = {c.x, c.y, c.z, c.w};
+            if (P.st.blend) o = blendColor(o, unpack01(col[s]), P.st);
+            col[s] = pack4(u8t(o.r * 255.f), u8t(o.g * 255.f), u8t(o.b * 255.f), u8t(o.a * 255.f));
+            nwr++;
+          }
+        }
+        head = tail = -1;
+        wcnt = 0;
+        waveSync();
+      }
+      if (have) {
+        /* append this lane's fragments of triangle pend_id */
+        const int base = jb + wcnt + qn - nI;
+        int q = 0;
+        for (int iy = 0; iy < 2; iy++)
+          for (int ix = 0; ix < 2; ix++) {
+            if (q >= nI || !((meta >> (20 + iy * 2 + ix)) & 1u)) continue;
+            const int slot = base + q++;
+            job_id[slot] = (uint32_t)pend_id;
+            job_meta[slot] = (meta & 0x1ffffu) | ((uint32_t)(iy * 2 + ix) << 8);
+            job_next[slot] = -1;
+#pragma unroll
+            for (int s = 0; s < S; s++) job_z[slot * S + s] = zs[s];
+            if (tail >= 0)
+              job_next[tail] = (short)slot;
+            else
+              head = slot;
+            tail = slot;
+          }
+        wcnt += tot;
+        have = false;
+        k++;
+        continue;
+      }
+      if (k == m) break;
+      const int id = (int)win[k];
+      const PrimSetup ps = P.prims[id];
+      if (ps.qx1 < wx0 || ps.qx0 > wx0 + kSub - 1 || ps.qy1 < wy0 || ps.qy0 > wy0 + kSub - 1) {
+        k++;
+        continue;
+      }
+      nI = 0;
+      meta = 0;
+      if (inFb) {
+        int qa, qb;
+        const int nqx = axisQuads(ps.min_x, ps.max_x, ps.bcx, px, qa, qb);
+        const int nqy = nqx ? axisQuads(ps.min_y, ps.max_y, ps.bcy, py, qa, qb) : 0;
+        if (nqx && nqy) {
+          const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
+          const float fx = (float)px, fy = (float)py;
+          /* coverage, z and the depth clip (RendererSoft.cpp:776-799) */
+          bool sin[5];
+          float4 sbc[5];
+          float sz[5];
+          int shade = 0;
+          bool pin;
+          if (S == 4) {
+#pragma unroll
+            for (int s = 0; s < 4; s++) sin[s] = bary(T, lx[s] + fx, ly[s] + fy, sbc[s]);
+            sin[4] = bary(T, fx + 0.5f, fy + 0.5f, sbc[4]);
+            pin = sin[0] || sin[1] || sin[2] || sin[3];
+            shade = 4;
+            if (!sin[4]) {
+#pragma unroll
+              for (int s = 3; s >= 0; s--)
+                if (sin[s]) shade = s;
+            }
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+              sz[s] = 0.f;
+              if (!sin[s]) continue;
+              sz[s] = dpps3(sbc[s], T.z);
+              if (sz[s] < 0.f || sz[s] > 1.f) sin[s] = false;
+            }
           } else {
-            const float4 pb = shadingBc<S>(T, (float)(qx + (j & 1)), (float)(qy + (j >> 1)));
-            fr.tc[j] = {interp(pb, va[0], vb[0], vc[0]), interp(pb, va[1], vb[1], vc[1])};
+            sin[0] = bary(T, fx + 0.5f, fy + 0.5f, sbc[0]);
+            pin = sin[0];
+            sz[0] = pin ? dpps3(sbc[0], T.z) : 0.f;
+          }
+          /* early-Z (RendererSoft.cpp:853-878): test without writing */
+          if (pin && P.st.depthTest) {
+            if (S == 4) {
+              bool any = false;
+#pragma unroll
+              for (int s = 0; s < 4; s++) {
+                if (!sin[s]) continue;
+                if (hasD) sin[s] = depthPass(gclamp(sz[s], 0.f, 1.f), dep[s], dfunc);
+                any = any || sin[s];
+              }
+              pin = any;
+            } else {
+              pin = hasD ? depthPass(gclamp(sz[0], 0.f, 1.f), dep[0], dfunc) : true;
+            }
+          }
+          if (pin) {
+            uint32_t smask = 0;
+            if (S == 4) {
+#pragma unroll
+              for (int s = 0; s < 4; s++) smask |= (sin[s] ? 1u : 0u) << s;
+            } else {
+              smask = 1u;
+            }
+#pragma unroll
+            for (int s = 0; s < S; s++) zs[s] = gclamp(sz[s], 0.f, 1.f);
+            uint32_t qm = 0;
+            for (int iy = 0; iy < nqy; iy++)
+              for (int ix = 0; ix < nqx; ix++) qm |= 1u << (20 + iy * 2 + ix);
+            nI = nqx * nqy;
+            meta = (uint32_t)pl | (smask << 10) | ((uint32_t)shade << 14) | qm;
           }
         }
       }
-      c = runFS<SHADING>(P, fr);
-      c = {gclamp(c.r, 0.f, 1.f), gclamp(c.g, 0.f, 1.f), gclamp(c.b, 0.f, 1.f), gclamp(c.a, 0.f, 1.f)};
-      nfs++;
-    }
-    const uint32_t packed = pack4(u8t(c.r * 255.f), u8t(c.g * 255.f), u8t(c.b * 255.f), u8t(c.a * 255.f));
-#pragma unroll
-    for (int t = 0; t < S; t++) {
-      if (win[t] != w) continue;
-      dep[t] = gclamp(dpps3(sbc[S == 4 ? t : 0], T.z), 0.f, 1.f);
-      if (hasC) {
-        col[t] = packed;
-        nwr++;
+      qn = waveInclusiveScan(nI);
+      tot = __shfl(qn, 63, 64);
+      if (tot == 0) {
+        k++;
+        continue;
       }
+      pend_id = id;
+      have = true;
     }
+    __syncthreads();
+    if (lp == 0) s_n = 0;
+    __syncthreads();
+    if (hi >= maxid) break;
+    lo = hi + 1;
   }
-  for (int s = 0; s < S; s++) P.depth[pix * S + s] = dep[s];
-  if (hasC) {
-    for (int s = 0; s < S; s++) P.color[pix * S + s] = col[s];
-    if (S == 4) {
-      int r[4];
-#pragma unroll
-      for (int ch = 0; ch < 4; ch++) r[ch] = (chan(col[0], ch) + chan(col[1], ch) + chan(col[2], ch) + chan(col[3], ch)) >> 2;
-      P.color_resolved[pix] = pack4(r[0], r[1], r[2], r[3]);
+
+  if (inFb) {
+    if (hasD)
+      for (int s = 0; s < S; s++) P.depth[pix * S + s] = dep[s];
+    if (hasC) {
+      for (int s = 0; s < S; s++) P.color[pix * S + s] = col[s];
+      if (S == 4) P.color_resolved[pix] = resolve4(col);
     }
   }
   if (nfs) atomicAdd(&P.counters[0], nfs);
   if (nwr) atomicAdd(&P.counters[1], nwr);
+  if (nculled) atomicAdd(&P.counters[6], nculled);
 }
 
 /* --------------------------------------------------------------------------- textures */

@@ -277,8 +277,8 @@ __device__ __forceinline__ float4 toScreen(const DrawParams &P, float4 c) {
 /* The quad context a fragment shader needs: its own varyings and the texture
  * coordinates of quad pixels 0, 1, 2 for the coarse derivative. */
 struct Frag {
-  const float *v; /* own interpolated varyings (indexed like the VS output) */
-  f2 tc[3];       /* v_texCoord of quad pixels p0, p1, p2 */
+  float v[kVaryStride]; /* own interpolated varyings (indexed like the VS output) */
+  f2 tc[3];             /* v_texCoord of quad pixels p0, p1, p2 */
 };
 
 /* ShaderSoft::getSampler2DLod (ShaderSoft.h:117-133), used only when the sampler filters with mipmaps */
@@ -325,7 +325,7 @@ __device__ __forceinline__ f3 normalFromMap(const DrawParams &P, const Frag &f, 
 }
 
 /* BlinnPhongSoft.h:165-189 */
-__device__ float shadowCalc(const DrawParams &P, float4 sp, f3 n, f3 lightDir) {
+__device__ __forceinline__ float shadowCalc(const DrawParams &P, float4 sp, f3 n, f3 lightDir) {
   f3 pc{sp.x / sp.w, sp.y / sp.w, sp.z / sp.w};
   float cur = pc.z;
   if (cur < 0.f || cur > 1.f) return 0.0f;
@@ -346,7 +346,7 @@ __device__ float shadowCalc(const DrawParams &P, float4 sp, f3 n, f3 lightDir) {
 }
 
 /* BlinnPhongSoft.h:191-242; `directional` = constant light vector, no range attenuation */
-__device__ RGBAf fsBlinnPhong(const DrawParams &P, const Frag &f, bool directional) {
+__device__ __forceinline__ RGBAf fsBlinnPhong(const DrawParams &P, const Frag &f, bool directional) {
   const float *v = f.v;
   const Uniforms &u = P.u;
   f2 uv{v[0], v[1]};
@@ -451,7 +451,7 @@ __device__ __forceinline__ PbrInputs pbrInputs(const DrawParams &P, const Frag &
 }
 
 /* PbrSoft.h:225-322 */
-__device__ RGBAf fsPbr(const DrawParams &P, const Frag &f) {
+__device__ __forceinline__ RGBAf fsPbr(const DrawParams &P, const Frag &f) {
   const float *v = f.v;
   const Uniforms &u = P.u;
   PbrInputs in = pbrInputs(P, f);

@@ -457,6 +457,31 @@ def random_triangles(n, width, height, seed, area_lo=0.25, area_hi=16.0, z_lo=0.
     return v, np.arange(m * 3, dtype=np.int32).reshape(-1, 3)
 
 
+def scene_c3(width=3840, height=2160, n=2_000_000, big=2000, seed=5, msaa=True, read_samples=False):
+    """C3 binning/raster stress: `n` random triangles (screen area log-uniform in 0.25-16 px, centres uniform,
+    NDC z uniform in [0, 1]) and `big` screen-covering triangles at random depths after them; per-vertex RGBA32F
+    colours, depth LESS, no culling, 3840x2160 MSAA-4x."""
+    v, i = random_triangles(n, width, height, seed, 0.25, 16.0, 0.0, 1.0, big=big)
+    st = S.Stream()
+    st.texture(2, width, height, usage=S.USAGE_COLOR, msaa=msaa)
+    st.tex_init(2)
+    st.texture(3, width, height, format=S.FMT_FLOAT32, usage=S.USAGE_DEPTH, msaa=msaa)
+    st.tex_init(3)
+    st.fbo(1, color=2, depth=3)
+    st.vao(1, v, i)
+    st.frame()
+    st.begin(1, True, True, (0, 0, 0, 0), 1.0)
+    st.viewport(0, 0, width, height)
+    rs = S.RenderStates(depth_test=True, depth_func=S.DEPTH_LESS)
+    st.draw(1, S.SHADING_VERTEX_COLOR, 0, rs, S.uniforms_model(np.eye(4), np.eye(4)), bytes(64), bytes(48))
+    st.end()
+    st.read(2)
+    st.read(3, which=1)
+    if read_samples:
+        st.read(2, which=1)
+    return Scene("c3", st, width, height, msaa, {"triangles": n + big})
+
+
 VARIANTS = ["blend_alpha", "blend_add_one", "blend_minmax", "blend_sub", "depth_lequal", "depth_gequal_rz",
             "depth_always_nomask", "depth_equal", "depth_notequal", "depth_never", "cull_off", "near_clip",
             "side_clip", "filter_nearest", "filter_linear", "filter_nn", "filter_ln", "filter_nl", "wrap_mirror",

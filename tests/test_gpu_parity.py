@@ -108,3 +108,18 @@ def test_random_triangles_with_screen_covering(renderer):
     st.read(2, which=1)
     sc = scenes.Scene("rand", st, w, h, True, {})
     assert_identical(_hip(renderer, sc), _oracle(sc), "random")
+
+
+def test_c3_shape_reduced_with_hierarchical_z(renderer):
+    """The C3 workload's shape at 640x360 (random small triangles, then screen-covering ones at random depths,
+    per-vertex colour, MSAA-4x, LESS, no culling): identical to the checker, and the per-tile hierarchical-Z
+    test skips most (tile, triangle) pairs."""
+    sc = scenes.scene_c3(640, 360, 40_000, 64, seed=7, read_samples=True)
+    print("[gpu] c3 reduced 640x360", flush=True)
+    renderer.resetStats()
+    got = _hip(renderer, sc)
+    st = renderer.stats()
+    out, _ = oracle.cpu_ref_replay(sc.stream.bytes(), sc.stream.output_bytes(), threads=min(16, os.cpu_count()),
+                                   exact_math=True)
+    assert_identical(got, sc.stream.split_outputs(out), "c3_reduced")
+    assert st["hiz_culled"] > st["tile_pairs"] // 2, st
