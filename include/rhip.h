@@ -94,6 +94,12 @@ typedef struct rhip_stats {
   long long hiz_culled; /* (tile, triangle) pairs skipped by the per-tile hierarchical-Z test */
 } rhip_stats;
 
+/* Error of a shader against the FP64 evaluation of the plain PBR shader, per colour channel, in units of 1/255. */
+typedef struct rhip_nn_error {
+  double mean, p99, max;
+  long long samples; /* fragments */
+} rhip_nn_error;
+
 /* ---------------------------------------------------------------- device (Renderer::create/destroy) */
 int rhip_device_create(int hip_device, rhip_device *out);              /* Renderer::create  (Renderer.h:27) */
 int rhip_device_destroy(rhip_device dev);                              /* Renderer::destroy (Renderer.h:28) */
@@ -142,6 +148,26 @@ int rhip_end_render_pass(rhip_device dev);                                      
 int rhip_wait_idle(rhip_device dev);                                                  /* waitIdle (Renderer.h:58) */
 
 /* ---------------------------------------------------------------- measurement */
+/* ---------------------------------------------------------------- learned PBR shader
+ * No counterpart in SoftGLRender: a 24-64-64-3 MLP on BF16 MFMA that replaces
+ * the image-based PBR shader (PbrSoft.h:225-322, point light off) for programs
+ * created with RHS_SHADING_NEURAL_PBR. Records are fragments captured from
+ * plain PBR draws (shading inputs, plain GPU colour, FP64 plain colour). */
+int rhip_neural_param_count(void);                                      /* 5955 */
+int rhip_neural_init(rhip_device dev, unsigned int seed);               /* fresh network */
+int rhip_neural_set_weights(rhip_device dev, const float *w, int n);    /* W1 (64x24), b1, W2 (64x64), b2, W3 (3x64), b3 */
+int rhip_neural_get_weights(rhip_device dev, float *w, int n);
+int rhip_neural_records(rhip_device dev, long long capacity);          /* (re)allocates and empties the record buffer */
+int rhip_neural_capture(rhip_device dev, int enable, int shift, int material); /* 1 in 2^shift fragments of PBR draws */
+int rhip_neural_record_count(rhip_device dev, long long *count);
+int rhip_neural_read_records(rhip_device dev, long long first, long long count, float *out);   /* 24 floats each */
+int rhip_neural_write_records(rhip_device dev, long long first, long long count, const float *in);
+int rhip_neural_gradient(rhip_device dev, long long first, int count, float *grad, double *loss); /* summed over records */
+int rhip_neural_train(rhip_device dev, long long first, long long count, int steps, int batch, float lr_start,
+                      float lr_end, unsigned int seed, double *losses /* steps entries, may be NULL */);
+int rhip_neural_evaluate(rhip_device dev, long long first, long long count, int material, rhip_nn_error *network,
+                         rhip_nn_error *plain, float *outputs /* count x 3, may be NULL */);
+
 int rhip_stats_get(rhip_device dev, rhip_stats *out); /* waits for queued work */
 int rhip_stats_reset(rhip_device dev);
 /* The HIP stream the device queues on (hipStream_t), for event timing. */

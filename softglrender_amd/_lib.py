@@ -57,6 +57,11 @@ class Stats(ctypes.Structure):
                                                   "hiz_culled")]
 
 
+class NnError(ctypes.Structure):
+    _fields_ = [("mean", ctypes.c_double), ("p99", ctypes.c_double), ("max", ctypes.c_double),
+                ("samples", ctypes.c_longlong)]
+
+
 # every exported symbol of rhip.h, with its argument types
 SIGNATURES = {
     "rhip_device_create": [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)],
@@ -91,6 +96,21 @@ SIGNATURES = {
     "rhip_profile_enable": [ctypes.c_void_p, ctypes.c_int],
     "rhip_profile_query": [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)],
+    "rhip_neural_param_count": [],
+    "rhip_neural_init": [ctypes.c_void_p, ctypes.c_uint],
+    "rhip_neural_set_weights": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int],
+    "rhip_neural_get_weights": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int],
+    "rhip_neural_records": [ctypes.c_void_p, ctypes.c_longlong],
+    "rhip_neural_capture": [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "rhip_neural_record_count": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)],
+    "rhip_neural_read_records": [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_void_p],
+    "rhip_neural_write_records": [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_void_p],
+    "rhip_neural_gradient": [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int, ctypes.c_void_p,
+                             ctypes.POINTER(ctypes.c_double)],
+    "rhip_neural_train": [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
+                          ctypes.c_float, ctypes.c_float, ctypes.c_uint, ctypes.c_void_p],
+    "rhip_neural_evaluate": [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_int,
+                             ctypes.POINTER(NnError), ctypes.POINTER(NnError), ctypes.c_void_p],
 }
 
 _lib = None

@@ -18,7 +18,7 @@
 #include <vector>
 
 #include "rhip.h"
-#include "rhip_kernels.h"
+#include "rhip_neural.h"
 
 using namespace rhip;
 
@@ -134,6 +134,21 @@ struct rhip_device_s {
   std::vector<Timed> timed;
   std::vector<hipEvent_t> event_pool;
   rhip_stats host_stats{};
+  /* learned shader: FP32 parameters, Adam moments, gradient, packed bf16 operands, captured records */
+  struct Neural {
+    float *params = nullptr, *grad = nullptr, *m1 = nullptr, *m2 = nullptr;
+    NnPacked *packed = nullptr;
+    double *scratch = nullptr; /* losses and evaluation sums */
+    unsigned int *hist = nullptr;
+    int step = 0;
+    bool ready = false;
+    DevBuf records;
+    long long rec_cap = 0;
+    unsigned long long *rec_count = nullptr;
+    bool capturing = false;
+    int shift = 0, material = 0;
+    uint32_t salt = 0;
+  } nn;
 };
 
 namespace {
@@ -390,6 +405,9 @@ int rhip_device_destroy(rhip_device d) {
   d->tex.clear();
   d->vao.clear();
   if (d->counters) hipFree(d->counters);
+  for (void *p : {(void *)d->nn.params, (void *)d->nn.grad, (void *)d->nn.m1, (void *)d->nn.m2, (void *)d->nn.packed,
+                  (void *)d->nn.scratch, (void *)d->nn.hist, (void *)d->nn.rec_count})
+    if (p) hipFree(p);
   hipStreamDestroy(d->stream);
   delete d;
   return RHIP_OK;
@@ -617,7 +635,7 @@ int rhip_fbo_destroy(rhip_device d, int id) {
 
 int rhip_program_create(rhip_device d, int shading, uint32_t defines, int *out) {
   if (!d || !out) return fail(RHIP_E_INVALID, "null argument");
-  if (shading < RHS_SHADING_BASIC || shading > RHS_SHADING_VERTEX_COLOR)
+  if (shading < RHS_SHADING_BASIC || shading > RHS_SHADING_NEURAL_PBR)
     return fail(RHIP_E_UNSUPPORTED, "shading model %d", shading);
   if (defines & ~31u) return fail(RHIP_E_INVALID, "unknown define bits 0x%x", defines);
   int id = d->next_id++;
@@ -705,6 +723,17 @@ int rhip_draw(rhip_device d, const rhip_draw_desc *dd) {
   if (!ct && !dt) return RHIP_OK;
   if (d->vp_w <= 0 || d->vp_h <= 0) return fail(RHIP_E_INVALID, "viewport not set");
 
+  if (pg.shading == RHS_SHADING_NEURAL_PBR) {
+    const bool two = !st.blend && dt && st.depthTest && st.depthMask &&
+                     (st.depthFunc == 1 || st.depthFunc == 3 || st.depthFunc == 4 || st.depthFunc == 6);
+    uint32_t enable_light = 0;
+    if (dd->uniform_material) memcpy(&enable_light, dd->uniform_material, 4);
+    if (!d->nn.ready) return fail(RHIP_E_INVALID, "learned shader: no network (rhip_neural_init or set_weights)");
+    if (!two || enable_light || (pg.defines & (RHS_DEF_AO_MAP | RHS_DEF_EMISSIVE_MAP)))
+      return fail(RHIP_E_UNSUPPORTED,
+                  "learned shader: image-based lighting only (no point light, AO or emissive map), opaque "
+                  "draws that write depth with LESS/LEQUAL/GREATER/GEQUAL");
+  }
   DrawParams P;
   memset(&P, 0, sizeof P);
   decodeUniforms(*dd, P.u);
@@ -828,12 +857,29 @@ int rhip_draw(rhip_device d, const rhip_draw_desc *dd) {
     }
     if (P.color) {
       const int grid = (int)std::min<size_t>(blocks(njobs, 256), 4096);
-      {
+      if (pg.shading == RHS_SHADING_NEURAL_PBR) {
+        Timer tm(d, samples == 4 ? "k_shade_neural_msaa4" : "k_shade_neural");
+        if (samples == 4)
+          RHIP_LAUNCH(k_shade_neural<4>, grid, 256, s, P, need_deriv ? 1 : 0, d->nn.packed);
+        else
+          RHIP_LAUNCH(k_shade_neural<1>, grid, 256, s, P, need_deriv ? 1 : 0, d->nn.packed);
+      } else {
         Timer tm(d, samples == 4 ? "k_shade_msaa4" : "k_shade");
         if (samples == 4)
           launchShade<4>(P, grid, need_deriv ? 1 : 0, s);
         else
           launchShade<1>(P, grid, need_deriv ? 1 : 0, s);
+      }
+      if (d->nn.capturing && pg.shading == RHS_SHADING_PBR) {
+        Timer tm(d, "k_capture");
+        d->nn.salt = d->nn.salt * 1664525u + 1013904223u;
+        float *rec = d->nn.records.as<float>();
+        if (samples == 4)
+          RHIP_LAUNCH(k_capture<4>, grid, 256, s, P, need_deriv ? 1 : 0, rec, d->nn.rec_cap, d->nn.rec_count,
+                      d->nn.shift, d->nn.salt, d->nn.material);
+        else
+          RHIP_LAUNCH(k_capture<1>, grid, 256, s, P, need_deriv ? 1 : 0, rec, d->nn.rec_cap, d->nn.rec_count,
+                      d->nn.shift, d->nn.salt, d->nn.material);
       }
       if (samples == 4) {
         Timer tm(d, "k_resolve");
@@ -883,6 +929,235 @@ int rhip_profile_query(rhip_device d, int index, char *name, int name_cap, doubl
   if (name && name_cap > 0) snprintf(name, name_cap, "%s", agg[index].first.c_str());
   if (total_ms) *total_ms = agg[index].second.first;
   if (count) *count = agg[index].second.second;
+  return RHIP_OK;
+}
+
+/* ------------------------------------------------------------------ learned shader */
+
+static int nnEnsure(rhip_device d) {
+  auto &n = d->nn;
+  if (n.params) return RHIP_OK;
+  const size_t pb = kNnParams * sizeof(float);
+  HIPCHK(hipMalloc(&n.params, pb));
+  HIPCHK(hipMalloc(&n.grad, pb));
+  HIPCHK(hipMalloc(&n.m1, pb));
+  HIPCHK(hipMalloc(&n.m2, pb));
+  HIPCHK(hipMalloc(&n.packed, sizeof(NnPacked)));
+  HIPCHK(hipMalloc(&n.scratch, 8 * sizeof(double)));
+  HIPCHK(hipMalloc(&n.hist, 2 * kNnHistBins * sizeof(unsigned int)));
+  HIPCHK(hipMalloc(&n.rec_count, sizeof(unsigned long long)));
+  HIPCHK(hipMemsetAsync(n.rec_count, 0, sizeof(unsigned long long), d->stream));
+  return RHIP_OK;
+}
+
+/* parameters in, Adam state and gradient cleared, operands repacked */
+static int nnLoad(rhip_device d, const float *w) {
+  auto &n = d->nn;
+  const size_t pb = kNnParams * sizeof(float);
+  HIPCHK(hipMemcpyAsync(n.params, w, pb, hipMemcpyHostToDevice, d->stream));
+  HIPCHK(hipMemsetAsync(n.m1, 0, pb, d->stream));
+  HIPCHK(hipMemsetAsync(n.m2, 0, pb, d->stream));
+  RHIP_LAUNCH(k_nn_adam, 1, 1024, d->stream, n.params, n.grad, n.m1, n.m2, n.packed, 0.f, 0.9f, 0.999f, 1e-8f, 0.f,
+              0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(d->stream));
+  n.step = 0;
+  n.ready = true;
+  return RHIP_OK;
+}
+
+int rhip_neural_param_count(void) { return kNnParams; }
+
+int rhip_neural_init(rhip_device d, unsigned int seed) {
+  int rc = checkDev(d);
+  if (rc || (rc = nnEnsure(d))) return rc;
+  /* uniform in +-sqrt(6 / fan_in) (fan_in 24, 64, 64), biases 0; a fixed LCG so runs repeat */
+  std::vector<float> w(kNnParams, 0.f);
+  uint64_t st = 0x9e3779b97f4a7c15ull ^ seed;
+  auto uni = [&]() {
+    st = st * 6364136223846793005ull + 1442695040888963407ull;
+    return (double)(st >> 11) * (1.0 / 9007199254740992.0);
+  };
+  auto fill = [&](int off, int cnt, int fan) {
+    const double a = std::sqrt(6.0 / fan);
+    for (int i = 0; i < cnt; i++) w[off + i] = (float)((2 * uni() - 1) * a);
+  };
+  fill(kOffW1, kNnHid * kNnIn, kNnIn);
+  fill(kOffW2, kNnHid * kNnHid, kNnHid);
+  fill(kOffW3, kNnOut * kNnHid, kNnHid);
+  return nnLoad(d, w.data());
+}
+
+int rhip_neural_set_weights(rhip_device d, const float *w, int n) {
+  int rc = checkDev(d);
+  if (rc || (rc = nnEnsure(d))) return rc;
+  if (!w || n != kNnParams) return fail(RHIP_E_INVALID, "learned shader: %d parameters expected", kNnParams);
+  return nnLoad(d, w);
+}
+
+int rhip_neural_get_weights(rhip_device d, float *w, int n) {
+  int rc = checkDev(d);
+  if (rc) return rc;
+  if (!d->nn.ready) return fail(RHIP_E_INVALID, "learned shader: no network");
+  if (!w || n != kNnParams) return fail(RHIP_E_INVALID, "learned shader: %d parameters expected", kNnParams);
+  HIPCHK(hipMemcpyAsync(w, d->nn.params, kNnParams * sizeof(float), hipMemcpyDeviceToHost, d->stream));
+  HIPCHK(hipStreamSynchronize(d->stream));
+  return RHIP_OK;
+}
+
+int rhip_neural_records(rhip_device d, long long capacity) {
+  int rc = checkDev(d);
+  if (rc || (rc = nnEnsure(d))) return rc;
+  if (capacity < 0) return fail(RHIP_E_INVALID, "negative capacity");
+  HIPCHK(hipStreamSynchronize(d->stream));
+  if ((rc = d->nn.records.ensure((size_t)std::max<long long>(capacity, 1) * kRecFloats * 4))) return rc;
+  d->nn.rec_cap = capacity;
+  HIPCHK(hipMemsetAsync(d->nn.rec_count, 0, sizeof(unsigned long long), d->stream));
+  return RHIP_OK;
+}
+
+int rhip_neural_capture(rhip_device d, int enable, int shift, int material) {
+  int rc = checkDev(d);
+  if (rc || (rc = nnEnsure(d))) return rc;
+  if (enable && d->nn.rec_cap <= 0) return fail(RHIP_E_INVALID, "learned shader: no record buffer");
+  if (shift < 0 || shift > 20) return fail(RHIP_E_INVALID, "sampling shift out of range");
+  d->nn.capturing = enable != 0;
+  d->nn.shift = shift;
+  d->nn.material = material;
+  return RHIP_OK;
+}
+
+int rhip_neural_record_count(rhip_device d, long long *count) {
+  int rc = checkDev(d);
+  if (rc || (rc = nnEnsure(d))) return rc;
+  unsigned long long c = 0;
+  HIPCHK(hipMemcpyAsync(&c, d->nn.rec_count, sizeof c, hipMemcpyDeviceToHost, d->stream));
+  HIPCHK(hipStreamSynchronize(d->stream));
+  *count = std::min<long long>((long long)c, d->nn.rec_cap);
+  return RHIP_OK;
+}
+
+int rhip_neural_read_records(rhip_device d, long long first, long long count, float *out) {
+  int rc = checkDev(d);
+  if (rc) return rc;
+  if (first < 0 || count < 0 || first + count > d->nn.rec_cap || !out)
+    return fail(RHIP_E_INVALID, "record range outside the buffer");
+  HIPCHK(hipMemcpyAsync(out, d->nn.records.as<float>() + (size_t)first * kRecFloats,
+                        (size_t)count * kRecFloats * 4, hipMemcpyDeviceToHost, d->stream));
+  HIPCHK(hipStreamSynchronize(d->stream));
+  return RHIP_OK;
+}
+
+int rhip_neural_write_records(rhip_device d, long long first, long long count, const float *in) {
+  int rc = checkDev(d);
+  if (rc) return rc;
+  if (first < 0 || count < 0 || first + count > d->nn.rec_cap || !in)
+    return fail(RHIP_E_INVALID, "record range outside the buffer");
+  HIPCHK(hipMemcpyAsync(d->nn.records.as<float>() + (size_t)first * kRecFloats, in, (size_t)count * kRecFloats * 4,
+                        hipMemcpyHostToDevice, d->stream));
+  HIPCHK(hipStreamSynchronize(d->stream));
+  return RHIP_OK;
+}
+
+static int nnRange(rhip_device d, long long first, long long count) {
+  if (!d->nn.ready) return fail(RHIP_E_INVALID, "learned shader: no network");
+  if (first < 0 || count <= 0 || first + count > d->nn.rec_cap) return fail(RHIP_E_INVALID, "record range invalid");
+  return RHIP_OK;
+}
+
+int rhip_neural_gradient(rhip_device d, long long first, int count, float *grad, double *loss) {
+  int rc = checkDev(d);
+  if (rc || (rc = nnRange(d, first, count))) return rc;
+  auto &n = d->nn;
+  HIPCHK(hipMemsetAsync(n.grad, 0, kNnParams * sizeof(float), d->stream));
+  HIPCHK(hipMemsetAsync(n.scratch, 0, sizeof(double), d->stream));
+  RHIP_LAUNCH(k_nn_grad, (unsigned)std::min<int>((count + 63) / 64, 1024), 64, d->stream, n.packed,
+              n.records.as<float>(), first, 0ll, count, 0u, n.grad, n.scratch);
+  HIPCHK(hipGetLastError());
+  if (grad) HIPCHK(hipMemcpyAsync(grad, n.grad, kNnParams * sizeof(float), hipMemcpyDeviceToHost, d->stream));
+  if (loss) HIPCHK(hipMemcpyAsync(loss, n.scratch, sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  HIPCHK(hipMemsetAsync(n.grad, 0, kNnParams * sizeof(float), d->stream));
+  HIPCHK(hipStreamSynchronize(d->stream));
+  return RHIP_OK;
+}
+
+int rhip_neural_train(rhip_device d, long long first, long long count, int steps, int batch, float lr_start,
+                      float lr_end, unsigned int seed, double *losses) {
+  int rc = checkDev(d);
+  if (rc || (rc = nnRange(d, first, count))) return rc;
+  if (steps < 0 || batch < 64 || count >= (1ll << 32)) return fail(RHIP_E_INVALID, "training arguments invalid");
+  auto &n = d->nn;
+  std::vector<double> lh;
+  double *dl = nullptr;
+  if (losses && steps > 0) {
+    HIPCHK(hipMalloc(&dl, steps * sizeof(double)));
+    HIPCHK(hipMemsetAsync(dl, 0, steps * sizeof(double), d->stream));
+  }
+  const unsigned grid = (unsigned)std::min<int>((batch + 63) / 64, 512);
+  for (int i = 0; i < steps; i++) {
+    /* cosine schedule from lr_start to lr_end */
+    const double t = steps > 1 ? (double)i / (steps - 1) : 0.0;
+    const float lr = (float)(lr_end + 0.5 * (lr_start - lr_end) * (1.0 + std::cos(3.14159265358979 * t)));
+    const uint32_t sd = seed * 0x9e3779b9u + (uint32_t)(n.step + 1) * 0x85ebca6bu;
+    RHIP_LAUNCH(k_nn_grad, grid, 64, d->stream, n.packed, n.records.as<float>(), first, count, batch, sd, n.grad,
+                dl ? dl + i : n.scratch);
+    n.step++;
+    RHIP_LAUNCH(k_nn_adam, 1, 1024, d->stream, n.params, n.grad, n.m1, n.m2, n.packed, lr, 0.9f, 0.999f, 1e-8f,
+                1.f / batch, n.step);
+  }
+  HIPCHK(hipGetLastError());
+  if (dl) {
+    lh.resize(steps);
+    HIPCHK(hipMemcpyAsync(lh.data(), dl, steps * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  }
+  HIPCHK(hipStreamSynchronize(d->stream));
+  if (dl) {
+    hipFree(dl);
+    for (int i = 0; i < steps; i++) losses[i] = lh[i] / ((double)batch * kNnOut);
+  }
+  return RHIP_OK;
+}
+
+static void histStats(const std::vector<unsigned int> &h, double sum, double mx, double n, rhip_nn_error *e) {
+  e->samples = (long long)n;
+  e->mean = n > 0 ? sum / (n * kNnOut) : 0.0;
+  e->max = mx;
+  /* p99 over channel values: upper edge of the bin holding the 99th percentile */
+  const double total = n * kNnOut, want = 0.99 * total;
+  double acc = 0;
+  e->p99 = 0;
+  for (int b = 0; b < kNnHistBins; b++) {
+    acc += h[b];
+    if (acc >= want) {
+      e->p99 = b == kNnHistBins - 1 ? mx : (b + 1) / 16.0;
+      break;
+    }
+  }
+}
+
+int rhip_neural_evaluate(rhip_device d, long long first, long long count, int material, rhip_nn_error *nn_err,
+                         rhip_nn_error *plain_err, float *outputs) {
+  int rc = checkDev(d);
+  if (rc || (rc = nnRange(d, first, count))) return rc;
+  if (count > INT32_MAX) return fail(RHIP_E_INVALID, "too many records");
+  auto &n = d->nn;
+  float *dout = nullptr;
+  if (outputs) HIPCHK(hipMalloc(&dout, (size_t)count * 3 * sizeof(float)));
+  HIPCHK(hipMemsetAsync(n.hist, 0, 2 * kNnHistBins * sizeof(unsigned int), d->stream));
+  HIPCHK(hipMemsetAsync(n.scratch, 0, 8 * sizeof(double), d->stream));
+  RHIP_LAUNCH(k_nn_eval, (unsigned)std::min<long long>((count + 255) / 256, 2048), 256, d->stream, n.packed,
+              n.records.as<float>(), first, (int)count, material, n.hist, n.hist + kNnHistBins, n.scratch, dout);
+  HIPCHK(hipGetLastError());
+  std::vector<unsigned int> h(2 * kNnHistBins);
+  double sums[8];
+  HIPCHK(hipMemcpyAsync(h.data(), n.hist, h.size() * 4, hipMemcpyDeviceToHost, d->stream));
+  HIPCHK(hipMemcpyAsync(sums, n.scratch, sizeof sums, hipMemcpyDeviceToHost, d->stream));
+  if (dout) HIPCHK(hipMemcpyAsync(outputs, dout, (size_t)count * 3 * sizeof(float), hipMemcpyDeviceToHost, d->stream));
+  HIPCHK(hipStreamSynchronize(d->stream));
+  if (dout) hipFree(dout);
+  std::vector<unsigned int> hn(h.begin(), h.begin() + kNnHistBins), hp(h.begin() + kNnHistBins, h.end());
+  if (nn_err) histStats(hn, sums[0], sums[2], sums[4], nn_err);
+  if (plain_err) histStats(hp, sums[1], sums[3], sums[4], plain_err);
   return RHIP_OK;
 }
 

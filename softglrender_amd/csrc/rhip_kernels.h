@@ -698,7 +698,8 @@ __device__ __forceinline__ bool varyUsed(const DrawParams &P, int j) {
   if (j >= 4 && j <= 6) return !nm;
   if (j >= 12 && j <= 14) return true;
   if (j >= 16 && j <= 18) return true;
-  if (SHADING == RHS_SHADING_PBR) return nm && ((j >= 20 && j <= 22) || (j >= 24 && j <= 26));
+  if (SHADING == RHS_SHADING_PBR || SHADING == RHS_SHADING_NEURAL_PBR)
+    return nm && ((j >= 20 && j <= 22) || (j >= 24 && j <= 26));
   if (j >= 20 && j <= 23) return P.u.enable_shadow != 0;
   return nm && ((j >= 24 && j <= 26) || (j >= 28 && j <= 30));
 }
@@ -707,7 +708,7 @@ __device__ __forceinline__ bool varyUsed(const DrawParams &P, int j) {
  * sample, `inst` which of the triangle's quads holding the pixel it was produced in
  * (low bit: x, high bit: y, in the reference's visiting order). RendererSoft.cpp:800-851. */
 template <int S, int SHADING>
-__device__ __forceinline__ RGBAf shadeAt(const DrawParams &P, const PrimSetup &ps, const Tri &T, int px, int py,
+__device__ __forceinline__ Frag makeFrag(const DrawParams &P, const PrimSetup &ps, const Tri &T, int px, int py,
                                          float4 b, int inst, int need_deriv) {
   const float w = dpps3(b, T.w);
   const float iw = 1.f / w;
@@ -736,7 +737,13 @@ __device__ __forceinline__ RGBAf shadeAt(const DrawParams &P, const PrimSetup &p
       }
     }
   }
-  return runFS<SHADING>(P, fr);
+  return fr;
+}
+
+template <int S, int SHADING>
+__device__ __forceinline__ RGBAf shadeAt(const DrawParams &P, const PrimSetup &ps, const Tri &T, int px, int py,
+                                         float4 b, int inst, int need_deriv) {
+  return runFS<SHADING>(P, makeFrag<S, SHADING>(P, ps, T, px, py, b, inst, need_deriv));
 }
 
 __device__ __forceinline__ uint32_t resolve4(const uint32_t *c) {

@@ -1,0 +1,752 @@
+/*
+ * rhip_neural.h -- the learned PBR shader (no counterpart in SoftGLRender).
+ *
+ * A 24-64-64-3 MLP (ReLU, linear RGB output) stands in for the image-based
+ * PBR shader (PbrSoft.h:225-322 with the point light off): from the normal,
+ * view direction, N.V, roughness, metallic, albedo and a 3-band encoding of
+ * the reflection vector it predicts the display colour, replacing the two
+ * cube-map lookups, the split-sum terms and the gamma curves. Weights are
+ * BF16 on MFMA (v_mfma_f32_16x16x32_bf16) with FP32 accumulation, for
+ * inference and for training; an FP32 master copy is updated with Adam. The
+ * plain shader evaluated in FP64 is the reference both for the training
+ * targets and for the error the network is judged by.
+ *
+ * Matrix convention: every MFMA operand is loaded as 8 consecutive bf16 of a
+ * row-major [row][k] matrix in LDS (lane l: row l%16, k (l/16)*8..+7). A
+ * layer computes H^T[unit][frag] = W[unit][k] . X[frag][k], so weights are
+ * the A operand and activations the B operand; the result tile gives lane l
+ * units (l/16)*4+v of fragment l%16.
+ */
+#pragma once
+#include "rhip_kernels.h"
+
+namespace rhip {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kNnIn = 24, kNnHid = 64, kNnOut = 3;
+constexpr int kNnK1 = 32; /* layer-1 K: 24 features, a constant 1 (bias column), zeros */
+constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kNnHid * kNnIn, kOffW2 = kOffB1 + kNnHid,
+              kOffB2 = kOffW2 + kNnHid * kNnHid, kOffW3 = kOffB2 + kNnHid, kOffB3 = kOffW3 + kNnOut * kNnHid;
+constexpr int kNnParams = kOffB3 + kNnOut; /* 5955 */
+
+/* bf16 operands packed for MFMA (device): W1 with b1 in column 24, W2, W3
+ * padded to 16 rows, and the transposes backward needs; biases of layers 2, 3. */
+struct NnPacked {
+  __bf16 w1[kNnHid * kNnK1];
+  __bf16 w2[kNnHid * kNnHid];
+  __bf16 w3[16 * kNnHid];
+  __bf16 w2t[kNnHid * kNnHid];
+  __bf16 w3t[kNnHid * 32];
+  float b2[kNnHid];
+  float b3[16];
+};
+
+/* A captured fragment: what the network reads, the plain GPU shader's colour
+ * and the FP64 plain-shader colour. */
+constexpr int kRecFloats = 24;
+enum {
+  kRecN = 0,       /* 3: shading normal */
+  kRecV = 3,       /* 3: view direction */
+  kRecAlbedo = 6,  /* 4: albedo texel as sampled (sRGB-encoded), alpha */
+  kRecRough = 10,
+  kRecMetal = 11,
+  kRecAo = 12,
+  kRecMaterial = 13,
+  kRecTarget = 14, /* 3: FP64 plain shader, clamped to [0, 1] */
+  kRecPlain = 17,  /* 3: the GPU plain shader, clamped to [0, 1] */
+};
+
+__device__ __forceinline__ f32x4 mma16(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+/* operand of tile (row0, k0) of a row-major bf16 matrix with `rs` elements per row */
+__device__ __forceinline__ bf16x8 ldOp(const __bf16 *m, int rs, int row0, int k0) {
+  const int l = threadIdx.x & 63;
+  return *(const bf16x8 *)(m + (row0 + (l & 15)) * rs + k0 + (l >> 4) * 8);
+}
+
+/* The 24 inputs (+ the bias 1) of one fragment. */
+__device__ __forceinline__ void nnFeatures(f3 N, f3 V, float rough, float metal, float ar, float ag, float ab,
+                                           float *f) {
+  const float nv = gmax(dot3(N, V), 0.f);
+  const float d2 = 2.f * dot3(N, V);
+  const f3 R{d2 * N.x - V.x, d2 * N.y - V.y, d2 * N.z - V.z};
+  f[0] = N.x, f[1] = N.y, f[2] = N.z;
+  f[3] = V.x, f[4] = V.y, f[5] = V.z;
+  f[6] = nv, f[7] = rough, f[8] = metal;
+  f[9] = ar, f[10] = ag, f[11] = ab;
+  f[12] = R.x, f[13] = R.y, f[14] = R.z;
+  const float pi = 3.14159265358979f;
+#pragma unroll
+  for (int b = 0; b < 3; b++) {
+    const float s = pi * (float)(1 << b);
+    f[15 + 3 * b] = sinf(s * R.x);
+    f[16 + 3 * b] = sinf(s * R.y);
+    f[17 + 3 * b] = sinf(s * R.z);
+  }
+  f[24] = 1.f;
+#pragma unroll
+  for (int k = 25; k < kNnK1; k++) f[k] = 0.f;
+}
+
+__device__ __forceinline__ void recFeatures(const float *r, float *f) {
+  nnFeatures({r[kRecN], r[kRecN + 1], r[kRecN + 2]}, {r[kRecV], r[kRecV + 1], r[kRecV + 2]}, r[kRecRough],
+             r[kRecMetal], r[kRecAlbedo], r[kRecAlbedo + 1], r[kRecAlbedo + 2], f);
+}
+
+/* Per-wavefront LDS of the forward pass: X[64][32], H[64][64] (bf16), Y[64][4] (f32). */
+struct NnWaveLds {
+  __bf16 x[64 * kNnK1];
+  __bf16 h[64 * kNnHid];
+  float y[64 * 4];
+};
+
+/* Forward pass of the 64 fragments of a wavefront: lane l's features `f`
+ * (kNnK1 floats) in, lane l's RGB out. Every lane of the wavefront must call it. */
+__device__ __forceinline__ void nnForward64(const NnPacked &W, NnWaveLds &L, const float *f, float *out) {
+  const int l = threadIdx.x & 63, g = l >> 4, c = l & 15;
+#pragma unroll
+  for (int k = 0; k < kNnK1; k++) L.x[l * kNnK1 + k] = (__bf16)f[k];
+  waveSync();
+  /* layer 1: H1^T = W1 . X^T (bias in column 24) */
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    const bf16x8 a = ldOp(W.w1, kNnK1, 16 * m, 0);
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      f32x4 d = {0.f, 0.f, 0.f, 0.f};
+      d = mma16(a, ldOp(L.x, kNnK1, 16 * n, 0), d);
+#pragma unroll
+      for (int v = 0; v < 4; v++) L.h[(16 * n + c) * kNnHid + 16 * m + 4 * g + v] = (__bf16)gmax(d[v], 0.f);
+    }
+  }
+  waveSync();
+  /* layer 2: H2^T = W2 . H1^T + b2, written over H1 once all of it is read */
+  bf16x8 hb[4][2];
+#pragma unroll
+  for (int n = 0; n < 4; n++)
+#pragma unroll
+    for (int kb = 0; kb < 2; kb++) hb[n][kb] = ldOp(L.h, kNnHid, 16 * n, 32 * kb);
+  waveSync();
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    const bf16x8 a0 = ldOp(W.w2, kNnHid, 16 * m, 0), a1 = ldOp(W.w2, kNnHid, 16 * m, 32);
+    f32x4 bias;
+#pragma unroll
+    for (int v = 0; v < 4; v++) bias[v] = W.b2[16 * m + 4 * g + v];
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      f32x4 d = mma16(a1, hb[n][1], mma16(a0, hb[n][0], bias));
+#pragma unroll
+      for (int v = 0; v < 4; v++) L.h[(16 * n + c) * kNnHid + 16 * m + 4 * g + v] = (__bf16)gmax(d[v], 0.f);
+    }
+  }
+  waveSync();
+  /* layer 3: Y^T = W3 . H2^T + b3 (rows 0..2) */
+  {
+    const bf16x8 a0 = ldOp(W.w3, kNnHid, 0, 0), a1 = ldOp(W.w3, kNnHid, 0, 32);
+    f32x4 bias;
+#pragma unroll
+    for (int v = 0; v < 4; v++) bias[v] = W.b3[4 * g + v];
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      f32x4 d = mma16(a1, ldOp(L.h, kNnHid, 16 * n, 32), mma16(a0, ldOp(L.h, kNnHid, 16 * n, 0), bias));
+      if (g == 0) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) L.y[(16 * n + c) * 4 + v] = d[v];
+      }
+    }
+  }
+  waveSync();
+  out[0] = L.y[l * 4 + 0];
+  out[1] = L.y[l * 4 + 1];
+  out[2] = L.y[l * 4 + 2];
+  waveSync();
+}
+
+/* --------------------------------------------------------------------------- FP64 plain shader */
+
+__device__ __forceinline__ double texelF64(const TexDesc &d, const TexLevel &l, int x, int y, int ch) {
+  return (double)chan(fetchWrap<uint32_t>(d, l, x, y, d.wrap), ch) / 255.0;
+}
+
+__device__ __forceinline__ void bilinearF64(const TexDesc &d, int layer, int level, double u, double v, double *o) {
+  const TexLevel &l = d.lv[layer][level];
+  const double fx0 = u * l.w - 0.5, fy0 = v * l.h - 0.5;
+  const double flx = floor(fx0), fly = floor(fy0);
+  const int x = (int)flx, y = (int)fly;
+  const double ax = fx0 - flx, ay = fy0 - fly;
+#pragma unroll
+  for (int ch = 0; ch < 3; ch++) {
+    const double t0 = texelF64(d, l, x, y, ch) * (1 - ax) + texelF64(d, l, x + 1, y, ch) * ax;
+    const double t1 = texelF64(d, l, x, y + 1, ch) * (1 - ax) + texelF64(d, l, x + 1, y + 1, ch) * ax;
+    o[ch] = t0 * (1 - ay) + t1 * ay;
+  }
+}
+
+/* textureCubeLodImpl (SamplerSoft.h:289-373) with exact interpolation weights and no
+ * intermediate 8-bit rounding: bilinear in a face, linear between mip levels. */
+__device__ void cubeF64(const TexDesc &d, double x, double y, double z, double lod, double *o) {
+  const double ax = fabs(x), ay = fabs(y), az = fabs(z);
+  double ma = 0, uc = 0, vc = 0;
+  int face = 0;
+  if (x > 0 && ax >= ay && ax >= az) { ma = ax; uc = -z; vc = y; face = 0; }
+  if (!(x > 0) && ax >= ay && ax >= az) { ma = ax; uc = z; vc = y; face = 1; }
+  if (y > 0 && ay >= ax && ay >= az) { ma = ay; uc = x; vc = -z; face = 2; }
+  if (!(y > 0) && ay >= ax && ay >= az) { ma = ay; uc = x; vc = z; face = 3; }
+  if (z > 0 && az >= ax && az >= ay) { ma = az; uc = x; vc = y; face = 4; }
+  if (!(z > 0) && az >= ax && az >= ay) { ma = az; uc = -x; vc = y; face = 5; }
+  vc = -vc;
+  const double u = 0.5 * (uc / ma + 1.0), v = 0.5 * (vc / ma + 1.0);
+  if (d.filter_min <= 1 || d.levels == 1) {
+    bilinearF64(d, face, 0, u, v, o);
+    return;
+  }
+  const int maxLevel = d.levels - 1;
+  const int hi = min(max((int)floor(lod), 0), maxLevel);
+  const int lo = min(hi + 1, maxLevel);
+  bilinearF64(d, face, hi, u, v, o);
+  if (hi == lo) return;
+  double t[3];
+  bilinearF64(d, face, lo, u, v, t);
+  const double f = lod - floor(lod);
+#pragma unroll
+  for (int ch = 0; ch < 3; ch++) o[ch] = o[ch] * (1 - f) + t[ch] * f;
+}
+
+/* The image-based PBR shader (PbrSoft.h:225-322, u_enableLight off) in FP64
+ * from the fragment's shading inputs; returns the display colour (not clamped). */
+__device__ void pbrIblF64(const DrawParams &P, const float *r, double *col) {
+  const double N[3] = {r[kRecN], r[kRecN + 1], r[kRecN + 2]};
+  const double V[3] = {r[kRecV], r[kRecV + 1], r[kRecV + 2]};
+  const double rough = r[kRecRough], metal = r[kRecMetal], ao = r[kRecAo];
+  double alb[3], F0[3];
+#pragma unroll
+  for (int ch = 0; ch < 3; ch++) {
+    alb[ch] = pow((double)r[kRecAlbedo + ch], 2.2);
+    F0[ch] = 0.04 * (1 - metal) + alb[ch] * metal;
+  }
+  const double ndv = N[0] * V[0] + N[1] * V[1] + N[2] * V[2];
+  const double nv = ndv > 0 ? ndv : 0;
+  const double p5 = pow(fmin(fmax(1 - nv, 0.0), 1.0), 5.0);
+  const double R[3] = {2 * ndv * N[0] - V[0], 2 * ndv * N[1] - V[1], 2 * ndv * N[2] - V[2]};
+  double irr[3], pf[3];
+  cubeF64(*P.tex[RHS_SLOT_IRRADIANCE], N[0], N[1], N[2], 0.0, irr);
+  cubeF64(*P.tex[RHS_SLOT_PREFILTER], R[0], R[1], R[2], rough * 4.0, pf);
+  /* Fresnel with roughness, then EnvBRDFApprox (PbrSoft.h:209-223) on it */
+  double F[3];
+#pragma unroll
+  for (int ch = 0; ch < 3; ch++) F[ch] = F0[ch] + (fmax(1 - rough, F0[ch]) - F0[ch]) * p5;
+  const double cx = 1 - rough, cy = 0.0425 - 0.0275 * rough, cz = 1.04 - 0.572 * rough, cw = -0.04 + 0.022 * rough;
+  const double a004 = fmin(cx * cx, exp2(-9.28 * nv)) * cx + cy;
+  const double bx = -1.04 * a004 + cz, by = (1.04 * a004 + cw) * fmax(0.0, fmin(1.0, 50.0 * F[1]));
+#pragma unroll
+  for (int ch = 0; ch < 3; ch++) {
+    const double kD = (1 - F[ch]) * (1 - metal);
+    const double amb = (kD * irr[ch] * alb[ch] + pf[ch] * (F[ch] * bx + by)) * ao;
+    col[ch] = pow(amb, 1.0 / 2.2);
+  }
+}
+
+/* --------------------------------------------------------------------------- capture */
+
+__device__ __forceinline__ uint32_t hash32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+/* Records 1 in 2^shift of the fragments the two-phase raster queued for a
+ * PBR draw: the shading inputs, the plain GPU colour and the FP64 colour. */
+template <int S>
+__global__ __launch_bounds__(256) void k_capture(DrawParams P, int need_deriv, float *rec, long long cap,
+                                                 unsigned long long *count, int shift, uint32_t salt, int material) {
+  const int n = P.dcount[2];
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    if (hash32((uint32_t)j ^ salt) & ((1u << shift) - 1u)) continue;
+    const uint2 jb = P.jobs[j];
+    const uint32_t pix = jb.x & 0x0fffffffu, w = jb.y;
+    const int px = (int)(pix % (uint32_t)P.fb_w), py = (int)(pix / (uint32_t)P.fb_w);
+    const int id = (int)(w >> 2), inst = (int)(w & 3u);
+    const PrimSetup ps = P.prims[id];
+    const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
+    const float fx = (float)px, fy = (float)py;
+    float4 bs;
+    if (S == 4) {
+      if (!bary(T, fx + 0.5f, fy + 0.5f, bs)) {
+#pragma unroll
+        for (int t = 3; t >= 0; t--) {
+          float4 b;
+          if (bary(T, sampleX(t) + fx, sampleY(t) + fy, b)) bs = b;
+        }
+      }
+    } else {
+      bary(T, fx + 0.5f, fy + 0.5f, bs);
+    }
+    const Frag fr = makeFrag<S, RHS_SHADING_PBR>(P, ps, T, px, py, bs, inst, need_deriv);
+    const PbrInputs in = pbrInputs(P, fr);
+    const RGBAf plain = fsPbr(P, fr);
+    const unsigned long long k = atomicAdd(count, 1ull);
+    if ((long long)k >= cap) continue;
+    float r[kRecFloats];
+    r[kRecN] = in.N.x, r[kRecN + 1] = in.N.y, r[kRecN + 2] = in.N.z;
+    r[kRecV] = in.V.x, r[kRecV + 1] = in.V.y, r[kRecV + 2] = in.V.z;
+    r[kRecAlbedo] = in.albedo_rgba.r, r[kRecAlbedo + 1] = in.albedo_rgba.g;
+    r[kRecAlbedo + 2] = in.albedo_rgba.b, r[kRecAlbedo + 3] = in.albedo_rgba.a;
+    r[kRecRough] = in.rough, r[kRecMetal] = in.metallic, r[kRecAo] = in.ao;
+    r[kRecMaterial] = (float)material;
+    double col[3];
+    pbrIblF64(P, r, col);
+    r[kRecTarget] = (float)fmin(fmax(col[0], 0.0), 1.0);
+    r[kRecTarget + 1] = (float)fmin(fmax(col[1], 0.0), 1.0);
+    r[kRecTarget + 2] = (float)fmin(fmax(col[2], 0.0), 1.0);
+    r[kRecPlain] = gclamp(plain.r, 0.f, 1.f);
+    r[kRecPlain + 1] = gclamp(plain.g, 0.f, 1.f);
+    r[kRecPlain + 2] = gclamp(plain.b, 0.f, 1.f);
+    for (int q = kRecPlain + 3; q < kRecFloats; q++) r[q] = 0.f;
+    float4 *o = (float4 *)(rec + (size_t)k * kRecFloats);
+#pragma unroll
+    for (int q = 0; q < kRecFloats / 4; q++) o[q] = make_float4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
+  }
+}
+
+/* --------------------------------------------------------------------------- inference in the pipeline */
+
+/* Colour of the fragments the two-phase raster queued, by the network: each
+ * lane gathers its fragment's inputs (albedo, normal and metal-rough maps as
+ * the plain shader samples them), then its wavefront evaluates the MLP for 64
+ * fragments at once. */
+template <int S>
+__global__ __launch_bounds__(256) void k_shade_neural(DrawParams P, int need_deriv, const NnPacked *W) {
+  __shared__ NnWaveLds lds[4];
+  __shared__ unsigned long long s_fs, s_wr;
+  if (threadIdx.x == 0) s_fs = s_wr = 0;
+  __syncthreads();
+  NnWaveLds &L = lds[threadIdx.x >> 6];
+  const int n = P.dcount[2];
+  unsigned long long nfs = 0, nwr = 0;
+  for (int b0 = blockIdx.x * blockDim.x; b0 < n; b0 += gridDim.x * blockDim.x) {
+    const int j = b0 + threadIdx.x;
+    const bool act = j < n;
+    float f[kNnK1];
+    uint32_t pix = 0, m = 0;
+    float alpha = 1.f;
+    if (act) {
+      const uint2 jb = P.jobs[j];
+      pix = jb.x & 0x0fffffffu;
+      m = jb.x >> 28;
+      const uint32_t w = jb.y;
+      const int px = (int)(pix % (uint32_t)P.fb_w), py = (int)(pix / (uint32_t)P.fb_w);
+      const int id = (int)(w >> 2), inst = (int)(w & 3u);
+      const PrimSetup ps = P.prims[id];
+      const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
+      const float fx = (float)px, fy = (float)py;
+      float4 bs;
+      if (S == 4) {
+        if (!bary(T, fx + 0.5f, fy + 0.5f, bs)) {
+#pragma unroll
+          for (int t = 3; t >= 0; t--) {
+            float4 b;
+            if (bary(T, sampleX(t) + fx, sampleY(t) + fy, b)) bs = b;
+          }
+        }
+      } else {
+        bary(T, fx + 0.5f, fy + 0.5f, bs);
+      }
+      const Frag fr = makeFrag<S, RHS_SHADING_NEURAL_PBR>(P, ps, T, px, py, bs, inst, need_deriv);
+      /* the inputs pbrInputs() reads, without the gamma of the albedo */
+      const f2 uv{fr.v[0], fr.v[1]};
+      const TexDesc *ta = P.tex[RHS_SLOT_ALBEDO];
+      const RGBAf alb = (P.defines & RHS_DEF_ALBEDO_MAP)
+                            ? tex2D(ta, uv, lodFor(ta, fr))
+                            : RGBAf{P.u.base_color[0], P.u.base_color[1], P.u.base_color[2], P.u.base_color[3]};
+      float metal = 0.f, rough = 1.f;
+      if (P.defines & RHS_DEF_METALROUGHNESS_MAP) {
+        const RGBAf mr = tex2D(P.tex[RHS_SLOT_METAL_ROUGHNESS], uv, 0.f);
+        metal = mr.b;
+        rough = mr.g;
+      }
+      const f3 N = normalFromMap(P, fr, 20, 24);
+      const f3 V = nrm3({fr.v[12], fr.v[13], fr.v[14]});
+      nnFeatures(N, V, rough, metal, alb.r, alb.g, alb.b, f);
+      alpha = alb.a;
+    } else {
+#pragma unroll
+      for (int k = 0; k < kNnK1; k++) f[k] = 0.f;
+    }
+    float y[3];
+    nnForward64(*W, L, f, y);
+    if (act) {
+      const float r = gclamp(y[0], 0.f, 1.f), g = gclamp(y[1], 0.f, 1.f), b = gclamp(y[2], 0.f, 1.f);
+      const uint32_t packed = pack4(u8t(r * 255.f), u8t(g * 255.f), u8t(b * 255.f), u8t(gclamp(alpha, 0.f, 1.f) * 255.f));
+      nfs++;
+#pragma unroll
+      for (int t = 0; t < S; t++)
+        if ((m >> t) & 1u) {
+          P.color[(size_t)pix * S + t] = packed;
+          nwr++;
+        }
+    }
+  }
+  if (nfs) atomicAdd(&s_fs, nfs);
+  if (nwr) atomicAdd(&s_wr, nwr);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (s_fs) atomicAdd(&P.counters[0], s_fs);
+    if (s_wr) atomicAdd(&P.counters[1], s_wr);
+  }
+}
+
+/* --------------------------------------------------------------------------- evaluation */
+
+constexpr int kNnHistBins = 1024; /* bins of 1/16 of 1/255, the last one open */
+
+__device__ __forceinline__ int histBin(float err255) {
+  const int b = (int)(err255 * 16.f);
+  return b < 0 ? 0 : (b >= kNnHistBins ? kNnHistBins - 1 : b);
+}
+
+/* Network and plain-shader errors against the FP64 colours over records
+ * [first, first + count) (of material `material`, or all if < 0): per-channel
+ * |error| * 255 into histograms, plus sums and maxima. Also writes the
+ * network outputs when `out` is given. One wavefront per 64 records. */
+__global__ __launch_bounds__(256) void k_nn_eval(const NnPacked *W, const float *rec, long long first, int count,
+                                                 int material, unsigned int *hist_nn, unsigned int *hist_plain,
+                                                 double *sums, float *out) {
+  __shared__ NnWaveLds lds[4];
+  __shared__ unsigned int h_nn[kNnHistBins], h_pl[kNnHistBins];
+  __shared__ double s_sum[6];
+  for (int i = threadIdx.x; i < kNnHistBins; i += blockDim.x) h_nn[i] = h_pl[i] = 0;
+  if (threadIdx.x < 6) s_sum[threadIdx.x] = 0;
+  __syncthreads();
+  NnWaveLds &L = lds[threadIdx.x >> 6];
+  double e_nn = 0, e_pl = 0, m_nn = 0, m_pl = 0, n_ok = 0;
+  for (int b0 = blockIdx.x * blockDim.x; b0 < count; b0 += gridDim.x * blockDim.x) {
+    const int i = b0 + threadIdx.x;
+    const float *r = rec + (size_t)(first + (i < count ? i : 0)) * kRecFloats;
+    const bool act = i < count && (material < 0 || (int)r[kRecMaterial] == material);
+    float f[kNnK1];
+    recFeatures(r, f);
+    float y[3];
+    nnForward64(*W, L, f, y);
+    if (out && i < count) {
+      out[(size_t)i * 3 + 0] = y[0];
+      out[(size_t)i * 3 + 1] = y[1];
+      out[(size_t)i * 3 + 2] = y[2];
+    }
+    if (!act) continue;
+    n_ok += 1;
+#pragma unroll
+    for (int ch = 0; ch < 3; ch++) {
+      const float en = fabsf(gclamp(y[ch], 0.f, 1.f) - r[kRecTarget + ch]) * 255.f;
+      const float ep = fabsf(r[kRecPlain + ch] - r[kRecTarget + ch]) * 255.f;
+      atomicAdd(&h_nn[histBin(en)], 1u);
+      atomicAdd(&h_pl[histBin(ep)], 1u);
+      e_nn += en;
+      e_pl += ep;
+      m_nn = fmax(m_nn, (double)en);
+      m_pl = fmax(m_pl, (double)ep);
+    }
+  }
+  atomicAdd(&s_sum[0], e_nn);
+  atomicAdd(&s_sum[1], e_pl);
+  atomicAdd(&s_sum[4], n_ok);
+  __syncthreads();
+  for (int i = threadIdx.x; i < kNnHistBins; i += blockDim.x) {
+    if (h_nn[i]) atomicAdd(&hist_nn[i], h_nn[i]);
+    if (h_pl[i]) atomicAdd(&hist_plain[i], h_pl[i]);
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(&sums[0], s_sum[0]);
+    atomicAdd(&sums[1], s_sum[1]);
+    atomicAdd(&sums[4], s_sum[4]);
+  }
+  /* maxima: through the integer bits of non-negative doubles */
+  atomicMax((unsigned long long *)&sums[2], (unsigned long long)__double_as_longlong(m_nn));
+  atomicMax((unsigned long long *)&sums[3], (unsigned long long)__double_as_longlong(m_pl));
+}
+
+/* --------------------------------------------------------------------------- training */
+
+/* Per-wavefront LDS of one training chunk (64 fragments). */
+struct NnTrainLds {
+  __bf16 x[64 * kNnK1];    /* X[frag][feat] */
+  __bf16 xt[kNnK1 * 64];   /* X^T[feat][frag] */
+  __bf16 h1[64 * kNnHid];  /* H1[frag][unit] */
+  __bf16 h1t[kNnHid * 64]; /* H1^T, later dH1^T */
+  __bf16 h2[64 * kNnHid];  /* H2, later dH2[frag][unit] */
+  __bf16 h2t[kNnHid * 64]; /* H2^T, later dH2^T */
+  __bf16 dy[64 * 32];      /* dY[frag][channel] (3 used) */
+  __bf16 dyt[16 * 64];     /* dY^T[channel][frag] */
+  float yf[64 * 4];        /* outputs of the chunk */
+  float g[kNnParams];      /* gradient sums of this block */
+};
+
+/* sum over the 16 rows of a product tile: MFMA with an all-ones operand */
+__device__ __forceinline__ bf16x8 onesOp() {
+  bf16x8 o;
+#pragma unroll
+  for (int k = 0; k < 8; k++) o[k] = (__bf16)1.f;
+  return o;
+}
+
+/* One training step's gradient of mean squared error over `batch` records
+ * drawn from [first, first + count) (or the `batch` records starting at
+ * `first` when `count` is 0): forward and backward on MFMA, gradients summed
+ * per block in LDS and added to `grad` (FP32, unscaled sum). Blocks are one
+ * wavefront; each takes chunks of 64 records. */
+__global__ __launch_bounds__(64) void k_nn_grad(const NnPacked *W, const float *rec, long long first, long long count,
+                                                int batch, uint32_t seed, float *grad, double *loss) {
+  __shared__ NnTrainLds L;
+  const int l = threadIdx.x, g = l >> 4, c = l & 15;
+  for (int i = l; i < kNnParams; i += 64) L.g[i] = 0.f;
+  double lsum = 0;
+  const bf16x8 ones = onesOp();
+  for (int c0 = blockIdx.x * 64; c0 < batch; c0 += gridDim.x * 64) {
+    const int bi = c0 + l;
+    const bool act = bi < batch;
+    long long ri = first + bi;
+    if (count > 0) ri = first + (long long)(hash32((uint32_t)bi * 0x9e3779b9u ^ seed) % (uint32_t)count);
+    const float *r = rec + (size_t)(act ? ri : first) * kRecFloats;
+    float f[kNnK1];
+    recFeatures(r, f);
+    if (!act) {
+#pragma unroll
+      for (int k = 0; k < kNnK1; k++) f[k] = 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < kNnK1; k++) {
+      L.x[l * kNnK1 + k] = (__bf16)f[k];
+      L.xt[k * 64 + l] = (__bf16)f[k];
+    }
+    waveSync();
+    /* forward: H1, H2 (and transposes), Y */
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      const bf16x8 a = ldOp(W->w1, kNnK1, 16 * m, 0);
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        f32x4 d = {0.f, 0.f, 0.f, 0.f};
+        d = mma16(a, ldOp(L.x, kNnK1, 16 * n, 0), d);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const __bf16 h = (__bf16)gmax(d[v], 0.f);
+          const int u = 16 * m + 4 * g + v, fr = 16 * n + c;
+          L.h1[fr * kNnHid + u] = h;
+          L.h1t[u * 64 + fr] = h;
+        }
+      }
+    }
+    waveSync();
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      const bf16x8 a0 = ldOp(W->w2, kNnHid, 16 * m, 0), a1 = ldOp(W->w2, kNnHid, 16 * m, 32);
+      f32x4 bias;
+#pragma unroll
+      for (int v = 0; v < 4; v++) bias[v] = W->b2[16 * m + 4 * g + v];
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        const f32x4 d = mma16(a1, ldOp(L.h1, kNnHid, 16 * n, 32), mma16(a0, ldOp(L.h1, kNnHid, 16 * n, 0), bias));
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const __bf16 h = (__bf16)gmax(d[v], 0.f);
+          const int u = 16 * m + 4 * g + v, fr = 16 * n + c;
+          L.h2[fr * kNnHid + u] = h;
+          L.h2t[u * 64 + fr] = h;
+        }
+      }
+    }
+    waveSync();
+    {
+      const bf16x8 a0 = ldOp(W->w3, kNnHid, 0, 0), a1 = ldOp(W->w3, kNnHid, 0, 32);
+      f32x4 bias;
+#pragma unroll
+      for (int v = 0; v < 4; v++) bias[v] = W->b3[4 * g + v];
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        const int fr = 16 * n + c;
+        const f32x4 d = mma16(a1, ldOp(L.h2, kNnHid, 16 * n, 32), mma16(a0, ldOp(L.h2, kNnHid, 16 * n, 0), bias));
+        if (g == 0) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) L.yf[fr * 4 + v] = d[v];
+        }
+      }
+    }
+    waveSync();
+    /* this lane's record: error, dY */
+    float dyv[3];
+#pragma unroll
+    for (int ch = 0; ch < 3; ch++) {
+      const float y = L.yf[l * 4 + ch];
+      const float e = act ? y - r[kRecTarget + ch] : 0.f;
+      lsum += (double)e * e;
+      dyv[ch] = 2.f * e;
+    }
+    waveSync();
+#pragma unroll
+    for (int ch = 0; ch < 32; ch++) L.dy[l * 32 + ch] = (__bf16)(ch < 3 ? dyv[ch] : 0.f);
+#pragma unroll
+    for (int ch = 0; ch < 16; ch++) L.dyt[ch * 64 + l] = (__bf16)(ch < 3 ? dyv[ch] : 0.f);
+    waveSync();
+    /* dW3 = dY^T . H2, db3 = row sums of dY^T */
+    {
+      const bf16x8 a0 = ldOp(L.dyt, 64, 0, 0), a1 = ldOp(L.dyt, 64, 0, 32);
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        f32x4 d = {0.f, 0.f, 0.f, 0.f};
+        d = mma16(a1, ldOp(L.h2t, 64, 16 * n, 32), mma16(a0, ldOp(L.h2t, 64, 16 * n, 0), d));
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int ch = 4 * g + v;
+          if (ch < kNnOut) L.g[kOffW3 + ch * kNnHid + 16 * n + c] += d[v];
+        }
+      }
+      f32x4 s = {0.f, 0.f, 0.f, 0.f};
+      s = mma16(a1, ones, mma16(a0, ones, s));
+      if (c == 0) {
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          if (4 * g + v < kNnOut) L.g[kOffB3 + 4 * g + v] += s[v];
+      }
+    }
+    /* dH2^T = W3^T . dY^T, masked by H2 > 0, written over H2 and H2^T */
+    {
+      bf16x8 bdy[4];
+#pragma unroll
+      for (int n = 0; n < 4; n++) bdy[n] = ldOp(L.dy, 32, 16 * n, 0);
+      waveSync();
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        const bf16x8 a = ldOp(W->w3t, 32, 16 * m, 0);
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+          f32x4 d = {0.f, 0.f, 0.f, 0.f};
+          d = mma16(a, bdy[n], d);
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int u = 16 * m + 4 * g + v, fr = 16 * n + c;
+            const bool on = (float)L.h2t[u * 64 + fr] > 0.f;
+            const __bf16 dh = (__bf16)(on ? d[v] : 0.f);
+            L.h2t[u * 64 + fr] = dh;
+            L.h2[fr * kNnHid + u] = dh;
+          }
+        }
+      }
+    }
+    waveSync();
+    /* dW2 = dH2^T . H1, db2 = row sums of dH2^T */
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      const bf16x8 a0 = ldOp(L.h2t, 64, 16 * m, 0), a1 = ldOp(L.h2t, 64, 16 * m, 32);
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        f32x4 d = {0.f, 0.f, 0.f, 0.f};
+        d = mma16(a1, ldOp(L.h1t, 64, 16 * n, 32), mma16(a0, ldOp(L.h1t, 64, 16 * n, 0), d));
+#pragma unroll
+        for (int v = 0; v < 4; v++) L.g[kOffW2 + (16 * m + 4 * g + v) * kNnHid + 16 * n + c] += d[v];
+      }
+      f32x4 s = {0.f, 0.f, 0.f, 0.f};
+      s = mma16(a1, ones, mma16(a0, ones, s));
+      if (c == 0) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) L.g[kOffB2 + 16 * m + 4 * g + v] += s[v];
+      }
+    }
+    waveSync();
+    /* dH1^T = W2^T . dH2^T, masked by H1 > 0, written over H1^T */
+    {
+      bf16x8 bd[4][2];
+#pragma unroll
+      for (int n = 0; n < 4; n++)
+#pragma unroll
+        for (int kb = 0; kb < 2; kb++) bd[n][kb] = ldOp(L.h2, kNnHid, 16 * n, 32 * kb);
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        const bf16x8 a0 = ldOp(W->w2t, kNnHid, 16 * m, 0), a1 = ldOp(W->w2t, kNnHid, 16 * m, 32);
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+          f32x4 d = {0.f, 0.f, 0.f, 0.f};
+          d = mma16(a1, bd[n][1], mma16(a0, bd[n][0], d));
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int u = 16 * m + 4 * g + v, fr = 16 * n + c;
+            const bool on = (float)L.h1t[u * 64 + fr] > 0.f;
+            L.h1t[u * 64 + fr] = (__bf16)(on ? d[v] : 0.f);
+          }
+        }
+      }
+    }
+    waveSync();
+    /* dW1 (with db1 in column 24) = dH1^T . X */
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      const bf16x8 a0 = ldOp(L.h1t, 64, 16 * m, 0), a1 = ldOp(L.h1t, 64, 16 * m, 32);
+#pragma unroll
+      for (int n = 0; n < 2; n++) {
+        f32x4 d = {0.f, 0.f, 0.f, 0.f};
+        d = mma16(a1, ldOp(L.xt, 64, 16 * n, 32), mma16(a0, ldOp(L.xt, 64, 16 * n, 0), d));
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int u = 16 * m + 4 * g + v, k = 16 * n + c;
+          if (k < kNnIn)
+            L.g[kOffW1 + u * kNnIn + k] += d[v];
+          else if (k == kNnIn)
+            L.g[kOffB1 + u] += d[v];
+        }
+      }
+    }
+    waveSync();
+  }
+  for (int i = l; i < kNnParams; i += 64)
+    if (L.g[i] != 0.f) atomicAdd(&grad[i], L.g[i]);
+  /* loss: sum over the wavefront, one atomic */
+  double t = lsum;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+  if (l == 0) atomicAdd(loss, t);
+}
+
+/* Adam on the FP32 parameters with the summed gradient (scaled by 1/batch),
+ * then the gradient cleared and the bf16 operands repacked. One block. */
+__global__ __launch_bounds__(1024) void k_nn_adam(float *p, float *grad, float *m1, float *m2, NnPacked *W,
+                                                  float lr, float b1, float b2, float eps, float scale, int step) {
+  const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+  for (int i = threadIdx.x; i < kNnParams; i += blockDim.x) {
+    if (step > 0) {
+      const float gi = grad[i] * scale;
+      const float a = b1 * m1[i] + (1.f - b1) * gi, b = b2 * m2[i] + (1.f - b2) * gi * gi;
+      m1[i] = a;
+      m2[i] = b;
+      p[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+    }
+    grad[i] = 0.f;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kNnHid * kNnK1; i += blockDim.x) {
+    const int u = i / kNnK1, k = i % kNnK1;
+    W->w1[i] = (__bf16)(k < kNnIn ? p[kOffW1 + u * kNnIn + k] : (k == kNnIn ? p[kOffB1 + u] : 0.f));
+  }
+  for (int i = threadIdx.x; i < kNnHid * kNnHid; i += blockDim.x) {
+    const int u = i / kNnHid, k = i % kNnHid;
+    W->w2[i] = (__bf16)p[kOffW2 + i];
+    W->w2t[i] = (__bf16)p[kOffW2 + k * kNnHid + u];
+  }
+  for (int i = threadIdx.x; i < 16 * kNnHid; i += blockDim.x) {
+    const int ch = i / kNnHid;
+    W->w3[i] = (__bf16)(ch < kNnOut ? p[kOffW3 + i] : 0.f);
+  }
+  for (int i = threadIdx.x; i < kNnHid * 32; i += blockDim.x) {
+    const int u = i / 32, ch = i % 32;
+    W->w3t[i] = (__bf16)(ch < kNnOut ? p[kOffW3 + ch * kNnHid + u] : 0.f);
+  }
+  for (int i = threadIdx.x; i < kNnHid; i += blockDim.x) W->b2[i] = p[kOffB2 + i];
+  for (int i = threadIdx.x; i < 16; i += blockDim.x) W->b3[i] = i < kNnOut ? p[kOffB3 + i] : 0.f;
+}
+
+}  // namespace rhip

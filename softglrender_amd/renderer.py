@@ -351,6 +351,63 @@ class RendererHIP:
     def resetStats(self):
         _lib.check(self.lib.rhip_stats_reset(self.dev))
 
+    # ------------------------------------------------------------- learned PBR shader (no reference counterpart)
+
+    NN_RECORD_FLOATS = 24
+
+    def neural_init(self, seed: int = 1):
+        _lib.check(self.lib.rhip_neural_init(self.dev, seed))
+
+    def neural_set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        _lib.check(self.lib.rhip_neural_set_weights(self.dev, w.ctypes.data, w.size))
+
+    def neural_get_weights(self):
+        w = np.zeros(self.lib.rhip_neural_param_count(), np.float32)
+        _lib.check(self.lib.rhip_neural_get_weights(self.dev, w.ctypes.data, w.size))
+        return w
+
+    def neural_records(self, capacity: int):
+        _lib.check(self.lib.rhip_neural_records(self.dev, capacity))
+
+    def neural_capture(self, enable: bool, shift: int = 0, material: int = 0):
+        _lib.check(self.lib.rhip_neural_capture(self.dev, int(enable), shift, material))
+
+    def neural_record_count(self) -> int:
+        n = ctypes.c_longlong()
+        _lib.check(self.lib.rhip_neural_record_count(self.dev, ctypes.byref(n)))
+        return n.value
+
+    def neural_read_records(self, first: int, count: int):
+        out = np.zeros((count, self.NN_RECORD_FLOATS), np.float32)
+        _lib.check(self.lib.rhip_neural_read_records(self.dev, first, count, out.ctypes.data))
+        return out
+
+    def neural_write_records(self, first: int, rec):
+        rec = np.ascontiguousarray(rec, dtype=np.float32).reshape(-1, self.NN_RECORD_FLOATS)
+        _lib.check(self.lib.rhip_neural_write_records(self.dev, first, rec.shape[0], rec.ctypes.data))
+
+    def neural_gradient(self, first: int, count: int):
+        g = np.zeros(self.lib.rhip_neural_param_count(), np.float32)
+        loss = ctypes.c_double()
+        _lib.check(self.lib.rhip_neural_gradient(self.dev, first, count, g.ctypes.data, ctypes.byref(loss)))
+        return g, loss.value
+
+    def neural_train(self, first: int, count: int, steps: int, batch: int = 16384, lr_start: float = 3e-3,
+                     lr_end: float = 1e-4, seed: int = 1):
+        losses = np.zeros(max(steps, 1), np.float64)
+        _lib.check(self.lib.rhip_neural_train(self.dev, first, count, steps, batch, lr_start, lr_end, seed,
+                                              losses.ctypes.data))
+        return losses[:steps]
+
+    def neural_evaluate(self, first: int, count: int, material: int = -1, outputs: bool = False):
+        a, b = _lib.NnError(), _lib.NnError()
+        out = np.zeros((count, 3), np.float32) if outputs else None
+        _lib.check(self.lib.rhip_neural_evaluate(self.dev, first, count, material, ctypes.byref(a), ctypes.byref(b),
+                                                 out.ctypes.data if outputs else None))
+        err = lambda e: {"mean": e.mean, "p99": e.p99, "max": e.max, "samples": e.samples}
+        return err(a), err(b), out
+
     def profile(self, on: bool = True):
         """Per-kernel HIP-event timing on the renderer's stream (clears previous records)."""
         _lib.check(self.lib.rhip_profile_enable(self.dev, int(on)))

@@ -1,0 +1,134 @@
+"""The learned PBR shader on the GPU: MFMA forward and gradient against the numpy network, FP64 targets
+against the numpy FP64 shader, training gain, and the shader in the pipeline."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import nn_ref as nn
+from softglrender_amd import neural, scenes
+from softglrender_amd import stream as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _synthetic_records(n, seed):
+    rng = np.random.default_rng(seed)
+    rec = np.zeros((n, 24), np.float32)
+    N = rng.normal(size=(n, 3)); N /= np.linalg.norm(N, axis=1, keepdims=True)
+    V = rng.normal(size=(n, 3)); V /= np.linalg.norm(V, axis=1, keepdims=True)
+    rec[:, nn.REC_N:nn.REC_N + 3], rec[:, nn.REC_V:nn.REC_V + 3] = N, V
+    rec[:, nn.REC_ALBEDO:nn.REC_ALBEDO + 4] = rng.uniform(0, 1, (n, 4))
+    rec[:, nn.REC_ROUGH], rec[:, nn.REC_METAL], rec[:, nn.REC_AO] = rng.uniform(0, 1, n), rng.uniform(0, 1, n), 1.0
+    rec[:, nn.REC_MATERIAL] = rng.integers(0, 3, n)
+    rec[:, nn.REC_TARGET:nn.REC_TARGET + 3] = rng.uniform(0, 1, (n, 3))
+    rec[:, nn.REC_PLAIN:nn.REC_PLAIN + 3] = rec[:, nn.REC_TARGET:nn.REC_TARGET + 3]
+    return rec
+
+
+def _random_params(seed):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(nn.N_PARAMS)
+    p[nn.OFF_W1:nn.OFF_B1] = rng.uniform(-1, 1, nn.N_HID * nn.N_IN) * math.sqrt(6 / 24)
+    p[nn.OFF_B1:nn.OFF_W2] = rng.uniform(-0.2, 0.2, nn.N_HID)
+    p[nn.OFF_W2:nn.OFF_B2] = rng.uniform(-1, 1, nn.N_HID * nn.N_HID) * math.sqrt(6 / 64)
+    p[nn.OFF_B2:nn.OFF_W3] = rng.uniform(-0.2, 0.2, nn.N_HID)
+    p[nn.OFF_W3:nn.OFF_B3] = rng.uniform(-1, 1, nn.N_OUT * nn.N_HID) * math.sqrt(6 / 64)
+    p[nn.OFF_B3:] = rng.uniform(-0.2, 0.2, nn.N_OUT)
+    return p.astype(np.float32)
+
+
+def test_mfma_forward_matches_numpy_network(renderer):
+    rec = _synthetic_records(1000, 1)
+    p = _random_params(2)
+    renderer.neural_records(len(rec))
+    renderer.neural_write_records(0, rec)
+    renderer.neural_set_weights(p)
+    _, _, out = renderer.neural_evaluate(0, len(rec), outputs=True)
+    want, _ = nn.forward(p.astype(np.float64), nn.features(rec), bf=True)
+    err = np.abs(out - want)
+    print(f"[gpu] forward: max |gpu - numpy| {err.max():.2e}, mean {err.mean():.2e}", flush=True)
+    assert err.max() < 2e-2 and err.mean() < 1e-3
+
+
+def test_mfma_gradient_matches_numpy_network(renderer):
+    rec = _synthetic_records(1000, 3)
+    p = _random_params(4)
+    renderer.neural_records(len(rec))
+    renderer.neural_write_records(0, rec)
+    renderer.neural_set_weights(p)
+    g, loss = renderer.neural_gradient(0, len(rec))
+    gw, lw = nn.gradient(p.astype(np.float64), nn.features(rec), rec[:, nn.REC_TARGET:nn.REC_TARGET + 3], bf=True)
+    rel = np.linalg.norm(g - gw) / np.linalg.norm(gw)
+    print(f"[gpu] gradient: relative L2 error {rel:.2e}, loss {loss:.6f} vs {lw:.6f}", flush=True)
+    assert abs(loss - lw) <= 1e-3 * lw
+    assert rel < 1e-2
+    for a, b in ((nn.OFF_W1, nn.OFF_B1), (nn.OFF_B1, nn.OFF_W2), (nn.OFF_W2, nn.OFF_B2), (nn.OFF_B2, nn.OFF_W3),
+                 (nn.OFF_W3, nn.OFF_B3), (nn.OFF_B3, nn.N_PARAMS)):
+        assert np.linalg.norm(g[a:b] - gw[a:b]) <= 2e-2 * np.linalg.norm(gw[a:b]) + 1e-6, (a, b)
+
+
+def _small_views(n_cams, seed, materials, normal, w=320, h=180):
+    cams = neural.random_cameras(n_cams, seed)
+    return neural.scene_views(cams, materials, normal, width=w, height=h, freq=40)
+
+
+@pytest.fixture(scope="module")
+def small_setup():
+    albedo, normal, mr = scenes.pbr_textures(256, 3)
+    mats = neural.neural_materials(2, 256, 11, base=(albedo, mr))
+    return mats, normal
+
+
+def test_capture_targets_match_numpy_fp64_shader(renderer, small_setup):
+    mats, normal = small_setup
+    renderer.neural_records(200_000)
+    ns = _small_views(2, 5, mats, normal)
+    first, count = neural.capture(renderer, ns, 0)
+    assert count > 1000
+    rec = renderer.neural_read_records(first, count)
+    irr, pref = scenes.ibl_maps(3 + 4)
+    want = np.clip(nn.pbr_ibl_f64(rec, irr, pref), 0, 1)
+    d = np.abs(rec[:, nn.REC_TARGET:nn.REC_TARGET + 3] - want)
+    plain = np.abs(rec[:, nn.REC_PLAIN:nn.REC_PLAIN + 3] - rec[:, nn.REC_TARGET:nn.REC_TARGET + 3]) * 255
+    print(f"[gpu] capture: {count} records, FP64 target vs numpy max {d.max():.2e}; plain GPU vs FP64 mean "
+          f"{plain.mean():.3f}/255 max {plain.max():.2f}/255", flush=True)
+    assert d.max() < 1e-6
+    assert plain.mean() < 1.0
+
+
+def test_training_gains_over_untrained_network(renderer, small_setup):
+    mats, normal = small_setup
+    renderer.neural_records(400_000)
+    tr_first, tr_count = neural.capture(renderer, _small_views(8, 21, mats, normal), 0)
+    ho_first, ho_count = neural.capture(renderer, _small_views(2, 99, mats, normal), 0)
+    renderer.neural_init(7)
+    before, plain, _ = renderer.neural_evaluate(ho_first, ho_count)
+    losses = renderer.neural_train(tr_first, tr_count, steps=1500, batch=4096, lr_start=3e-3, lr_end=1e-4, seed=3)
+    after, _, _ = renderer.neural_evaluate(ho_first, ho_count)
+    print(f"[gpu] training: loss {losses[0]:.4f} -> {losses[-1]:.6f}; held-out mean error {before['mean']:.2f} -> "
+          f"{after['mean']:.3f}/255, p99 {after['p99']:.2f}/255 (plain shader: mean {plain['mean']:.3f})", flush=True)
+    assert losses[-1] < losses[0] / 20
+    assert after["mean"] < before["mean"] / 10
+    assert after["mean"] < 3.0
+
+
+def test_neural_draw_renders_the_network(renderer, small_setup):
+    """After training, the learned shader's image is close to the plain shader's image of the same view."""
+    mats, normal = small_setup
+    renderer.neural_records(300_000)
+    tr_first, tr_count = neural.capture(renderer, _small_views(8, 21, mats, normal), 0)
+    renderer.neural_init(7)
+    renderer.neural_train(tr_first, tr_count, steps=1500, batch=4096, seed=3)
+    from softglrender_amd.replay import replay
+    cams = neural.random_cameras(1, 77)
+    imgs = []
+    for sh in (S.SHADING_PBR, S.SHADING_NEURAL_PBR):
+        ns = neural.scene_views(cams, mats[:1], normal, width=320, height=180, freq=40, shading=sh, reads=True)
+        outs, _ = replay(renderer, ns.scene.stream.bytes(), layouts=ns.scene.meta.get("layouts"))
+        imgs.append(outs[0].astype(np.float64))
+    cov = np.any(imgs[0] != imgs[0][0, 0], axis=-1)
+    diff = np.abs(imgs[0] - imgs[1])[cov][:, :3]
+    print(f"[gpu] neural draw: {cov.sum()} pixels, mean |neural - plain| {diff.mean():.3f}/255", flush=True)
+    assert cov.sum() > 5000
+    assert diff.mean() < 4.0
