@@ -99,6 +99,82 @@ def raster_algorithmic_bytes(d):
     return fb + 12 * d["tris"] + d["verts"] * (16 + 4 * d["vary"])
 
 
+def run_neural(r, quick=False, log=lambda m: None):
+    """C5: the learned PBR shader on the C2 scene (point light off for both shaders). Trains on fragments of
+    64 random cameras x 8 random materials, evaluates per material on 32 held-out cameras against the FP64
+    plain shader, and times learned against plain shading of the C2 view."""
+    import numpy as np
+    from softglrender_amd import neural, scenes
+    from softglrender_amd import stream as S
+    t0 = time.time()
+    albedo, normal, mr = scenes.pbr_textures(2048, 3)
+    mats = neural.neural_materials(8, 1024, 11, base=(albedo, mr))
+    verts = scenes.icosphere(224, 1.0, displace=0.1, seed=3)
+    n_train, n_held, shift = (8, 4, 5) if quick else (64, 32, 5)
+    log(f"[neural] materials and geometry in {time.time() - t0:.1f}s")
+    r.neural_records(24_000_000)
+    t1 = time.time()
+    tr = neural.capture(r, neural.scene_views(neural.random_cameras(n_train, 1), mats, normal, verts=verts), shift)
+    ho = neural.capture(r, neural.scene_views(neural.random_cameras(n_held, 2), mats, normal, verts=verts), shift)
+    t_cap = time.time() - t1
+    log(f"[neural] captured {tr[1]} training and {ho[1]} held-out fragments in {t_cap:.1f}s")
+    r.neural_init(1)
+    steps = 6000 if quick else 60000
+    t2 = time.time()
+    losses = r.neural_train(tr[0], tr[1], steps=steps, batch=16384, lr_start=1e-2, lr_end=2e-5, seed=1)
+    t_train = time.time() - t2
+    log(f"[neural] {steps} steps in {t_train:.1f}s, loss {losses[0]:.5f} -> {losses[-1]:.6f}")
+    table = neural.fallback_table(r, ho[0], ho[1], len(mats))
+    e_nn, e_pl, _ = r.neural_evaluate(ho[0], ho[1])
+
+    # shading throughput on the C2 view (camera at (0, 0, 2.6), material 0)
+    cam = np.array([[0.0, 0.0, 2.6]])
+    times = {}
+    for name, sh in (("plain", S.SHADING_PBR), ("neural", S.SHADING_NEURAL_PBR)):
+        ns = neural.scene_views(cam, mats[:1], normal, verts=verts, shading=sh)
+        from softglrender_amd.replay import Replayer, parse
+        rp = Replayer(r, layouts=ns.scene.meta.get("layouts"))
+        ops = list(parse(ns.scene.stream.bytes()))
+        k = [i for i, (op, _) in enumerate(ops) if op == S.OP_FRAME][0]
+        sink = []
+        for op, p in ops[:k]:
+            rp.exec(op, p, sink)
+        frame = [(op, p) for op, p in ops[k + 1:] if op != S.OP_READ]
+        for _ in range(2):
+            for op, p in frame:
+                rp.exec(op, p, sink)
+        r.waitIdle()
+        r.profile(True)
+        reps = 20
+        t = time.perf_counter()
+        for _ in range(reps):
+            for op, p in frame:
+                rp.exec(op, p, sink)
+        r.waitIdle()
+        frame_ms = (time.perf_counter() - t) * 1e3 / reps
+        prof = r.profileResults()
+        r.profile(False)
+        kerns = ["k_shade_msaa4"] if name == "plain" else ["k_nn_features_msaa4", "k_nn_infer_msaa4"]
+        times[name] = {"shade_kernel_ms": round(sum(prof[k][0] / prof[k][1] for k in kerns), 4),
+                       "frame_ms": round(frame_ms, 4)}
+        if name == "neural":
+            times[name].update({k: round(prof[k][0] / prof[k][1], 4) for k in kerns})
+    speed = times["plain"]["shade_kernel_ms"] / times["neural"]["shade_kernel_ms"]
+    rnd = lambda e: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in e.items()}
+    return {
+        "network": "24-64-64-3 ReLU MLP, BF16 MFMA (v_mfma_f32_16x16x32_bf16), FP32 accumulation",
+        "scene": "c2 geometry and IBL, image-based PBR (point light off for both shaders)",
+        "train_fragments": int(tr[1]), "heldout_fragments": int(ho[1]),
+        "train_cameras": n_train, "heldout_cameras": n_held, "materials": len(mats),
+        "steps": steps, "batch": 16384, "train_s": round(t_train, 2), "loss_first_last": [losses[0], losses[-1]],
+        "error_vs_fp64_255": {"network": rnd(e_nn), "plain_gpu_shader": rnd(e_pl)},
+        "per_material": [{"material": t["material"], "network_mean": round(t["network"]["mean"], 3),
+                          "network_p99": round(t["network"]["p99"], 3), "fallback": t["fallback"]} for t in table],
+        "fallback_materials": [t["material"] for t in table if t["fallback"]],
+        "shading": times, "shading_speedup": round(speed, 3),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +182,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-neural", action="store_true", help="skip the learned-shader (c5) measurement")
+    ap.add_argument("--neural-quick", action="store_true", help="reduced c5 run (fewer cameras and steps)")
     ap.add_argument("--cpu-frames", type=int, default=3, help="timed reference frames (after one warm-up)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -237,6 +315,14 @@ def main():
         except Exception as e:  # report, do not hide
             cpu = {"error": repr(e)}
 
+    neural_res = None
+    if args.config == "c2" and not args.no_neural:
+        try:
+            neural_res = run_neural(r, quick=args.neural_quick,
+                                    log=lambda m: print(m, file=sys.stderr, flush=True))
+        except Exception as e:  # report, do not hide
+            neural_res = {"error": repr(e)}
+
     line = {
         "metric": "ms/frame 1080p MSAA4x 1M tris; Gfrag/s; neural vs plain PBR at <=1/255 err",
         "value": round(ms_per_frame, 4),
@@ -261,6 +347,7 @@ def main():
         "kernels_ms_per_frame": {k: round(v[0] / args.steps, 4) for k, v in prof.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "neural": neural_res,
     }
     print(json.dumps(line), flush=True)
     r.destroy()

@@ -37,6 +37,7 @@ N_PARAMS = OFF_B3 + N_OUT
 # capture record layout (24 floats)
 REC_N, REC_V, REC_ALBEDO, REC_ROUGH, REC_METAL, REC_AO, REC_MATERIAL, REC_TARGET, REC_PLAIN = 0, 3, 6, 10, 11, 12, 13, \
     14, 17
+REC_PIXEL, REC_MASK = 20, 21
 
 
 def bf16(x):
@@ -73,7 +74,8 @@ def _dot3_f32(a, b):
 
 def features(rec):
     """The 24 network inputs of capture records (float32 like the kernels): N, V, max(N.V, 0), roughness,
-    metallic, albedo (as sampled), R = 2 (N.V) N - V, sin(pi R), sin(2 pi R), sin(4 pi R)."""
+    metallic, albedo (as sampled), R = 2 (N.V) N - V, sin(pi R), sin(2 pi R), sin(4 pi R) (the last two
+    by double-angle identities from sin(pi R) and cos(pi R), as the kernels compute them)."""
     rec = np.asarray(rec, dtype=np.float32)
     N = rec[:, REC_N:REC_N + 3]
     V = rec[:, REC_V:REC_V + 3]
@@ -88,9 +90,12 @@ def features(rec):
     f[:, 8] = rec[:, REC_METAL]
     f[:, 9:12] = rec[:, REC_ALBEDO:REC_ALBEDO + 3]
     f[:, 12:15] = R
-    for b in range(3):
-        s = np.float32(np.pi * (1 << b))
-        f[:, 15 + 3 * b:18 + 3 * b] = np.sin((s * R).astype(np.float32))
+    # sin(pi R), then sin(2 pi R) = 2 s c and sin(4 pi R) = 2 sin(2 pi R) cos(2 pi R), cos(2 pi R) = 1 - 2 s^2
+    a = (np.float32(np.pi) * R).astype(np.float32)
+    s1, c1 = np.sin(a).astype(np.float32), np.cos(a).astype(np.float32)
+    s2 = (np.float32(2) * s1 * c1).astype(np.float32)
+    c2 = (np.float32(1) - np.float32(2) * s1 * s1).astype(np.float32)
+    f[:, 15:18], f[:, 18:21], f[:, 21:24] = s1, s2, (np.float32(2) * s2 * c2).astype(np.float32)
     return f
 
 

@@ -142,7 +142,7 @@ struct rhip_device_s {
     unsigned int *hist = nullptr;
     int step = 0;
     bool ready = false;
-    DevBuf records;
+    DevBuf records, feat;
     long long rec_cap = 0;
     unsigned long long *rec_count = nullptr;
     bool capturing = false;
@@ -309,18 +309,22 @@ struct Timer {
 
 /* Two-phase raster: coarse hierarchical Z, then visibility and depth per tile. */
 template <int S>
-void launchVis(const DrawParams &P, int ntiles, int ncoarse, hipStream_t st) {
+void launchVis(rhip_device d, const DrawParams &P, int ntiles, int ncoarse, hipStream_t st) {
   const int f = P.st.depthFunc;
-  if (f == 4 || f == 6) {
-    RHIP_LAUNCH(k_hiz_coarse<S RHIP_COMMA true>, ncoarse, 256, st, P);
-  } else {
-    RHIP_LAUNCH(k_hiz_coarse<S RHIP_COMMA false>, ncoarse, 256, st, P);
+  {
+    Timer tm(d, "k_hiz_coarse");
+    if (f == 4 || f == 6) {
+      RHIP_LAUNCH(k_hiz_coarse<S RHIP_COMMA true>, ncoarse, 256, st, P);
+    } else {
+      RHIP_LAUNCH(k_hiz_coarse<S RHIP_COMMA false>, ncoarse, 256, st, P);
+    }
   }
+  Timer tm(d, S == 4 ? "k_raster_vis_msaa4" : "k_raster_vis");
   switch (f) {
-    case 1: RHIP_LAUNCH(k_raster_vis<S RHIP_COMMA VIS_LESS>, ntiles, 256, st, P); break;
-    case 3: RHIP_LAUNCH(k_raster_vis<S RHIP_COMMA VIS_LEQUAL>, ntiles, 256, st, P); break;
-    case 4: RHIP_LAUNCH(k_raster_vis<S RHIP_COMMA VIS_GREATER>, ntiles, 256, st, P); break;
-    default: RHIP_LAUNCH(k_raster_vis<S RHIP_COMMA VIS_GEQUAL>, ntiles, 256, st, P); break;
+    case 1: RHIP_LAUNCH(k_raster_vis<S RHIP_COMMA VIS_LESS RHIP_COMMA kTile RHIP_COMMA 4>, ntiles, 256, st, P); break;
+    case 3: RHIP_LAUNCH(k_raster_vis<S RHIP_COMMA VIS_LEQUAL RHIP_COMMA kTile RHIP_COMMA 4>, ntiles, 256, st, P); break;
+    case 4: RHIP_LAUNCH(k_raster_vis<S RHIP_COMMA VIS_GREATER RHIP_COMMA kTile RHIP_COMMA 4>, ntiles, 256, st, P); break;
+    default: RHIP_LAUNCH(k_raster_vis<S RHIP_COMMA VIS_GEQUAL RHIP_COMMA kTile RHIP_COMMA 4>, ntiles, 256, st, P); break;
   }
 }
 
@@ -800,6 +804,8 @@ int rhip_draw(rhip_device d, const rhip_draw_desc *dd) {
     return fail(RHIP_E_UNSUPPORTED, "draw too large (%zu triangles)", T);
   const int nblk = (int)blocks(T, 256);
   P.ntiles = ntiles;
+  P.tile = kTile;
+  P.coarse_tiles = kCoarse;
   P.coarse_x = (P.tiles_x + kCoarse - 1) / kCoarse;
   P.coarse_y = (P.tiles_y + kCoarse - 1) / kCoarse;
   P.nbins = ntiles + P.coarse_x * P.coarse_y + 1;
@@ -848,21 +854,28 @@ int rhip_draw(rhip_device d, const rhip_draw_desc *dd) {
     P.hiz_bound = d->hiz.as<uint32_t>();
     P.hiz_count = (int *)P.hiz_bound + ncoarse;
     P.hiz_surv = P.hiz_count + ncoarse;
-    {
-      Timer tm(d, samples == 4 ? "k_raster_vis_msaa4" : "k_raster_vis");
-      if (samples == 4)
-        launchVis<4>(P, ntiles, ncoarse, s);
-      else
-        launchVis<1>(P, ntiles, ncoarse, s);
-    }
+    if (samples == 4)
+      launchVis<4>(d, P, ntiles, ncoarse, s);
+    else
+      launchVis<1>(d, P, ntiles, ncoarse, s);
     if (P.color) {
       const int grid = (int)std::min<size_t>(blocks(njobs, 256), 4096);
       if (pg.shading == RHS_SHADING_NEURAL_PBR) {
-        Timer tm(d, samples == 4 ? "k_shade_neural_msaa4" : "k_shade_neural");
+        if ((rc = d->nn.feat.ensure((njobs + 64) * kNnK1 * 2))) return rc;
+        __bf16 *feat = d->nn.feat.as<__bf16>();
+        {
+          Timer tm(d, samples == 4 ? "k_nn_features_msaa4" : "k_nn_features");
+          if (samples == 4)
+            RHIP_LAUNCH(k_nn_features<4>, grid, 256, s, P, need_deriv ? 1 : 0, feat);
+          else
+            RHIP_LAUNCH(k_nn_features<1>, grid, 256, s, P, need_deriv ? 1 : 0, feat);
+        }
+        Timer tm(d, samples == 4 ? "k_nn_infer_msaa4" : "k_nn_infer");
+        const unsigned ig = (unsigned)std::min<size_t>((njobs + 255) / 256, 1024); /* about one resident set */
         if (samples == 4)
-          RHIP_LAUNCH(k_shade_neural<4>, grid, 256, s, P, need_deriv ? 1 : 0, d->nn.packed);
+          RHIP_LAUNCH(k_nn_infer<4>, ig, 256, s, P, d->nn.packed, feat);
         else
-          RHIP_LAUNCH(k_shade_neural<1>, grid, 256, s, P, need_deriv ? 1 : 0, d->nn.packed);
+          RHIP_LAUNCH(k_nn_infer<1>, ig, 256, s, P, d->nn.packed, feat);
       } else {
         Timer tm(d, samples == 4 ? "k_shade_msaa4" : "k_shade");
         if (samples == 4)

@@ -12,7 +12,7 @@ namespace rhip {
 
 constexpr int kMaxLevels = 16;
 constexpr int kMaxLayers = 6;
-constexpr int kTile = 16;         /* screen tile edge in pixels (one workgroup) */
+constexpr int kTile = 16;         /* default screen tile edge in pixels (one workgroup) */
 constexpr int kSub = 8;           /* 8x8 sub-tile per wavefront */
 constexpr int kVaryStride = 32;   /* floats per vertex in the varyings buffer */
 constexpr int kMaxClipVerts = 12; /* new vertices one clipped triangle can create */
@@ -110,7 +110,9 @@ struct DrawParams {
   int *dcount; /* device scalars: [0] appended fan triangles, [1] primitive count, [2] queued fragments */
   /* primitives */
   PrimSetup *prims;
-  /* bins: ntiles fine bins, then coarse_x * coarse_y coarse bins, then the global bin */
+  /* bins: ntiles fine bins of tile x tile pixels, then coarse_x * coarse_y coarse bins of
+   * coarse_tiles x coarse_tiles tiles, then the global bin */
+  int tile, coarse_tiles;
   int ntiles, coarse_x, coarse_y, nbins;
   int *bin_count;  /* [nbins] */
   int *bin_start;  /* [nbins + 1] */

@@ -216,16 +216,17 @@ __device__ __forceinline__ void axisRange(float mn, float mx, int bc, int *first
 /* Tiles touched by a live primitive's quads, clipped to the framebuffer. */
 __device__ __forceinline__ void tileRect(const DrawParams &P, const PrimSetup &s, int &tx0, int &ty0, int &tx1,
                                          int &ty1) {
-  tx0 = max(s.qx0, 0) / kTile;
-  tx1 = min(s.qx1, P.fb_w - 1) / kTile;
-  ty0 = max(s.qy0, 0) / kTile;
-  ty1 = min(s.qy1, P.fb_h - 1) / kTile;
+  tx0 = max(s.qx0, 0) / P.tile;
+  tx1 = min(s.qx1, P.fb_w - 1) / P.tile;
+  ty0 = max(s.qy0, 0) / P.tile;
+  ty1 = min(s.qy1, P.fb_h - 1) / P.tile;
 }
 
 /* Which bins list a primitive: its tiles (0), its coarse bins, or the global bin. */
-__device__ __forceinline__ int binClass(int tx0, int ty0, int tx1, int ty1) {
+__device__ __forceinline__ int binClass(const DrawParams &P, int tx0, int ty0, int tx1, int ty1) {
   if ((tx1 - tx0 + 1) * (ty1 - ty0 + 1) <= kBinFine) return 0;
-  if ((tx1 / kCoarse - tx0 / kCoarse + 1) * (ty1 / kCoarse - ty0 / kCoarse + 1) <= kBinCoarse) return kPrimCoarse;
+  const int c = P.coarse_tiles;
+  if ((tx1 / c - tx0 / c + 1) * (ty1 / c - ty0 / c + 1) <= kBinCoarse) return kPrimCoarse;
   return kPrimGlobal;
 }
 
@@ -243,8 +244,8 @@ __device__ __forceinline__ void binVisit(const DrawParams &P, int id, int cls, i
   if (cls == kPrimGlobal) {
     put(P.nbins - 1);
   } else if (cls == kPrimCoarse) {
-    for (int cy = ty0 / kCoarse; cy <= ty1 / kCoarse; cy++)
-      for (int cx = tx0 / kCoarse; cx <= tx1 / kCoarse; cx++) put(P.ntiles + cy * P.coarse_x + cx);
+    for (int cy = ty0 / P.coarse_tiles; cy <= ty1 / P.coarse_tiles; cy++)
+      for (int cx = tx0 / P.coarse_tiles; cx <= tx1 / P.coarse_tiles; cx++) put(P.ntiles + cy * P.coarse_x + cx);
   } else {
     for (int ty = ty0; ty <= ty1; ty++)
       for (int tx = tx0; tx <= tx1; tx++) put(ty * P.tiles_x + tx);
@@ -311,8 +312,8 @@ __device__ BinRect setupPrim(const DrawParams &P, int id, int i0, int i1, int i2
     const int x0 = max(fx, 0), x1 = min(lx + 1, P.fb_w - 1);
     const int y0 = max(fy, 0), y1 = min(ly + 1, P.fb_h - 1);
     if (x0 <= x1 && y0 <= y1) {
-      r = {0, x0 / kTile, y0 / kTile, x1 / kTile, y1 / kTile};
-      r.cls = binClass(r.tx0, r.ty0, r.tx1, r.ty1);
+      r = {0, x0 / P.tile, y0 / P.tile, x1 / P.tile, y1 / P.tile};
+      r.cls = binClass(P, r.tx0, r.ty0, r.tx1, r.ty1);
       flags |= kPrimLive | r.cls;
     }
   }
@@ -560,7 +561,7 @@ struct TileBins {
 
 __device__ __forceinline__ TileBins tileBins(const DrawParams &P, int tile) {
   const int tx = tile % P.tiles_x, ty = tile / P.tiles_x;
-  const int c = P.ntiles + (ty / kCoarse) * P.coarse_x + tx / kCoarse;
+  const int c = P.ntiles + (ty / P.coarse_tiles) * P.coarse_x + tx / P.coarse_tiles;
   const int g = P.nbins - 1;
   TileBins b;
   b.s[0] = P.bin_start[tile];
@@ -697,7 +698,7 @@ __device__ __forceinline__ bool varyUsed(const DrawParams &P, int j) {
   if (j < 2) return true;
   if (j >= 4 && j <= 6) return !nm;
   if (j >= 12 && j <= 14) return true;
-  if (j >= 16 && j <= 18) return true;
+  if (j >= 16 && j <= 18) return SHADING != RHS_SHADING_NEURAL_PBR; /* the learned shader has no point light */
   if (SHADING == RHS_SHADING_PBR || SHADING == RHS_SHADING_NEURAL_PBR)
     return nm && ((j >= 20 && j <= 22) || (j >= 24 && j <= 26));
   if (j >= 20 && j <= 23) return P.u.enable_shadow != 0;
@@ -931,8 +932,9 @@ __global__ __launch_bounds__(256) void k_hiz_coarse(DrawParams P) {
   __shared__ int s_cnt;
   const int c = blockIdx.x;
   const int cx = c % P.coarse_x, cy = c / P.coarse_x;
-  const int x0 = cx * kCoarse * kTile, y0 = cy * kCoarse * kTile;
-  const int x1 = min(x0 + kCoarse * kTile, P.fb_w) - 1, y1 = min(y0 + kCoarse * kTile, P.fb_h) - 1;
+  const int cpx = P.coarse_tiles * P.tile;
+  const int x0 = cx * cpx, y0 = cy * cpx;
+  const int x1 = min(x0 + cpx, P.fb_w) - 1, y1 = min(y0 + cpx, P.fb_h) - 1;
   const int gb = P.nbins - 1, cb = P.ntiles + c;
   const int cs = P.bin_start[cb], cn = P.bin_start[cb + 1] - cs;
   const int gs = P.bin_start[gb], gn = P.bin_start[gb + 1] - gs;
@@ -970,32 +972,41 @@ __global__ __launch_bounds__(256) void k_hiz_coarse(DrawParams P) {
   }
 }
 
-template <int S, int MODE>
-__global__ __launch_bounds__(256) void k_raster_vis(DrawParams P) {
-  constexpr int NPIX = kTile * kTile;
+/* One tile of the two-phase raster: TILE x TILE pixels, 64 * WAVES lanes,
+ * PPL pixels per lane (lane l owns pixels l, l + lanes, ...). */
+template <int S, int MODE, int TILE, int WAVES>
+__device__ void visTile(const DrawParams &P, int tile) {
+  constexpr int NPIX = TILE * TILE, NT = 64 * WAVES, PPL = NPIX / NT;
+  static_assert(PPL >= 1 && PPL * NT == NPIX, "tile and wavefronts do not divide");
   constexpr bool MX = visMax<MODE>();
   __shared__ unsigned long long keys[NPIX * S];
-  __shared__ int big[NPIX];
-  __shared__ uint32_t big_z[NPIX];
+  __shared__ int big[NT];
+  __shared__ uint32_t big_z[NT];
   __shared__ uint32_t s_bound, s_bound2;
   __shared__ int s_n, s_base;
-  const int tile = blockIdx.x;
-  const int tx0 = (tile % P.tiles_x) * kTile, ty0 = (tile / P.tiles_x) * kTile;
-  const int X1 = min(tx0 + kTile, P.fb_w) - 1, Y1 = min(ty0 + kTile, P.fb_h) - 1;
+  const int tx0 = (tile % P.tiles_x) * TILE, ty0 = (tile / P.tiles_x) * TILE;
+  const int X1 = min(tx0 + TILE, P.fb_w) - 1, Y1 = min(ty0 + TILE, P.fb_h) - 1;
   const int lp = threadIdx.x;
-  const int px = tx0 + (lp & (kTile - 1)), py = ty0 + lp / kTile;
-  const bool inFb = px < P.fb_w && py < P.fb_h;
-  const size_t pix = (size_t)py * P.fb_w + px;
   const bool hasC = P.color != nullptr;
   const TileBins tb = tileBins(P, tile);
-  const int coarse = (ty0 / (kCoarse * kTile)) * P.coarse_x + tx0 / (kCoarse * kTile);
+  const int cpx = P.coarse_tiles * TILE;
+  const int coarse = (ty0 / cpx) * P.coarse_x + tx0 / cpx;
   const float lx[4] = {0.375f, 0.875f, 0.125f, 0.625f};
   const float ly[4] = {0.875f, 0.625f, 0.375f, 0.125f};
 
   if (tb.total() == 0) return;
-  float dep[S];
+  __syncthreads(); /* LDS of a previous tile of this block is no longer read */
+  float dep[PPL][S];
 #pragma unroll
-  for (int s = 0; s < S; s++) dep[s] = inFb ? P.depth[pix * S + s] : 0.f;
+  for (int q = 0; q < PPL; q++) {
+    const int pl = lp + q * NT, px = tx0 + pl % TILE, py = ty0 + pl / TILE;
+    const bool in = px < P.fb_w && py < P.fb_h;
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      dep[q][s] = in ? P.depth[((size_t)py * P.fb_w + px) * S + s] : 0.f;
+      keys[pl * S + s] = visInit<MODE>(dep[q][s], in);
+    }
+  }
   /* big triangles: the coarse bin's survivors, or all of its and the global bin's entries */
   const int hc = P.hiz_count[coarse];
   const int nbig = hc >= 0 ? hc : tb.n[1] + tb.n[2];
@@ -1005,17 +1016,19 @@ __global__ __launch_bounds__(256) void k_raster_vis(DrawParams P) {
     s_bound2 = MX ? 0xffffffffu : 0u;
     s_n = 0;
   }
-#pragma unroll
-  for (int s = 0; s < S; s++) keys[lp * S + s] = visInit<MODE>(dep[s], inFb);
   __syncthreads();
-  /* the tile's own depths: atomicMin (LESS) of their deepest, combined below */
-  if (inFb) hizFromDepth<S>(dep, MX, &s_bound2);
+  /* the tile's own depths bound it too */
+#pragma unroll
+  for (int q = 0; q < PPL; q++) {
+    const int pl = lp + q * NT;
+    if (tx0 + pl % TILE < P.fb_w && ty0 + pl / TILE < P.fb_h) hizFromDepth<S>(dep[q], MX, &s_bound2);
+  }
   __syncthreads();
   if (lp == 0) s_bound = MX ? max(s_bound, s_bound2) : min(s_bound, s_bound2);
   __syncthreads();
 
   /* big triangles that cover the tile tighten the bound */
-  for (int i = lp; i < nbig; i += NPIX) {
+  for (int i = lp; i < nbig; i += NT) {
     const int id = bigAt(i);
     const PrimSetup ps = P.prims[id];
     float zmin, zmax;
@@ -1030,16 +1043,22 @@ __global__ __launch_bounds__(256) void k_raster_vis(DrawParams P) {
   const uint32_t bound = s_bound;
   unsigned long long nculled = 0;
 
-  /* phase 1a: surviving big triangles one at a time, one lane per pixel; a
+  /* phase 1a: surviving big triangles one at a time, lanes over pixels; a
    * wavefront skips a triangle no pixel of it can take at its current depths */
-  uint32_t mine = 0; /* this lane's deepest (LESS) / shallowest (GREATER) current depth key */
+  auto laneLimit = [&]() {
+    uint32_t mine = MX ? 0xffffffffu : 0u;
 #pragma unroll
-  for (int s = 0; s < S; s++) {
-    const uint32_t k = (uint32_t)(keys[lp * S + s] >> 32);
-    mine = s == 0 ? k : (MX ? min(mine, k) : max(mine, k));
-  }
+    for (int q = 0; q < PPL; q++)
+#pragma unroll
+      for (int s = 0; s < S; s++) {
+        const uint32_t k = (uint32_t)(keys[(lp + q * NT) * S + s] >> 32);
+        mine = MX ? min(mine, k) : max(mine, k);
+      }
+    return mine;
+  };
+  uint32_t mine = laneLimit();
   uint32_t wave_lim = MX ? waveMinU(mine) : waveMaxU(mine);
-  for (int c0 = 0; c0 < nbig; c0 += NPIX) {
+  for (int c0 = 0; c0 < nbig; c0 += NT) {
     if (c0 + lp < nbig) {
       const int id = bigAt(c0 + lp);
       const PrimSetup &ps = P.prims[id];
@@ -1060,15 +1079,14 @@ __global__ __launch_bounds__(256) void k_raster_vis(DrawParams P) {
       if (MX ? zk < wave_lim : zk > wave_lim) continue; /* wave-uniform */
       const int id = big[j];
       const PrimSetup ps = P.prims[id];
-      if (inFb && px >= ps.qx0 && px <= ps.qx1 && py >= ps.qy0 && py <= ps.qy1) {
-        const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
-        visPixel<S, MODE>(ps, T, id, px, py, &keys[lp * S]);
+      const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
 #pragma unroll
-        for (int s = 0; s < S; s++) {
-          const uint32_t k = (uint32_t)(keys[lp * S + s] >> 32);
-          mine = s == 0 ? k : (MX ? min(mine, k) : max(mine, k));
-        }
+      for (int q = 0; q < PPL; q++) {
+        const int pl = lp + q * NT, px = tx0 + pl % TILE, py = ty0 + pl / TILE;
+        if (px < P.fb_w && py < P.fb_h && px >= ps.qx0 && px <= ps.qx1 && py >= ps.qy0 && py <= ps.qy1)
+          visPixel<S, MODE>(ps, T, id, px, py, &keys[pl * S]);
       }
+      mine = laneLimit();
       wave_lim = MX ? waveMinU(mine) : waveMaxU(mine);
     }
     __syncthreads();
@@ -1086,7 +1104,7 @@ __global__ __launch_bounds__(256) void k_raster_vis(DrawParams P) {
   const uint32_t bound2 = MX ? max(bound, s_bound2) : min(bound, s_bound2);
 
   /* phase 1b: the tile's own triangles, one lane per triangle */
-  for (int i = lp; i < tb.n[0]; i += NPIX) {
+  for (int i = lp; i < tb.n[0]; i += NT) {
     const int id = P.bin_items[tb.s[0] + i];
     const PrimSetup ps = P.prims[id];
     if (zCulled(ps, bound2, MX)) {
@@ -1099,26 +1117,31 @@ __global__ __launch_bounds__(256) void k_raster_vis(DrawParams P) {
     const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
     for (int y = y0; y <= y1; y++)
       for (int x = x0; x <= x1; x++)
-        visPixel<S, MODE>(ps, T, id, x, y, &keys[((y - ty0) * kTile + (x - tx0)) * S]);
+        visPixel<S, MODE>(ps, T, id, x, y, &keys[((y - ty0) * TILE + (x - tx0)) * S]);
   }
   __syncthreads();
 
   /* phase 2: each pixel's distinct winners write their depth here; colour is
    * queued, one job per (pixel, winner), for k_shade */
-  uint32_t win[S];
-#pragma unroll
-  for (int s = 0; s < S; s++) {
-    const uint32_t low = (uint32_t)keys[lp * S + s];
-    const uint32_t ord = (MODE == VIS_LESS || MODE == VIS_GEQUAL) ? low : ~low;
-    win[s] = (inFb && ord) ? ord - 1u : 0xffffffffu;
-  }
+  uint32_t win[PPL][S];
   int nj = 0;
 #pragma unroll
-  for (int s = 0; s < S; s++) {
-    bool seen = win[s] == 0xffffffffu;
+  for (int q = 0; q < PPL; q++) {
+    const int pl = lp + q * NT, px = tx0 + pl % TILE, py = ty0 + pl / TILE;
+    const bool in = px < P.fb_w && py < P.fb_h;
 #pragma unroll
-    for (int t = 0; t < s; t++) seen = seen || win[t] == win[s];
-    nj += seen ? 0 : 1;
+    for (int s = 0; s < S; s++) {
+      const uint32_t low = (uint32_t)keys[pl * S + s];
+      const uint32_t ord = (MODE == VIS_LESS || MODE == VIS_GEQUAL) ? low : ~low;
+      win[q][s] = (in && ord) ? ord - 1u : 0xffffffffu;
+    }
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      bool seen = win[q][s] == 0xffffffffu;
+#pragma unroll
+      for (int t = 0; t < s; t++) seen = seen || win[q][t] == win[q][s];
+      nj += seen ? 0 : 1;
+    }
   }
   int base = 0;
   if (hasC) {
@@ -1130,14 +1153,18 @@ __global__ __launch_bounds__(256) void k_raster_vis(DrawParams P) {
     __syncthreads();
     base = s_base + off;
   }
-  if (inFb) {
+#pragma unroll
+  for (int q = 0; q < PPL; q++) {
+    const int pl = lp + q * NT, px = tx0 + pl % TILE, py = ty0 + pl / TILE;
+    if (px >= P.fb_w || py >= P.fb_h) continue;
+    const size_t pix = (size_t)py * P.fb_w + px;
 #pragma unroll
     for (int s = 0; s < S; s++) {
-      const uint32_t w = win[s];
+      const uint32_t w = win[q][s];
       if (w == 0xffffffffu) continue;
       bool seen = false;
 #pragma unroll
-      for (int t = 0; t < s; t++) seen = seen || win[t] == w;
+      for (int t = 0; t < s; t++) seen = seen || win[q][t] == w;
       if (seen) continue;
       const int id = (int)(w >> 2);
       const PrimSetup ps = P.prims[id];
@@ -1145,21 +1172,26 @@ __global__ __launch_bounds__(256) void k_raster_vis(DrawParams P) {
       uint32_t m = 0;
 #pragma unroll
       for (int t = s; t < S; t++) {
-        if (win[t] != w) continue;
+        if (win[q][t] != w) continue;
         m |= 1u << t;
         float4 bc;
         if (S == 4)
           bary(T, lx[t] + (float)px, ly[t] + (float)py, bc);
         else
           bary(T, (float)px + 0.5f, (float)py + 0.5f, bc);
-        dep[t] = gclamp(dpps3(bc, T.z), 0.f, 1.f);
+        dep[q][t] = gclamp(dpps3(bc, T.z), 0.f, 1.f);
       }
       if (hasC) P.jobs[base++] = make_uint2((uint32_t)pix | (m << 28), w);
     }
 #pragma unroll
-    for (int s = 0; s < S; s++) P.depth[pix * S + s] = dep[s];
+    for (int s = 0; s < S; s++) P.depth[pix * S + s] = dep[q][s];
   }
   if (nculled) atomicAdd(&P.counters[6], nculled);
+}
+
+template <int S, int MODE, int TILE, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_raster_vis(DrawParams P) {
+  visTile<S, MODE, TILE, WAVES>(P, blockIdx.x);
 }
 
 /* Colour of the fragments the two-phase raster queued: one lane per (pixel,

@@ -56,6 +56,8 @@ enum {
   kRecMaterial = 13,
   kRecTarget = 14, /* 3: FP64 plain shader, clamped to [0, 1] */
   kRecPlain = 17,  /* 3: the GPU plain shader, clamped to [0, 1] */
+  kRecPixel = 20,  /* pixel index y * width + x */
+  kRecMask = 21,   /* samples the fragment covers (bit mask) */
 };
 
 __device__ __forceinline__ f32x4 mma16(bf16x8 a, bf16x8 b, f32x4 c) {
@@ -79,13 +81,16 @@ __device__ __forceinline__ void nnFeatures(f3 N, f3 V, float rough, float metal,
   f[6] = nv, f[7] = rough, f[8] = metal;
   f[9] = ar, f[10] = ag, f[11] = ab;
   f[12] = R.x, f[13] = R.y, f[14] = R.z;
+  /* sin(pi R), then sin(2 pi R) and sin(4 pi R) by the double-angle identities */
   const float pi = 3.14159265358979f;
+  const float r3[3] = {R.x, R.y, R.z};
 #pragma unroll
-  for (int b = 0; b < 3; b++) {
-    const float s = pi * (float)(1 << b);
-    f[15 + 3 * b] = sinf(s * R.x);
-    f[16 + 3 * b] = sinf(s * R.y);
-    f[17 + 3 * b] = sinf(s * R.z);
+  for (int i = 0; i < 3; i++) {
+    const float s1 = __sinf(pi * r3[i]), c1 = __cosf(pi * r3[i]);
+    const float s2 = 2.f * s1 * c1, c2 = 1.f - 2.f * s1 * s1;
+    f[15 + i] = s1;
+    f[18 + i] = s2;
+    f[21 + i] = 2.f * s2 * c2;
   }
   f[24] = 1.f;
 #pragma unroll
@@ -97,30 +102,74 @@ __device__ __forceinline__ void recFeatures(const float *r, float *f) {
              r[kRecMetal], r[kRecAlbedo], r[kRecAlbedo + 1], r[kRecAlbedo + 2], f);
 }
 
-/* Per-wavefront LDS of the forward pass: X[64][32], H[64][64] (bf16), Y[64][4] (f32). */
+/* Inference LDS. Rows are padded by 8 bf16 so that the 16 rows one MFMA operand
+ * load touches start in different banks. */
+constexpr int kXs = kNnK1 + 8, kHs = kNnHid + 8;
 struct NnWaveLds {
-  __bf16 x[64 * kNnK1];
-  __bf16 h[64 * kNnHid];
+  __bf16 x[64 * kXs];
+  __bf16 h[64 * kHs];
   float y[64 * 4];
 };
+/* the block's copy of the packed operands (read by all its wavefronts) */
+struct NnBlockW {
+  __bf16 w1[kNnHid * kXs];
+  __bf16 w2[kNnHid * kHs];
+  __bf16 w3[16 * kHs];
+  float b2[kNnHid];
+  float b3[16];
+};
+
+/* Copies the packed operands into the block's LDS (all threads of the block). */
+__device__ __forceinline__ void nnLoadBlockW(const NnPacked &W, NnBlockW &B) {
+  for (int i = threadIdx.x; i < kNnHid * kNnK1 / 8; i += blockDim.x) {
+    const int r = i / (kNnK1 / 8), k = (i % (kNnK1 / 8)) * 8;
+    *(bf16x8 *)&B.w1[r * kXs + k] = *(const bf16x8 *)&W.w1[r * kNnK1 + k];
+  }
+  for (int i = threadIdx.x; i < kNnHid * kNnHid / 8; i += blockDim.x) {
+    const int r = i / (kNnHid / 8), k = (i % (kNnHid / 8)) * 8;
+    *(bf16x8 *)&B.w2[r * kHs + k] = *(const bf16x8 *)&W.w2[r * kNnHid + k];
+  }
+  for (int i = threadIdx.x; i < 16 * kNnHid / 8; i += blockDim.x) {
+    const int r = i / (kNnHid / 8), k = (i % (kNnHid / 8)) * 8;
+    *(bf16x8 *)&B.w3[r * kHs + k] = *(const bf16x8 *)&W.w3[r * kNnHid + k];
+  }
+  for (int i = threadIdx.x; i < kNnHid; i += blockDim.x) B.b2[i] = W.b2[i];
+  for (int i = threadIdx.x; i < 16; i += blockDim.x) B.b3[i] = W.b3[i];
+}
+
+/* four consecutive units of a result tile, ReLU'd, as one 8-byte store */
+__device__ __forceinline__ void storeRelu4(__bf16 *dst, f32x4 d) {
+  typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+  bf16x4 o;
+#pragma unroll
+  for (int v = 0; v < 4; v++) o[v] = (__bf16)gmax(d[v], 0.f);
+  *(bf16x4 *)dst = o;
+}
 
 /* Forward pass of the 64 fragments of a wavefront: lane l's features `f`
  * (kNnK1 floats) in, lane l's RGB out. Every lane of the wavefront must call it. */
-__device__ __forceinline__ void nnForward64(const NnPacked &W, NnWaveLds &L, const float *f, float *out) {
+__device__ __forceinline__ void nnForward64(const NnBlockW &W, NnWaveLds &L, const float *f, float *out) {
   const int l = threadIdx.x & 63, g = l >> 4, c = l & 15;
 #pragma unroll
-  for (int k = 0; k < kNnK1; k++) L.x[l * kNnK1 + k] = (__bf16)f[k];
+  for (int k0 = 0; k0 < kNnK1; k0 += 8) {
+    bf16x8 v;
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = (__bf16)f[k0 + k];
+    *(bf16x8 *)&L.x[l * kXs + k0] = v;
+  }
   waveSync();
   /* layer 1: H1^T = W1 . X^T (bias in column 24) */
+  bf16x8 xb[4];
+#pragma unroll
+  for (int n = 0; n < 4; n++) xb[n] = ldOp(L.x, kXs, 16 * n, 0);
 #pragma unroll
   for (int m = 0; m < 4; m++) {
-    const bf16x8 a = ldOp(W.w1, kNnK1, 16 * m, 0);
+    const bf16x8 a = ldOp(W.w1, kXs, 16 * m, 0);
 #pragma unroll
     for (int n = 0; n < 4; n++) {
       f32x4 d = {0.f, 0.f, 0.f, 0.f};
-      d = mma16(a, ldOp(L.x, kNnK1, 16 * n, 0), d);
-#pragma unroll
-      for (int v = 0; v < 4; v++) L.h[(16 * n + c) * kNnHid + 16 * m + 4 * g + v] = (__bf16)gmax(d[v], 0.f);
+      d = mma16(a, xb[n], d);
+      storeRelu4(&L.h[(16 * n + c) * kHs + 16 * m + 4 * g], d);
     }
   }
   waveSync();
@@ -129,41 +178,36 @@ __device__ __forceinline__ void nnForward64(const NnPacked &W, NnWaveLds &L, con
 #pragma unroll
   for (int n = 0; n < 4; n++)
 #pragma unroll
-    for (int kb = 0; kb < 2; kb++) hb[n][kb] = ldOp(L.h, kNnHid, 16 * n, 32 * kb);
+    for (int kb = 0; kb < 2; kb++) hb[n][kb] = ldOp(L.h, kHs, 16 * n, 32 * kb);
   waveSync();
 #pragma unroll
   for (int m = 0; m < 4; m++) {
-    const bf16x8 a0 = ldOp(W.w2, kNnHid, 16 * m, 0), a1 = ldOp(W.w2, kNnHid, 16 * m, 32);
+    const bf16x8 a0 = ldOp(W.w2, kHs, 16 * m, 0), a1 = ldOp(W.w2, kHs, 16 * m, 32);
     f32x4 bias;
 #pragma unroll
     for (int v = 0; v < 4; v++) bias[v] = W.b2[16 * m + 4 * g + v];
 #pragma unroll
-    for (int n = 0; n < 4; n++) {
-      f32x4 d = mma16(a1, hb[n][1], mma16(a0, hb[n][0], bias));
-#pragma unroll
-      for (int v = 0; v < 4; v++) L.h[(16 * n + c) * kNnHid + 16 * m + 4 * g + v] = (__bf16)gmax(d[v], 0.f);
-    }
+    for (int n = 0; n < 4; n++) storeRelu4(&L.h[(16 * n + c) * kHs + 16 * m + 4 * g],
+                                           mma16(a1, hb[n][1], mma16(a0, hb[n][0], bias)));
   }
   waveSync();
   /* layer 3: Y^T = W3 . H2^T + b3 (rows 0..2) */
   {
-    const bf16x8 a0 = ldOp(W.w3, kNnHid, 0, 0), a1 = ldOp(W.w3, kNnHid, 0, 32);
+    const bf16x8 a0 = ldOp(W.w3, kHs, 0, 0), a1 = ldOp(W.w3, kHs, 0, 32);
     f32x4 bias;
 #pragma unroll
     for (int v = 0; v < 4; v++) bias[v] = W.b3[4 * g + v];
 #pragma unroll
     for (int n = 0; n < 4; n++) {
-      f32x4 d = mma16(a1, ldOp(L.h, kNnHid, 16 * n, 32), mma16(a0, ldOp(L.h, kNnHid, 16 * n, 0), bias));
-      if (g == 0) {
-#pragma unroll
-        for (int v = 0; v < 4; v++) L.y[(16 * n + c) * 4 + v] = d[v];
-      }
+      const f32x4 d = mma16(a1, ldOp(L.h, kHs, 16 * n, 32), mma16(a0, ldOp(L.h, kHs, 16 * n, 0), bias));
+      if (g == 0) *(f32x4 *)&L.y[(16 * n + c) * 4] = d;
     }
   }
   waveSync();
-  out[0] = L.y[l * 4 + 0];
-  out[1] = L.y[l * 4 + 1];
-  out[2] = L.y[l * 4 + 2];
+  const f32x4 y = *(const f32x4 *)&L.y[l * 4];
+  out[0] = y[0];
+  out[1] = y[1];
+  out[2] = y[2];
   waveSync();
 }
 
@@ -289,6 +333,7 @@ __global__ __launch_bounds__(256) void k_capture(DrawParams P, int need_deriv, f
     } else {
       bary(T, fx + 0.5f, fy + 0.5f, bs);
     }
+    const uint32_t smask = jb.x >> 28;
     const Frag fr = makeFrag<S, RHS_SHADING_PBR>(P, ps, T, px, py, bs, inst, need_deriv);
     const PbrInputs in = pbrInputs(P, fr);
     const RGBAf plain = fsPbr(P, fr);
@@ -309,7 +354,9 @@ __global__ __launch_bounds__(256) void k_capture(DrawParams P, int need_deriv, f
     r[kRecPlain] = gclamp(plain.r, 0.f, 1.f);
     r[kRecPlain + 1] = gclamp(plain.g, 0.f, 1.f);
     r[kRecPlain + 2] = gclamp(plain.b, 0.f, 1.f);
-    for (int q = kRecPlain + 3; q < kRecFloats; q++) r[q] = 0.f;
+    r[kRecPixel] = (float)pix;
+    r[kRecMask] = (float)smask;
+    for (int q = kRecMask + 1; q < kRecFloats; q++) r[q] = 0.f;
     float4 *o = (float4 *)(rec + (size_t)k * kRecFloats);
 #pragma unroll
     for (int q = 0; q < kRecFloats / 4; q++) o[q] = make_float4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
@@ -318,73 +365,139 @@ __global__ __launch_bounds__(256) void k_capture(DrawParams P, int need_deriv, f
 
 /* --------------------------------------------------------------------------- inference in the pipeline */
 
-/* Colour of the fragments the two-phase raster queued, by the network: each
- * lane gathers its fragment's inputs (albedo, normal and metal-rough maps as
- * the plain shader samples them), then its wavefront evaluates the MLP for 64
- * fragments at once. */
+/* Learned shading, part 1: the inputs of every fragment the two-phase raster
+ * queued (albedo, normal and metal-rough maps sampled as the plain shader
+ * samples them), as a 32-wide bf16 row: 24 features, the bias 1, zeros, and
+ * the albedo alpha in the last column (which the network's weights ignore). */
 template <int S>
-__global__ __launch_bounds__(256) void k_shade_neural(DrawParams P, int need_deriv, const NnPacked *W) {
-  __shared__ NnWaveLds lds[4];
+__global__ __launch_bounds__(256) void k_nn_features(DrawParams P, int need_deriv, __bf16 *feat) {
+  const int n = P.dcount[2];
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const uint2 jb = P.jobs[j];
+    const uint32_t pix = jb.x & 0x0fffffffu, w = jb.y;
+    const int px = (int)(pix % (uint32_t)P.fb_w), py = (int)(pix / (uint32_t)P.fb_w);
+    const int id = (int)(w >> 2), inst = (int)(w & 3u);
+    const PrimSetup ps = P.prims[id];
+    const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
+    const float fx = (float)px, fy = (float)py;
+    float4 bs;
+    if (S == 4) {
+      if (!bary(T, fx + 0.5f, fy + 0.5f, bs)) {
+#pragma unroll
+        for (int t = 3; t >= 0; t--) {
+          float4 b;
+          if (bary(T, sampleX(t) + fx, sampleY(t) + fy, b)) bs = b;
+        }
+      }
+    } else {
+      bary(T, fx + 0.5f, fy + 0.5f, bs);
+    }
+    const Frag fr = makeFrag<S, RHS_SHADING_NEURAL_PBR>(P, ps, T, px, py, bs, inst, need_deriv);
+    const f2 uv{fr.v[0], fr.v[1]};
+    const TexDesc *ta = P.tex[RHS_SLOT_ALBEDO];
+    const RGBAf alb = (P.defines & RHS_DEF_ALBEDO_MAP)
+                          ? tex2D(ta, uv, lodFor(ta, fr))
+                          : RGBAf{P.u.base_color[0], P.u.base_color[1], P.u.base_color[2], P.u.base_color[3]};
+    float metal = 0.f, rough = 1.f;
+    if (P.defines & RHS_DEF_METALROUGHNESS_MAP) {
+      const RGBAf mr = tex2D(P.tex[RHS_SLOT_METAL_ROUGHNESS], uv, 0.f);
+      metal = mr.b;
+      rough = mr.g;
+    }
+    const f3 N = normalFromMap(P, fr, 20, 24);
+    const f3 V = nrm3({fr.v[12], fr.v[13], fr.v[14]});
+    float f[kNnK1];
+    nnFeatures(N, V, rough, metal, alb.r, alb.g, alb.b, f);
+    f[kNnK1 - 1] = alb.a;
+    bf16x8 *o = (bf16x8 *)(feat + (size_t)j * kNnK1);
+#pragma unroll
+    for (int q = 0; q < kNnK1 / 8; q++) {
+      bf16x8 v;
+#pragma unroll
+      for (int k = 0; k < 8; k++) v[k] = (__bf16)f[8 * q + k];
+      o[q] = v;
+    }
+  }
+}
+
+/* Learned shading, part 2: the network on 64 fragments per wavefront, layer-1
+ * operands read straight from the feature rows; the colour goes to every
+ * sample the fragment won. */
+template <int S>
+__global__ __launch_bounds__(256) void k_nn_infer(DrawParams P, const NnPacked *W, const __bf16 *feat) {
+  __shared__ __bf16 hbuf[4][64 * kHs];
   __shared__ unsigned long long s_fs, s_wr;
   if (threadIdx.x == 0) s_fs = s_wr = 0;
   __syncthreads();
-  NnWaveLds &L = lds[threadIdx.x >> 6];
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 4, c = l & 15;
+  __bf16 *H = hbuf[wave];
   const int n = P.dcount[2];
+  const int nchunks = (n + 63) / 64;
   unsigned long long nfs = 0, nwr = 0;
-  for (int b0 = blockIdx.x * blockDim.x; b0 < n; b0 += gridDim.x * blockDim.x) {
-    const int j = b0 + threadIdx.x;
-    const bool act = j < n;
-    float f[kNnK1];
-    uint32_t pix = 0, m = 0;
-    float alpha = 1.f;
-    if (act) {
-      const uint2 jb = P.jobs[j];
-      pix = jb.x & 0x0fffffffu;
-      m = jb.x >> 28;
-      const uint32_t w = jb.y;
-      const int px = (int)(pix % (uint32_t)P.fb_w), py = (int)(pix / (uint32_t)P.fb_w);
-      const int id = (int)(w >> 2), inst = (int)(w & 3u);
-      const PrimSetup ps = P.prims[id];
-      const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
-      const float fx = (float)px, fy = (float)py;
-      float4 bs;
-      if (S == 4) {
-        if (!bary(T, fx + 0.5f, fy + 0.5f, bs)) {
-#pragma unroll
-          for (int t = 3; t >= 0; t--) {
-            float4 b;
-            if (bary(T, sampleX(t) + fx, sampleY(t) + fy, b)) bs = b;
-          }
-        }
-      } else {
-        bary(T, fx + 0.5f, fy + 0.5f, bs);
-      }
-      const Frag fr = makeFrag<S, RHS_SHADING_NEURAL_PBR>(P, ps, T, px, py, bs, inst, need_deriv);
-      /* the inputs pbrInputs() reads, without the gamma of the albedo */
-      const f2 uv{fr.v[0], fr.v[1]};
-      const TexDesc *ta = P.tex[RHS_SLOT_ALBEDO];
-      const RGBAf alb = (P.defines & RHS_DEF_ALBEDO_MAP)
-                            ? tex2D(ta, uv, lodFor(ta, fr))
-                            : RGBAf{P.u.base_color[0], P.u.base_color[1], P.u.base_color[2], P.u.base_color[3]};
-      float metal = 0.f, rough = 1.f;
-      if (P.defines & RHS_DEF_METALROUGHNESS_MAP) {
-        const RGBAf mr = tex2D(P.tex[RHS_SLOT_METAL_ROUGHNESS], uv, 0.f);
-        metal = mr.b;
-        rough = mr.g;
-      }
-      const f3 N = normalFromMap(P, fr, 20, 24);
-      const f3 V = nrm3({fr.v[12], fr.v[13], fr.v[14]});
-      nnFeatures(N, V, rough, metal, alb.r, alb.g, alb.b, f);
-      alpha = alb.a;
-    } else {
-#pragma unroll
-      for (int k = 0; k < kNnK1; k++) f[k] = 0.f;
+  for (int ch = blockIdx.x * 4 + wave; ch < nchunks; ch += gridDim.x * 4) {
+    /* this lane's fragment: its job record and alpha, fetched while the network runs */
+    const int jme = ch * 64 + l;
+    const bool mine = jme < n;
+    uint32_t jx = 0;
+    float alpha = 0.f;
+    if (mine) {
+      jx = P.jobs[jme].x;
+      alpha = (float)feat[(size_t)jme * kNnK1 + kNnK1 - 1];
     }
-    float y[3];
-    nnForward64(*W, L, f, y);
-    if (act) {
-      const float r = gclamp(y[0], 0.f, 1.f), g = gclamp(y[1], 0.f, 1.f), b = gclamp(y[2], 0.f, 1.f);
-      const uint32_t packed = pack4(u8t(r * 255.f), u8t(g * 255.f), u8t(b * 255.f), u8t(gclamp(alpha, 0.f, 1.f) * 255.f));
+    const __bf16 *X = feat + (size_t)ch * 64 * kNnK1;
+    bf16x8 xb[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) xb[q] = ldOp(X, kNnK1, 16 * q, 0);
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      const bf16x8 a = ldOp(W->w1, kNnK1, 16 * m, 0);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        f32x4 d = {0.f, 0.f, 0.f, 0.f};
+        storeRelu4(&H[(16 * q + c) * kHs + 16 * m + 4 * g], mma16(a, xb[q], d));
+      }
+    }
+    waveSync();
+    bf16x8 hb[4][2];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int kb = 0; kb < 2; kb++) hb[q][kb] = ldOp(H, kHs, 16 * q, 32 * kb);
+    waveSync();
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      const bf16x8 a0 = ldOp(W->w2, kNnHid, 16 * m, 0), a1 = ldOp(W->w2, kNnHid, 16 * m, 32);
+      f32x4 bias;
+#pragma unroll
+      for (int v = 0; v < 4; v++) bias[v] = W->b2[16 * m + 4 * g + v];
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        storeRelu4(&H[(16 * q + c) * kHs + 16 * m + 4 * g], mma16(a1, hb[q][1], mma16(a0, hb[q][0], bias)));
+    }
+    waveSync();
+    /* layer 3: lanes 0..15 hold channels 0..3 of fragments 16q + c */
+    f32x4 yq[4];
+    {
+      const bf16x8 a0 = ldOp(W->w3, kNnHid, 0, 0), a1 = ldOp(W->w3, kNnHid, 0, 32);
+      f32x4 bias;
+#pragma unroll
+      for (int v = 0; v < 4; v++) bias[v] = W->b3[4 * g + v];
+#pragma unroll
+      for (int q = 0; q < 4; q++) yq[q] = mma16(a1, ldOp(H, kHs, 16 * q, 32), mma16(a0, ldOp(H, kHs, 16 * q, 0), bias));
+    }
+    waveSync();
+    /* outputs to their fragments' lanes through the (now free) hidden-layer LDS */
+    float *Y = (float *)H;
+    if (g == 0) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) *(f32x4 *)&Y[(16 * q + c) * 4] = yq[q];
+    }
+    waveSync();
+    if (mine) {
+      const f32x4 y = *(const f32x4 *)&Y[l * 4];
+      const uint32_t pix = jx & 0x0fffffffu, m = jx >> 28;
+      const uint32_t packed = pack4(u8t(gclamp(y[0], 0.f, 1.f) * 255.f), u8t(gclamp(y[1], 0.f, 1.f) * 255.f),
+                                    u8t(gclamp(y[2], 0.f, 1.f) * 255.f), u8t(gclamp(alpha, 0.f, 1.f) * 255.f));
       nfs++;
 #pragma unroll
       for (int t = 0; t < S; t++)
@@ -393,6 +506,7 @@ __global__ __launch_bounds__(256) void k_shade_neural(DrawParams P, int need_der
           nwr++;
         }
     }
+    waveSync();
   }
   if (nfs) atomicAdd(&s_fs, nfs);
   if (nwr) atomicAdd(&s_wr, nwr);
@@ -420,10 +534,12 @@ __global__ __launch_bounds__(256) void k_nn_eval(const NnPacked *W, const float 
                                                  int material, unsigned int *hist_nn, unsigned int *hist_plain,
                                                  double *sums, float *out) {
   __shared__ NnWaveLds lds[4];
+  __shared__ NnBlockW bw;
   __shared__ unsigned int h_nn[kNnHistBins], h_pl[kNnHistBins];
   __shared__ double s_sum[6];
   for (int i = threadIdx.x; i < kNnHistBins; i += blockDim.x) h_nn[i] = h_pl[i] = 0;
   if (threadIdx.x < 6) s_sum[threadIdx.x] = 0;
+  nnLoadBlockW(*W, bw);
   __syncthreads();
   NnWaveLds &L = lds[threadIdx.x >> 6];
   double e_nn = 0, e_pl = 0, m_nn = 0, m_pl = 0, n_ok = 0;
@@ -434,7 +550,7 @@ __global__ __launch_bounds__(256) void k_nn_eval(const NnPacked *W, const float 
     float f[kNnK1];
     recFeatures(r, f);
     float y[3];
-    nnForward64(*W, L, f, y);
+    nnForward64(bw, L, f, y);
     if (out && i < count) {
       out[(size_t)i * 3 + 0] = y[0];
       out[(size_t)i * 3 + 1] = y[1];
