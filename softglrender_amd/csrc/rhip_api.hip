@@ -123,7 +123,7 @@ struct rhip_device_s {
   int cur_fbo = 0;
   int vp_x = 0, vp_y = 0, vp_w = 0, vp_h = 0;
   /* per-draw arenas */
-  DevBuf clip, frag, mask, vary, clip_raw, clip_poly, blk, prims, bins, bin_items, scalars, jobs, hiz;
+  DevBuf clip, frag, mask, vary, clip_raw, clip_poly, fan_first, blk, prims, bins, bin_items, scalars, jobs, hiz;
   unsigned long long *counters = nullptr; /* device */
   /* optional per-kernel timing: an event pair around every launch */
   bool profiling = false;
@@ -812,6 +812,7 @@ int rhip_draw(rhip_device d, const rhip_draw_desc *dd) {
   if ((rc = d->clip.ensure(vcap * 16)) || (rc = d->frag.ensure(vcap * 16)) || (rc = d->mask.ensure(vcap * 4)) ||
       (rc = d->vary.ensure(vcap * kVaryStride * 4)) || (rc = d->clip_raw.ensure(T * kMaxClipVerts * va.stride)) ||
       (rc = d->clip_poly.ensure(T * (kMaxPoly + 1) * 4)) || (rc = d->blk.ensure((size_t)nblk * 3 * 4)) ||
+      (rc = d->fan_first.ensure(T * 4)) ||
       (rc = d->prims.ensure(pcap * sizeof(PrimSetup))) || (rc = d->bins.ensure(((size_t)P.nbins * 3 + 1) * 4)) ||
       (rc = d->bin_items.ensure(pcap * kMaxBinItems * 4)) || (rc = d->scalars.ensure(64)))
     return rc;
@@ -821,6 +822,7 @@ int rhip_draw(rhip_device d, const rhip_draw_desc *dd) {
   P.vary = d->vary.as<float>();
   P.clip_raw = d->clip_raw.as<float>();
   P.clip_poly = d->clip_poly.as<int>();
+  P.fan_first = d->fan_first.as<int>();
   P.nblk = nblk;
   P.blk_clip = d->blk.as<int>();
   P.blk_fans = P.blk_clip + nblk;
@@ -849,11 +851,12 @@ int rhip_draw(rhip_device d, const rhip_draw_desc *dd) {
   if (twoPhase) {
     const int ncoarse = P.coarse_x * P.coarse_y;
     const size_t njobs = (size_t)fbw * fbh * samples;
-    if ((rc = d->jobs.ensure(njobs * 8)) || (rc = d->hiz.ensure((size_t)ncoarse * (2 + kHizCap) * 4))) return rc;
+    if ((rc = d->jobs.ensure(njobs * 8)) || (rc = d->hiz.ensure((size_t)ncoarse * (3 + kHizCap) * 4))) return rc;
     P.jobs = d->jobs.as<uint2>();
     P.hiz_bound = d->hiz.as<uint32_t>();
     P.hiz_count = (int *)P.hiz_bound + ncoarse;
-    P.hiz_surv = P.hiz_count + ncoarse;
+    P.hiz_seed = P.hiz_count + ncoarse;
+    P.hiz_surv = P.hiz_seed + ncoarse;
     if (samples == 4)
       launchVis<4>(d, P, ntiles, ncoarse, s);
     else

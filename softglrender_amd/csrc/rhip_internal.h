@@ -68,6 +68,7 @@ struct PrimSetup {
   int bcx, bcy;        /* 32-px block counts */
   int qx0, qy0, qx1, qy1; /* pixel range touched by the triangle's quads (inclusive), empty if qx1 < qx0 */
   float zlo, zhi;         /* bounds of every depth the triangle's samples can take */
+  int src;                /* triangle a clipped polygon came from (its fans share it), -1 if not clipped */
 };
 
 /* PrimSetup.flags */
@@ -103,6 +104,7 @@ struct DrawParams {
   float *vary;
   /* clipping: triangle t owns new-vertex slots nverts + t * kMaxClipVerts ... */
   int *clip_poly; /* kMaxPoly vertex ids + count per triangle of a block that clips */
+  int *fan_first; /* per clipped triangle: id of its second fan triangle (the first keeps its id) */
   int *blk_clip;  /* per 256-triangle block: triangles that need clipping */
   int *blk_fans;  /* per block: fan triangles its clipped triangles append */
   int *blk_fan_off;
@@ -123,6 +125,7 @@ struct DrawParams {
   uint32_t *hiz_bound; /* per coarse bin: depth bound from big triangles covering it */
   int *hiz_count;      /* per coarse bin: big triangles that bound leaves, -1 if over kHizCap */
   int *hiz_surv;       /* per coarse bin: those triangles (kHizCap slots) */
+  int *hiz_seed;       /* per coarse bin: nearest clipped polygon covering it, or -1 */
   unsigned long long *counters; /* [0] fs invocations, [1] samples written, [2] clip overflows,
                                    [3] bin items, [4] primitives binned, [5] clipped triangles,
                                    [6] (tile, triangle) pairs skipped by the hierarchical-Z test */

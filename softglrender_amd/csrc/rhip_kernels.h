@@ -260,8 +260,9 @@ struct BinRect {
 
 /* Fills the setup record of primitive `id` from three screen-space vertices;
  * returns the bins it goes to (the caller counts it there). */
-__device__ BinRect setupPrim(const DrawParams &P, int id, int i0, int i1, int i2) {
+__device__ BinRect setupPrim(const DrawParams &P, int id, int i0, int i1, int i2, int src = -1) {
   PrimSetup s;
+  s.src = src;
   s.v[0] = i0;
   s.v[1] = i1;
   s.v[2] = i2;
@@ -473,7 +474,7 @@ __device__ int clipTriangle(const DrawParams &P, int t) {
   }
   for (int q = 0; q < nin; q++) poly[q] = in[q];
   poly[kMaxPoly] = nin;
-  binCount(P, setupPrim(P, t, in[0], in[1], in[2]));
+  binCount(P, setupPrim(P, t, in[0], in[1], in[2], t));
   return nin - 3;
 }
 
@@ -515,7 +516,8 @@ __global__ __launch_bounds__(256) void k_clip_fans(DrawParams P) {
   const int nf = n > 3 ? n - 3 : 0;
   const int inc = blockInclusiveScan<256>(nf, part);
   const int base = P.ntris + P.blk_fan_off[b] + inc - nf;
-  for (int i = 3; i < n; i++) binCount(P, setupPrim(P, base + i - 3, poly[0], poly[i - 1], poly[i]));
+  if (n > 3) P.fan_first[t] = base;
+  for (int i = 3; i < n; i++) binCount(P, setupPrim(P, base + i - 3, poly[0], poly[i - 1], poly[i], t));
 }
 
 /* --------------------------------------------------------------------------- binning */
@@ -820,8 +822,10 @@ __device__ __forceinline__ Tri makeTri(const float4 &f0, const float4 &f1, const
  * beyond the bound cannot win or pass at any sample and is skipped; results are
  * unchanged. GREATER / GEQUAL mirror it. */
 
-/* True if the triangle covers every sample of pixels [x0, x1] x [y0, y1] with at
- * least a pixel of margin, inside its quads and, multisampled, inside the depth
+/* True if the triangle covers every sample of pixels [x0, x1] x [y0, y1] -- all
+ * of which lie in [x0 + 1/8, x1 + 7/8] x [y0 + 1/8, y1 + 7/8] -- with 1/16 of a
+ * pixel to spare (the single-precision edge functions err by about 1e-3 pixel
+ * at 4K coordinates), inside its quads and, multisampled, inside the depth
  * range; [zmin, zmax] then bounds its depth over them. Evaluated in double. */
 template <int S>
 __device__ bool coversRect(const PrimSetup &ps, const float4 &a, const float4 &b, const float4 &c, int x0, int y0,
@@ -836,7 +840,7 @@ __device__ bool coversRect(const PrimSetup &ps, const float4 &a, const float4 &b
   double lo = INFINITY, hi = -INFINITY;
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    const double X = (k & 1) ? x1 + 2.0 : x0 - 1.0, Y = (k & 2) ? y1 + 2.0 : y0 - 1.0;
+    const double X = (k & 1) ? x1 + 0.9375 : x0 + 0.0625, Y = (k & 2) ? y1 + 0.9375 : y0 + 0.0625;
     const double e0 = ((bx - ax) * (Y - ay) - (by - ay) * (X - ax)) * sg;
     const double e1 = ((cx - bx) * (Y - by) - (cy - by) * (X - bx)) * sg;
     const double e2 = ((ax - cx) * (Y - cy) - (ay - cy) * (X - cx)) * sg;
@@ -846,6 +850,58 @@ __device__ bool coversRect(const PrimSetup &ps, const float4 &a, const float4 &b
     hi = fmax(hi, z);
   }
   const double sl = 1e-5 + 16.0 * 1.19209290e-07 * (fabs((double)a.z) + fabs((double)b.z) + fabs((double)c.z));
+  lo -= sl;
+  hi += sl;
+  if (!(lo == lo) || !(hi == hi)) return false;
+  if (S == 4 && (lo < 0.0 || hi > 1.0)) return false;
+  zmin = (float)lo;
+  zmax = (float)hi;
+  return true;
+}
+
+/* The same test for the convex polygon clipped triangle t became (RendererSoft.cpp:489-559);
+ * the depth range is taken over the planes of all its fan triangles. It only chooses which
+ * triangles go first: the bound that culls comes from the depths their fans actually leave. */
+template <int S>
+__device__ bool polyCoversRect(const DrawParams &P, int t, int x0, int y0, int x1, int y1, float &zmin,
+                               float &zmax) {
+  const int *poly = P.clip_poly + (size_t)t * (kMaxPoly + 1);
+  const int n = poly[kMaxPoly];
+  if (n < 3) return false;
+  double vx[kMaxPoly], vy[kMaxPoly], vz[kMaxPoly];
+  double area = 0, zs = 0;
+  for (int k = 0; k < n; k++) {
+    const float4 f = P.frag[poly[k]];
+    vx[k] = f.x, vy[k] = f.y, vz[k] = f.z;
+    zs += fabs((double)f.z);
+  }
+  for (int k = 0; k < n; k++) {
+    const int k1 = k + 1 == n ? 0 : k + 1;
+    area += vx[k] * vy[k1] - vx[k1] * vy[k];
+  }
+  if (!(fabs(area) > 2.0)) return false;
+  const double sg = area > 0 ? 1.0 : -1.0;
+  double lo = INFINITY, hi = -INFINITY;
+  for (int c = 0; c < 4; c++) {
+    const double X = (c & 1) ? x1 + 0.9375 : x0 + 0.0625, Y = (c & 2) ? y1 + 0.9375 : y0 + 0.0625;
+    for (int k = 0; k < n; k++) {
+      const int k1 = k + 1 == n ? 0 : k + 1;
+      const double ex = vx[k1] - vx[k], ey = vy[k1] - vy[k];
+      if (ex * ex + ey * ey < 1e-12) continue; /* clipping can repeat a vertex */
+      const double e = (ex * (Y - vy[k]) - ey * (X - vx[k])) * sg;
+      if (!(e > 0)) return false;
+    }
+    for (int k = 1; k + 1 < n; k++) {
+      const double ar = (vx[k] - vx[0]) * (vy[k + 1] - vy[0]) - (vy[k] - vy[0]) * (vx[k + 1] - vx[0]);
+      if (!(fabs(ar) > 0)) continue;
+      const double dzdx = ((vz[k] - vz[0]) * (vy[k + 1] - vy[0]) - (vz[k + 1] - vz[0]) * (vy[k] - vy[0])) / ar;
+      const double dzdy = ((vz[k + 1] - vz[0]) * (vx[k] - vx[0]) - (vz[k] - vz[0]) * (vx[k + 1] - vx[0])) / ar;
+      const double z = vz[0] + dzdx * (X - vx[0]) + dzdy * (Y - vy[0]);
+      lo = fmin(lo, z);
+      hi = fmax(hi, z);
+    }
+  }
+  const double sl = 1e-5 + 16.0 * 1.19209290e-07 * zs;
   lo -= sl;
   hi += sl;
   if (!(lo == lo) || !(hi == hi)) return false;
@@ -928,8 +984,8 @@ constexpr int kHizCap = 1024;
 
 template <int S, bool MX>
 __global__ __launch_bounds__(256) void k_hiz_coarse(DrawParams P) {
-  __shared__ uint32_t s_bound;
-  __shared__ int s_cnt;
+  __shared__ uint32_t s_bound, s_pz;
+  __shared__ int s_cnt, s_pt;
   const int c = blockIdx.x;
   const int cx = c % P.coarse_x, cy = c / P.coarse_x;
   const int cpx = P.coarse_tiles * P.tile;
@@ -958,6 +1014,34 @@ __global__ __launch_bounds__(256) void k_hiz_coarse(DrawParams P) {
   }
   __syncthreads();
   const uint32_t bound = s_bound;
+  /* the nearest clipped polygon covering the whole bin: its tiles rasterize it first */
+  uint32_t my_pz = MX ? 0u : 0xffffffffu;
+  int my_pt = -1;
+  for (int i = lp; i < n; i += blockDim.x) {
+    const int id = i < cn ? P.bin_items[cs + i] : P.bin_items[gs + i - cn];
+    if (P.prims[id].src != id) continue;
+    float zmin, zmax;
+    if (polyCoversRect<S>(P, id, x0, y0, x1, y1, zmin, zmax)) {
+      const uint32_t k = MX ? depthKey(gclamp(zmin, 0.f, 1.f)) : depthKey(gclamp(zmax, 0.f, 1.f));
+      if (MX ? k > my_pz : k < my_pz) {
+        my_pz = k;
+        my_pt = id;
+      }
+    }
+  }
+  if (lp == 0) {
+    s_pz = MX ? 0u : 0xffffffffu;
+    s_pt = -1;
+  }
+  __syncthreads();
+  if (my_pt >= 0) {
+    if (MX)
+      atomicMax(&s_pz, my_pz);
+    else
+      atomicMin(&s_pz, my_pz);
+  }
+  __syncthreads();
+  if (my_pt >= 0 && my_pz == s_pz) s_pt = my_pt;
   for (int i = lp; i < n; i += blockDim.x) {
     const int id = i < cn ? P.bin_items[cs + i] : P.bin_items[gs + i - cn];
     const PrimSetup &ps = P.prims[id];
@@ -969,6 +1053,7 @@ __global__ __launch_bounds__(256) void k_hiz_coarse(DrawParams P) {
   if (lp == 0) {
     P.hiz_bound[c] = bound;
     P.hiz_count[c] = s_cnt <= kHizCap ? s_cnt : -1;
+    P.hiz_seed[c] = s_pt;
   }
 }
 
@@ -1040,6 +1125,74 @@ __device__ void visTile(const DrawParams &P, int tile) {
     }
   }
   __syncthreads();
+  /* the nearest clipped polygon that covers the tile (no single fan of it may):
+   * its fans go first, and the depths they leave bound what follows */
+  __shared__ uint32_t s_pz;
+  __shared__ int s_pt;
+  if (lp == 0) {
+    s_pz = MX ? 0u : 0xffffffffu;
+    s_pt = -1;
+  }
+  __syncthreads();
+  uint32_t my_pz = MX ? 0u : 0xffffffffu;
+  int my_pt = -1;
+  const int cseed = P.hiz_seed[coarse];
+  for (int i = lp; i < (cseed < 0 && hc < 0 ? nbig : 0); i += NT) {
+    const int id = bigAt(i);
+    const PrimSetup &ps = P.prims[id];
+    if (ps.src != id) continue; /* each polygon once, through its first fan */
+    float zmin, zmax;
+    if (polyCoversRect<S>(P, id, tx0, ty0, X1, Y1, zmin, zmax)) {
+      const uint32_t k = MX ? depthKey(gclamp(zmin, 0.f, 1.f)) : depthKey(gclamp(zmax, 0.f, 1.f));
+      if (MX ? k > my_pz : k < my_pz) {
+        my_pz = k;
+        my_pt = id;
+      }
+    }
+  }
+  if (my_pt >= 0) {
+    if (MX)
+      atomicMax(&s_pz, my_pz);
+    else
+      atomicMin(&s_pz, my_pz);
+  }
+  __syncthreads();
+  if (my_pt >= 0 && my_pz == s_pz) s_pt = my_pt; /* any one of equals */
+  __syncthreads();
+  const int seed = cseed >= 0 ? cseed : s_pt;
+  if (seed >= 0) {
+    const int nf = P.clip_poly[(size_t)seed * (kMaxPoly + 1) + kMaxPoly] - 2;
+    for (int k = 0; k < nf; k++) {
+      const int id = k == 0 ? seed : P.fan_first[seed] + k - 1;
+      const PrimSetup ps = P.prims[id];
+      if (!(ps.flags & kPrimLive)) continue;
+      const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
+#pragma unroll
+      for (int q = 0; q < PPL; q++) {
+        const int pl = lp + q * NT, px = tx0 + pl % TILE, py = ty0 + pl / TILE;
+        if (px < P.fb_w && py < P.fb_h && px >= ps.qx0 && px <= ps.qx1 && py >= ps.qy0 && py <= ps.qy1)
+          visPixel<S, MODE>(ps, T, id, px, py, &keys[pl * S]);
+      }
+    }
+    /* the deepest (LESS) depth any sample holds now bounds the tile */
+    uint32_t lim = MX ? 0xffffffffu : 0u;
+#pragma unroll
+    for (int q = 0; q < PPL; q++)
+#pragma unroll
+      for (int s = 0; s < S; s++) {
+        const uint32_t k = (uint32_t)(keys[(lp + q * NT) * S + s] >> 32);
+        lim = MX ? min(lim, k) : max(lim, k);
+      }
+    if (lp == 0) s_pz = MX ? 0xffffffffu : 0u;
+    __syncthreads();
+    if (MX)
+      atomicMin(&s_pz, lim); /* GREATER: the tile's shallowest sample */
+    else
+      atomicMax(&s_pz, lim); /* LESS: the tile's deepest sample */
+    __syncthreads();
+    if (lp == 0) s_bound = MX ? max(s_bound, s_pz) : min(s_bound, s_pz);
+    __syncthreads();
+  }
   const uint32_t bound = s_bound;
   unsigned long long nculled = 0;
 
