@@ -168,6 +168,51 @@ int rhip_neural_train(rhip_device dev, long long first, long long count, int ste
 int rhip_neural_evaluate(rhip_device dev, long long first, long long count, int material, rhip_nn_error *network,
                          rhip_nn_error *plain, float *outputs /* count x 3, may be NULL */);
 
+/* ---------------------------------------------------------------- batched views
+ * No counterpart in SoftGLRender, whose Viewer renders one view per frame
+ * (Viewer.cpp:100-168): many independent views, each one render pass of one
+ * draw into its own framebuffer, run as one captured HIP graph in which every
+ * stage is one launch over all views and the raster is a persistent kernel
+ * over (view, tile) work items. A view's result is the same as
+ * beginRenderPass(fbo, clear) + setViewPort + draw + endRenderPass on it. */
+typedef struct rhip_batch_view {
+  int fbo;                 /* framebuffer; no two views of a batch share an attachment */
+  int viewport[4];         /* x, y, width, height */
+  rhip_clear_states clear; /* applied first, as beginRenderPass does */
+  rhip_draw_desc draw;     /* uniform blocks are copied at creation */
+} rhip_batch_view;
+
+/* Screen tile (= fine bin) edge in pixels (8, 16, 32 or 64) and wavefronts
+ * per tile workgroup of the raster. */
+typedef struct rhip_batch_config {
+  int tile, waves;
+} rhip_batch_config;
+
+typedef struct rhip_batch_timing {
+  rhip_batch_config config;
+  double ms; /* mean time of one batch */
+} rhip_batch_timing;
+
+/* Views must all take the two-phase raster (opaque, depth test and write with
+ * LESS/LEQUAL/GREATER/GEQUAL) with one sample count and one shading model, and
+ * the resources they use must outlive the batch. config NULL: the
+ * configuration an earlier rhip_batch_tune found for the same workload in this
+ * process, else tile 16 with 4 wavefronts. */
+int rhip_batch_create(rhip_device dev, const rhip_batch_view *views, int n, const rhip_batch_config *config,
+                      int *out_batch);
+int rhip_batch_run(rhip_device dev, int batch); /* queues one replay of the batch's graph */
+/* Autotuner: rebuilds the batch for every valid configuration, times `reps`
+ * replays of each, keeps the fastest (also for later batches of the same
+ * workload) and reports every timing (up to `cap`; `*count` gets the number). */
+int rhip_batch_tune(rhip_device dev, int batch, int reps, rhip_batch_timing *timings, int cap, int *count,
+                    rhip_batch_config *best);
+int rhip_batch_get_config(rhip_device dev, int batch, rhip_batch_config *config);
+int rhip_batch_set_config(rhip_device dev, int batch, const rhip_batch_config *config); /* rebuilds the graph */
+/* Waits, then gives each view's completion time in the last replay, in
+ * milliseconds from the replay's first kernel (device clock). */
+int rhip_batch_view_times(rhip_device dev, int batch, double *ms, int n);
+int rhip_batch_destroy(rhip_device dev, int batch);
+
 int rhip_stats_get(rhip_device dev, rhip_stats *out); /* waits for queued work */
 int rhip_stats_reset(rhip_device dev);
 /* The HIP stream the device queues on (hipStream_t), for event timing. */

@@ -62,6 +62,18 @@ class NnError(ctypes.Structure):
                 ("samples", ctypes.c_longlong)]
 
 
+class BatchView(ctypes.Structure):
+    _fields_ = [("fbo", ctypes.c_int), ("viewport", ctypes.c_int * 4), ("clear", ClearStates), ("draw", DrawDesc)]
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("tile", ctypes.c_int), ("waves", ctypes.c_int)]
+
+
+class BatchTiming(ctypes.Structure):
+    _fields_ = [("config", BatchConfig), ("ms", ctypes.c_double)]
+
+
 # every exported symbol of rhip.h, with its argument types
 SIGNATURES = {
     "rhip_device_create": [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)],
@@ -111,6 +123,15 @@ SIGNATURES = {
                           ctypes.c_float, ctypes.c_float, ctypes.c_uint, ctypes.c_void_p],
     "rhip_neural_evaluate": [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_int,
                              ctypes.POINTER(NnError), ctypes.POINTER(NnError), ctypes.c_void_p],
+    "rhip_batch_create": [ctypes.c_void_p, ctypes.POINTER(BatchView), ctypes.c_int, ctypes.POINTER(BatchConfig),
+                          ctypes.POINTER(ctypes.c_int)],
+    "rhip_batch_run": [ctypes.c_void_p, ctypes.c_int],
+    "rhip_batch_tune": [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(BatchTiming), ctypes.c_int,
+                        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(BatchConfig)],
+    "rhip_batch_get_config": [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(BatchConfig)],
+    "rhip_batch_set_config": [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(BatchConfig)],
+    "rhip_batch_view_times": [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int],
+    "rhip_batch_destroy": [ctypes.c_void_p, ctypes.c_int],
 }
 
 _lib = None

@@ -22,8 +22,9 @@ HIP_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contr
              "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 
 TARGETS = {
-    "librhip.so": ["rhip_api.hip"],
+    "librhip.so": ["rhip_api.hip", "rhip_batch.hip", "rhip_batch_vis1.hip", "rhip_batch_vis4.hip"],
 }
+OBJ_DIR = os.path.join(LIB_DIR, "obj")
 
 HOST = os.path.join(HERE, "host")
 # RendererHIP (the C++ Renderer implementation) and its stream driver, on librhip
@@ -39,14 +40,28 @@ def _run(cmd):
 
 
 def build(verbose: bool = False) -> None:
-    os.makedirs(LIB_DIR, exist_ok=True)
+    os.makedirs(OBJ_DIR, exist_ok=True)
     for lib, srcs in TARGETS.items():
         out = os.path.join(LIB_DIR, lib)
         tmp = out + ".tmp"
-        cmd = [HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *[os.path.join(CSRC, s) for s in srcs]]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        _run(cmd)
+        # translation units compile in parallel, then link
+        objs, procs = [], []
+        for src in srcs:
+            obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
+            cmd = [HIPCC, *HIP_FLAGS, "-c", "-o", obj, os.path.join(CSRC, src)]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+            objs.append(obj)
+        failed = None
+        for cmd, pr in procs:
+            txt, _ = pr.communicate()
+            if pr.returncode != 0:
+                sys.stderr.write(txt)
+                failed = failed or cmd
+        if failed:
+            raise RuntimeError(f"build failed: {' '.join(failed[-2:])}")
+        _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
         os.replace(tmp, out)
     out = os.path.join(LIB_DIR, "librenderer_hip.so")
     cmd = ["g++", *HOST_FLAGS, "-o", out + ".tmp", os.path.join(HOST, "RendererHIP.cpp"), os.path.join(HOST, "replay.cpp"),

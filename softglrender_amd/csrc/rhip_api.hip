@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "rhip.h"
+#include "rhip_host.h"
 #include "rhip_neural.h"
 
 using namespace rhip;
@@ -30,7 +31,7 @@ using namespace rhip;
 #define RHIP_LAUNCH(kern, grid, block, stream, ...) kern<<<(grid), (block), 0, (stream)>>>(__VA_ARGS__)
 #endif
 
-namespace {
+namespace rhip {
 
 thread_local std::string g_err;
 
@@ -44,126 +45,9 @@ int fail(int code, const char *fmt, ...) {
   g_err = buf;
   return code;
 }
-
-#define HIPCHK(expr)                                                                          \
-  do {                                                                                        \
-    hipError_t e_ = (expr);                                                                   \
-    if (e_ != hipSuccess) return fail(RHIP_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
-  } while (0)
-
-/* A grow-only device buffer. */
-struct DevBuf {
-  void *p = nullptr;
-  size_t cap = 0;
-  int ensure(size_t bytes) {
-    if (bytes <= cap) return RHIP_OK;
-    if (p) hipFree(p);
-    p = nullptr;
-    cap = 0;
-    size_t want = std::max(bytes, (size_t)4096);
-    want = want + want / 4;
-    if (hipMalloc(&p, want) != hipSuccess) return fail(RHIP_E_OOM, "device allocation of %zu bytes failed", want);
-    cap = want;
-    return RHIP_OK;
-  }
-  template <typename T>
-  T *as() const {
-    return (T *)p;
-  }
-  ~DevBuf() {
-    if (p) hipFree(p);
-  }
-};
-
-struct Texture {
-  rhip_texture_desc desc{};
-  rhip_sampler_desc samp{};
-  TexDesc hd{};               /* host copy of the descriptor */
-  TexDesc *dd = nullptr;      /* device copy */
-  uint8_t *data = nullptr;    /* all levels of all layers */
-  size_t bytes = 0;
-  uint8_t *resolved = nullptr; /* resolve target of a multisampled colour texture */
-  bool initialised = false;
-  ~Texture() {
-    if (dd) hipFree(dd);
-    if (data) hipFree(data);
-    if (resolved) hipFree(resolved);
-  }
-};
-
-struct Vao {
-  float *verts = nullptr;
-  int32_t *idx = nullptr;
-  int stride = 0, nverts = 0, nidx = 0;
-  ~Vao() {
-    if (verts) hipFree(verts);
-    if (idx) hipFree(idx);
-  }
-};
-
-struct Fbo {
-  int color = 0, color_layer = 0, color_level = 0, depth = 0;
-};
-
-struct Program {
-  int shading = 0;
-  uint32_t defines = 0;
-};
-
-}  // namespace
-
-struct rhip_device_s {
-  int hip_device = 0;
-  hipStream_t stream = nullptr;
-  int next_id = 1;
-  std::map<int, std::unique_ptr<Texture>> tex;
-  std::map<int, std::unique_ptr<Vao>> vao;
-  std::map<int, Fbo> fbo;
-  std::map<int, Program> prog;
-  int cur_fbo = 0;
-  int vp_x = 0, vp_y = 0, vp_w = 0, vp_h = 0;
-  /* per-draw arenas */
-  DevBuf clip, frag, mask, vary, clip_raw, clip_poly, fan_first, blk, prims, bins, bin_items, scalars, jobs, hiz;
-  unsigned long long *counters = nullptr; /* device */
-  /* optional per-kernel timing: an event pair around every launch */
-  bool profiling = false;
-  struct Timed {
-    const char *name;
-    hipEvent_t a, b;
-  };
-  std::vector<Timed> timed;
-  std::vector<hipEvent_t> event_pool;
-  rhip_stats host_stats{};
-  /* learned shader: FP32 parameters, Adam moments, gradient, packed bf16 operands, captured records */
-  struct Neural {
-    float *params = nullptr, *grad = nullptr, *m1 = nullptr, *m2 = nullptr;
-    NnPacked *packed = nullptr;
-    double *scratch = nullptr; /* losses and evaluation sums */
-    unsigned int *hist = nullptr;
-    int step = 0;
-    bool ready = false;
-    DevBuf records, feat;
-    long long rec_cap = 0;
-    unsigned long long *rec_count = nullptr;
-    bool capturing = false;
-    int shift = 0, material = 0;
-    uint32_t salt = 0;
-  } nn;
-};
+}  // namespace rhip
 
 namespace {
-
-int checkDev(rhip_device d) {
-  if (!d) return fail(RHIP_E_INVALID, "null device");
-  hipError_t e = hipSetDevice(d->hip_device);
-  if (e != hipSuccess) return fail(RHIP_E_HIP, "hipSetDevice: %s", hipGetErrorString(e));
-  return RHIP_OK;
-}
-
-Texture *getTex(rhip_device d, int id) {
-  auto it = d->tex.find(id);
-  return it == d->tex.end() ? nullptr : it->second.get();
-}
 
 /* Texel index in a level's storage order (host side of texelIndex). */
 size_t hostTexelIndex(const TexLevel &l, int layout, int x, int y) {
@@ -246,8 +130,10 @@ int fillLevel(rhip_device d, Texture &t, int layer, int level, const uint8_t *sr
   return RHIP_OK;
 }
 
+}  // namespace
+
 /* decode the 256 / 64 / 48-byte uniform blocks (Material.h:63-95) */
-void decodeUniforms(const rhip_draw_desc &dd, Uniforms &u) {
+void rhip::decodeUniforms(const rhip_draw_desc &dd, Uniforms &u) {
   const uint8_t *m = (const uint8_t *)dd.uniform_model, *s = (const uint8_t *)dd.uniform_scene,
                 *t = (const uint8_t *)dd.uniform_material;
   memset(&u, 0, sizeof u);
@@ -274,7 +160,8 @@ void decodeUniforms(const rhip_draw_desc &dd, Uniforms &u) {
   }
 }
 
-unsigned blocks(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+namespace {
 
 hipEvent_t takeEvent(rhip_device d) {
   if (!d->event_pool.empty()) {
@@ -383,6 +270,153 @@ void launchOrdered(const DrawParams &P, int ntiles, int need_deriv, int flush_ea
 
 }  // namespace
 
+namespace rhip {
+
+int drawSetup(rhip_device d, int fbo, const int vp[4], const rhip_draw_desc &dd, int tile, DrawParams &P,
+              DrawShape &sh) {
+  sh = DrawShape{};
+  memset(&P, 0, sizeof P);
+  auto vit = d->vao.find(dd.vao);
+  auto pit = d->prog.find(dd.program);
+  if (vit == d->vao.end()) return fail(RHIP_E_INVALID, "invalid vao %d", dd.vao);
+  if (pit == d->prog.end()) return fail(RHIP_E_INVALID, "invalid program %d", dd.program);
+  const Vao &va = *vit->second;
+  const Program &pg = pit->second;
+  const RhsStates &st = dd.states;
+  if (st.primitiveType != 2 || st.polygonMode != 2)
+    return fail(RHIP_E_UNSUPPORTED, "only filled triangles are implemented (primitive %d, polygon mode %d)",
+                st.primitiveType, st.polygonMode);
+  if (st.depthFunc < 0 || st.depthFunc > 7 || st.blendFuncRgb < 0 || st.blendFuncRgb > 4)
+    return fail(RHIP_E_INVALID, "invalid render states");
+  auto fit = d->fbo.find(fbo);
+  if (fit == d->fbo.end()) return fail(RHIP_E_INVALID, "invalid framebuffer %d", fbo);
+  const Fbo &f = fit->second;
+  Texture *ct = f.color ? getTex(d, f.color) : nullptr;
+  Texture *dt = f.depth ? getTex(d, f.depth) : nullptr;
+  if (!ct && !dt) {
+    sh.empty = true;
+    return RHIP_OK;
+  }
+  if (vp[2] <= 0 || vp[3] <= 0) return fail(RHIP_E_INVALID, "viewport not set");
+  sh.two_phase = !st.blend && dt && st.depthTest && st.depthMask &&
+                 (st.depthFunc == 1 || st.depthFunc == 3 || st.depthFunc == 4 || st.depthFunc == 6);
+
+  if (pg.shading == RHS_SHADING_NEURAL_PBR) {
+    uint32_t enable_light = 0;
+    if (dd.uniform_material) memcpy(&enable_light, dd.uniform_material, 4);
+    if (!d->nn.ready) return fail(RHIP_E_INVALID, "learned shader: no network (rhip_neural_init or set_weights)");
+    if (!sh.two_phase || enable_light || (pg.defines & (RHS_DEF_AO_MAP | RHS_DEF_EMISSIVE_MAP)))
+      return fail(RHIP_E_UNSUPPORTED,
+                  "learned shader: image-based lighting only (no point light, AO or emissive map), opaque "
+                  "draws that write depth with LESS/LEQUAL/GREATER/GEQUAL");
+  }
+  decodeUniforms(dd, P.u);
+  P.st = st;
+  P.shading = pg.shading;
+  P.defines = pg.defines;
+  for (int s = 0; s < RHS_SLOT_COUNT; s++) {
+    int id = dd.sampler_tex[s];
+    if (!id) continue;
+    Texture *t = getTex(d, id);
+    if (!t) return fail(RHIP_E_INVALID, "sampler slot %d: invalid texture %d", s, id);
+    if (!t->initialised) return fail(RHIP_E_INVALID, "sampler slot %d: texture %d has no image", s, id);
+    P.tex[s] = t->dd;
+    if (t->samp.filter_min > 1 && s != RHS_SLOT_METAL_ROUGHNESS && s <= RHS_SLOT_AO) P.need_deriv = 1;
+  }
+  P.vp_w = (float)vp[2];
+  P.vp_h = (float)vp[3];
+  P.o_x = (float)vp[0] + P.vp_w / 2.f;
+  P.o_y = (float)vp[1] + P.vp_h / 2.f;
+  P.p_x = P.vp_w / 2.f;
+  P.p_y = P.vp_h / 2.f;
+  int fbw = 0, fbh = 0, samples = 1;
+  if (ct) {
+    const TexLevel &lv = ct->hd.lv[f.color_layer][f.color_level];
+    P.color = (uint32_t *)(ct->data + lv.offset);
+    P.color_resolved = (uint32_t *)ct->resolved;
+    fbw = lv.w;
+    fbh = lv.h;
+    samples = ct->hd.msaa ? 4 : 1;
+  }
+  if (dt) {
+    const TexLevel &lv = dt->hd.lv[0][0];
+    P.depth = (float *)(dt->data + lv.offset);
+    if (!ct) {
+      fbw = lv.w;
+      fbh = lv.h;
+      samples = dt->hd.msaa ? 4 : 1;
+    } else if (lv.w != fbw || lv.h != fbh || (dt->hd.msaa ? 4 : 1) != samples) {
+      return fail(RHIP_E_UNSUPPORTED, "colour and depth attachments differ in size or sample count");
+    }
+  }
+  P.fb_w = fbw;
+  P.fb_h = fbh;
+  P.samples = samples;
+  P.tile = tile;
+  P.coarse_tiles = kCoarse;
+  P.tiles_x = (fbw + tile - 1) / tile;
+  P.tiles_y = (fbh + tile - 1) / tile;
+  P.ntiles = P.tiles_x * P.tiles_y;
+  P.coarse_x = (P.tiles_x + kCoarse - 1) / kCoarse;
+  P.coarse_y = (P.tiles_y + kCoarse - 1) / kCoarse;
+  P.nbins = P.ntiles + P.coarse_x * P.coarse_y + 1;
+  P.verts = va.verts;
+  P.stride_f = va.stride / 4;
+  P.nverts = va.nverts;
+  P.idx = va.idx;
+  P.ntris = va.nidx / 3;
+  P.counters = d->counters;
+  sh.V = (size_t)va.nverts;
+  sh.T = (size_t)P.ntris;
+  sh.stride = va.stride;
+  P.nblk = (int)blocks(sh.T, 256);
+  const size_t vcap = sh.V + sh.T * kMaxClipVerts, pcap = sh.T * (1 + kMaxFans);
+  if (pcap * kMaxBinItems >= (size_t)INT32_MAX || vcap >= (size_t)INT32_MAX)
+    return fail(RHIP_E_UNSUPPORTED, "draw too large (%zu triangles)", sh.T);
+  return RHIP_OK;
+}
+
+/* Arenas sized for the worst case, so that no count has to come back to the
+ * host: every triangle may clip (kMaxClipVerts new vertices, kMaxFans extra
+ * triangles) and every primitive is listed in at most kMaxBinItems bins. */
+size_t carveArenas(DrawParams &P, const DrawShape &sh, char *base) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char *p = base ? base + off : nullptr;
+    off += (bytes + 255) & ~(size_t)255;
+    return p;
+  };
+  const size_t V = sh.V, T = sh.T, vcap = V + T * kMaxClipVerts, pcap = T * (1 + kMaxFans);
+  const int nblk = P.nblk;
+  P.clip = (float4 *)take(vcap * 16);
+  P.frag = (float4 *)take(vcap * 16);
+  P.mask = (int *)take(vcap * 4);
+  P.vary = (float *)take(vcap * kVaryStride * 4);
+  P.clip_raw = (float *)take(T * kMaxClipVerts * sh.stride);
+  P.clip_poly = (int *)take(T * (kMaxPoly + 1) * 4);
+  P.fan_first = (int *)take(T * 4);
+  P.blk_clip = (int *)take((size_t)nblk * 3 * 4);
+  P.blk_fans = P.blk_clip ? P.blk_clip + nblk : nullptr;
+  P.blk_fan_off = P.blk_fans ? P.blk_fans + nblk : nullptr;
+  P.dcount = (int *)take(64);
+  P.prims = (PrimSetup *)take(pcap * sizeof(PrimSetup));
+  P.bin_count = (int *)take(((size_t)P.nbins * 3 + 1) * 4);
+  P.bin_start = P.bin_count ? P.bin_count + P.nbins : nullptr;
+  P.bin_cursor = P.bin_start ? P.bin_start + P.nbins + 1 : nullptr;
+  P.bin_items = (int *)take(pcap * kMaxBinItems * 4);
+  if (sh.two_phase) {
+    const int ncoarse = P.coarse_x * P.coarse_y;
+    P.jobs = (uint2 *)take((size_t)P.fb_w * P.fb_h * P.samples * 8);
+    P.hiz_bound = (uint32_t *)take((size_t)ncoarse * (3 + kHizCap) * 4);
+    P.hiz_count = P.hiz_bound ? (int *)P.hiz_bound + ncoarse : nullptr;
+    P.hiz_seed = P.hiz_count ? P.hiz_count + ncoarse : nullptr;
+    P.hiz_surv = P.hiz_seed ? P.hiz_seed + ncoarse : nullptr;
+  }
+  return off;
+}
+
+}  // namespace rhip
+
 extern "C" {
 
 const char *rhip_last_error(void) { return g_err.c_str(); }
@@ -406,6 +440,7 @@ int rhip_device_destroy(rhip_device d) {
   if (!d) return RHIP_OK;
   hipSetDevice(d->hip_device);
   hipStreamSynchronize(d->stream);
+  d->batches.clear();
   d->tex.clear();
   d->vao.clear();
   if (d->counters) hipFree(d->counters);
@@ -709,131 +744,19 @@ int rhip_draw(rhip_device d, const rhip_draw_desc *dd) {
   if (rc) return rc;
   if (!dd) return fail(RHIP_E_INVALID, "null draw");
   if (!d->cur_fbo) return RHIP_OK; /* RendererSoft::draw returns when no framebuffer is bound (:137-139) */
-  auto vit = d->vao.find(dd->vao);
-  auto pit = d->prog.find(dd->program);
-  if (vit == d->vao.end()) return fail(RHIP_E_INVALID, "invalid vao %d", dd->vao);
-  if (pit == d->prog.end()) return fail(RHIP_E_INVALID, "invalid program %d", dd->program);
-  const Vao &va = *vit->second;
-  const Program &pg = pit->second;
-  const RhsStates &st = dd->states;
-  if (st.primitiveType != 2 || st.polygonMode != 2)
-    return fail(RHIP_E_UNSUPPORTED, "only filled triangles are implemented (primitive %d, polygon mode %d)",
-                st.primitiveType, st.polygonMode);
-  if (st.depthFunc < 0 || st.depthFunc > 7 || st.blendFuncRgb < 0 || st.blendFuncRgb > 4)
-    return fail(RHIP_E_INVALID, "invalid render states");
-  const Fbo &f = d->fbo[d->cur_fbo];
-  Texture *ct = f.color ? getTex(d, f.color) : nullptr;
-  Texture *dt = f.depth ? getTex(d, f.depth) : nullptr;
-  if (!ct && !dt) return RHIP_OK;
-  if (d->vp_w <= 0 || d->vp_h <= 0) return fail(RHIP_E_INVALID, "viewport not set");
-
-  if (pg.shading == RHS_SHADING_NEURAL_PBR) {
-    const bool two = !st.blend && dt && st.depthTest && st.depthMask &&
-                     (st.depthFunc == 1 || st.depthFunc == 3 || st.depthFunc == 4 || st.depthFunc == 6);
-    uint32_t enable_light = 0;
-    if (dd->uniform_material) memcpy(&enable_light, dd->uniform_material, 4);
-    if (!d->nn.ready) return fail(RHIP_E_INVALID, "learned shader: no network (rhip_neural_init or set_weights)");
-    if (!two || enable_light || (pg.defines & (RHS_DEF_AO_MAP | RHS_DEF_EMISSIVE_MAP)))
-      return fail(RHIP_E_UNSUPPORTED,
-                  "learned shader: image-based lighting only (no point light, AO or emissive map), opaque "
-                  "draws that write depth with LESS/LEQUAL/GREATER/GEQUAL");
-  }
+  const int vp[4] = {d->vp_x, d->vp_y, d->vp_w, d->vp_h};
   DrawParams P;
-  memset(&P, 0, sizeof P);
-  decodeUniforms(*dd, P.u);
-  P.st = st;
-  P.shading = pg.shading;
-  P.defines = pg.defines;
-  bool need_deriv = false;
-  for (int s = 0; s < RHS_SLOT_COUNT; s++) {
-    int id = dd->sampler_tex[s];
-    if (!id) continue;
-    Texture *t = getTex(d, id);
-    if (!t) return fail(RHIP_E_INVALID, "sampler slot %d: invalid texture %d", s, id);
-    if (!t->initialised) return fail(RHIP_E_INVALID, "sampler slot %d: texture %d has no image", s, id);
-    P.tex[s] = t->dd;
-    if (t->samp.filter_min > 1 && s != RHS_SLOT_METAL_ROUGHNESS && s <= RHS_SLOT_AO) need_deriv = true;
-  }
-  P.vp_w = (float)d->vp_w;
-  P.vp_h = (float)d->vp_h;
-  P.o_x = (float)d->vp_x + P.vp_w / 2.f;
-  P.o_y = (float)d->vp_y + P.vp_h / 2.f;
-  P.p_x = P.vp_w / 2.f;
-  P.p_y = P.vp_h / 2.f;
-  int fbw = 0, fbh = 0, samples = 1;
-  if (ct) {
-    const TexLevel &lv = ct->hd.lv[f.color_layer][f.color_level];
-    P.color = (uint32_t *)(ct->data + lv.offset);
-    P.color_resolved = (uint32_t *)ct->resolved;
-    fbw = lv.w;
-    fbh = lv.h;
-    samples = ct->hd.msaa ? 4 : 1;
-  }
-  if (dt) {
-    const TexLevel &lv = dt->hd.lv[0][0];
-    P.depth = (float *)(dt->data + lv.offset);
-    if (!ct) {
-      fbw = lv.w;
-      fbh = lv.h;
-      samples = dt->hd.msaa ? 4 : 1;
-    } else if (lv.w != fbw || lv.h != fbh || (dt->hd.msaa ? 4 : 1) != samples) {
-      return fail(RHIP_E_UNSUPPORTED, "colour and depth attachments differ in size or sample count");
-    }
-  }
-  P.fb_w = fbw;
-  P.fb_h = fbh;
-  P.samples = samples;
-  P.tiles_x = (fbw + kTile - 1) / kTile;
-  P.tiles_y = (fbh + kTile - 1) / kTile;
-  const int ntiles = P.tiles_x * P.tiles_y;
-  P.verts = va.verts;
-  P.stride_f = va.stride / 4;
-  P.nverts = va.nverts;
-  P.idx = va.idx;
-  P.ntris = va.nidx / 3;
-  P.counters = d->counters;
+  DrawShape sh;
+  if ((rc = drawSetup(d, d->cur_fbo, vp, *dd, kTile, P, sh))) return rc;
+  if (sh.empty) return RHIP_OK;
+  if ((rc = d->arena.ensure(carveArenas(P, sh, nullptr)))) return rc;
+  carveArenas(P, sh, d->arena.as<char>());
+  if (sh.T == 0) return RHIP_OK;
+  const RhsStates &st = dd->states;
+  const int ntiles = P.ntiles, nblk = P.nblk, fbw = P.fb_w, fbh = P.fb_h, samples = P.samples;
+  const bool need_deriv = P.need_deriv != 0;
+  const size_t V = sh.V;
   hipStream_t s = d->stream;
-  const size_t V = (size_t)va.nverts, T = (size_t)P.ntris;
-
-  /* Arenas sized for the worst case, so that no count has to come back to the
-   * host: every triangle may clip (kMaxClipVerts new vertices, kMaxFans extra
-   * triangles) and every primitive is listed in at most kMaxBinItems bins. */
-  const size_t vcap = V + T * kMaxClipVerts;
-  const size_t pcap = T * (1 + kMaxFans);
-  if (pcap * kMaxBinItems >= (size_t)INT32_MAX || vcap >= (size_t)INT32_MAX)
-    return fail(RHIP_E_UNSUPPORTED, "draw too large (%zu triangles)", T);
-  const int nblk = (int)blocks(T, 256);
-  P.ntiles = ntiles;
-  P.tile = kTile;
-  P.coarse_tiles = kCoarse;
-  P.coarse_x = (P.tiles_x + kCoarse - 1) / kCoarse;
-  P.coarse_y = (P.tiles_y + kCoarse - 1) / kCoarse;
-  P.nbins = ntiles + P.coarse_x * P.coarse_y + 1;
-  if ((rc = d->clip.ensure(vcap * 16)) || (rc = d->frag.ensure(vcap * 16)) || (rc = d->mask.ensure(vcap * 4)) ||
-      (rc = d->vary.ensure(vcap * kVaryStride * 4)) || (rc = d->clip_raw.ensure(T * kMaxClipVerts * va.stride)) ||
-      (rc = d->clip_poly.ensure(T * (kMaxPoly + 1) * 4)) || (rc = d->blk.ensure((size_t)nblk * 3 * 4)) ||
-      (rc = d->fan_first.ensure(T * 4)) ||
-      (rc = d->prims.ensure(pcap * sizeof(PrimSetup))) || (rc = d->bins.ensure(((size_t)P.nbins * 3 + 1) * 4)) ||
-      (rc = d->bin_items.ensure(pcap * kMaxBinItems * 4)) || (rc = d->scalars.ensure(64)))
-    return rc;
-  P.clip = d->clip.as<float4>();
-  P.frag = d->frag.as<float4>();
-  P.mask = d->mask.as<int>();
-  P.vary = d->vary.as<float>();
-  P.clip_raw = d->clip_raw.as<float>();
-  P.clip_poly = d->clip_poly.as<int>();
-  P.fan_first = d->fan_first.as<int>();
-  P.nblk = nblk;
-  P.blk_clip = d->blk.as<int>();
-  P.blk_fans = P.blk_clip + nblk;
-  P.blk_fan_off = P.blk_fans + nblk;
-  P.dcount = d->scalars.as<int>();
-  P.prims = d->prims.as<PrimSetup>();
-  P.bin_count = d->bins.as<int>();
-  P.bin_start = P.bin_count + P.nbins;
-  P.bin_cursor = P.bin_start + P.nbins + 1;
-  P.bin_items = d->bin_items.as<int>();
-  if (T == 0) return RHIP_OK;
 
   RHIP_LAUNCH(k_fill_u32, blocks(P.nbins, 256), 256, s, (uint32_t *)P.bin_count, (size_t)P.nbins, 0u);
   { Timer tm(d, "k_vertex"); RHIP_LAUNCH(k_vertex, blocks(V, 256), 256, s, P); }
@@ -846,24 +769,16 @@ int rhip_draw(rhip_device d, const rhip_draw_desc *dd) {
   HIPCHK(hipGetLastError());
 
   /* draws whose result follows from depths and order alone take the two-phase raster */
-  const bool twoPhase = !st.blend && P.depth && st.depthTest && st.depthMask &&
-                        (st.depthFunc == 1 || st.depthFunc == 3 || st.depthFunc == 4 || st.depthFunc == 6);
-  if (twoPhase) {
+  if (sh.two_phase) {
     const int ncoarse = P.coarse_x * P.coarse_y;
     const size_t njobs = (size_t)fbw * fbh * samples;
-    if ((rc = d->jobs.ensure(njobs * 8)) || (rc = d->hiz.ensure((size_t)ncoarse * (3 + kHizCap) * 4))) return rc;
-    P.jobs = d->jobs.as<uint2>();
-    P.hiz_bound = d->hiz.as<uint32_t>();
-    P.hiz_count = (int *)P.hiz_bound + ncoarse;
-    P.hiz_seed = P.hiz_count + ncoarse;
-    P.hiz_surv = P.hiz_seed + ncoarse;
     if (samples == 4)
       launchVis<4>(d, P, ntiles, ncoarse, s);
     else
       launchVis<1>(d, P, ntiles, ncoarse, s);
     if (P.color) {
       const int grid = (int)std::min<size_t>(blocks(njobs, 256), 4096);
-      if (pg.shading == RHS_SHADING_NEURAL_PBR) {
+      if (P.shading == RHS_SHADING_NEURAL_PBR) {
         if ((rc = d->nn.feat.ensure((njobs + 64) * kNnK1 * 2))) return rc;
         __bf16 *feat = d->nn.feat.as<__bf16>();
         {
@@ -886,7 +801,7 @@ int rhip_draw(rhip_device d, const rhip_draw_desc *dd) {
         else
           launchShade<1>(P, grid, need_deriv ? 1 : 0, s);
       }
-      if (d->nn.capturing && pg.shading == RHS_SHADING_PBR) {
+      if (d->nn.capturing && P.shading == RHS_SHADING_PBR) {
         Timer tm(d, "k_capture");
         d->nn.salt = d->nn.salt * 1664525u + 1013904223u;
         float *rec = d->nn.records.as<float>();

@@ -1,0 +1,167 @@
+/*
+ * rhip_host.h -- host-side state of librhip shared by its translation units:
+ * the device object, its resources and per-draw arenas.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "rhip.h"
+#include "rhip_internal.h"
+
+namespace rhip {
+
+struct NnPacked;
+struct Batch;
+
+/* Records the message of a failure for rhip_last_error and returns `code`. */
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+
+#define HIPCHK(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) return fail(RHIP_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+/* A grow-only device buffer. */
+struct DevBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes) {
+    if (bytes <= cap) return RHIP_OK;
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max(bytes, (size_t)4096);
+    want = want + std::min(want / 4, (size_t)1 << 30); /* room to grow, at most 1 GiB */
+    if (hipMalloc(&p, want) != hipSuccess) return fail(RHIP_E_OOM, "device allocation of %zu bytes failed", want);
+    cap = want;
+    return RHIP_OK;
+  }
+  template <typename T>
+  T *as() const {
+    return (T *)p;
+  }
+  ~DevBuf() {
+    if (p) hipFree(p);
+  }
+};
+
+struct Texture {
+  rhip_texture_desc desc{};
+  rhip_sampler_desc samp{};
+  TexDesc hd{};               /* host copy of the descriptor */
+  TexDesc *dd = nullptr;      /* device copy */
+  uint8_t *data = nullptr;    /* all levels of all layers */
+  size_t bytes = 0;
+  uint8_t *resolved = nullptr; /* resolve target of a multisampled colour texture */
+  bool initialised = false;
+  ~Texture() {
+    if (dd) hipFree(dd);
+    if (data) hipFree(data);
+    if (resolved) hipFree(resolved);
+  }
+};
+
+struct Vao {
+  float *verts = nullptr;
+  int32_t *idx = nullptr;
+  int stride = 0, nverts = 0, nidx = 0;
+  ~Vao() {
+    if (verts) hipFree(verts);
+    if (idx) hipFree(idx);
+  }
+};
+
+struct Fbo {
+  int color = 0, color_layer = 0, color_level = 0, depth = 0;
+};
+
+struct Program {
+  int shading = 0;
+  uint32_t defines = 0;
+};
+
+}  // namespace rhip
+
+struct rhip_device_s {
+  int hip_device = 0;
+  hipStream_t stream = nullptr;
+  int next_id = 1;
+  std::map<int, std::unique_ptr<rhip::Texture>> tex;
+  std::map<int, std::unique_ptr<rhip::Vao>> vao;
+  std::map<int, rhip::Fbo> fbo;
+  std::map<int, rhip::Program> prog;
+  int cur_fbo = 0;
+  int vp_x = 0, vp_y = 0, vp_w = 0, vp_h = 0;
+  /* per-draw arenas */
+  rhip::DevBuf arena;
+  unsigned long long *counters = nullptr; /* device */
+  /* optional per-kernel timing: an event pair around every launch */
+  bool profiling = false;
+  struct Timed {
+    const char *name;
+    hipEvent_t a, b;
+  };
+  std::vector<Timed> timed;
+  std::vector<hipEvent_t> event_pool;
+  std::map<int, std::shared_ptr<rhip::Batch>> batches; /* captured multi-view batches */
+  rhip_stats host_stats{};
+  /* learned shader: FP32 parameters, Adam moments, gradient, packed bf16 operands, captured records */
+  struct Neural {
+    float *params = nullptr, *grad = nullptr, *m1 = nullptr, *m2 = nullptr;
+    rhip::NnPacked *packed = nullptr;
+    double *scratch = nullptr; /* losses and evaluation sums */
+    unsigned int *hist = nullptr;
+    int step = 0;
+    bool ready = false;
+    rhip::DevBuf records, feat;
+    long long rec_cap = 0;
+    unsigned long long *rec_count = nullptr;
+    bool capturing = false;
+    int shift = 0, material = 0;
+    uint32_t salt = 0;
+  } nn;
+};
+
+namespace rhip {
+
+inline int checkDev(rhip_device d) {
+  if (!d) return fail(RHIP_E_INVALID, "null device");
+  hipError_t e = hipSetDevice(d->hip_device);
+  if (e != hipSuccess) return fail(RHIP_E_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+  return RHIP_OK;
+}
+
+inline Texture *getTex(rhip_device d, int id) {
+  auto it = d->tex.find(id);
+  return it == d->tex.end() ? nullptr : it->second.get();
+}
+
+inline unsigned blocks(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+/* What the host knows of a draw besides its DrawParams. */
+struct DrawShape {
+  size_t V = 0, T = 0; /* vertices, triangles */
+  int stride = 0;      /* vertex bytes */
+  bool empty = false;  /* no attachment: nothing to do */
+  bool two_phase = false; /* visibility first, then shading (opaque draws that write depth) */
+};
+
+/* Validates a draw into framebuffer `fbo` through viewport vp = (x, y, w, h)
+ * with screen tiles of `tile` pixels, and fills P except for its arenas. */
+int drawSetup(rhip_device d, int fbo, const int vp[4], const rhip_draw_desc &dd, int tile, DrawParams &P,
+              DrawShape &sh);
+
+/* Points P's arenas into memory from `base` on (sizes only when base is null);
+ * returns the bytes they take. */
+size_t carveArenas(DrawParams &P, const DrawShape &sh, char *base);
+
+void decodeUniforms(const rhip_draw_desc &dd, Uniforms &u);
+
+}  // namespace rhip

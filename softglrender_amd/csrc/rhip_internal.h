@@ -80,7 +80,7 @@ struct DrawParams {
   RhsStates st;
   int shading;
   uint32_t defines;
-  int nvary;
+  int need_deriv; /* a sampler of the draw needs dFdx/dFdy (mipmapped minification) */
   /* viewport (RendererSoft.cpp:92-113) */
   float vp_w, vp_h, o_x, o_y, p_x, p_y;
   /* framebuffer */

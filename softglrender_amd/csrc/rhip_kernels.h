@@ -23,7 +23,7 @@ namespace rhip {
 
 /* --------------------------------------------------------------------------- clears */
 
-__global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {
+static __global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t stride = (size_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) p[i] = v;
@@ -31,8 +31,8 @@ __global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {
 
 /* --------------------------------------------------------------------------- vertex stage */
 
-__global__ __launch_bounds__(256) void k_vertex(DrawParams P) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void vertexBody(const DrawParams &P, int blk) {
+  int i = blk * blockDim.x + threadIdx.x;
   if (i >= P.nverts) return;
   float a[16];
   const float *src = P.verts + (size_t)i * P.stride_f;
@@ -48,6 +48,8 @@ __global__ __launch_bounds__(256) void k_vertex(DrawParams P) {
 #pragma unroll
   for (int k = 0; k < kVaryStride / 4; k++) o[k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
 }
+
+static __global__ __launch_bounds__(256) void k_vertex(DrawParams P) { vertexBody(P, blockIdx.x); }
 
 /* --------------------------------------------------------------------------- block helpers */
 
@@ -354,8 +356,8 @@ __device__ __forceinline__ bool validTri(const DrawParams &P, int i0, int i1, in
 
 /* processPolygonAssembly (RendererSoft.cpp:428-437) + the clip test: triangles
  * with a vertex outside the frustum are set up later by k_clip. */
-__global__ __launch_bounds__(256) void k_setup(DrawParams P) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void setupBody(const DrawParams &P, int blk) {
+  const int t = blk * blockDim.x + threadIdx.x;
   int needs = 0;
   BinRect r = {-1, 0, 0, 0, 0};
   if (t < P.ntris) {
@@ -371,8 +373,10 @@ __global__ __launch_bounds__(256) void k_setup(DrawParams P) {
   }
   binWave<false>(P, t, r);
   const int n = __syncthreads_count(needs);
-  if (threadIdx.x == 0) P.blk_clip[blockIdx.x] = n;
+  if (threadIdx.x == 0) P.blk_clip[blk] = n;
 }
+
+static __global__ __launch_bounds__(256) void k_setup(DrawParams P) { setupBody(P, blockIdx.x); }
 
 /* --------------------------------------------------------------------------- clipping */
 
@@ -479,9 +483,8 @@ __device__ int clipTriangle(const DrawParams &P, int t) {
 }
 
 /* One block per 256-triangle block of k_setup; blocks with nothing to clip return at once. */
-__global__ __launch_bounds__(256) void k_clip(DrawParams P) {
+__device__ __forceinline__ void clipBody(const DrawParams &P, int b) {
   __shared__ int fans;
-  const int b = blockIdx.x;
   if (P.blk_clip[b] == 0) {
     if (threadIdx.x == 0) P.blk_fans[b] = 0;
     return;
@@ -495,8 +498,10 @@ __global__ __launch_bounds__(256) void k_clip(DrawParams P) {
   if (threadIdx.x == 0) P.blk_fans[b] = fans;
 }
 
+static __global__ __launch_bounds__(256) void k_clip(DrawParams P) { clipBody(P, blockIdx.x); }
+
 /* One block of 1024: exclusive offsets of every block's fan triangles, and the primitive count. */
-__global__ __launch_bounds__(1024) void k_scan_fans(DrawParams P) {
+__device__ __forceinline__ void scanFansBody(const DrawParams &P) {
   const int total = blockScanArray(P.blk_fans, P.blk_fan_off, P.nblk, nullptr);
   if (threadIdx.x == 0) {
     P.dcount[0] = total;
@@ -505,10 +510,11 @@ __global__ __launch_bounds__(1024) void k_scan_fans(DrawParams P) {
   }
 }
 
+static __global__ __launch_bounds__(1024) void k_scan_fans(DrawParams P) { scanFansBody(P); }
+
 /* fans appended after all original primitives, in clipped-triangle order (:550-558) */
-__global__ __launch_bounds__(256) void k_clip_fans(DrawParams P) {
+__device__ __forceinline__ void clipFansBody(const DrawParams &P, int b) {
   __shared__ int part[256];
-  const int b = blockIdx.x;
   if (P.blk_fans[b] == 0) return;
   const int t = b * blockDim.x + threadIdx.x;
   const int *poly = P.clip_poly + (size_t)t * (kMaxPoly + 1);
@@ -520,10 +526,12 @@ __global__ __launch_bounds__(256) void k_clip_fans(DrawParams P) {
   for (int i = 3; i < n; i++) binCount(P, setupPrim(P, base + i - 3, poly[0], poly[i - 1], poly[i], t));
 }
 
+static __global__ __launch_bounds__(256) void k_clip_fans(DrawParams P) { clipFansBody(P, blockIdx.x); }
+
 /* --------------------------------------------------------------------------- binning */
 
 /* One block of 1024: bin starts from the counts setup made; cursors cleared. */
-__global__ __launch_bounds__(1024) void k_bin_scan(DrawParams P) {
+__device__ __forceinline__ void binScanBody(const DrawParams &P) {
   const int total = blockScanArray(P.bin_count, P.bin_start, P.nbins, P.bin_cursor);
   if (threadIdx.x == 0) {
     P.bin_start[P.nbins] = total;
@@ -532,10 +540,12 @@ __global__ __launch_bounds__(1024) void k_bin_scan(DrawParams P) {
   }
 }
 
+static __global__ __launch_bounds__(1024) void k_bin_scan(DrawParams P) { binScanBody(P); }
+
 /* Lists every live primitive in its bins (wavefronts walk the ids together). */
-__global__ __launch_bounds__(256) void k_bin_emit(DrawParams P) {
+__device__ __forceinline__ void binEmitBody(const DrawParams &P, int blk, int nblocks) {
   const int n = P.dcount[1];
-  for (int b0 = blockIdx.x * blockDim.x; b0 < n; b0 += gridDim.x * blockDim.x) {
+  for (int b0 = blk * blockDim.x; b0 < n; b0 += nblocks * blockDim.x) {
     const int id = b0 + threadIdx.x;
     BinRect r = {-1, 0, 0, 0, 0};
     if (id < n) {
@@ -548,6 +558,8 @@ __global__ __launch_bounds__(256) void k_bin_emit(DrawParams P) {
     binWave<true>(P, id, r);
   }
 }
+
+static __global__ __launch_bounds__(256) void k_bin_emit(DrawParams P) { binEmitBody(P, blockIdx.x, gridDim.x); }
 
 /* The entries of one tile: its own bin, its coarse bin, the global bin. */
 struct TileBins {
@@ -983,10 +995,9 @@ __device__ __forceinline__ void visPixel(const PrimSetup &ps, const Tri &T, int 
 constexpr int kHizCap = 1024;
 
 template <int S, bool MX>
-__global__ __launch_bounds__(256) void k_hiz_coarse(DrawParams P) {
+__device__ __forceinline__ void hizCoarseBody(const DrawParams &P, int c) {
   __shared__ uint32_t s_bound, s_pz;
   __shared__ int s_cnt, s_pt;
-  const int c = blockIdx.x;
   const int cx = c % P.coarse_x, cy = c / P.coarse_x;
   const int cpx = P.coarse_tiles * P.tile;
   const int x0 = cx * cpx, y0 = cy * cpx;
@@ -1055,6 +1066,11 @@ __global__ __launch_bounds__(256) void k_hiz_coarse(DrawParams P) {
     P.hiz_count[c] = s_cnt <= kHizCap ? s_cnt : -1;
     P.hiz_seed[c] = s_pt;
   }
+}
+
+template <int S, bool MX>
+__global__ __launch_bounds__(256) void k_hiz_coarse(DrawParams P) {
+  hizCoarseBody<S, MX>(P, blockIdx.x);
 }
 
 /* One tile of the two-phase raster: TILE x TILE pixels, 64 * WAVES lanes,
@@ -1350,15 +1366,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_raster_vis(DrawParams P) {
 /* Colour of the fragments the two-phase raster queued: one lane per (pixel,
  * winning triangle), every sample the triangle won gets its shader output. */
 template <int S, int SHADING>
-__global__ __launch_bounds__(256) void k_shade(DrawParams P, int need_deriv) {
-  __shared__ unsigned long long s_fs, s_wr;
-  if (threadIdx.x == 0) s_fs = s_wr = 0;
-  __syncthreads();
+__device__ __forceinline__ void shadeJob(const DrawParams &P, int j, int need_deriv, unsigned long long &nfs,
+                                         unsigned long long &nwr) {
   const float lx[4] = {0.375f, 0.875f, 0.125f, 0.625f};
   const float ly[4] = {0.875f, 0.625f, 0.375f, 0.125f};
-  const int n = P.dcount[2];
-  unsigned long long nfs = 0, nwr = 0;
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+  {
     const uint2 jb = P.jobs[j];
     const uint32_t pix = jb.x & 0x0fffffffu, m = jb.x >> 28, w = jb.y;
     const int px = (int)(pix % (uint32_t)P.fb_w), py = (int)(pix / (uint32_t)P.fb_w);
@@ -1390,6 +1402,13 @@ __global__ __launch_bounds__(256) void k_shade(DrawParams P, int need_deriv) {
         nwr++;
       }
   }
+}
+
+/* Adds a block's fragment-shader and sample-write counts to the draw counters. */
+__device__ __forceinline__ void addShadeCounts(const DrawParams &P, unsigned long long nfs, unsigned long long nwr) {
+  __shared__ unsigned long long s_fs, s_wr;
+  if (threadIdx.x == 0) s_fs = s_wr = 0;
+  __syncthreads();
   if (nfs) atomicAdd(&s_fs, nfs);
   if (nwr) atomicAdd(&s_wr, nwr);
   __syncthreads();
@@ -1399,15 +1418,26 @@ __global__ __launch_bounds__(256) void k_shade(DrawParams P, int need_deriv) {
   }
 }
 
+template <int S, int SHADING>
+__global__ __launch_bounds__(256) void k_shade(DrawParams P, int need_deriv) {
+  const int n = P.dcount[2];
+  unsigned long long nfs = 0, nwr = 0;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+    shadeJob<S, SHADING>(P, j, need_deriv, nfs, nwr);
+  addShadeCounts(P, nfs, nwr);
+}
+
 /* multiSampleResolve (RendererSoft.cpp:880-912) of a whole 4x colour target */
-__global__ __launch_bounds__(256) void k_resolve(DrawParams P) {
+__device__ __forceinline__ void resolveBody(const DrawParams &P, int blk, int nblocks) {
   const size_t n = (size_t)P.fb_w * P.fb_h;
-  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
+  for (size_t p = (size_t)blk * blockDim.x + threadIdx.x; p < n; p += (size_t)nblocks * blockDim.x) {
     const uint4 c = ((const uint4 *)P.color)[p];
     const uint32_t cs[4] = {c.x, c.y, c.z, c.w};
     P.color_resolved[p] = resolve4(cs);
   }
 }
+
+static __global__ __launch_bounds__(256) void k_resolve(DrawParams P) { resolveBody(P, blockIdx.x, gridDim.x); }
 
 /* --------------------------------------------------------------------------- ordered raster
  *
@@ -1722,7 +1752,7 @@ __global__ __launch_bounds__(256) void k_raster_ord(DrawParams P, int need_deriv
 /* generateMipmaps / sampleBufferBilinear (SamplerSoft.h:89-110, 240-252): level i is a
  * clamp-to-edge bilinear resample of level i-1 at (x*r + r/2, y*r + r/2). */
 template <typename T>
-__global__ void k_mipgen(const TexDesc *d, int layer, int level) {
+static __global__ void k_mipgen(const TexDesc *d, int layer, int level) {
   const TexLevel &src = d->lv[layer][level - 1];
   const TexLevel &dst = d->lv[layer][level];
   int x = blockIdx.x * blockDim.x + threadIdx.x;

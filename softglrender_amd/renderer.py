@@ -235,6 +235,9 @@ class RendererHIP:
         self.device = device
         self.dev = None
         self._vao = self._prog = self._res = self._ps = None
+        self._batch = None  # views being recorded by beginBatch / endBatch
+        self._pass = None
+        self._vp = (0, 0, 0, 0)
 
     def type(self):
         return RENDERER_HIP
@@ -296,10 +299,15 @@ class RendererHIP:
     def beginRenderPass(self, frameBuffer: FrameBufferHIP, states: ClearStates):
         cs = _lib.ClearStates(int(states.depthFlag), int(states.colorFlag),
                               (ctypes.c_float * 4)(*states.clearColor), states.clearDepth)
+        if self._batch is not None:
+            self._pass = [frameBuffer.handle, cs]
+            return
         _lib.check(self.lib.rhip_begin_render_pass(self.dev, frameBuffer.handle, ctypes.byref(cs)))
 
     def setViewPort(self, x, y, width, height):
-        _lib.check(self.lib.rhip_set_viewport(self.dev, x, y, width, height))
+        self._vp = (x, y, width, height)
+        if self._batch is None:
+            _lib.check(self.lib.rhip_set_viewport(self.dev, x, y, width, height))
 
     def setVertexArrayObject(self, vao):
         self._vao = vao
@@ -317,6 +325,17 @@ class RendererHIP:
     def draw(self):
         if self._vao is None or self._prog is None or self._prog.handle is None or self._ps is None:
             return
+        dd, keep = self._draw_desc()
+        if self._batch is not None:
+            if self._pass is None:
+                raise RuntimeError("batched draw outside a render pass")
+            fbo, cs = self._pass
+            self._batch.append((_lib.BatchView(fbo, (ctypes.c_int * 4)(*self._vp), cs, dd), keep))
+            self._pass[1] = _lib.ClearStates()  # later draws of the pass clear nothing
+            return
+        _lib.check(self.lib.rhip_draw(self.dev, ctypes.byref(dd)))
+
+    def _draw_desc(self):
         rs = self._ps.renderStates
         st = _lib.RhsStates(int(rs.blend), rs.blend_func_rgb, rs.blend_src_rgb, rs.blend_dst_rgb,
                             rs.blend_func_alpha, rs.blend_src_alpha, rs.blend_dst_alpha, int(rs.depth_test),
@@ -334,10 +353,31 @@ class RendererHIP:
         dd = _lib.DrawDesc(self._vao.handle, self._prog.handle, st, ctypes.cast(keep[0], ctypes.c_void_p),
                            ctypes.cast(keep[1], ctypes.c_void_p), ctypes.cast(keep[2], ctypes.c_void_p),
                            (ctypes.c_int * 8)(*samplers))
-        _lib.check(self.lib.rhip_draw(self.dev, ctypes.byref(dd)))
+        return dd, keep
 
     def endRenderPass(self):
+        if self._batch is not None:
+            self._pass = None
+            return
         _lib.check(self.lib.rhip_end_render_pass(self.dev))
+
+    # ------------------------------------------------------------- batched views (no reference counterpart)
+
+    def beginBatch(self):
+        """Render passes and draws up to endBatch() are recorded, not run: each draw becomes one view of a batch."""
+        self._batch = []
+        self._pass = None
+
+    def endBatch(self, tile=None, waves=None) -> "BatchHIP":
+        """Creates the batch of the recorded views (one graph of launches over all of them)."""
+        views, self._batch, self._pass = self._batch, None, None
+        if not views:
+            raise RuntimeError("no draws recorded")
+        arr = (_lib.BatchView * len(views))(*[v for v, _ in views])
+        cfg = ctypes.byref(_lib.BatchConfig(tile, waves)) if tile is not None else None
+        h = ctypes.c_int()
+        _lib.check(self.lib.rhip_batch_create(self.dev, arr, len(views), cfg, ctypes.byref(h)))
+        return BatchHIP(self, h.value, len(views))
 
     def waitIdle(self):
         _lib.check(self.lib.rhip_wait_idle(self.dev))
@@ -424,6 +464,52 @@ class RendererHIP:
 
     def stream(self) -> int:
         return self.lib.rhip_device_stream(self.dev)
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class BatchHIP:
+    """A captured batch of views: run() replays it; tune() is the autotuner."""
+
+    def __init__(self, r: RendererHIP, handle: int, n_views: int):
+        self.r, self.handle, self.n_views = r, handle, n_views
+
+    def run(self):
+        _lib.check(self.r.lib.rhip_batch_run(self.r.dev, self.handle))
+
+    def config(self):
+        c = _lib.BatchConfig()
+        _lib.check(self.r.lib.rhip_batch_get_config(self.r.dev, self.handle, ctypes.byref(c)))
+        return c.tile, c.waves
+
+    def set_config(self, tile, waves):
+        _lib.check(self.r.lib.rhip_batch_set_config(self.r.dev, self.handle, ctypes.byref(_lib.BatchConfig(tile, waves))))
+
+    def tune(self, reps=3):
+        """Times every (tile, waves) configuration and keeps the fastest; returns
+        ((tile, waves), [((tile, waves), ms per batch), ...])."""
+        cap = 64
+        arr = (_lib.BatchTiming * cap)()
+        n = ctypes.c_int()
+        best = _lib.BatchConfig()
+        _lib.check(self.r.lib.rhip_batch_tune(self.r.dev, self.handle, reps, arr, cap, ctypes.byref(n),
+                                              ctypes.byref(best)))
+        return (best.tile, best.waves), [((t.config.tile, t.config.waves), t.ms) for t in arr[:min(n.value, cap)]]
+
+    def view_times(self):
+        """Per-view completion times (ms from the start) of the last replay."""
+        out = np.zeros(self.n_views, np.float64)
+        _lib.check(self.r.lib.rhip_batch_view_times(self.r.dev, self.handle, out.ctypes.data, self.n_views))
+        return out
+
+    def destroy(self):
+        if self.handle and self.r.dev:
+            _lib.check(self.r.lib.rhip_batch_destroy(self.r.dev, self.handle))
+        self.handle = 0
 
     def __del__(self):
         try:

@@ -158,3 +158,48 @@ def replay(renderer: RendererHIP, data: bytes, repeats: int = 1, layouts: dict |
     for op, p in reads_cmds:
         rp.exec(op, p, outs)
     return outs, times
+
+
+def split(data: bytes):
+    """(setup, frame, reads) command lists of a stream."""
+    setup, frame, reads = [], [], []
+    in_frame = False
+    for op, p in parse(data):
+        if op == S.OP_FRAME:
+            in_frame = True
+        elif op == S.OP_READ:
+            reads.append((op, p))
+        elif in_frame:
+            frame.append((op, p))
+        else:
+            setup.append((op, p))
+    return setup, frame, reads
+
+
+def replay_batched(renderer: RendererHIP, data: bytes, layouts: dict | None = None, tile=None, waves=None):
+    """Runs the setup part, then records the frame part as one batch (each draw
+    one view). Returns (replayer, batch, read commands); the batch has not run yet."""
+    rp = Replayer(renderer, layouts)
+    setup, frame, reads = split(data)
+    sink = []
+    for op, p in setup:
+        rp.exec(op, p, sink)
+    renderer.waitIdle()
+    renderer.beginBatch()
+    try:
+        for op, p in frame:
+            rp.exec(op, p, sink)
+    except BaseException:
+        renderer._batch = None
+        raise
+    batch = renderer.endBatch(tile, waves)
+    batch.keep = rp  # the resources the batch uses live as long as it does
+    return rp, batch, reads
+
+
+def run_reads(rp: Replayer, reads):
+    """Executes read commands; returns the arrays they read."""
+    outs = []
+    for op, p in reads:
+        rp.exec(op, p, outs)
+    return outs

@@ -269,6 +269,68 @@ def scene_c1(width=640, height=480, seed=1, msaa=False, lon=64, lat=32, read_msa
     return Scene("c1", st, width, height, msaa, dict(triangles=int(idx.shape[0]), vertices=int(verts.shape[0])))
 
 
+def scene_c4(n_scenes=64, n_views=256, sizes=(256, 512, 1024), tri_lo=1_000, tri_hi=500_000, tex_size=1024,
+             seed=9, reads=False):
+    """C4: `n_views` views of `n_scenes` seeded scenes. Scene i is a displaced icosphere with a
+    triangle count log-uniform in [tri_lo, tri_hi] (frequency round(sqrt(T / 20))) and its own
+    tex_size^2 RGBA8 checker with full mips (Tiled layout, trilinear). View v shows scene v mod
+    n_scenes at a size drawn from `sizes` (square), from an eye uniform on a radius-3 sphere looking at
+    the origin (fovy 60, near 0.1); Blinn-Phong with a point light, depth LESS, back-face culling, no
+    MSAA. Every view renders into its own colour and depth textures."""
+    rng = np.random.default_rng(seed)
+    st = S.Stream()
+    layouts = {}
+    tris = np.exp(rng.uniform(math.log(tri_lo), math.log(tri_hi), n_scenes))
+    freqs = [max(1, int(round(math.sqrt(t / 20.0)))) for t in tris]
+    mip = dict(mipmaps=True, filter_min=S.FILTER_LINEAR_MIPMAP_LINEAR, filter_mag=S.FILTER_LINEAR,
+               wrap=S.WRAP_REPEAT, usage=S.USAGE_SAMPLER | S.USAGE_UPLOAD)
+    n_tris = []
+    for i, f in enumerate(freqs):
+        v, idx = icosphere(f, 1.0, displace=0.1, seed=seed * 1000 + i)
+        st.vao(1 + i, v, idx)
+        n_tris.append(int(idx.shape[0]))
+        tid = 1 + i
+        st.texture(tid, tex_size, tex_size, **mip)
+        st.tex_data(tid, [checker_texture(tex_size, 16, seed * 1000 + i)])
+        layouts[tid] = 1  # Tiled
+    view_sizes = rng.choice(np.asarray(sizes), n_views)
+    d = rng.normal(size=(n_views, 3))
+    eyes = 3.0 * d / np.linalg.norm(d, axis=1, keepdims=True)
+    base = 1 + n_scenes
+    for v in range(n_views):
+        w = int(view_sizes[v])
+        st.texture(base + 2 * v, w, w, usage=S.USAGE_COLOR | S.USAGE_SAMPLER)
+        st.tex_init(base + 2 * v)
+        st.texture(base + 2 * v + 1, w, w, format=S.FMT_FLOAT32, usage=S.USAGE_DEPTH)
+        st.tex_init(base + 2 * v + 1)
+        st.fbo(1 + v, color=base + 2 * v, depth=base + 2 * v + 1)
+    st.frame()
+    proj = projection(math.radians(60.0), 1.0, 0.1)
+    model = np.eye(4)
+    rs = _common_states(S.DEPTH_LESS, cull=True)
+    for v in range(n_views):
+        w = int(view_sizes[v])
+        i = v % n_scenes
+        eye = tuple(eyes[v])
+        up = (0, 1, 0) if abs(eye[1]) < 0.99 * 3.0 else (0, 0, 1)
+        view = look_at(eye, (0, 0, 0), up)
+        st.begin(1 + v, True, True, (0.05, 0.05, 0.08, 1.0), 1.0)
+        st.viewport(0, 0, w, w)
+        st.draw(1 + i, S.SHADING_BLINN_PHONG, S.DEF_ALBEDO, rs, S.uniforms_model(model, proj @ view @ model),
+                S.uniforms_scene((0.15, 0.15, 0.15), eye, (2.0, 3.0, 4.0), (1.0, 1.0, 1.0)),
+                S.uniforms_material(1, 0, 0, 1.0, 0.5, (1, 1, 1, 1)), samplers=[(S.SLOT_ALBEDO, 1 + i)])
+        st.end()
+    if reads:
+        for v in range(n_views):
+            st.read(base + 2 * v, which=0)
+            st.read(base + 2 * v + 1, which=0)
+    pixels = int(sum(int(s) ** 2 for s in view_sizes))
+    return Scene("c4", st, int(max(sizes)), int(max(sizes)), False,
+                 dict(layouts=layouts, scene_triangles=n_tris, view_sizes=[int(s) for s in view_sizes],
+                      view_scene=[v % n_scenes for v in range(n_views)], pixels=pixels,
+                      view_triangles=int(sum(n_tris[v % n_scenes] for v in range(n_views)))))
+
+
 # ----------------------------------------------------------------------------- IBL inputs
 
 def _sh_basis(d):

@@ -1,0 +1,122 @@
+"""Batched views on the GPU (rhip_batch_*): every view of a batch must be
+identical, bit for bit, to the CPU checker's result for the same render pass
+run on its own, for every (tile, wavefronts) configuration the autotuner can
+pick and on every replay of the captured graph."""
+import numpy as np
+import pytest
+
+import oracle
+from softglrender_amd import scenes
+from softglrender_amd import stream as S
+from softglrender_amd._lib import RhipError
+from softglrender_amd.replay import replay, replay_batched, run_reads
+
+pytestmark = pytest.mark.gpu
+
+
+def _small(seed, reads=True):
+    """The batched-views workload scaled down: 5 scenes (200-20k triangles, 128^2 textures), 10 views of
+    40^2 to 150^2 pixels."""
+    return scenes.scene_c4(n_scenes=5, n_views=10, sizes=(40, 96, 150), tri_lo=200, tri_hi=20_000, tex_size=128,
+                           seed=seed, reads=reads)
+
+
+def _oracle(sc):
+    out, _ = oracle.cpu_ref_replay(sc.stream.bytes(), sc.stream.output_bytes(), exact_math=True)
+    return sc.stream.split_outputs(out)
+
+
+def assert_identical(got, want, label):
+    assert len(got) == len(want), label
+    for i, (g, w) in enumerate(zip(got, want)):
+        assert g.shape == w.shape, (label, i, g.shape, w.shape)
+        bad = int((g.view(np.uint8) != w.view(np.uint8)).sum())
+        assert bad == 0, f"{label}: output {i} differs in {bad} bytes"
+
+
+def test_batch_identical_to_checker(renderer):
+    sc = _small(3)
+    print("[gpu] batch seed 3", flush=True)
+    rp, b, reads = replay_batched(renderer, sc.stream.bytes(), layouts=sc.meta["layouts"])
+    b.run()
+    got = run_reads(rp, reads)
+    want = _oracle(sc)
+    assert_identical(got, want, "batch")
+    covered = sum(int((w[..., 3] > 0).sum()) for w in want[0::2])
+    assert covered > 5000, covered
+    b.destroy()
+
+
+def test_batch_identical_to_separate_draws(renderer):
+    sc = _small(6)
+    print("[gpu] batch vs draws seed 6", flush=True)
+    seq, _ = replay(renderer, sc.stream.bytes(), layouts=sc.meta["layouts"])
+    rp, b, reads = replay_batched(renderer, sc.stream.bytes(), layouts=sc.meta["layouts"])
+    b.run()
+    assert_identical(run_reads(rp, reads), seq, "batch vs separate draws")
+
+
+def test_batch_every_configuration(renderer):
+    sc = _small(4)
+    want = _oracle(sc)
+    rp, b, reads = replay_batched(renderer, sc.stream.bytes(), layouts=sc.meta["layouts"])
+    best, timings = b.tune(reps=1)
+    configs = [c for c, _ in timings]
+    print(f"[gpu] tuned {len(configs)} configurations, best {best}", flush=True)
+    assert len(configs) == 12 and best in configs
+    assert all(ms > 0 for _, ms in timings)
+    assert b.config() == best
+    for tile, waves in configs:
+        print(f"[gpu] batch tile {tile} waves {waves}", flush=True)
+        b.set_config(tile, waves)
+        b.run()
+        assert_identical(run_reads(rp, reads), want, f"tile {tile} waves {waves}")
+
+
+def test_batch_replays_and_view_times(renderer):
+    sc = _small(5)
+    rp, b, reads = replay_batched(renderer, sc.stream.bytes(), layouts=sc.meta["layouts"])
+    b.run()
+    first = run_reads(rp, reads)
+    other = scenes.scene_c1(width=160, height=120)
+    for k in range(3):
+        print(f"[gpu] replay {k}", flush=True)
+        replay(renderer, other.stream.bytes())  # eager work between replays
+        b.run()
+        assert_identical(run_reads(rp, reads), first, f"replay {k}")
+    t = b.view_times()
+    assert t.shape == (10,) and np.all(t > 0) and np.all(t < 1000.0), t
+
+
+def _two_pass_stream(same_target, blend=False):
+    st = S.Stream()
+    v, idx = scenes.uv_sphere(16, 8)
+    st.vao(1, v, idx)
+    for t in (1, 3):
+        st.texture(t, 32, 32, usage=S.USAGE_COLOR)
+        st.tex_init(t)
+        st.texture(t + 1, 32, 32, format=S.FMT_FLOAT32, usage=S.USAGE_DEPTH)
+        st.tex_init(t + 1)
+    st.fbo(1, color=1, depth=2)
+    st.fbo(2, color=3, depth=4)
+    st.frame()
+    m = np.eye(4)
+    mvp = scenes.projection(1.0, 1.0, 0.1) @ scenes.look_at((0, 0, 3), (0, 0, 0), (0, 1, 0))
+    for f in (1, 1 if same_target else 2):
+        st.begin(f, True, True, (0, 0, 0, 0), 1.0)
+        st.viewport(0, 0, 32, 32)
+        st.draw(1, S.SHADING_BASIC, 0, scenes._common_states(S.DEPTH_LESS, blend=blend), S.uniforms_model(m, mvp),
+                S.uniforms_scene((0, 0, 0), (0, 0, 3), (0, 0, 0), (1, 1, 1)),
+                S.uniforms_material(0, 0, 0, 1.0, 0.5, (1, 1, 1, 1)))
+        st.end()
+    return st.bytes()
+
+
+def test_batch_rejects_what_it_cannot_run(renderer):
+    with pytest.raises(RhipError, match="written by another view"):
+        replay_batched(renderer, _two_pass_stream(same_target=True))
+    with pytest.raises(RhipError, match="two-phase|opaque"):
+        replay_batched(renderer, _two_pass_stream(same_target=False, blend=True))
+    rp, b, _ = replay_batched(renderer, _two_pass_stream(same_target=False))
+    b.run()
+    renderer.waitIdle()
