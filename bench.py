@@ -99,6 +99,141 @@ def raster_algorithmic_bytes(d):
     return fb + 12 * d["tris"] + d["verts"] * (16 + 4 * d["vary"])
 
 
+class _HipEvents:
+    """HIP events on the renderer's stream (ctypes on the HIP runtime librhip links)."""
+
+    def __init__(self, r):
+        import ctypes
+        self.ct = ctypes
+        self.hip = ctypes.CDLL("libamdhip64.so.7")
+        self.stream = ctypes.c_void_p(r.lib.rhip_device_stream(r.dev))
+        self.events = []
+
+    def record(self):
+        e = self.ct.c_void_p()
+        if self.hip.hipEventCreate(self.ct.byref(e)) != 0:
+            raise RuntimeError("hipEventCreate failed")
+        self.hip.hipEventRecord(e, self.stream)
+        self.events.append(e)
+        return e
+
+    def ms(self, a, b):
+        self.hip.hipEventSynchronize(b)
+        out = self.ct.c_float()
+        self.hip.hipEventElapsedTime(self.ct.byref(out), a, b)
+        return float(out.value)
+
+    def close(self):
+        for e in self.events:
+            self.hip.hipEventDestroy(e)
+        self.events = []
+
+
+def _pct(x, q):
+    import numpy as np
+    return round(float(np.percentile(np.asarray(x), q)), 4)
+
+
+def run_batched(r, quick=False, reps=5, log=lambda m: None):
+    """Batched views: 256 views over 64 seeded scenes (displaced icospheres of 1k-500k triangles, 1024^2
+    Tiled textures, views of 256^2/512^2/1024^2, Blinn-Phong, no MSAA). The same render passes run once as
+    256 separate draws and once as one captured batch, after the autotuner picked the batch's tile size
+    and wavefronts per tile. Reports views/s and per-view latency (p50/p99) of both."""
+    import hashlib
+    from softglrender_amd import scenes
+    from softglrender_amd.replay import Replayer, split
+
+    t0 = time.time()
+    if quick:
+        sc = scenes.scene_c4(n_scenes=16, n_views=64, tri_hi=100_000)
+    else:
+        sc = scenes.scene_c4()
+    data = sc.stream.bytes()
+    setup, frame, _ = split(data)
+    log(f"[bench] batched views: {len(sc.meta['view_sizes'])} views, {sc.meta['view_triangles']} view-triangles, "
+        f"{sc.meta['pixels']} pixels; generated in {time.time() - t0:.1f}s")
+    rp = Replayer(r, layouts=sc.meta["layouts"])
+    sink = []
+    for op, p in setup:
+        rp.exec(op, p, sink)
+    r.waitIdle()
+    # the frame is one (begin, viewport, draw, end) group per view
+    views = [frame[i:i + 4] for i in range(0, len(frame), 4)]
+    nv = len(views)
+    ev = _HipEvents(r)
+
+    def separate(per_view):
+        marks = []
+        for cmds in views:
+            a = ev.record() if per_view else None
+            for op, p in cmds:
+                rp.exec(op, p, sink)
+            marks.append((a, ev.record() if per_view else None))
+        return marks
+
+    separate(False)  # warm-up
+    r.waitIdle()
+    a = ev.record()
+    for _ in range(reps):
+        separate(False)
+    b = ev.record()
+    sep_ms = ev.ms(a, b) / reps
+    marks = separate(True)
+    r.waitIdle()
+    own = [ev.ms(x, y) for x, y in marks]
+    since = [ev.ms(marks[0][0], y) for _, y in marks]
+    ev.close()
+    out_tex = sorted(t for t in rp.tex if t > len(sc.meta["scene_triangles"]))[0::2][:8]
+    digest = lambda: hashlib.sha256(b"".join(rp.tex[t].readImage().tobytes() for t in out_tex)).hexdigest()
+    ref_digest = digest()
+    log(f"[bench] separate draws: {sep_ms:.3f} ms per {nv} views")
+
+    r.beginBatch()
+    for op, p in frame:
+        rp.exec(op, p, sink)
+    batch = r.endBatch()
+    t_tune = time.time()
+    best, table = batch.tune(reps=3)
+    t_tune = time.time() - t_tune
+    log(f"[bench] autotuner: best tile {best[0]} waves {best[1]} in {t_tune:.1f}s: {table}")
+    batch.run()  # warm-up
+    r.waitIdle()
+    a = ev.record()
+    for _ in range(reps):
+        batch.run()
+    b = ev.record()
+    bat_ms = ev.ms(a, b) / reps
+    ev.close()
+    batch.run()
+    lat = batch.view_times()
+    same = digest() == ref_digest
+    log(f"[bench] batch: {bat_ms:.3f} ms per {nv} views; identical to separate draws: {same}")
+    draws = frame_draws(data)
+    algo = sum(raster_algorithmic_bytes(d) for d in draws)
+    res = {
+        "workload": f"{nv} views over {len(sc.meta['scene_triangles'])} scenes, "
+                    f"{sc.meta['view_triangles']} triangles and {sc.meta['pixels']} pixels per batch",
+        "views": nv,
+        "batched": {"ms_per_batch": round(bat_ms, 4), "views_per_s": round(nv / (bat_ms * 1e-3), 1),
+                    "latency_ms_p50": _pct(lat, 50), "latency_ms_p99": _pct(lat, 99),
+                    "latency_note": "completion of each view, from the batch's first kernel (device clock)",
+                    "achieved_gbs": round(algo / (bat_ms * 1e-3) / 1e9, 2)},
+        "separate_draws": {"ms_per_batch": round(sep_ms, 4), "views_per_s": round(nv / (sep_ms * 1e-3), 1),
+                           "latency_ms_p50": _pct(own, 50), "latency_ms_p99": _pct(own, 99),
+                           "completion_ms_p50": _pct(since, 50), "completion_ms_p99": _pct(since, 99),
+                           "latency_note": "each draw's own duration (HIP events around its commands); "
+                                           "completion = from the first draw's start"},
+        "speedup": round(sep_ms / bat_ms, 3),
+        "autotuner": {"best": {"tile": best[0], "waves": best[1]},
+                      "ms_per_batch": {f"tile{t}_waves{w}": round(ms, 4) for (t, w), ms in table},
+                      "seconds": round(t_tune, 1)},
+        "algorithmic_bytes_per_batch": int(algo),
+        "identical_to_separate_draws": same,
+    }
+    batch.destroy()
+    return res
+
+
 def run_neural(r, quick=False, log=lambda m: None):
     """C5: the learned PBR shader on the C2 scene (point light off for both shaders). Trains on fragments of
     64 random cameras x 8 random materials, evaluates per material on 32 held-out cameras against the FP64
@@ -180,11 +315,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4"],
+                    help="c4: only the batched-views measurement (views/s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-neural", action="store_true", help="skip the learned-shader (c5) measurement")
     ap.add_argument("--neural-quick", action="store_true", help="reduced c5 run (fewer cameras and steps)")
     ap.add_argument("--cpu-frames", type=int, default=3, help="timed reference frames (after one warm-up)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-views (c4) measurement")
+    ap.add_argument("--batched-quick", action="store_true", help="reduced c4 run (64 views over 16 scenes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build renders on one GPU; --gpus must be 1 (multi-GPU sharding is a design "
@@ -195,6 +333,21 @@ def main():
     from softglrender_amd.renderer import RendererHIP
     from softglrender_amd.replay import Replayer, parse
     from softglrender_amd import stream as S
+
+    if args.config == "c4":
+        r = RendererHIP(0)
+        r.create()
+        res = run_batched(r, quick=args.batched_quick, reps=args.steps,
+                          log=lambda m: print(m, file=sys.stderr, flush=True))
+        b = res["batched"]
+        print(json.dumps({"metric": "views/s, 256 views of 64 scenes in one batch", "value": b["views_per_s"],
+                          "unit": "views/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+                          "ms_per_step": b["ms_per_batch"], "higher_is_better": True, "scaling": "weak",
+                          "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scenes)",
+                          "config": {"workload": "c4_256_views_64_scenes_blinnphong_tiled_textures"},
+                          "batched_views": res}), flush=True)
+        r.destroy()
+        return
 
     t0 = time.time()
     if args.config == "c2":
@@ -323,6 +476,13 @@ def main():
         except Exception as e:  # report, do not hide
             neural_res = {"error": repr(e)}
 
+    batched_res = None
+    if args.config == "c2" and not args.no_batched:
+        try:
+            batched_res = run_batched(r, quick=args.batched_quick, log=lambda m: print(m, file=sys.stderr, flush=True))
+        except Exception as e:  # report, do not hide
+            batched_res = {"error": repr(e)}
+
     line = {
         "metric": "ms/frame 1080p MSAA4x 1M tris; Gfrag/s; neural vs plain PBR at <=1/255 err",
         "value": round(ms_per_frame, 4),
@@ -348,6 +508,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "neural": neural_res,
+        "batched_views": batched_res,
     }
     print(json.dumps(line), flush=True)
     r.destroy()
