@@ -129,6 +129,15 @@ class _HipEvents:
         self.events = []
 
 
+def _measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 counters on the C2 frame, or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "c2_hbm_traffic.json")) as f:
+            return json.load(f)["kernels"][kernel]["hbm_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def _pct(x, q):
     import numpy as np
     return round(float(np.percentile(np.asarray(x), q)), 4)
@@ -423,7 +432,10 @@ def main():
             algo = sum(raster_algorithmic_bytes(d) for d in mine) / len(mine)
             achieved = algo / (avg_ms * 1e-3) / 1e9
             roofline = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                        "traffic": _measured_traffic(name) if args.config == "c2" else None,
+                        "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE runs of this "
+                                        "command, scaled on k_resolve (profiles/c2_hbm_traffic.md)",
                         "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(algo)}
 
     cpu = None

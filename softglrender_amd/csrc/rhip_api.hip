@@ -194,6 +194,11 @@ struct Timer {
   }
 };
 
+/* Wavefronts per tile of a single draw's two-phase raster. */
+#ifndef RHIP_VIS_WAVES
+#define RHIP_VIS_WAVES 4
+#endif
+
 /* Two-phase raster: coarse hierarchical Z, then visibility and depth per tile. */
 template <int S>
 void launchVis(rhip_device d, const DrawParams &P, int ntiles, int ncoarse, hipStream_t st) {
@@ -208,10 +213,10 @@ void launchVis(rhip_device d, const DrawParams &P, int ntiles, int ncoarse, hipS
   }
   Timer tm(d, S == 4 ? "k_raster_vis_msaa4" : "k_raster_vis");
   switch (f) {
-    case 1: RHIP_LAUNCH(k_raster_vis<S RHIP_COMMA VIS_LESS RHIP_COMMA kTile RHIP_COMMA 4>, ntiles, 256, st, P); break;
-    case 3: RHIP_LAUNCH(k_raster_vis<S RHIP_COMMA VIS_LEQUAL RHIP_COMMA kTile RHIP_COMMA 4>, ntiles, 256, st, P); break;
-    case 4: RHIP_LAUNCH(k_raster_vis<S RHIP_COMMA VIS_GREATER RHIP_COMMA kTile RHIP_COMMA 4>, ntiles, 256, st, P); break;
-    default: RHIP_LAUNCH(k_raster_vis<S RHIP_COMMA VIS_GEQUAL RHIP_COMMA kTile RHIP_COMMA 4>, ntiles, 256, st, P); break;
+    case 1: RHIP_LAUNCH(k_raster_vis<S RHIP_COMMA VIS_LESS RHIP_COMMA kTile RHIP_COMMA RHIP_VIS_WAVES>, ntiles, 64 * RHIP_VIS_WAVES, st, P); break;
+    case 3: RHIP_LAUNCH(k_raster_vis<S RHIP_COMMA VIS_LEQUAL RHIP_COMMA kTile RHIP_COMMA RHIP_VIS_WAVES>, ntiles, 64 * RHIP_VIS_WAVES, st, P); break;
+    case 4: RHIP_LAUNCH(k_raster_vis<S RHIP_COMMA VIS_GREATER RHIP_COMMA kTile RHIP_COMMA RHIP_VIS_WAVES>, ntiles, 64 * RHIP_VIS_WAVES, st, P); break;
+    default: RHIP_LAUNCH(k_raster_vis<S RHIP_COMMA VIS_GEQUAL RHIP_COMMA kTile RHIP_COMMA RHIP_VIS_WAVES>, ntiles, 64 * RHIP_VIS_WAVES, st, P); break;
   }
 }
 
