@@ -319,6 +319,8 @@ int drawSetup(rhip_device d, int fbo, const int vp[4], const rhip_draw_desc &dd,
   P.st = st;
   P.shading = pg.shading;
   P.defines = pg.defines;
+  P.vary_groups = varyGroups(P.shading, P.defines, P.u.enable_shadow);
+  P.vary_stride = 4 * __builtin_popcount(P.vary_groups);
   for (int s = 0; s < RHS_SLOT_COUNT; s++) {
     int id = dd.sampler_tex[s];
     if (!id) continue;
@@ -396,7 +398,7 @@ size_t carveArenas(DrawParams &P, const DrawShape &sh, char *base) {
   P.clip = (float4 *)take(vcap * 16);
   P.frag = (float4 *)take(vcap * 16);
   P.mask = (int *)take(vcap * 4);
-  P.vary = (float *)take(vcap * kVaryStride * 4);
+  P.vary = (float *)take(vcap * P.vary_stride * 4);
   P.clip_raw = (float *)take(T * kMaxClipVerts * sh.stride);
   P.clip_poly = (int *)take(T * (kMaxPoly + 1) * 4);
   P.fan_first = (int *)take(T * 4);

@@ -14,7 +14,7 @@ constexpr int kMaxLevels = 16;
 constexpr int kMaxLayers = 6;
 constexpr int kTile = 16;         /* default screen tile edge in pixels (one workgroup) */
 constexpr int kSub = 8;           /* 8x8 sub-tile per wavefront */
-constexpr int kVaryStride = 32;   /* floats per vertex in the varyings buffer */
+constexpr int kVaryStride = 32;   /* varying slots of a vertex shader output (the reference layout) */
 constexpr int kMaxClipVerts = 12; /* new vertices one clipped triangle can create */
 constexpr int kMaxPoly = 9;       /* polygon vertices after six planes */
 constexpr int kMaxFans = kMaxPoly - 3; /* extra triangles one clipped triangle can append */
@@ -71,6 +71,34 @@ struct PrimSetup {
   int src;                /* triangle a clipped polygon came from (its fans share it), -1 if not clipped */
 };
 
+/* Varying slots a fragment shader reads, in the reference's per-shader slot layout
+ * (BlinnPhongSoft.h:58-68, PbrSoft.h:63-72): 0-1 uv, 4-6 normal, 12-14 camera
+ * direction, 16-18 light direction, 20-23 shadow position (Blinn-Phong), then the
+ * TBN normal and tangent when a normal map is bound. World position (8-10) only
+ * feeds the vertex shader's own outputs. */
+__host__ __device__ inline bool varySlotUsed(int shading, uint32_t defines, int enable_shadow, int j) {
+  if (shading == RHS_SHADING_VERTEX_COLOR) return j < 4;
+  if (shading == RHS_SHADING_BASIC) return false;
+  const bool nm = (defines & RHS_DEF_NORMAL_MAP) != 0;
+  if (j < 2) return true;
+  if (j >= 4 && j <= 6) return !nm;
+  if (j >= 12 && j <= 14) return true;
+  if (j >= 16 && j <= 18) return shading != RHS_SHADING_NEURAL_PBR; /* the learned shader has no point light */
+  if (shading == RHS_SHADING_PBR || shading == RHS_SHADING_NEURAL_PBR)
+    return nm && ((j >= 20 && j <= 22) || (j >= 24 && j <= 26));
+  if (j >= 20 && j <= 23) return enable_shadow != 0;
+  return nm && ((j >= 24 && j <= 26) || (j >= 28 && j <= 30));
+}
+
+/* Varyings are stored per vertex as the groups of 4 slots that hold a read slot,
+ * packed in slot order: bit g is set when slots 4g..4g+3 are stored. */
+__host__ __device__ inline uint32_t varyGroups(int shading, uint32_t defines, int enable_shadow) {
+  uint32_t g = 0;
+  for (int j = 0; j < kVaryStride; j++)
+    if (varySlotUsed(shading, defines, enable_shadow, j)) g |= 1u << (j >> 2);
+  return g;
+}
+
 /* PrimSetup.flags */
 constexpr int kPrimLive = 1, kPrimFront = 2, kPrimCoarse = 4, kPrimGlobal = 8;
 
@@ -81,6 +109,8 @@ struct DrawParams {
   int shading;
   uint32_t defines;
   int need_deriv; /* a sampler of the draw needs dFdx/dFdy (mipmapped minification) */
+  uint32_t vary_groups; /* stored groups of 4 varying slots (varyGroups) */
+  int vary_stride;      /* floats per vertex in `vary`: 4 per stored group */
   /* viewport (RendererSoft.cpp:92-113) */
   float vp_w, vp_h, o_x, o_y, p_x, p_y;
   /* framebuffer */

@@ -31,6 +31,20 @@ static __global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {
 
 /* --------------------------------------------------------------------------- vertex stage */
 
+/* Float offset of stored varying group g within a vertex's record. */
+__device__ __forceinline__ int varyGroupOffset(const DrawParams &P, int g) {
+  return 4 * __popc(P.vary_groups & ((1u << g) - 1u));
+}
+
+/* Stores the groups of vertex i's varyings that a fragment shader reads. */
+__device__ __forceinline__ void storeVaryings(const DrawParams &P, int i, const float *v) {
+  float *o = P.vary + (size_t)i * P.vary_stride;
+#pragma unroll
+  for (int g = 0; g < kVaryStride / 4; g++)
+    if ((P.vary_groups >> g) & 1u)
+      *(float4 *)(o + varyGroupOffset(P, g)) = make_float4(v[4 * g], v[4 * g + 1], v[4 * g + 2], v[4 * g + 3]);
+}
+
 __device__ __forceinline__ void vertexBody(const DrawParams &P, int blk) {
   int i = blk * blockDim.x + threadIdx.x;
   if (i >= P.nverts) return;
@@ -44,9 +58,7 @@ __device__ __forceinline__ void vertexBody(const DrawParams &P, int blk) {
   P.clip[i] = c;
   P.mask[i] = clipMask(c);
   P.frag[i] = toScreen(P, c);
-  float4 *o = (float4 *)(P.vary + (size_t)i * kVaryStride);
-#pragma unroll
-  for (int k = 0; k < kVaryStride / 4; k++) o[k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+  storeVaryings(P, i, v);
 }
 
 static __global__ __launch_bounds__(256) void k_vertex(DrawParams P) { vertexBody(P, blockIdx.x); }
@@ -455,8 +467,7 @@ __device__ int clipTriangle(const DrawParams &P, int t) {
         P.clip[nv] = c;
         P.mask[nv] = clipMask(c);
         P.frag[nv] = toScreen(P, c);
-        float *vo = P.vary + (size_t)nv * kVaryStride;
-        for (int j = 0; j < kVaryStride; j++) vo[j] = v[j];
+        storeVaryings(P, nv, v);
         out[nout++] = nv;
       }
       ip = ic;
@@ -706,17 +717,7 @@ __device__ __forceinline__ RGBAf runFS(const DrawParams &P, const Frag &f) {
 /* Varyings a fragment shader reads (the others are never interpolated). */
 template <int SHADING>
 __device__ __forceinline__ bool varyUsed(const DrawParams &P, int j) {
-  if (SHADING == RHS_SHADING_VERTEX_COLOR) return j < 4;
-  if (SHADING == RHS_SHADING_BASIC) return false;
-  bool nm = (P.defines & RHS_DEF_NORMAL_MAP) != 0;
-  if (j < 2) return true;
-  if (j >= 4 && j <= 6) return !nm;
-  if (j >= 12 && j <= 14) return true;
-  if (j >= 16 && j <= 18) return SHADING != RHS_SHADING_NEURAL_PBR; /* the learned shader has no point light */
-  if (SHADING == RHS_SHADING_PBR || SHADING == RHS_SHADING_NEURAL_PBR)
-    return nm && ((j >= 20 && j <= 22) || (j >= 24 && j <= 26));
-  if (j >= 20 && j <= 23) return P.u.enable_shadow != 0;
-  return nm && ((j >= 24 && j <= 26) || (j >= 28 && j <= 30));
+  return varySlotUsed(SHADING, P.defines, P.u.enable_shadow, j);
 }
 
 /* Shading of one fragment: `b` is the raw (screen-space) barycentric of its shading
@@ -728,14 +729,17 @@ __device__ __forceinline__ Frag makeFrag(const DrawParams &P, const PrimSetup &p
   const float w = dpps3(b, T.w);
   const float iw = 1.f / w;
   b = make_float4(b.x * (iw * T.w[0]), b.y * (iw * T.w[1]), b.z * (iw * T.w[2]), b.w * (iw * 1.f));
-  const float *va = P.vary + (size_t)ps.v[0] * kVaryStride;
-  const float *vb = P.vary + (size_t)ps.v[1] * kVaryStride;
-  const float *vc = P.vary + (size_t)ps.v[2] * kVaryStride;
+  const float *va = P.vary + (size_t)ps.v[0] * P.vary_stride;
+  const float *vb = P.vary + (size_t)ps.v[1] * P.vary_stride;
+  const float *vc = P.vary + (size_t)ps.v[2] * P.vary_stride;
   Frag fr;
 #pragma unroll
-  for (int j = 0; j < kVaryStride; j++) fr.v[j] = varyUsed<SHADING>(P, j) ? interp(b, va[j], vb[j], vc[j]) : 0.f;
+  for (int j = 0; j < kVaryStride; j++) {
+    const int o = varyGroupOffset(P, j >> 2) + (j & 3);
+    fr.v[j] = varyUsed<SHADING>(P, j) ? interp(b, va[o], vb[o], vc[o]) : 0.f;
+  }
   fr.tc[0] = fr.tc[1] = fr.tc[2] = {0.f, 0.f};
-  if (need_deriv) {
+  if (need_deriv && (P.vary_groups & 1u)) { /* texture coordinates are slots 0-1 (group 0, offset 0) */
     /* coarse derivatives from quad pixels 0, 1, 2 (RendererSoft.cpp:329-334) */
     int qx0, qx1, qy0, qy1;
     axisQuads(ps.min_x, ps.max_x, ps.bcx, px, qx0, qx1);
