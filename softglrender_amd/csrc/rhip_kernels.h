@@ -21,6 +21,41 @@
 
 namespace rhip {
 
+/* --------------------------------------------------------------------------- XCD placement
+ *
+ * The MI355X deals a launch's workgroups to its eight XCDs in turn (workgroup i
+ * runs on XCD i mod 8), and each XCD has its own L2. Neighbouring blocks of work
+ * share data (triangles share vertices, tiles share big triangles, fragments
+ * share texels), so the kernels whose work items cost about the same (setup,
+ * clipping, shading) renumber their workgroups to give each XCD one contiguous
+ * range of the work; the rasters keep the round-robin order, which spreads tiles
+ * of uneven cost over all XCDs. The renumbering is a bijection: which hardware
+ * runs what changes, results do not. */
+constexpr int kXcds = 8;
+
+/* Logical block of workgroup b of n: XCD x gets one contiguous range. */
+__device__ __forceinline__ int xcdBlock(int b, int n) {
+  const int per = n / kXcds, extra = n % kXcds, x = b % kXcds, k = b / kXcds;
+  return x * per + min(x, extra) + k;
+}
+
+/* For grid-stride loops over n items: this workgroup's XCD owns items [lo, hi),
+ * a share in proportion to the workgroups it holds (none when the grid has fewer
+ * than one per XCD); the workgroup is number `idx` of the `cnt` that XCD holds. */
+struct XcdShare {
+  int lo, hi, idx, cnt;
+};
+__device__ __forceinline__ XcdShare xcdShare(int n) {
+  const int g = (int)gridDim.x, x = blockIdx.x % kXcds;
+  const int first = xcdBlock(x, g); /* logical number of this XCD's first workgroup */
+  XcdShare r;
+  r.idx = blockIdx.x / kXcds;
+  r.cnt = (g - x + kXcds - 1) / kXcds;
+  r.lo = (int)((long long)n * first / g);
+  r.hi = (int)((long long)n * (first + r.cnt) / g);
+  return r;
+}
+
 /* --------------------------------------------------------------------------- clears */
 
 static __global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {
@@ -388,7 +423,7 @@ __device__ __forceinline__ void setupBody(const DrawParams &P, int blk) {
   if (threadIdx.x == 0) P.blk_clip[blk] = n;
 }
 
-static __global__ __launch_bounds__(256) void k_setup(DrawParams P) { setupBody(P, blockIdx.x); }
+static __global__ __launch_bounds__(256) void k_setup(DrawParams P) { setupBody(P, xcdBlock(blockIdx.x, gridDim.x)); }
 
 /* --------------------------------------------------------------------------- clipping */
 
@@ -509,7 +544,7 @@ __device__ __forceinline__ void clipBody(const DrawParams &P, int b) {
   if (threadIdx.x == 0) P.blk_fans[b] = fans;
 }
 
-static __global__ __launch_bounds__(256) void k_clip(DrawParams P) { clipBody(P, blockIdx.x); }
+static __global__ __launch_bounds__(256) void k_clip(DrawParams P) { clipBody(P, xcdBlock(blockIdx.x, gridDim.x)); }
 
 /* One block of 1024: exclusive offsets of every block's fan triangles, and the primitive count. */
 __device__ __forceinline__ void scanFansBody(const DrawParams &P) {
@@ -537,7 +572,9 @@ __device__ __forceinline__ void clipFansBody(const DrawParams &P, int b) {
   for (int i = 3; i < n; i++) binCount(P, setupPrim(P, base + i - 3, poly[0], poly[i - 1], poly[i], t));
 }
 
-static __global__ __launch_bounds__(256) void k_clip_fans(DrawParams P) { clipFansBody(P, blockIdx.x); }
+static __global__ __launch_bounds__(256) void k_clip_fans(DrawParams P) {
+  clipFansBody(P, xcdBlock(blockIdx.x, gridDim.x));
+}
 
 /* --------------------------------------------------------------------------- binning */
 
@@ -1364,7 +1401,7 @@ __device__ void visTile(const DrawParams &P, int tile) {
 
 template <int S, int MODE, int TILE, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_raster_vis(DrawParams P) {
-  visTile<S, MODE, TILE, WAVES>(P, blockIdx.x);
+  visTile<S, MODE, TILE, WAVES>(P, blockIdx.x); /* tiles stay dealt round-robin: their costs are uneven */
 }
 
 /* Colour of the fragments the two-phase raster queued: one lane per (pixel,
@@ -1426,7 +1463,8 @@ template <int S, int SHADING>
 __global__ __launch_bounds__(256) void k_shade(DrawParams P, int need_deriv) {
   const int n = P.dcount[2];
   unsigned long long nfs = 0, nwr = 0;
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+  const XcdShare x = xcdShare(n);
+  for (int j = x.lo + x.idx * blockDim.x + threadIdx.x; j < x.hi; j += x.cnt * blockDim.x)
     shadeJob<S, SHADING>(P, j, need_deriv, nfs, nwr);
   addShadeCounts(P, nfs, nwr);
 }

@@ -372,7 +372,8 @@ __global__ __launch_bounds__(256) void k_capture(DrawParams P, int need_deriv, f
 template <int S>
 __global__ __launch_bounds__(256) void k_nn_features(DrawParams P, int need_deriv, __bf16 *feat) {
   const int n = P.dcount[2];
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+  const XcdShare x = xcdShare((n + 63) / 64); /* whole 64-row chunks, as k_nn_infer reads them */
+  for (int j = x.lo * 64 + x.idx * blockDim.x + threadIdx.x; j < min(n, x.hi * 64); j += x.cnt * blockDim.x) {
     const uint2 jb = P.jobs[j];
     const uint32_t pix = jb.x & 0x0fffffffu, w = jb.y;
     const int px = (int)(pix % (uint32_t)P.fb_w), py = (int)(pix / (uint32_t)P.fb_w);
@@ -434,7 +435,8 @@ __global__ __launch_bounds__(256) void k_nn_infer(DrawParams P, const NnPacked *
   const int n = P.dcount[2];
   const int nchunks = (n + 63) / 64;
   unsigned long long nfs = 0, nwr = 0;
-  for (int ch = blockIdx.x * 4 + wave; ch < nchunks; ch += gridDim.x * 4) {
+  const XcdShare xs = xcdShare(nchunks); /* the chunks k_nn_features wrote on this XCD */
+  for (int ch = xs.lo + xs.idx * 4 + wave; ch < xs.hi; ch += xs.cnt * 4) {
     /* this lane's fragment: its job record and alpha, fetched while the network runs */
     const int jme = ch * 64 + l;
     const bool mine = jme < n;
