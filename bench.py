@@ -298,7 +298,7 @@ def run_neural(r, quick=False, log=lambda m: None):
         frame_ms = (time.perf_counter() - t) * 1e3 / reps
         prof = r.profileResults()
         r.profile(False)
-        kerns = ["k_shade_msaa4"] if name == "plain" else ["k_nn_features_msaa4", "k_nn_infer_msaa4"]
+        kerns = ["k_shade_msaa4"] if name == "plain" else ["k_nn_shade_msaa4"]
         times[name] = {"shade_kernel_ms": round(sum(prof[k][0] / prof[k][1] for k in kerns), 4),
                        "frame_ms": round(frame_ms, 4)}
         if name == "neural":

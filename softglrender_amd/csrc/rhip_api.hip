@@ -373,6 +373,7 @@ int drawSetup(rhip_device d, int fbo, const int vp[4], const rhip_draw_desc &dd,
   P.idx = va.idx;
   P.ntris = va.nidx / 3;
   P.counters = d->counters;
+  P.gamma_lut = d->gamma_lut;
   sh.V = (size_t)va.nverts;
   sh.T = (size_t)P.ntris;
   sh.stride = va.stride;
@@ -439,6 +440,9 @@ int rhip_device_create(int hip_device, rhip_device *out) {
   HIPCHK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
   HIPCHK(hipMalloc(&d->counters, 8 * sizeof(unsigned long long)));
   HIPCHK(hipMemset(d->counters, 0, 8 * sizeof(unsigned long long)));
+  HIPCHK(hipMalloc(&d->gamma_lut, 512 * sizeof(float)));
+  RHIP_LAUNCH(k_gamma_tables, 1, 256, d->stream, d->gamma_lut);
+  HIPCHK(hipGetLastError());
   *out = d.release();
   return RHIP_OK;
 }
@@ -451,6 +455,7 @@ int rhip_device_destroy(rhip_device d) {
   d->tex.clear();
   d->vao.clear();
   if (d->counters) hipFree(d->counters);
+  if (d->gamma_lut) hipFree(d->gamma_lut);
   for (void *p : {(void *)d->nn.params, (void *)d->nn.grad, (void *)d->nn.m1, (void *)d->nn.m2, (void *)d->nn.packed,
                   (void *)d->nn.scratch, (void *)d->nn.hist, (void *)d->nn.rec_count})
     if (p) hipFree(p);
@@ -786,21 +791,12 @@ int rhip_draw(rhip_device d, const rhip_draw_desc *dd) {
     if (P.color) {
       const int grid = (int)std::min<size_t>(blocks(njobs, 256), 4096);
       if (P.shading == RHS_SHADING_NEURAL_PBR) {
-        if ((rc = d->nn.feat.ensure((njobs + 64) * kNnK1 * 2))) return rc;
-        __bf16 *feat = d->nn.feat.as<__bf16>();
-        {
-          Timer tm(d, samples == 4 ? "k_nn_features_msaa4" : "k_nn_features");
-          if (samples == 4)
-            RHIP_LAUNCH(k_nn_features<4>, grid, 256, s, P, need_deriv ? 1 : 0, feat);
-          else
-            RHIP_LAUNCH(k_nn_features<1>, grid, 256, s, P, need_deriv ? 1 : 0, feat);
-        }
-        Timer tm(d, samples == 4 ? "k_nn_infer_msaa4" : "k_nn_infer");
-        const unsigned ig = (unsigned)std::min<size_t>((njobs + 255) / 256, 1024); /* about one resident set */
+        Timer tm(d, samples == 4 ? "k_nn_shade_msaa4" : "k_nn_shade");
+        const unsigned ig = (unsigned)std::min<size_t>((njobs + 255) / 256, 2048);
         if (samples == 4)
-          RHIP_LAUNCH(k_nn_infer<4>, ig, 256, s, P, d->nn.packed, feat);
+          RHIP_LAUNCH(k_nn_shade<4>, ig, 256, s, P, d->nn.packed, need_deriv ? 1 : 0);
         else
-          RHIP_LAUNCH(k_nn_infer<1>, ig, 256, s, P, d->nn.packed, feat);
+          RHIP_LAUNCH(k_nn_shade<1>, ig, 256, s, P, d->nn.packed, need_deriv ? 1 : 0);
       } else {
         Timer tm(d, samples == 4 ? "k_shade_msaa4" : "k_shade");
         if (samples == 4)

@@ -102,6 +102,7 @@ struct rhip_device_s {
   /* per-draw arenas */
   rhip::DevBuf arena;
   unsigned long long *counters = nullptr; /* device */
+  float *gamma_lut = nullptr;             /* device: k_gamma_tables */
   /* optional per-kernel timing: an event pair around every launch */
   bool profiling = false;
   struct Timed {
@@ -120,7 +121,7 @@ struct rhip_device_s {
     unsigned int *hist = nullptr;
     int step = 0;
     bool ready = false;
-    rhip::DevBuf records, feat;
+    rhip::DevBuf records;
     long long rec_cap = 0;
     unsigned long long *rec_count = nullptr;
     bool capturing = false;

@@ -109,6 +109,7 @@ struct DrawParams {
   int shading;
   uint32_t defines;
   int need_deriv; /* a sampler of the draw needs dFdx/dFdy (mipmapped minification) */
+  const float *gamma_lut; /* [0, 256): rpowf(k / 255, 2.2); [256, 512): the PBR output-byte thresholds */
   uint32_t vary_groups; /* stored groups of 4 varying slots (varyGroups) */
   int vary_stride;      /* floats per vertex in `vary`: 4 per stored group */
   /* viewport (RendererSoft.cpp:92-113) */

@@ -21,6 +21,28 @@
 
 namespace rhip {
 
+/* --------------------------------------------------------------------------- shading tables */
+
+/* The output byte of a PBR colour channel c in the two-phase raster (gamma 1/2.2, clamp, truncation). */
+__device__ __forceinline__ int gammaOutExact(float c) { return u8t(gclamp(rpowf(c, 1.0f / 2.2f), 0.f, 1.f) * 255.f); }
+
+/* One block of 256: t[k] = rpowf(k / 255, 2.2) (input gamma of RGBA8 albedo), and
+ * t[256 + k] = the smallest non-negative float c with gammaOutExact(c) >= k, by
+ * bisection over float bit patterns (order-preserving for non-negative floats). */
+static __global__ void k_gamma_tables(float *t) {
+  const int k = threadIdx.x;
+  t[k] = rpowf((float)k / 255.f, 2.2f);
+  uint32_t lo = 0, hi = 0x7f800000u; /* +inf gives byte 255 */
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (gammaOutExact(__uint_as_float(mid)) >= k)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  t[256 + k] = __uint_as_float(lo);
+}
+
 /* --------------------------------------------------------------------------- XCD placement
  *
  * The MI355X deals a launch's workgroups to its eight XCDs in turn (workgroup i
@@ -580,6 +602,8 @@ static __global__ __launch_bounds__(256) void k_clip_fans(DrawParams P) {
 
 /* One block of 1024: bin starts from the counts setup made; cursors cleared. */
 __device__ __forceinline__ void binScanBody(const DrawParams &P) {
+  /* the PBR shader's linear base colour, rpowf(base colour, 2.2), once per draw */
+  if (threadIdx.x < 3) P.dcount[4 + threadIdx.x] = __float_as_int(rpowf(P.u.base_color[threadIdx.x], 2.2f));
   const int total = blockScanArray(P.bin_count, P.bin_start, P.nbins, P.bin_cursor);
   if (threadIdx.x == 0) {
     P.bin_start[P.nbins] = total;
@@ -736,7 +760,7 @@ __device__ __forceinline__ float interp(float4 b, float v0, float v1, float v2) 
   return fmaf(v2, b.z, fmaf(v1, b.y, v0 * b.x));
 }
 
-template <int SHADING>
+template <int SHADING, bool U8OUT = false>
 __device__ __forceinline__ RGBAf runFS(const DrawParams &P, const Frag &f) {
   if (SHADING == RHS_SHADING_BASIC) {
     return {P.u.base_color[0], P.u.base_color[1], P.u.base_color[2], P.u.base_color[3]};
@@ -745,7 +769,7 @@ __device__ __forceinline__ RGBAf runFS(const DrawParams &P, const Frag &f) {
   } else if (SHADING == RHS_SHADING_BLINN_PHONG_DIR) {
     return fsBlinnPhong(P, f, true);
   } else if (SHADING == RHS_SHADING_PBR) {
-    return fsPbr(P, f);
+    return fsPbr<U8OUT>(P, f);
   } else {
     return {f.v[0], f.v[1], f.v[2], f.v[3]};
   }
@@ -796,10 +820,10 @@ __device__ __forceinline__ Frag makeFrag(const DrawParams &P, const PrimSetup &p
   return fr;
 }
 
-template <int S, int SHADING>
+template <int S, int SHADING, bool U8OUT = false>
 __device__ __forceinline__ RGBAf shadeAt(const DrawParams &P, const PrimSetup &ps, const Tri &T, int px, int py,
                                          float4 b, int inst, int need_deriv) {
-  return runFS<SHADING>(P, makeFrag<S, SHADING>(P, ps, T, px, py, b, inst, need_deriv));
+  return runFS<SHADING, U8OUT>(P, makeFrag<S, SHADING>(P, ps, T, px, py, b, inst, need_deriv));
 }
 
 __device__ __forceinline__ uint32_t resolve4(const uint32_t *c) {
@@ -1432,7 +1456,7 @@ __device__ __forceinline__ void shadeJob(const DrawParams &P, int j, int need_de
     } else {
       bary(T, fx + 0.5f, fy + 0.5f, bs);
     }
-    RGBAf c = shadeAt<S, SHADING>(P, ps, T, px, py, bs, inst, need_deriv);
+    RGBAf c = shadeAt<S, SHADING, true>(P, ps, T, px, py, bs, inst, need_deriv); /* written as bytes, no blending */
     c = {gclamp(c.r, 0.f, 1.f), gclamp(c.g, 0.f, 1.f), gclamp(c.b, 0.f, 1.f), gclamp(c.a, 0.f, 1.f)};
     const uint32_t packed = pack4(u8t(c.r * 255.f), u8t(c.g * 255.f), u8t(c.b * 255.f), u8t(c.a * 255.f));
     nfs++;

@@ -142,7 +142,7 @@ __device__ __forceinline__ void storeRelu4(__bf16 *dst, f32x4 d) {
   typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
   bf16x4 o;
 #pragma unroll
-  for (int v = 0; v < 4; v++) o[v] = (__bf16)gmax(d[v], 0.f);
+  for (int v = 0; v < 4; v++) o[v] = (__bf16)fmaxf(d[v], 0.f); /* one v_max_f32 (a NaN would become 0) */
   *(bf16x4 *)dst = o;
 }
 
@@ -365,68 +365,55 @@ __global__ __launch_bounds__(256) void k_capture(DrawParams P, int need_deriv, f
 
 /* --------------------------------------------------------------------------- inference in the pipeline */
 
-/* Learned shading, part 1: the inputs of every fragment the two-phase raster
- * queued (albedo, normal and metal-rough maps sampled as the plain shader
- * samples them), as a 32-wide bf16 row: 24 features, the bias 1, zeros, and
- * the albedo alpha in the last column (which the network's weights ignore). */
+/* The network's input row for job j of the two-phase raster (albedo, normal and
+ * metal-rough maps sampled as the plain shader samples them): 24 features, the
+ * bias 1, zeros, and the albedo alpha in the last column (which the weights ignore). */
 template <int S>
-__global__ __launch_bounds__(256) void k_nn_features(DrawParams P, int need_deriv, __bf16 *feat) {
-  const int n = P.dcount[2];
-  const XcdShare x = xcdShare((n + 63) / 64); /* whole 64-row chunks, as k_nn_infer reads them */
-  for (int j = x.lo * 64 + x.idx * blockDim.x + threadIdx.x; j < min(n, x.hi * 64); j += x.cnt * blockDim.x) {
-    const uint2 jb = P.jobs[j];
-    const uint32_t pix = jb.x & 0x0fffffffu, w = jb.y;
-    const int px = (int)(pix % (uint32_t)P.fb_w), py = (int)(pix / (uint32_t)P.fb_w);
-    const int id = (int)(w >> 2), inst = (int)(w & 3u);
-    const PrimSetup ps = P.prims[id];
-    const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
-    const float fx = (float)px, fy = (float)py;
-    float4 bs;
-    if (S == 4) {
-      if (!bary(T, fx + 0.5f, fy + 0.5f, bs)) {
+__device__ __forceinline__ void nnJobRow(const DrawParams &P, int j, int need_deriv, float *f) {
+  const uint2 jb = P.jobs[j];
+  const uint32_t pix = jb.x & 0x0fffffffu, w = jb.y;
+  const int px = (int)(pix % (uint32_t)P.fb_w), py = (int)(pix / (uint32_t)P.fb_w);
+  const int id = (int)(w >> 2), inst = (int)(w & 3u);
+  const PrimSetup ps = P.prims[id];
+  const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
+  const float fx = (float)px, fy = (float)py;
+  float4 bs;
+  if (S == 4) {
+    if (!bary(T, fx + 0.5f, fy + 0.5f, bs)) {
 #pragma unroll
-        for (int t = 3; t >= 0; t--) {
-          float4 b;
-          if (bary(T, sampleX(t) + fx, sampleY(t) + fy, b)) bs = b;
-        }
+      for (int t = 3; t >= 0; t--) {
+        float4 b;
+        if (bary(T, sampleX(t) + fx, sampleY(t) + fy, b)) bs = b;
       }
-    } else {
-      bary(T, fx + 0.5f, fy + 0.5f, bs);
     }
-    const Frag fr = makeFrag<S, RHS_SHADING_NEURAL_PBR>(P, ps, T, px, py, bs, inst, need_deriv);
-    const f2 uv{fr.v[0], fr.v[1]};
-    const TexDesc *ta = P.tex[RHS_SLOT_ALBEDO];
-    const RGBAf alb = (P.defines & RHS_DEF_ALBEDO_MAP)
-                          ? tex2D(ta, uv, lodFor(ta, fr))
-                          : RGBAf{P.u.base_color[0], P.u.base_color[1], P.u.base_color[2], P.u.base_color[3]};
-    float metal = 0.f, rough = 1.f;
-    if (P.defines & RHS_DEF_METALROUGHNESS_MAP) {
-      const RGBAf mr = tex2D(P.tex[RHS_SLOT_METAL_ROUGHNESS], uv, 0.f);
-      metal = mr.b;
-      rough = mr.g;
-    }
-    const f3 N = normalFromMap(P, fr, 20, 24);
-    const f3 V = nrm3({fr.v[12], fr.v[13], fr.v[14]});
-    float f[kNnK1];
-    nnFeatures(N, V, rough, metal, alb.r, alb.g, alb.b, f);
-    f[kNnK1 - 1] = alb.a;
-    bf16x8 *o = (bf16x8 *)(feat + (size_t)j * kNnK1);
-#pragma unroll
-    for (int q = 0; q < kNnK1 / 8; q++) {
-      bf16x8 v;
-#pragma unroll
-      for (int k = 0; k < 8; k++) v[k] = (__bf16)f[8 * q + k];
-      o[q] = v;
-    }
+  } else {
+    bary(T, fx + 0.5f, fy + 0.5f, bs);
   }
+  const Frag fr = makeFrag<S, RHS_SHADING_NEURAL_PBR>(P, ps, T, px, py, bs, inst, need_deriv);
+  const f2 uv{fr.v[0], fr.v[1]};
+  const TexDesc *ta = P.tex[RHS_SLOT_ALBEDO];
+  const RGBAf alb = (P.defines & RHS_DEF_ALBEDO_MAP)
+                        ? tex2D(ta, uv, lodFor(ta, fr))
+                        : RGBAf{P.u.base_color[0], P.u.base_color[1], P.u.base_color[2], P.u.base_color[3]};
+  float metal = 0.f, rough = 1.f;
+  if (P.defines & RHS_DEF_METALROUGHNESS_MAP) {
+    const RGBAf mr = tex2D(P.tex[RHS_SLOT_METAL_ROUGHNESS], uv, 0.f);
+    metal = mr.b;
+    rough = mr.g;
+  }
+  const f3 N = normalFromMap(P, fr, 20, 24);
+  const f3 V = nrm3({fr.v[12], fr.v[13], fr.v[14]});
+  nnFeatures(N, V, rough, metal, alb.r, alb.g, alb.b, f);
+  f[kNnK1 - 1] = alb.a;
 }
 
-/* Learned shading, part 2: the network on 64 fragments per wavefront, layer-1
- * operands read straight from the feature rows; the colour goes to every
- * sample the fragment won. */
+/* Learned shading of the fragments the two-phase raster queued, one kernel:
+ * each wavefront takes 64 jobs, every lane writes its job's input row to the
+ * wavefront's LDS tile, then the network runs on MFMA from that tile (weights
+ * from L2) and each lane writes its fragment's colour to every sample it won. */
 template <int S>
-__global__ __launch_bounds__(256) void k_nn_infer(DrawParams P, const NnPacked *W, const __bf16 *feat) {
-  __shared__ __bf16 hbuf[4][64 * kHs];
+__global__ __launch_bounds__(256) void k_nn_shade(DrawParams P, const NnPacked *W, int need_deriv) {
+  __shared__ __bf16 hbuf[4][64 * kHs]; /* per wavefront: input rows, then hidden activations, then outputs */
   __shared__ unsigned long long s_fs, s_wr;
   if (threadIdx.x == 0) s_fs = s_wr = 0;
   __syncthreads();
@@ -435,21 +422,34 @@ __global__ __launch_bounds__(256) void k_nn_infer(DrawParams P, const NnPacked *
   const int n = P.dcount[2];
   const int nchunks = (n + 63) / 64;
   unsigned long long nfs = 0, nwr = 0;
-  const XcdShare xs = xcdShare(nchunks); /* the chunks k_nn_features wrote on this XCD */
+  const XcdShare xs = xcdShare(nchunks);
   for (int ch = xs.lo + xs.idx * 4 + wave; ch < xs.hi; ch += xs.cnt * 4) {
-    /* this lane's fragment: its job record and alpha, fetched while the network runs */
     const int jme = ch * 64 + l;
     const bool mine = jme < n;
     uint32_t jx = 0;
     float alpha = 0.f;
-    if (mine) {
-      jx = P.jobs[jme].x;
-      alpha = (float)feat[(size_t)jme * kNnK1 + kNnK1 - 1];
+    {
+      float f[kNnK1];
+#pragma unroll
+      for (int k = 0; k < kNnK1; k++) f[k] = 0.f;
+      if (mine) {
+        jx = P.jobs[jme].x;
+        nnJobRow<S>(P, jme, need_deriv, f);
+        alpha = (float)(__bf16)f[kNnK1 - 1];
+      }
+#pragma unroll
+      for (int q = 0; q < kNnK1 / 8; q++) {
+        bf16x8 v;
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = (__bf16)f[8 * q + k];
+        *(bf16x8 *)&H[l * kHs + 8 * q] = v;
+      }
     }
-    const __bf16 *X = feat + (size_t)ch * 64 * kNnK1;
+    waveSync();
     bf16x8 xb[4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) xb[q] = ldOp(X, kNnK1, 16 * q, 0);
+    for (int q = 0; q < 4; q++) xb[q] = ldOp(H, kHs, 16 * q, 0);
+    waveSync(); /* every lane holds its operands before the tile is overwritten */
 #pragma unroll
     for (int m = 0; m < 4; m++) {
       const bf16x8 a = ldOp(W->w1, kNnK1, 16 * m, 0);
@@ -469,9 +469,7 @@ __global__ __launch_bounds__(256) void k_nn_infer(DrawParams P, const NnPacked *
 #pragma unroll
     for (int m = 0; m < 4; m++) {
       const bf16x8 a0 = ldOp(W->w2, kNnHid, 16 * m, 0), a1 = ldOp(W->w2, kNnHid, 16 * m, 32);
-      f32x4 bias;
-#pragma unroll
-      for (int v = 0; v < 4; v++) bias[v] = W->b2[16 * m + 4 * g + v];
+      const f32x4 bias = *(const f32x4 *)&W->b2[16 * m + 4 * g];
 #pragma unroll
       for (int q = 0; q < 4; q++)
         storeRelu4(&H[(16 * q + c) * kHs + 16 * m + 4 * g], mma16(a1, hb[q][1], mma16(a0, hb[q][0], bias)));
@@ -481,14 +479,12 @@ __global__ __launch_bounds__(256) void k_nn_infer(DrawParams P, const NnPacked *
     f32x4 yq[4];
     {
       const bf16x8 a0 = ldOp(W->w3, kNnHid, 0, 0), a1 = ldOp(W->w3, kNnHid, 0, 32);
-      f32x4 bias;
-#pragma unroll
-      for (int v = 0; v < 4; v++) bias[v] = W->b3[4 * g + v];
+      const f32x4 bias = *(const f32x4 *)&W->b3[4 * g];
 #pragma unroll
       for (int q = 0; q < 4; q++) yq[q] = mma16(a1, ldOp(H, kHs, 16 * q, 32), mma16(a0, ldOp(H, kHs, 16 * q, 0), bias));
     }
     waveSync();
-    /* outputs to their fragments' lanes through the (now free) hidden-layer LDS */
+    /* outputs to their fragments' lanes through the (now free) LDS tile */
     float *Y = (float *)H;
     if (g == 0) {
 #pragma unroll

@@ -36,6 +36,31 @@ __device__ __forceinline__ f3 cross3(f3 a, f3 b) {
 __device__ __forceinline__ float rpowf(float x, float y) { return (float)pow((double)x, (double)y); }
 __device__ __forceinline__ float rlog2f(float x) { return (float)log2((double)x); }
 __device__ __forceinline__ float rexp2f(float x) { return (float)exp2((double)x); }
+
+/* rpowf(x, N) for the integer exponents the shaders use, without the general pow
+ * when it can be avoided (Ziv's strategy): the product in double carries a relative
+ * error below 2^-47 (x^5: two roundings; x^128: x^2 is exact, then six squarings,
+ * at most 63 double ulps). If both ends of that error interval round to the same
+ * float, so does the exact power, and that float is rpowf's; otherwise (about one
+ * call in a million) the general pow decides. Zero, subnormal and overflowing results
+ * are covered by the same test. */
+template <int N>
+__device__ __forceinline__ float rpowi(float x) {
+  static_assert(N == 5 || N == 128, "exponent");
+  const double a = x;
+  double r;
+  if (N == 5) {
+    const double a2 = a * a;
+    r = a2 * a2 * a;
+  } else {
+    r = a;
+#pragma unroll
+    for (int i = 0; i < 7; i++) r = r * r;
+  }
+  const float f = (float)r;
+  if ((float)(r * (1.0 - 0x1p-44)) == f && (float)(r * (1.0 + 0x1p-44)) == f) return f;
+  return rpowf(x, (float)N);
+}
 __device__ __forceinline__ float gmax(float a, float b) { return a < b ? b : a; } /* glm::max */
 __device__ __forceinline__ float gmin(float a, float b) { return b < a ? b : a; } /* glm::min */
 /* glm::clamp = min(max(x, lo), hi) with glm's comparisons (keeps -0 and NaN like the reference) */
@@ -375,7 +400,7 @@ __device__ __forceinline__ RGBAf fsBlinnPhong(const DrawParams &P, const Frag &f
     f3 C = nrm3({v[12], v[13], v[14]});
     f3 H = nrm3(add3(L, C));
     float sa = gmax(dot3(N, H), 0.0f);
-    float sp = u.k_specular * rpowf(sa, 128.f);
+    float sp = u.k_specular * rpowi<128>(sa);
     spe = {sp, sp, sp};
     if (u.enable_shadow) {
       float s = 1.0f - shadowCalc(P, make_float4(v[20], v[21], v[22], v[23]), N, ldv);
@@ -414,6 +439,28 @@ __device__ __forceinline__ f3 envBRDFApprox(f3 spec, float r, float nv) {
   return {spec.x * ax + ay, spec.y * ax + ay, spec.z * ax + ay};
 }
 
+/* rpowf(x, 2.2) of a channel sampled from an RGBA8 texture: x is k / 255 for a
+ * byte k, and the device computed rpowf of all 256 once (k_gamma_tables). */
+__device__ __forceinline__ float gammaIn(const DrawParams &P, float x) {
+  return P.gamma_lut[(int)(x * 255.f + 0.5f)];
+}
+
+/* The byte the two-phase raster writes for a PBR colour channel c without an
+ * emissive map: u8(clamp(rpowf(c, 1/2.2), 0, 1) * 255), monotonic in c. The
+ * device found, with rpowf itself, the smallest c giving each byte k
+ * (k_gamma_tables); a hardware log2/exp2 estimate picks the starting byte and
+ * the thresholds correct it. Returned as a float that truncates back to k. */
+__device__ __forceinline__ float gammaOutByte(const DrawParams &P, float c) {
+  if (!(c > 0.f) || !(c < INFINITY)) return rpowf(c, 1.0f / 2.2f);
+  const float *T = P.gamma_lut + 256;
+  const float e = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(c) * (1.0f / 2.2f));
+  int k = (int)(gmin(e, 1.f) * 255.f);
+  k = k < 0 ? 0 : (k > 255 ? 255 : k);
+  while (k < 255 && c >= T[k + 1]) k++;
+  while (k > 0 && c < T[k]) k--;
+  return ((float)k + 0.5f) * (1.f / 255.f);
+}
+
 /* Material inputs of the PBR shader, shared with the learned shader. */
 struct PbrInputs {
   RGBAf albedo_rgba;
@@ -428,10 +475,14 @@ __device__ __forceinline__ PbrInputs pbrInputs(const DrawParams &P, const Frag &
   PbrInputs in;
   f2 uv{v[0], v[1]};
   const TexDesc *ta = P.tex[RHS_SLOT_ALBEDO];
-  in.albedo_rgba = (P.defines & RHS_DEF_ALBEDO_MAP)
-                       ? tex2D(ta, uv, lodFor(ta, f))
-                       : RGBAf{u.base_color[0], u.base_color[1], u.base_color[2], u.base_color[3]};
-  in.albedo = {rpowf(in.albedo_rgba.r, 2.2f), rpowf(in.albedo_rgba.g, 2.2f), rpowf(in.albedo_rgba.b, 2.2f)};
+  if (P.defines & RHS_DEF_ALBEDO_MAP) {
+    in.albedo_rgba = tex2D(ta, uv, lodFor(ta, f));
+    in.albedo = {gammaIn(P, in.albedo_rgba.r), gammaIn(P, in.albedo_rgba.g), gammaIn(P, in.albedo_rgba.b)};
+  } else {
+    /* rpowf(base colour, 2.2), computed once per draw by k_bin_scan */
+    in.albedo_rgba = RGBAf{u.base_color[0], u.base_color[1], u.base_color[2], u.base_color[3]};
+    in.albedo = {__int_as_float(P.dcount[4]), __int_as_float(P.dcount[5]), __int_as_float(P.dcount[6])};
+  }
   in.metallic = 0.0f;
   in.rough = 1.0f;
   if (P.defines & RHS_DEF_METALROUGHNESS_MAP) {
@@ -450,7 +501,9 @@ __device__ __forceinline__ PbrInputs pbrInputs(const DrawParams &P, const Frag &
   return in;
 }
 
-/* PbrSoft.h:225-322 */
+/* PbrSoft.h:225-322. U8OUT: the caller only truncates the colour to bytes with
+ * nothing blended in between, so the gamma step may return byte representatives. */
+template <bool U8OUT = false>
 __device__ __forceinline__ RGBAf fsPbr(const DrawParams &P, const Frag &f) {
   const float *v = f.v;
   const Uniforms &u = P.u;
@@ -470,7 +523,7 @@ __device__ __forceinline__ RGBAf fsPbr(const DrawParams &P, const Frag &f) {
     float nv = gmax(dot3(N, V), 0.0f), nl = gmax(dot3(N, L), 0.0f);
     float G = schlickGGX(nl, rough) * schlickGGX(nv, rough);
     float ct = gmax(dot3(H, V), 0.0f);
-    float p5 = rpowf(gclamp(1.0f - ct, 0.0f, 1.0f), 5.0f);
+    float p5 = rpowi<5>(gclamp(1.0f - ct, 0.0f, 1.0f));
     f3 F{F0.x + (1.0f - F0.x) * p5, F0.y + (1.0f - F0.y) * p5, F0.z + (1.0f - F0.z) * p5};
     float ng = NDF * G;
     f3 num = scl3(F, ng);
@@ -483,7 +536,7 @@ __device__ __forceinline__ RGBAf fsPbr(const DrawParams &P, const Frag &f) {
   f3 ambient;
   if (u.enable_ibl) {
     float nv = gmax(dot3(N, V), 0.0f);
-    float p5 = rpowf(gclamp(1.0f - nv, 0.0f, 1.0f), 5.0f);
+    float p5 = rpowi<5>(gclamp(1.0f - nv, 0.0f, 1.0f));
     float omr = 1.0f - rough;
     f3 F{F0.x + (gmax(omr, F0.x) - F0.x) * p5, F0.y + (gmax(omr, F0.y) - F0.y) * p5,
          F0.z + (gmax(omr, F0.z) - F0.z) * p5};
@@ -498,7 +551,10 @@ __device__ __forceinline__ RGBAf fsPbr(const DrawParams &P, const Frag &f) {
     ambient = scl3(mul3({u.ambient[0], u.ambient[1], u.ambient[2]}, albedo), in.ao);
   }
   f3 col = add3(ambient, Lo);
-  col = {rpowf(col.x, 1.0f / 2.2f), rpowf(col.y, 1.0f / 2.2f), rpowf(col.z, 1.0f / 2.2f)};
+  if (U8OUT && !(P.defines & RHS_DEF_EMISSIVE_MAP))
+    col = {gammaOutByte(P, col.x), gammaOutByte(P, col.y), gammaOutByte(P, col.z)};
+  else
+    col = {rpowf(col.x, 1.0f / 2.2f), rpowf(col.y, 1.0f / 2.2f), rpowf(col.z, 1.0f / 2.2f)};
   if (P.defines & RHS_DEF_EMISSIVE_MAP) {
     const TexDesc *te = P.tex[RHS_SLOT_EMISSIVE];
     RGBAf e = tex2D(te, {v[0], v[1]}, lodFor(te, f));
