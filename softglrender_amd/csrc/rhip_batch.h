@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void kb_hiz_coarse(const DrawParams *__restric
  * | tile) in queue order until the queue is empty, so every wave reaches the
  * exit once `*queue` passes `nitems`. */
 template <int S, int MODE, int TILE, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void kb_raster_vis(const DrawParams *__restrict__ Pv,
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4, 8))) void kb_raster_vis(const DrawParams *__restrict__ Pv,
                                                             const uint32_t *__restrict__ items, int nitems,
                                                             int *queue) {
   __shared__ int s_item;
@@ -136,7 +136,7 @@ static __global__ __launch_bounds__(1024) void kb_job_chunks(const DrawParams *_
 /* Shading of every view's queued fragments, one 64-fragment chunk per
  * wavefront step, chunks in view order. */
 template <int S, int SHADING>
-__global__ __launch_bounds__(256) void kb_shade(const DrawParams *__restrict__ Pv, const int *__restrict__ chunk_first,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void kb_shade(const DrawParams *__restrict__ Pv, const int *__restrict__ chunk_first,
                                                 int nv, int *chunks_done, unsigned long long *view_time) {
   const int lane = threadIdx.x & 63;
   const int total = chunk_first[nv];

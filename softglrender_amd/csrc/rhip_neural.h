@@ -412,7 +412,7 @@ __device__ __forceinline__ void nnJobRow(const DrawParams &P, int j, int need_de
  * wavefront's LDS tile, then the network runs on MFMA from that tile (weights
  * from L2) and each lane writes its fragment's colour to every sample it won. */
 template <int S>
-__global__ __launch_bounds__(256) void k_nn_shade(DrawParams P, const NnPacked *W, int need_deriv) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_nn_shade(DrawParams P, const NnPacked *W, int need_deriv) {
   __shared__ __bf16 hbuf[4][64 * kHs]; /* per wavefront: input rows, then hidden activations, then outputs */
   __shared__ unsigned long long s_fs, s_wr;
   if (threadIdx.x == 0) s_fs = s_wr = 0;
