@@ -1089,13 +1089,14 @@ __device__ __forceinline__ void hizCoarseBody(const DrawParams &P, int c) {
     }
   }
   __syncthreads();
-  const uint32_t bound = s_bound;
-  /* the nearest clipped polygon covering the whole bin: its tiles rasterize it first */
+  /* the nearest clipped polygon covering the whole bin: its tiles rasterize it
+   * first, and since its live fans leave a sample depth no farther than its
+   * depth range over the bin, that range bounds the bin too */
   uint32_t my_pz = MX ? 0u : 0xffffffffu;
   int my_pt = -1;
   for (int i = lp; i < n; i += blockDim.x) {
     const int id = i < cn ? P.bin_items[cs + i] : P.bin_items[gs + i - cn];
-    if (P.prims[id].src != id) continue;
+    if (P.prims[id].src != id || !(P.prims[id].flags & kPrimLive)) continue;
     float zmin, zmax;
     if (polyCoversRect<S>(P, id, x0, y0, x1, y1, zmin, zmax)) {
       const uint32_t k = MX ? depthKey(gclamp(zmin, 0.f, 1.f)) : depthKey(gclamp(zmax, 0.f, 1.f));
@@ -1118,6 +1119,8 @@ __device__ __forceinline__ void hizCoarseBody(const DrawParams &P, int c) {
   }
   __syncthreads();
   if (my_pt >= 0 && my_pz == s_pz) s_pt = my_pt;
+  __syncthreads();
+  const uint32_t bound = s_pt < 0 ? s_bound : (MX ? max(s_bound, s_pz) : min(s_bound, s_pz));
   for (int i = lp; i < n; i += blockDim.x) {
     const int id = i < cn ? P.bin_items[cs + i] : P.bin_items[gs + i - cn];
     const PrimSetup &ps = P.prims[id];
