@@ -1340,21 +1340,73 @@ __device__ void visTile(const DrawParams &P, int tile) {
   __syncthreads();
   const uint32_t bound2 = MX ? max(bound, s_bound2) : min(bound, s_bound2);
 
-  /* phase 1b: the tile's own triangles, one lane per triangle */
-  for (int i = lp; i < tb.n[0]; i += NT) {
-    const int id = P.bin_items[tb.s[0] + i];
-    const PrimSetup ps = P.prims[id];
-    if (zCulled(ps, bound2, MX)) {
-      nculled++;
-      continue;
+  /* phase 1b: the tile's own triangles. A wavefront takes 64 at a time, stages
+   * what visPixel needs of each in LDS and spreads their (triangle, pixel) pairs
+   * evenly over its lanes, 64 pairs per round, instead of each lane walking its
+   * own triangle's pixels (the keys' atomics make the order irrelevant). */
+  {
+    __shared__ float4 s_pr[WAVES][64][6];
+    __shared__ int s_start[WAVES][64];
+    __shared__ unsigned long long s_marks[WAVES];
+    const int wv = lp >> 6, lane = lp & 63;
+    for (int b0 = wv * 64; b0 < tb.n[0]; b0 += NT) {
+      int n = 0;
+      if (b0 + lane < tb.n[0]) {
+        const int id = P.bin_items[tb.s[0] + b0 + lane];
+        const PrimSetup ps = P.prims[id];
+        if (zCulled(ps, bound2, MX)) {
+          nculled++;
+        } else {
+          const int x0 = max(ps.qx0, tx0), x1 = min(ps.qx1, X1);
+          const int y0 = max(ps.qy0, ty0), y1 = min(ps.qy1, Y1);
+          if (x0 <= x1 && y0 <= y1) {
+            n = (x1 - x0 + 1) * (y1 - y0 + 1);
+            const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
+            float4 *d = s_pr[wv][lane];
+            d[0] = make_float4(ps.min_x, ps.min_y, ps.max_x, ps.max_y);
+            d[1] = make_float4(__int_as_float(ps.bcx), __int_as_float(ps.bcy), __int_as_float(id),
+                               __int_as_float(x0));
+            d[2] = make_float4(__int_as_float(y0), __int_as_float(x1 - x0 + 1), T.v0x, T.v0y);
+            d[3] = make_float4(T.a0, T.a1, T.b0, T.b1);
+            d[4] = make_float4(T.uz, T.z[0], T.z[1], T.z[2]);
+          }
+        }
+      }
+      const int inc = waveInclusiveScan(n);
+      const int off = inc - n, total = __builtin_amdgcn_readlane(inc, 63);
+      s_pr[wv][lane][5].x = __int_as_float(off);
+      int carry = 0;
+      for (int r0 = 0; r0 < total; r0 += 64) {
+        if (lane == 0) s_marks[wv] = 0ull;
+        waveSync();
+        if (n > 0 && off >= r0 && off < r0 + 64) {
+          s_start[wv][off - r0] = lane;
+          atomicOr(&s_marks[wv], 1ull << (off - r0));
+        }
+        waveSync();
+        /* the owner of pair r0 + lane: the last triangle starting at or before it */
+        const unsigned long long m = s_marks[wv] & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+        const int owner = m ? s_start[wv][63 - __clzll(m)] : carry;
+        carry = __builtin_amdgcn_readlane(owner, 63);
+        const int t = r0 + lane;
+        if (t < total) {
+          const float4 *d = s_pr[wv][owner];
+          const float4 d0 = d[0], d1 = d[1], d2 = d[2], d3 = d[3], d4 = d[4];
+          PrimSetup ps;
+          ps.min_x = d0.x, ps.min_y = d0.y, ps.max_x = d0.z, ps.max_y = d0.w;
+          ps.bcx = __float_as_int(d1.x), ps.bcy = __float_as_int(d1.y);
+          Tri T;
+          T.v0x = d2.z, T.v0y = d2.w, T.a0 = d3.x, T.a1 = d3.y, T.b0 = d3.z, T.b1 = d3.w;
+          T.uz = d4.x, T.z[0] = d4.y, T.z[1] = d4.z, T.z[2] = d4.w;
+          const int w = __float_as_int(d2.y), r = t - __float_as_int(d[5].x);
+          const int ry = r / w, rx = r - ry * w;
+          const int x = __float_as_int(d1.w) + rx, y = __float_as_int(d2.x) + ry;
+          visPixel<S, MODE>(ps, T, __float_as_int(d1.z), x, y, &keys[((y - ty0) * TILE + (x - tx0)) * S]);
+        }
+        waveSync(); /* marks and starts are read before the next round resets them */
+      }
+      waveSync(); /* staged triangles are read before the next batch overwrites them */
     }
-    const int x0 = max(ps.qx0, tx0), x1 = min(ps.qx1, X1);
-    const int y0 = max(ps.qy0, ty0), y1 = min(ps.qy1, Y1);
-    if (x0 > x1 || y0 > y1) continue;
-    const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
-    for (int y = y0; y <= y1; y++)
-      for (int x = x0; x <= x1; x++)
-        visPixel<S, MODE>(ps, T, id, x, y, &keys[((y - ty0) * TILE + (x - tx0)) * S]);
   }
   __syncthreads();
 
