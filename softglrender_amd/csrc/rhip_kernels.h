@@ -168,28 +168,6 @@ __device__ __forceinline__ void waveSync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-/* atomicAdd(&a[b], 1) for every lane with `active`, merged for lanes of the
- * wavefront that share b (up to four distinct b per call; the rest add alone).
- * Returns the lane's old value. Every lane of the wavefront must call it. */
-__device__ __forceinline__ int waveAtomicInc(int *a, int b, bool active) {
-  const int lane = threadIdx.x & 63;
-  unsigned long long pending = __ballot(active);
-  int res = 0;
-  for (int r = 0; r < 4 && pending; r++) {
-    const int leader = __ffsll(pending) - 1;
-    const int bl = __shfl(b, leader, 64);
-    const bool mine = ((pending >> lane) & 1ull) && b == bl;
-    const unsigned long long same = __ballot(mine);
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&a[bl], __popcll(same));
-    base = __shfl(base, leader, 64);
-    if (mine) res = base + __popcll(same & ((1ull << lane) - 1ull));
-    pending &= ~same;
-  }
-  if ((pending >> lane) & 1ull) res = atomicAdd(&a[b], 1);
-  return res;
-}
-
 /* Exclusive prefix sum of in[0..n) into out[0..n) by one block of 1024 threads,
  * 8192 entries at a time with coalesced loads and stores; also clears clear[0..n)
  * when given. Returns the total (to every thread). */
@@ -390,7 +368,10 @@ __device__ BinRect setupPrim(const DrawParams &P, int id, int i0, int i1, int i2
     }
   }
   s.flags = flags;
-  P.prims[id] = s;
+  if (flags & kPrimLive)
+    P.prims[id] = s;
+  else
+    P.prims[id].flags = flags; /* nothing reads more of a primitive that is not live */
   return r;
 }
 
@@ -399,24 +380,95 @@ __device__ __forceinline__ void binCount(const DrawParams &P, const BinRect &r) 
   if (r.cls >= 0) binVisit<false>(P, 0, r.cls, r.tx0, r.ty0, r.tx1, r.ty1);
 }
 
-/* Visits the bins of every lane's primitive; fine bins with wavefront-merged
- * atomics. Every lane of the wavefront must call it (r.cls < 0: nothing). */
-template <bool EMIT>
-__device__ __forceinline__ void binWave(const DrawParams &P, int id, const BinRect &r) {
-  if (r.cls > 0) binVisit<EMIT>(P, id, r.cls, r.tx0, r.ty0, r.tx1, r.ty1);
+/* Bins of a primitive's BinRect: how many, and the k-th (row-major). */
+__device__ __forceinline__ int binsOf(const DrawParams &P, const BinRect &r) {
+  if (r.cls < 0) return 0;
+  if (r.cls == kPrimGlobal) return 1;
+  if (r.cls == kPrimCoarse) {
+    const int c = P.coarse_tiles;
+    return (r.tx1 / c - r.tx0 / c + 1) * (r.ty1 / c - r.ty0 / c + 1);
+  }
+  return (r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1);
+}
+__device__ __forceinline__ int binAt(const DrawParams &P, const BinRect &r, int k) {
+  if (r.cls == kPrimGlobal) return P.nbins - 1;
+  if (r.cls == kPrimCoarse) {
+    const int c = P.coarse_tiles, cx0 = r.tx0 / c, w = r.tx1 / c - cx0 + 1;
+    return P.ntiles + (r.ty0 / c + k / w) * P.coarse_x + cx0 + k % w;
+  }
   const int w = r.tx1 - r.tx0 + 1;
-  const int nt = r.cls == 0 ? w * (r.ty1 - r.ty0 + 1) : 0;
-  const int most = (int)waveMaxU((uint32_t)nt);
-  for (int k = 0; k < most; k++) {
-    const bool act = k < nt;
-    const int b = act ? (r.ty0 + k / w) * P.tiles_x + r.tx0 + k % w : 0;
-    if (EMIT) {
-      const int slot = waveAtomicInc(P.bin_cursor, b, act);
-      if (act) P.bin_items[P.bin_start[b] + slot] = id;
-    } else {
-      waveAtomicInc(P.bin_count, b, act);
+  return (r.ty0 + k / w) * P.tiles_x + r.tx0 + k % w;
+}
+
+/* A workgroup's bins in LDS: open addressing, a bin's slot found within
+ * kBinProbe probes or not at all (slots only ever go from empty to a bin, so a
+ * bin one thread could not place no other thread can either). */
+constexpr int kBinHash = 512, kBinProbe = 16;
+struct BinHash {
+  int bin[kBinHash];
+  int cnt[kBinHash];
+  int base[kBinHash];
+};
+
+__device__ __forceinline__ int binSlot(BinHash &H, int b, bool insert) {
+  const int h = (int)(((uint32_t)b * 2654435761u) >> 23);
+  for (int p = 0; p < kBinProbe; p++) {
+    const int e = (h + p) & (kBinHash - 1);
+    const int cur = __hip_atomic_load(&H.bin[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (cur == b) return e;
+    if (cur == -1) {
+      if (!insert) return -1;
+      const int old = atomicCAS(&H.bin[e], -1, b);
+      if (old == -1 || old == b) return e;
     }
   }
+  return -1;
+}
+
+/* Counts (EMIT false) or lists (EMIT true) every thread's primitive in its bins
+ * with one global atomic per distinct bin of the workgroup: pairs are counted
+ * per bin in LDS, each bin reserves its range once, and the pairs fill it. A
+ * bin the table cannot hold takes a global atomic per pair. Every thread of
+ * the workgroup must call it (r.cls < 0: no bins). */
+template <bool EMIT>
+__device__ void binBlock(const DrawParams &P, int id, const BinRect &r) {
+  __shared__ BinHash H;
+  for (int i = threadIdx.x; i < kBinHash; i += blockDim.x) {
+    H.bin[i] = -1;
+    H.cnt[i] = 0;
+  }
+  __syncthreads();
+  const int nb = binsOf(P, r);
+  for (int k = 0; k < nb; k++) {
+    const int b = binAt(P, r, k);
+    const int e = binSlot(H, b, true);
+    if (e >= 0) {
+      atomicAdd(&H.cnt[e], 1);
+    } else if (EMIT) {
+      P.bin_items[P.bin_start[b] + atomicAdd(&P.bin_cursor[b], 1)] = id;
+    } else {
+      atomicAdd(&P.bin_count[b], 1);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kBinHash; i += blockDim.x) {
+    const int c = H.cnt[i];
+    if (!c) continue;
+    if (EMIT) {
+      H.base[i] = P.bin_start[H.bin[i]] + atomicAdd(&P.bin_cursor[H.bin[i]], c);
+      H.cnt[i] = 0; /* now the fill position within the reserved range */
+    } else {
+      atomicAdd(&P.bin_count[H.bin[i]], c);
+    }
+  }
+  if (EMIT) {
+    __syncthreads();
+    for (int k = 0; k < nb; k++) {
+      const int e = binSlot(H, binAt(P, r, k), false);
+      if (e >= 0) P.bin_items[H.base[e] + atomicAdd(&H.cnt[e], 1)] = id;
+    }
+  }
+  __syncthreads(); /* the table is free for the next call */
 }
 
 __device__ __forceinline__ bool validTri(const DrawParams &P, int i0, int i1, int i2) {
@@ -440,7 +492,7 @@ __device__ __forceinline__ void setupBody(const DrawParams &P, int blk) {
       r = setupPrim(P, t, i0, i1, i2);
     }
   }
-  binWave<false>(P, t, r);
+  binBlock<false>(P, t, r);
   const int n = __syncthreads_count(needs);
   if (threadIdx.x == 0) P.blk_clip[blk] = n;
 }
@@ -627,7 +679,7 @@ __device__ __forceinline__ void binEmitBody(const DrawParams &P, int blk, int nb
         r.cls = ps.flags & (kPrimCoarse | kPrimGlobal);
       }
     }
-    binWave<true>(P, id, r);
+    binBlock<true>(P, id, r);
   }
 }
 
