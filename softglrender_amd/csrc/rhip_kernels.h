@@ -1265,37 +1265,44 @@ __device__ void visTile(const DrawParams &P, int tile) {
    * its fans go first, and the depths they leave bound what follows */
   __shared__ uint32_t s_pz;
   __shared__ int s_pt;
-  if (lp == 0) {
-    s_pz = MX ? 0u : 0xffffffffu;
-    s_pt = -1;
-  }
-  __syncthreads();
-  uint32_t my_pz = MX ? 0u : 0xffffffffu;
-  int my_pt = -1;
+  /* A polygon covering the whole coarse bin already bounds its tiles through
+   * hiz_bound; rasterizing it first in every tile costs more than the tighter
+   * per-tile bound saves (4K stress frame: raster 1.11 -> 0.84 ms). Tiles of a
+   * bin without one, whose big list overflowed, look for their own. */
   const int cseed = P.hiz_seed[coarse];
-  for (int i = lp; i < (cseed < 0 && hc < 0 ? nbig : 0); i += NT) {
-    const int id = bigAt(i);
-    const PrimSetup &ps = P.prims[id];
-    if (ps.src != id) continue; /* each polygon once, through its first fan */
-    float zmin, zmax;
-    if (polyCoversRect<S>(P, id, tx0, ty0, X1, Y1, zmin, zmax)) {
-      const uint32_t k = MX ? depthKey(gclamp(zmin, 0.f, 1.f)) : depthKey(gclamp(zmax, 0.f, 1.f));
-      if (MX ? k > my_pz : k < my_pz) {
-        my_pz = k;
-        my_pt = id;
+  int seed = -1; /* block-uniform */
+  if (cseed < 0 && hc < 0 && nbig > 0) {
+    if (lp == 0) {
+      s_pz = MX ? 0u : 0xffffffffu;
+      s_pt = -1;
+    }
+    __syncthreads();
+    uint32_t my_pz = MX ? 0u : 0xffffffffu;
+    int my_pt = -1;
+    for (int i = lp; i < nbig; i += NT) {
+      const int id = bigAt(i);
+      const PrimSetup &ps = P.prims[id];
+      if (ps.src != id) continue; /* each polygon once, through its first fan */
+      float zmin, zmax;
+      if (polyCoversRect<S>(P, id, tx0, ty0, X1, Y1, zmin, zmax)) {
+        const uint32_t k = MX ? depthKey(gclamp(zmin, 0.f, 1.f)) : depthKey(gclamp(zmax, 0.f, 1.f));
+        if (MX ? k > my_pz : k < my_pz) {
+          my_pz = k;
+          my_pt = id;
+        }
       }
     }
+    if (my_pt >= 0) {
+      if (MX)
+        atomicMax(&s_pz, my_pz);
+      else
+        atomicMin(&s_pz, my_pz);
+    }
+    __syncthreads();
+    if (my_pt >= 0 && my_pz == s_pz) s_pt = my_pt; /* any one of equals */
+    __syncthreads();
+    seed = s_pt;
   }
-  if (my_pt >= 0) {
-    if (MX)
-      atomicMax(&s_pz, my_pz);
-    else
-      atomicMin(&s_pz, my_pz);
-  }
-  __syncthreads();
-  if (my_pt >= 0 && my_pz == s_pz) s_pt = my_pt; /* any one of equals */
-  __syncthreads();
-  const int seed = cseed >= 0 ? cseed : s_pt;
   if (seed >= 0) {
     const int nf = P.clip_poly[(size_t)seed * (kMaxPoly + 1) + kMaxPoly] - 2;
     for (int k = 0; k < nf; k++) {
