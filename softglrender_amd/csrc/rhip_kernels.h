@@ -1637,6 +1637,7 @@ static __global__ __launch_bounds__(256) void k_resolve(DrawParams P) { resolveB
  * writes the queue is drained after every triangle. */
 constexpr int kOrdWin = 2048;
 constexpr int kOrdJobsW = 256; /* queue entries per wavefront (64 lanes x 4 quads at most per triangle) */
+constexpr int kOrdStage = 256; /* triangles of a window whose walk data is staged in LDS */
 
 template <int S, int SHADING>
 __global__ __launch_bounds__(256) void k_raster_ord(DrawParams P, int need_deriv, int flush_each) {
@@ -1646,6 +1647,7 @@ __global__ __launch_bounds__(256) void k_raster_ord(DrawParams P, int need_deriv
   __shared__ short job_next[4 * kOrdJobsW];
   __shared__ float4 job_col[4 * kOrdJobsW];
   __shared__ float job_z[4 * kOrdJobsW * S];
+  __shared__ float4 s_stage[kOrdStage][5]; /* quad range, box, block counts and edge/depth setup per triangle */
   __shared__ uint32_t s_bound;
   __shared__ int s_n, s_cnt;
   const int tile = blockIdx.x;
@@ -1742,6 +1744,20 @@ __global__ __launch_bounds__(256) void k_raster_ord(DrawParams P, int need_deriv
     for (int i = m + lp; i < mp; i += 256) win[i] = 0xffffffffu;
     __syncthreads();
     blockSortU32(win, mp);
+    /* what the walk reads of the window's first triangles, staged once per window
+     * instead of two dependent global loads per triangle and wavefront */
+    const int ms = min(m, kOrdStage);
+    for (int i = lp; i < ms; i += 256) {
+      const PrimSetup ps = P.prims[win[i]];
+      const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
+      s_stage[i][0] = make_float4(__int_as_float(ps.qx0), __int_as_float(ps.qx1), __int_as_float(ps.qy0),
+                                  __int_as_float(ps.qy1));
+      s_stage[i][1] = make_float4(ps.min_x, ps.max_x, ps.min_y, ps.max_y);
+      s_stage[i][2] = make_float4(__int_as_float(ps.bcx), __int_as_float(ps.bcy), T.v0x, T.v0y);
+      s_stage[i][3] = make_float4(T.a0, T.a1, T.b0, T.b1);
+      s_stage[i][4] = make_float4(T.uz, T.z[0], T.z[1], T.z[2]);
+    }
+    __syncthreads();
 
     /* walk the window in order; the queue is drained at one place */
     int k = 0, tot = 0, nI = 0, pend_id = 0;
@@ -1825,7 +1841,21 @@ __global__ __launch_bounds__(256) void k_raster_ord(DrawParams P, int need_deriv
       }
       if (k == m) break;
       const int id = (int)win[k];
-      const PrimSetup ps = P.prims[id];
+      PrimSetup ps;
+      Tri T;
+      if (k < ms) { /* wave-uniform k: broadcast LDS reads */
+        const float4 d0 = s_stage[k][0], d1 = s_stage[k][1], d2 = s_stage[k][2], d3 = s_stage[k][3],
+                     d4 = s_stage[k][4];
+        ps.qx0 = __float_as_int(d0.x), ps.qx1 = __float_as_int(d0.y);
+        ps.qy0 = __float_as_int(d0.z), ps.qy1 = __float_as_int(d0.w);
+        ps.min_x = d1.x, ps.max_x = d1.y, ps.min_y = d1.z, ps.max_y = d1.w;
+        ps.bcx = __float_as_int(d2.x), ps.bcy = __float_as_int(d2.y);
+        T.v0x = d2.z, T.v0y = d2.w, T.a0 = d3.x, T.a1 = d3.y, T.b0 = d3.z, T.b1 = d3.w;
+        T.uz = d4.x, T.z[0] = d4.y, T.z[1] = d4.z, T.z[2] = d4.w;
+      } else {
+        ps = P.prims[id];
+        T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
+      }
       if (ps.qx1 < wx0 || ps.qx0 > wx0 + kSub - 1 || ps.qy1 < wy0 || ps.qy0 > wy0 + kSub - 1) {
         k++;
         continue;
@@ -1837,7 +1867,6 @@ __global__ __launch_bounds__(256) void k_raster_ord(DrawParams P, int need_deriv
         const int nqx = axisQuads(ps.min_x, ps.max_x, ps.bcx, px, qa, qb);
         const int nqy = nqx ? axisQuads(ps.min_y, ps.max_y, ps.bcy, py, qa, qb) : 0;
         if (nqx && nqy) {
-          const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
           const float fx = (float)px, fy = (float)py;
           /* coverage, z and the depth clip (RendererSoft.cpp:776-799) */
           bool sin[5];
