@@ -236,6 +236,57 @@ void launchShade(const DrawParams &P, int grid, int need_deriv, hipStream_t st) 
   }
 }
 
+/* Shading of the listed ordered raster's fragments, and its fallback for tiles that did not fit. */
+template <int S>
+void launchOrdShade(const DrawParams &P, int grid, int need_deriv, hipStream_t st) {
+  switch (P.shading) {
+    case RHS_SHADING_BASIC: RHIP_LAUNCH(k_ord_shade<S RHIP_COMMA RHS_SHADING_BASIC>, grid, 256, st, P, need_deriv); break;
+    case RHS_SHADING_BLINN_PHONG:
+      RHIP_LAUNCH(k_ord_shade<S RHIP_COMMA RHS_SHADING_BLINN_PHONG>, grid, 256, st, P, need_deriv);
+      break;
+    case RHS_SHADING_BLINN_PHONG_DIR:
+      RHIP_LAUNCH(k_ord_shade<S RHIP_COMMA RHS_SHADING_BLINN_PHONG_DIR>, grid, 256, st, P, need_deriv);
+      break;
+    case RHS_SHADING_PBR: RHIP_LAUNCH(k_ord_shade<S RHIP_COMMA RHS_SHADING_PBR>, grid, 256, st, P, need_deriv); break;
+    default: RHIP_LAUNCH(k_ord_shade<S RHIP_COMMA RHS_SHADING_VERTEX_COLOR>, grid, 256, st, P, need_deriv); break;
+  }
+}
+
+template <int S>
+void launchOrdRest(const DrawParams &P, int grid, int need_deriv, hipStream_t st) {
+  switch (P.shading) {
+    case RHS_SHADING_BASIC: RHIP_LAUNCH(k_ord_rest<S RHIP_COMMA RHS_SHADING_BASIC>, grid, 256, st, P, need_deriv); break;
+    case RHS_SHADING_BLINN_PHONG:
+      RHIP_LAUNCH(k_ord_rest<S RHIP_COMMA RHS_SHADING_BLINN_PHONG>, grid, 256, st, P, need_deriv);
+      break;
+    case RHS_SHADING_BLINN_PHONG_DIR:
+      RHIP_LAUNCH(k_ord_rest<S RHIP_COMMA RHS_SHADING_BLINN_PHONG_DIR>, grid, 256, st, P, need_deriv);
+      break;
+    case RHS_SHADING_PBR: RHIP_LAUNCH(k_ord_rest<S RHIP_COMMA RHS_SHADING_PBR>, grid, 256, st, P, need_deriv); break;
+    default: RHIP_LAUNCH(k_ord_rest<S RHIP_COMMA RHS_SHADING_VERTEX_COLOR>, grid, 256, st, P, need_deriv); break;
+  }
+}
+
+/* Listed ordered raster of a draw that writes no depth. */
+template <int S>
+void launchListed(rhip_device d, const DrawParams &P, int need_deriv, hipStream_t st) {
+  const size_t npx = (size_t)P.fb_w * P.fb_h;
+  {
+    Timer tm(d, S == 4 ? "k_ord_list_msaa4" : "k_ord_list");
+    RHIP_LAUNCH(k_ord_list<S>, std::min(P.ntiles, 768), 256, st, P);
+  }
+  if (P.color) {
+    {
+      Timer tm(d, S == 4 ? "k_ord_shade_msaa4" : "k_ord_shade");
+      launchOrdShade<S>(P, (int)std::min<size_t>(blocks(npx, 256), 4096), need_deriv, st);
+    }
+    Timer tm(d, S == 4 ? "k_ord_apply_msaa4" : "k_ord_apply");
+    RHIP_LAUNCH(k_ord_apply<S>, (unsigned)std::min<size_t>(blocks(npx, 256), 4096), 256, st, P);
+  }
+  Timer tm(d, S == 4 ? "k_ord_rest_msaa4" : "k_ord_rest");
+  launchOrdRest<S>(P, std::min(P.ntiles, 512), need_deriv, st);
+}
+
 /* Ordered raster (every state the two-phase raster does not take). */
 template <int S>
 void launchOrdered(const DrawParams &P, int ntiles, int need_deriv, int flush_each, hipStream_t st) {
@@ -305,6 +356,8 @@ int drawSetup(rhip_device d, int fbo, const int vp[4], const rhip_draw_desc &dd,
   if (vp[2] <= 0 || vp[3] <= 0) return fail(RHIP_E_INVALID, "viewport not set");
   sh.two_phase = !st.blend && dt && st.depthTest && st.depthMask &&
                  (st.depthFunc == 1 || st.depthFunc == 3 || st.depthFunc == 4 || st.depthFunc == 6);
+  /* a draw that writes no depth: every fragment's tests see the depths the draw started with */
+  sh.listed = !sh.two_phase && !(dt && st.depthTest && st.depthMask);
 
   if (pg.shading == RHS_SHADING_NEURAL_PBR) {
     uint32_t enable_light = 0;
@@ -420,6 +473,16 @@ size_t carveArenas(DrawParams &P, const DrawShape &sh, char *base) {
     P.hiz_seed = P.hiz_count ? P.hiz_count + ncoarse : nullptr;
     P.hiz_surv = P.hiz_seed ? P.hiz_seed + ncoarse : nullptr;
   }
+  if (sh.listed) {
+    const size_t px = (size_t)P.fb_w * P.fb_h;
+    const size_t cap = std::min<size_t>(2 * px + 65536, (size_t)1 << 30);
+    P.ord_cap = (int)cap;
+    P.jobs = (uint2 *)take(cap * 8);
+    P.ord_col = (float4 *)take(cap * 16);
+    P.ord_span = (int2 *)take(px * 8);
+    P.ord_state = (int *)take((size_t)P.ntiles * 4);
+    P.ord_list = (int *)take((size_t)P.ntiles * 4);
+  }
   return off;
 }
 
@@ -438,8 +501,8 @@ int rhip_device_create(int hip_device, rhip_device *out) {
   std::unique_ptr<rhip_device_s> d(new rhip_device_s());
   d->hip_device = hip_device;
   HIPCHK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
-  HIPCHK(hipMalloc(&d->counters, 8 * sizeof(unsigned long long)));
-  HIPCHK(hipMemset(d->counters, 0, 8 * sizeof(unsigned long long)));
+  HIPCHK(hipMalloc(&d->counters, kCounterSlots * kCounterStride * sizeof(unsigned long long)));
+  HIPCHK(hipMemset(d->counters, 0, kCounterSlots * kCounterStride * sizeof(unsigned long long)));
   HIPCHK(hipMalloc(&d->gamma_lut, 512 * sizeof(float)));
   RHIP_LAUNCH(k_gamma_tables, 1, 256, d->stream, d->gamma_lut);
   HIPCHK(hipGetLastError());
@@ -770,8 +833,7 @@ int rhip_draw(rhip_device d, const rhip_draw_desc *dd) {
   const size_t V = sh.V;
   hipStream_t s = d->stream;
 
-  RHIP_LAUNCH(k_fill_u32, blocks(P.nbins, 256), 256, s, (uint32_t *)P.bin_count, (size_t)P.nbins, 0u);
-  { Timer tm(d, "k_vertex"); RHIP_LAUNCH(k_vertex, blocks(V, 256), 256, s, P); }
+  { Timer tm(d, "k_vertex"); RHIP_LAUNCH(k_vertex, std::max(blocks(V, 256), 1u), 256, s, P); }
   { Timer tm(d, "k_setup"); RHIP_LAUNCH(k_setup, nblk, 256, s, P); }
   { Timer tm(d, "k_clip"); RHIP_LAUNCH(k_clip, nblk, 256, s, P); }
   RHIP_LAUNCH(k_scan_fans, 1, 1024, s, P);
@@ -820,6 +882,11 @@ int rhip_draw(rhip_device d, const rhip_draw_desc *dd) {
         RHIP_LAUNCH(k_resolve, (unsigned)std::min<size_t>(blocks((size_t)fbw * fbh, 256), 4096), 256, s, P);
       }
     }
+  } else if (sh.listed) {
+    if (samples == 4)
+      launchListed<4>(d, P, need_deriv ? 1 : 0, s);
+    else
+      launchListed<1>(d, P, need_deriv ? 1 : 0, s);
   } else {
     /* NOTEQUAL with depth writes: early-Z must see every earlier write */
     const int flush_each = (P.depth && st.depthTest && st.depthMask && st.depthFunc == 5) ? 1 : 0;
@@ -1099,9 +1166,11 @@ int rhip_stats_get(rhip_device d, rhip_stats *out) {
   int rc = checkDev(d);
   if (rc) return rc;
   if (!out) return fail(RHIP_E_INVALID, "null output");
-  unsigned long long c[8];
-  HIPCHK(hipMemcpyAsync(c, d->counters, sizeof c, hipMemcpyDeviceToHost, d->stream));
+  unsigned long long all[kCounterSlots * kCounterStride], c[8] = {};
+  HIPCHK(hipMemcpyAsync(all, d->counters, sizeof all, hipMemcpyDeviceToHost, d->stream));
   HIPCHK(hipStreamSynchronize(d->stream));
+  for (int k = 0; k < kCounterSlots; k++)
+    for (int i = 0; i < 8; i++) c[i] += all[k * kCounterStride + i];
   *out = d->host_stats;
   out->fs_invocations = (long long)c[0];
   out->samples_written = (long long)c[1];
@@ -1115,7 +1184,7 @@ int rhip_stats_get(rhip_device d, rhip_stats *out) {
 int rhip_stats_reset(rhip_device d) {
   int rc = checkDev(d);
   if (rc) return rc;
-  HIPCHK(hipMemsetAsync(d->counters, 0, 8 * sizeof(unsigned long long), d->stream));
+  HIPCHK(hipMemsetAsync(d->counters, 0, kCounterSlots * kCounterStride * sizeof(unsigned long long), d->stream));
   d->host_stats = rhip_stats();
   return RHIP_OK;
 }

@@ -152,6 +152,7 @@ struct DrawShape {
   int stride = 0;      /* vertex bytes */
   bool empty = false;  /* no attachment: nothing to do */
   bool two_phase = false; /* visibility first, then shading (opaque draws that write depth) */
+  bool listed = false;    /* ordered draws that write no depth: list, shade, then blend in order */
 };
 
 /* Validates a draw into framebuffer `fbo` through viewport vp = (x, y, w, h)

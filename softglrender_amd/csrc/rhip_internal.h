@@ -14,6 +14,10 @@ constexpr int kMaxLevels = 16;
 constexpr int kMaxLayers = 6;
 constexpr int kTile = 16;         /* default screen tile edge in pixels (one workgroup) */
 constexpr int kSub = 8;           /* 8x8 sub-tile per wavefront */
+/* statistics counters: kCounterSlots copies of the 8 counters, kCounterStride
+ * words (128 bytes) apart; a workgroup adds to copy blockIdx % kCounterSlots
+ * and the host sums the copies */
+constexpr int kCounterSlots = 64, kCounterStride = 16;
 constexpr int kVaryStride = 32;   /* varying slots of a vertex shader output (the reference layout) */
 constexpr int kMaxClipVerts = 12; /* new vertices one clipped triangle can create */
 constexpr int kMaxPoly = 9;       /* polygon vertices after six planes */
@@ -140,7 +144,8 @@ struct DrawParams {
   int *blk_fans;  /* per block: fan triangles its clipped triangles append */
   int *blk_fan_off;
   int nblk;
-  int *dcount; /* device scalars: [0] appended fan triangles, [1] primitive count, [2] queued fragments */
+  int *dcount; /* device scalars: [0] appended fan triangles, [1] primitive count, [2] queued fragments,
+                [3] listed fragments, [7] tiles left to the ordered raster */
   /* primitives */
   PrimSetup *prims;
   /* bins: ntiles fine bins of tile x tile pixels, then coarse_x * coarse_y coarse bins of
@@ -152,11 +157,18 @@ struct DrawParams {
   int *bin_cursor; /* [nbins] */
   int *bin_items;  /* primitive ids, unordered within a bin */
   /* two-phase raster */
-  uint2 *jobs;          /* (pixel | sample mask << 28, winner) per visible fragment; count in dcount[2] */
+  uint2 *jobs;          /* (pixel | sample mask << 28, winner) per visible fragment; count in dcount[2]
+                           (the listed ordered raster: dcount[3]) */
   uint32_t *hiz_bound; /* per coarse bin: depth bound from big triangles covering it */
   int *hiz_count;      /* per coarse bin: big triangles that bound leaves, -1 if over kHizCap */
   int *hiz_surv;       /* per coarse bin: those triangles (kHizCap slots) */
   int *hiz_seed;       /* per coarse bin: nearest clipped polygon covering it, or -1 */
+  /* listed ordered raster (draws that write no depth): fragments in jobs[0, dcount[3]) */
+  float4 *ord_col;  /* shaded colour per listed fragment */
+  int2 *ord_span;   /* per pixel of a listed tile: its fragments jobs[x, x + y), in API order */
+  int *ord_state;   /* per tile: kOrdEmpty, kOrdListed or kOrdOverflow */
+  int *ord_list;    /* tiles whose fragments did not fit, dcount[7] of them */
+  int ord_cap;      /* jobs slots */
   unsigned long long *counters; /* [0] fs invocations, [1] samples written, [2] clip overflows,
                                    [3] bin items, [4] primitives binned, [5] clipped triangles,
                                    [6] (tile, triangle) pairs skipped by the hierarchical-Z test */

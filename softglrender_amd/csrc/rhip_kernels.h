@@ -118,7 +118,11 @@ __device__ __forceinline__ void vertexBody(const DrawParams &P, int blk) {
   storeVaryings(P, i, v);
 }
 
-static __global__ __launch_bounds__(256) void k_vertex(DrawParams P) { vertexBody(P, blockIdx.x); }
+/* The vertex stage of one draw; it also clears the draw's bin counts, which k_setup fills next. */
+static __global__ __launch_bounds__(256) void k_vertex(DrawParams P) {
+  vertexBody(P, blockIdx.x);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.nbins; i += gridDim.x * blockDim.x) P.bin_count[i] = 0;
+}
 
 /* --------------------------------------------------------------------------- block helpers */
 
@@ -159,6 +163,20 @@ __device__ __forceinline__ uint32_t waveMinU(uint32_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, 64));
   return v;
+}
+
+/* This workgroup's copy of statistics counter i. */
+__device__ __forceinline__ unsigned long long *counterAt(const DrawParams &P, int i) {
+  return P.counters + (blockIdx.x & (kCounterSlots - 1)) * kCounterStride + i;
+}
+
+/* Adds the lanes' v (each below 2^26) to counter i, one atomic per wavefront;
+ * every lane of the wavefront calls it. */
+__device__ __forceinline__ void waveAddCounter(const DrawParams &P, int i, unsigned long long v) {
+  uint32_t w = (uint32_t)v;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) w += (uint32_t)__shfl_xor((int)w, o, 64);
+  if ((threadIdx.x & 63) == 0 && w) atomicAdd(counterAt(P, i), (unsigned long long)w);
 }
 
 /* Orders LDS accesses of the lanes of one wavefront around this point. */
@@ -523,7 +541,7 @@ __device__ int clipTriangle(const DrawParams &P, int t) {
     poly[kMaxPoly] = 0;
     return 0;
   }
-  atomicAdd(&P.counters[5], 1ull);
+  atomicAdd(counterAt(P, 5), 1ull);
   const int k = t;
   int in[kMaxPoly + 1], out[kMaxPoly + 1];
   int nin = 3, nnew = 0;
@@ -585,7 +603,7 @@ __device__ int clipTriangle(const DrawParams &P, int t) {
     if (full) {
       /* more vertices than a convex polygon can have: rounding produced extra
        * plane crossings; the triangle is dropped and the overflow counted */
-      atomicAdd(&P.counters[2], 1ull);
+      atomicAdd(counterAt(P, 2), 1ull);
       break;
     }
     for (int q = 0; q < nout; q++) in[q] = out[q];
@@ -627,6 +645,8 @@ __device__ __forceinline__ void scanFansBody(const DrawParams &P) {
     P.dcount[0] = total;
     P.dcount[1] = P.ntris + total;
     P.dcount[2] = 0;
+    P.dcount[3] = 0;
+    P.dcount[7] = 0;
   }
 }
 
@@ -659,8 +679,8 @@ __device__ __forceinline__ void binScanBody(const DrawParams &P) {
   const int total = blockScanArray(P.bin_count, P.bin_start, P.nbins, P.bin_cursor);
   if (threadIdx.x == 0) {
     P.bin_start[P.nbins] = total;
-    atomicAdd(&P.counters[3], (unsigned long long)total);
-    atomicAdd(&P.counters[4], (unsigned long long)P.dcount[1]);
+    atomicAdd(counterAt(P, 3), (unsigned long long)total);
+    atomicAdd(counterAt(P, 4), (unsigned long long)P.dcount[1]);
   }
 }
 
@@ -1534,7 +1554,7 @@ __device__ void visTile(const DrawParams &P, int tile) {
 #pragma unroll
     for (int s = 0; s < S; s++) P.depth[pix * S + s] = dep[q][s];
   }
-  if (nculled) atomicAdd(&P.counters[6], nculled);
+  waveAddCounter(P, 6, nculled);
 }
 
 /* Four wavefronts per SIMD (at most 128 registers, a few spilled) hide the
@@ -1595,8 +1615,8 @@ __device__ __forceinline__ void addShadeCounts(const DrawParams &P, unsigned lon
   if (nwr) atomicAdd(&s_wr, nwr);
   __syncthreads();
   if (threadIdx.x == 0) {
-    if (s_fs) atomicAdd(&P.counters[0], s_fs);
-    if (s_wr) atomicAdd(&P.counters[1], s_wr);
+    if (s_fs) atomicAdd(counterAt(P, 0), s_fs);
+    if (s_wr) atomicAdd(counterAt(P, 1), s_wr);
   }
 }
 
@@ -1640,7 +1660,7 @@ constexpr int kOrdJobsW = 256; /* queue entries per wavefront (64 lanes x 4 quad
 constexpr int kOrdStage = 256; /* triangles of a window whose walk data is staged in LDS */
 
 template <int S, int SHADING>
-__global__ __launch_bounds__(256) void k_raster_ord(DrawParams P, int need_deriv, int flush_each) {
+__device__ __forceinline__ void ordTile(const DrawParams &P, int tile, int need_deriv, int flush_each) {
   __shared__ uint32_t win[kOrdWin];
   __shared__ uint32_t job_id[4 * kOrdJobsW];
   __shared__ uint32_t job_meta[4 * kOrdJobsW];
@@ -1650,7 +1670,6 @@ __global__ __launch_bounds__(256) void k_raster_ord(DrawParams P, int need_deriv
   __shared__ float4 s_stage[kOrdStage][5]; /* quad range, box, block counts and edge/depth setup per triangle */
   __shared__ uint32_t s_bound;
   __shared__ int s_n, s_cnt;
-  const int tile = blockIdx.x;
   const int tx0 = (tile % P.tiles_x) * kTile, ty0 = (tile / P.tiles_x) * kTile;
   const int X1 = min(tx0 + kTile, P.fb_w) - 1, Y1 = min(ty0 + kTile, P.fb_h) - 1;
   const int lp = threadIdx.x, wave = lp >> 6, lane = lp & 63;
@@ -1954,9 +1973,441 @@ __global__ __launch_bounds__(256) void k_raster_ord(DrawParams P, int need_deriv
       if (S == 4) P.color_resolved[pix] = resolve4(col);
     }
   }
-  if (nfs) atomicAdd(&P.counters[0], nfs);
-  if (nwr) atomicAdd(&P.counters[1], nwr);
-  if (nculled) atomicAdd(&P.counters[6], nculled);
+  waveAddCounter(P, 0, nfs);
+  waveAddCounter(P, 1, nwr);
+  waveAddCounter(P, 6, nculled);
+}
+
+/* The ordered raster of every tile, one workgroup per tile. */
+template <int S, int SHADING>
+__global__ __launch_bounds__(256) void k_raster_ord(DrawParams P, int need_deriv, int flush_each) {
+  ordTile<S, SHADING>(P, blockIdx.x, need_deriv, flush_each);
+}
+
+/* --------------------------------------------------------------------------- listed ordered raster
+ *
+ * For a draw that writes no depth, every fragment's depth test sees the depths
+ * the draw started with, so which samples a fragment reaches does not depend on
+ * the others; only the per-sample writes (RendererSoft.cpp:358-407) depend on
+ * their order. k_ord_list walks a tile's triangles in primitive-id order and
+ * lists each fragment that covers a sample and passes the depth clip and the
+ * depth test, pixel by pixel in API order; k_ord_shade runs the fragment shader
+ * of every listed fragment at once over the whole GPU; k_ord_apply makes each
+ * pixel's writes in order. A tile whose fragments do not fit is marked, and
+ * k_ord_rest gives it the ordered raster instead. */
+constexpr int kOrdEmpty = 0, kOrdListed = 1, kOrdOverflow = 2;
+constexpr int kListCap = 512;              /* fragments one wavefront's 8x8 quarter can list */
+constexpr uint32_t kListVoid = 0xffffffffu; /* a jobs slot reserved by a tile that overflowed */
+
+constexpr int kListStage = 128; /* triangles of a window whose walk data is staged in LDS */
+
+/* One tile by one workgroup, one wavefront per 8x8 quarter. A wavefront takes
+ * the window's triangles 64 at a time and spreads their (triangle, pixel) pairs
+ * over its lanes, 64 pairs per round; fragments are listed in round and lane
+ * order, which is primitive-id order for every pixel. */
+template <int S>
+__device__ __forceinline__ void listTile(const DrawParams &P, int tile) {
+  __shared__ uint32_t win[kOrdWin];
+  __shared__ float4 s_stage[kListStage][5];
+  __shared__ uint2 s_frag[4][kListCap];
+  __shared__ float s_dep[kTile * kTile * S];
+  __shared__ int s_cnt[kTile * kTile];
+  __shared__ int4 s_run[4][64]; /* per triangle of a wavefront's batch: first pair, x0, y0, width */
+  __shared__ int s_start[4][64];
+  __shared__ unsigned long long s_marks[4];
+  __shared__ uint32_t s_bound;
+  __shared__ int s_n, s_c, s_over, s_base, s_wn[4];
+  const int tx0 = (tile % P.tiles_x) * kTile, ty0 = (tile / P.tiles_x) * kTile;
+  const int X1 = min(tx0 + kTile, P.fb_w) - 1, Y1 = min(ty0 + kTile, P.fb_h) - 1;
+  const int lp = threadIdx.x, wave = lp >> 6, lane = lp & 63;
+  const int wx0 = tx0 + (wave & 1) * kSub, wy0 = ty0 + (wave >> 1) * kSub;
+  const int wx1 = min(wx0 + kSub - 1, X1), wy1 = min(wy0 + kSub - 1, Y1);
+  const int px = wx0 + (lane & 7), py = wy0 + (lane >> 3);
+  const int pl = (py - ty0) * kTile + (px - tx0);
+  const bool inFb = px < P.fb_w && py < P.fb_h;
+  const size_t pix = (size_t)py * P.fb_w + px;
+  const bool hasD = P.depth != nullptr;
+  const TileBins tb = tileBins(P, tile);
+  const int n = tb.total();
+  if (n == 0) {
+    if (lp == 0) P.ord_state[tile] = kOrdEmpty;
+    return;
+  }
+  const float lx[4] = {0.375f, 0.875f, 0.125f, 0.625f};
+  const float ly[4] = {0.875f, 0.625f, 0.375f, 0.125f};
+  float dep[S];
+#pragma unroll
+  for (int s = 0; s < S; s++) {
+    dep[s] = (inFb && hasD) ? P.depth[pix * S + s] : 0.f;
+    s_dep[pl * S + s] = dep[s];
+  }
+  s_cnt[pl] = 0;
+  const int dfunc = P.st.depthFunc;
+  const bool dtest = P.st.depthTest && hasD;
+  const bool hiz = dtest && (dfunc == 1 || dfunc == 3 || dfunc == 4 || dfunc == 6);
+  const bool mx = dfunc == 4 || dfunc == 6;
+  if (lp == 0) {
+    s_bound = mx ? 0xffffffffu : 0u;
+    s_n = 0;
+    s_over = 0;
+  }
+  __syncthreads();
+  if (hiz && inFb) hizFromDepth<S>(dep, mx, &s_bound);
+  __syncthreads();
+  const uint32_t bound = s_bound;
+
+  const int maxid = P.dcount[1] - 1;
+  unsigned long long nculled = 0;
+  int wn = 0;        /* fragments this wavefront has listed */
+  bool full = false; /* its list overflowed (wave-uniform) */
+  int lo = 0;
+  while (lo <= maxid) {
+    int hi = maxid;
+    if (n > kOrdWin) {
+      /* the largest hi with at most kOrdWin entries in [lo, hi] */
+      int L = lo, R = maxid;
+      while (L < R) {
+        const int mid = L + (R - L + 1) / 2;
+        if (lp == 0) s_c = 0;
+        __syncthreads();
+        int cnt = 0;
+        for (int i = lp; i < n; i += 256) {
+          const int id = tb.at(P, i);
+          cnt += (id >= lo && id <= mid) ? 1 : 0;
+        }
+        if (cnt) atomicAdd(&s_c, cnt);
+        __syncthreads();
+        const int c = s_c;
+        __syncthreads();
+        if (c <= kOrdWin)
+          L = mid;
+        else
+          R = mid - 1;
+      }
+      hi = L;
+    }
+    for (int i = lp; i < n; i += 256) {
+      const int id = tb.at(P, i);
+      if (id < lo || id > hi) continue;
+      const PrimSetup &ps = P.prims[id];
+      if (i >= tb.n[0] && (ps.qx0 > X1 || ps.qx1 < tx0 || ps.qy0 > Y1 || ps.qy1 < ty0)) continue;
+      if (hiz && zCulled(ps, bound, mx)) {
+        nculled++;
+        continue;
+      }
+      win[atomicAdd(&s_n, 1)] = (uint32_t)id;
+    }
+    __syncthreads();
+    const int m = s_n;
+    int mp = 1;
+    while (mp < m) mp <<= 1;
+    for (int i = m + lp; i < mp; i += 256) win[i] = 0xffffffffu;
+    __syncthreads();
+    blockSortU32(win, mp);
+    const int ms = min(m, kListStage);
+    for (int i = lp; i < ms; i += 256) {
+      const PrimSetup ps = P.prims[win[i]];
+      const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
+      s_stage[i][0] = make_float4(__int_as_float(ps.qx0), __int_as_float(ps.qx1), __int_as_float(ps.qy0),
+                                  __int_as_float(ps.qy1));
+      s_stage[i][1] = make_float4(ps.min_x, ps.max_x, ps.min_y, ps.max_y);
+      s_stage[i][2] = make_float4(__int_as_float(ps.bcx), __int_as_float(ps.bcy), T.v0x, T.v0y);
+      s_stage[i][3] = make_float4(T.a0, T.a1, T.b0, T.b1);
+      s_stage[i][4] = make_float4(T.uz, T.z[0], T.z[1], T.z[2]);
+    }
+    __syncthreads();
+
+    for (int c0 = 0; c0 < m && !full; c0 += 64) {
+      /* this lane's triangle of the batch: its pixels in the quarter */
+      const int k = c0 + lane;
+      int np = 0, x0 = 0, y0 = 0, w = 1;
+      if (k < m) {
+        int qx0, qx1, qy0, qy1;
+        if (k < ms) {
+          const float4 d0 = s_stage[k][0];
+          qx0 = __float_as_int(d0.x), qx1 = __float_as_int(d0.y), qy0 = __float_as_int(d0.z), qy1 = __float_as_int(d0.w);
+        } else {
+          const PrimSetup &ps = P.prims[win[k]];
+          qx0 = ps.qx0, qx1 = ps.qx1, qy0 = ps.qy0, qy1 = ps.qy1;
+        }
+        x0 = max(qx0, wx0);
+        y0 = max(qy0, wy0);
+        const int x1 = min(qx1, wx1), y1 = min(qy1, wy1);
+        if (x0 <= x1 && y0 <= y1) {
+          w = x1 - x0 + 1;
+          np = w * (y1 - y0 + 1);
+        }
+      }
+      const int inc = waveInclusiveScan(np);
+      const int off = inc - np, total = __builtin_amdgcn_readlane(inc, 63);
+      s_run[wave][lane] = make_int4(off, x0, y0, w);
+      int carry = 0;
+      for (int r0 = 0; r0 < total; r0 += 64) {
+        if (lane == 0) s_marks[wave] = 0ull;
+        waveSync();
+        if (np > 0 && off >= r0 && off < r0 + 64) {
+          s_start[wave][off - r0] = lane;
+          atomicOr(&s_marks[wave], 1ull << (off - r0));
+        }
+        waveSync();
+        /* the owner of pair r0 + lane: the last triangle starting at or before it */
+        const unsigned long long mk = s_marks[wave] & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+        const int owner = mk ? s_start[wave][63 - __clzll(mk)] : carry;
+        carry = __builtin_amdgcn_readlane(owner, 63);
+        const int t = r0 + lane;
+        int nI = 0, id = 0, x = 0, y = 0;
+        uint32_t word = 0, qm = 0;
+        if (t < total) {
+          const int kk = c0 + owner;
+          id = (int)win[kk];
+          const int4 run = s_run[wave][owner];
+          const int r = t - run.x, ry = r / run.w, rx = r - ry * run.w;
+          x = run.y + rx;
+          y = run.z + ry;
+          float mnx, mxx, mny, mxy;
+          int bcx, bcy;
+          Tri T;
+          if (kk < ms) {
+            const float4 d1 = s_stage[kk][1], d2 = s_stage[kk][2], d3 = s_stage[kk][3], d4 = s_stage[kk][4];
+            mnx = d1.x, mxx = d1.y, mny = d1.z, mxy = d1.w;
+            bcx = __float_as_int(d2.x), bcy = __float_as_int(d2.y);
+            T.v0x = d2.z, T.v0y = d2.w, T.a0 = d3.x, T.a1 = d3.y, T.b0 = d3.z, T.b1 = d3.w;
+            T.uz = d4.x, T.z[0] = d4.y, T.z[1] = d4.z, T.z[2] = d4.w;
+          } else {
+            const PrimSetup ps = P.prims[id];
+            mnx = ps.min_x, mxx = ps.max_x, mny = ps.min_y, mxy = ps.max_y;
+            bcx = ps.bcx, bcy = ps.bcy;
+            T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
+          }
+          int qa, qb;
+          const int nqx = axisQuads(mnx, mxx, bcx, x, qa, qb);
+          const int nqy = nqx ? axisQuads(mny, mxy, bcy, y, qa, qb) : 0;
+          if (nqx && nqy) {
+            const float fx = (float)x, fy = (float)y;
+            const float *dp = &s_dep[((y - ty0) * kTile + (x - tx0)) * S];
+            /* coverage, z and the depth clip (RendererSoft.cpp:776-799), then
+             * the depth test, which nothing in this draw can change */
+            bool sin[4];
+            float sz[4];
+            bool pin;
+            if (S == 4) {
+              pin = false;
+#pragma unroll
+              for (int s = 0; s < 4; s++) {
+                float4 bc;
+                sin[s] = bary(T, lx[s] + fx, ly[s] + fy, bc);
+                sz[s] = 0.f;
+                pin = pin || sin[s];
+                if (!sin[s]) continue;
+                sz[s] = dpps3(bc, T.z);
+                if (sz[s] < 0.f || sz[s] > 1.f) sin[s] = false;
+              }
+            } else {
+              float4 bc;
+              sin[0] = bary(T, fx + 0.5f, fy + 0.5f, bc);
+              pin = sin[0];
+              sz[0] = pin ? dpps3(bc, T.z) : 0.f;
+            }
+            if (pin && P.st.depthTest) {
+              if (S == 4) {
+                bool any = false;
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                  if (!sin[s]) continue;
+                  if (hasD) sin[s] = depthPass(gclamp(sz[s], 0.f, 1.f), dp[s], dfunc);
+                  any = any || sin[s];
+                }
+                pin = any;
+              } else {
+                pin = hasD ? depthPass(gclamp(sz[0], 0.f, 1.f), dp[0], dfunc) : true;
+              }
+            }
+            if (pin) {
+              uint32_t smask = 1u;
+              if (S == 4) {
+                smask = 0;
+#pragma unroll
+                for (int s = 0; s < 4; s++) smask |= (sin[s] ? 1u : 0u) << s;
+              }
+              for (int iy = 0; iy < nqy; iy++)
+                for (int ix = 0; ix < nqx; ix++) qm |= 1u << (iy * 2 + ix);
+              nI = nqx * nqy;
+              word = (uint32_t)((size_t)y * P.fb_w + x) | (smask << 28);
+            }
+          }
+        }
+        const int pos = waveInclusiveScan(nI);
+        const int tot = __builtin_amdgcn_readlane(pos, 63);
+        if (wn + tot > kListCap) {
+          full = true;
+          break;
+        }
+        if (nI) {
+          int q = wn + pos - nI;
+          for (int f = 0; f < 4; f++)
+            if ((qm >> f) & 1u) s_frag[wave][q++] = make_uint2(word, ((uint32_t)id << 2) | (uint32_t)f);
+          atomicAdd(&s_cnt[(y - ty0) * kTile + (x - tx0)], nI);
+        }
+        wn += tot;
+        waveSync(); /* marks and starts are read before the next round resets them */
+      }
+      waveSync(); /* the batch's runs are read before the next batch overwrites them */
+    }
+    if (full && lane == 0) s_over = 1;
+    __syncthreads();
+    const bool stop = s_over != 0;
+    if (lp == 0) s_n = 0;
+    __syncthreads();
+    if (stop || hi >= maxid) break;
+    lo = hi + 1;
+  }
+
+  /* each pixel's fragments, contiguous and in order, at one range of the draw's list */
+  const int c = s_cnt[pl];
+  const int inc = waveInclusiveScan(c);
+  if (lane == 63) s_wn[wave] = inc;
+  __syncthreads();
+  const int all = s_wn[0] + s_wn[1] + s_wn[2] + s_wn[3];
+  if (lp == 0) s_base = (s_over || all == 0) ? 0 : atomicAdd(&P.dcount[3], all);
+  __syncthreads();
+  const int gb = s_base;
+  if (s_over || gb + all > P.ord_cap) {
+    if (!s_over) /* the range reserved past the end is never written: mark what lies before it */
+      for (int i = gb + lp; i < min(gb + all, P.ord_cap); i += 256) P.jobs[i] = make_uint2(kListVoid, 0u);
+    if (lp == 0) {
+      P.ord_state[tile] = kOrdOverflow;
+      P.ord_list[atomicAdd(&P.dcount[7], 1)] = tile;
+    }
+    return;
+  }
+  int base = gb + inc - c;
+  for (int v = 0; v < wave; v++) base += s_wn[v];
+  const uint32_t me = (uint32_t)pix;
+  int r = 0;
+  for (int e = 0; e < wn; e++) {
+    const uint2 f = s_frag[wave][e];
+    if (c > 0 && (f.x & 0x0fffffffu) == me) P.jobs[base + r++] = f;
+  }
+  if (inFb) P.ord_span[pix] = make_int2(base, c);
+  if (lp == 0) P.ord_state[tile] = kOrdListed;
+  waveAddCounter(P, 6, nculled);
+}
+
+/* Tiles dealt round-robin to a grid that fits the GPU at once: most tiles of a
+ * draw like this are empty, and the busy ones sit together. A workgroup finds
+ * which of its next 64 tiles have triangles with one load per tile, in
+ * parallel, and walks only those. */
+template <int S>
+__global__ __launch_bounds__(256) void k_ord_list(DrawParams P) {
+  __shared__ int s_tiles[64];
+  __shared__ int s_nt;
+  for (int t0 = blockIdx.x; t0 < P.ntiles; t0 += 64 * gridDim.x) {
+    if (threadIdx.x == 0) s_nt = 0;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      const int tile = t0 + (int)threadIdx.x * (int)gridDim.x;
+      if (tile < P.ntiles) {
+        if (tileBins(P, tile).total() > 0)
+          s_tiles[atomicAdd(&s_nt, 1)] = tile;
+        else
+          P.ord_state[tile] = kOrdEmpty;
+      }
+    }
+    __syncthreads();
+    const int nt = s_nt;
+    for (int i = 0; i < nt; i++) {
+      listTile<S>(P, s_tiles[i]);
+      __syncthreads();
+    }
+  }
+}
+
+/* The fragment shader of every listed fragment: the centre sample if it is
+ * inside, else the first inside sample (RendererSoft.cpp:776-802). */
+template <int S, int SHADING>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_ord_shade(DrawParams P, int need_deriv) {
+  const int n = min(P.dcount[3], P.ord_cap);
+  const XcdShare xs = xcdShare(n);
+  for (int j = xs.lo + xs.idx * blockDim.x + threadIdx.x; j < xs.hi; j += xs.cnt * blockDim.x) {
+    const uint2 jb = P.jobs[j];
+    if (jb.x == kListVoid) continue;
+    const uint32_t pix = jb.x & 0x0fffffffu;
+    const int x = (int)(pix % (uint32_t)P.fb_w), y = (int)(pix / (uint32_t)P.fb_w);
+    const int id = (int)(jb.y >> 2), inst = (int)(jb.y & 3u);
+    const PrimSetup ps = P.prims[id];
+    const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
+    const float fx = (float)x, fy = (float)y;
+    float4 bs;
+    if (S == 4) {
+      if (!bary(T, fx + 0.5f, fy + 0.5f, bs)) {
+#pragma unroll
+        for (int t = 3; t >= 0; t--) {
+          float4 b;
+          if (bary(T, sampleX(t) + fx, sampleY(t) + fy, b)) bs = b;
+        }
+      }
+    } else {
+      bary(T, fx + 0.5f, fy + 0.5f, bs);
+    }
+    const RGBAf c = shadeAt<S, SHADING>(P, ps, T, x, y, bs, inst, need_deriv);
+    P.ord_col[j] = make_float4(gclamp(c.r, 0.f, 1.f), gclamp(c.g, 0.f, 1.f), gclamp(c.b, 0.f, 1.f),
+                               gclamp(c.a, 0.f, 1.f));
+  }
+}
+
+/* Each pixel's listed fragments written to its samples in order, blended when
+ * blending is on; the 4x target is resolved. Tiles left to k_ord_rest are skipped. */
+template <int S>
+__global__ __launch_bounds__(256) void k_ord_apply(DrawParams P) {
+  const uint32_t npx = (uint32_t)P.fb_w * (uint32_t)P.fb_h, w = (uint32_t)P.fb_w;
+  unsigned long long nfs = 0, nwr = 0;
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npx; p += gridDim.x * blockDim.x) {
+    const uint32_t y = p / w, x = p - y * w;
+    const int st = P.ord_state[(y / kTile) * P.tiles_x + x / kTile];
+    if (st == kOrdOverflow) continue;
+    if (S == 1 && st != kOrdListed) continue;
+    uint32_t col[S];
+    if (S == 4) {
+      const uint4 c4 = ((const uint4 *)P.color)[p];
+      col[0] = c4.x, col[1] = c4.y, col[2] = c4.z, col[3] = c4.w;
+    } else {
+      col[0] = P.color[p];
+    }
+    if (st == kOrdListed) {
+      const int2 sp = P.ord_span[p];
+      for (int j = sp.x; j < sp.x + sp.y; j++) {
+        const uint32_t m = P.jobs[j].x >> 28;
+        const float4 c = P.ord_col[j];
+        nfs++;
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+          if (!((m >> s) & 1u)) continue;
+          RGBAf o = {c.x, c.y, c.z, c.w};
+          if (P.st.blend) o = blendColor(o, unpack01(col[s]), P.st);
+          col[s] = pack4(u8t(o.r * 255.f), u8t(o.g * 255.f), u8t(o.b * 255.f), u8t(o.a * 255.f));
+          nwr++;
+        }
+      }
+      if (sp.y) {
+        if (S == 4)
+          ((uint4 *)P.color)[p] = make_uint4(col[0], col[1], col[2], col[3]);
+        else
+          P.color[p] = col[0];
+      }
+    }
+    if (S == 4) P.color_resolved[p] = resolve4(col);
+  }
+  addShadeCounts(P, nfs, nwr);
+}
+
+/* The ordered raster of the tiles whose fragments did not fit the list. */
+template <int S, int SHADING>
+__global__ __launch_bounds__(256) void k_ord_rest(DrawParams P, int need_deriv) {
+  const int n = P.dcount[7];
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    ordTile<S, SHADING>(P, P.ord_list[i], need_deriv, 0);
+    __syncthreads();
+  }
 }
 
 /* --------------------------------------------------------------------------- textures */

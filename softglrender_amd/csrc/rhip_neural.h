@@ -510,8 +510,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
   if (nwr) atomicAdd(&s_wr, nwr);
   __syncthreads();
   if (threadIdx.x == 0) {
-    if (s_fs) atomicAdd(&P.counters[0], s_fs);
-    if (s_wr) atomicAdd(&P.counters[1], s_wr);
+    if (s_fs) atomicAdd(counterAt(P, 0), s_fs);
+    if (s_wr) atomicAdd(counterAt(P, 1), s_wr);
   }
 }
 
