@@ -281,10 +281,16 @@ void launchListed(rhip_device d, const DrawParams &P, int need_deriv, hipStream_
       launchOrdShade<S>(P, (int)std::min<size_t>(blocks(npx, 256), 4096), need_deriv, st);
     }
     Timer tm(d, S == 4 ? "k_ord_apply_msaa4" : "k_ord_apply");
-    RHIP_LAUNCH(k_ord_apply<S>, (unsigned)std::min<size_t>(blocks(npx, 256), 4096), 256, st, P);
+    RHIP_LAUNCH(k_ord_apply<S>, std::min(P.ntiles, 1024), 256, st, P);
   }
-  Timer tm(d, S == 4 ? "k_ord_rest_msaa4" : "k_ord_rest");
-  launchOrdRest<S>(P, std::min(P.ntiles, 512), need_deriv, st);
+  {
+    Timer tm(d, S == 4 ? "k_ord_rest_msaa4" : "k_ord_rest");
+    launchOrdRest<S>(P, std::min(P.ntiles, 512), need_deriv, st);
+  }
+  if (S == 4 && P.color) {
+    Timer tm(d, "k_resolve");
+    RHIP_LAUNCH(k_resolve, (unsigned)std::min<size_t>(blocks(npx, 256), 4096), 256, st, P);
+  }
 }
 
 /* Ordered raster (every state the two-phase raster does not take). */

@@ -145,7 +145,7 @@ struct DrawParams {
   int *blk_fan_off;
   int nblk;
   int *dcount; /* device scalars: [0] appended fan triangles, [1] primitive count, [2] queued fragments,
-                [3] listed fragments, [7] tiles left to the ordered raster */
+                [3] listed fragments, [7] tiles left to the ordered raster, [8] tiles with listed fragments */
   /* primitives */
   PrimSetup *prims;
   /* bins: ntiles fine bins of tile x tile pixels, then coarse_x * coarse_y coarse bins of
@@ -167,7 +167,8 @@ struct DrawParams {
   float4 *ord_col;  /* shaded colour per listed fragment */
   int2 *ord_span;   /* per pixel of a listed tile: its fragments jobs[x, x + y), in API order */
   int *ord_state;   /* per tile: kOrdEmpty, kOrdListed or kOrdOverflow */
-  int *ord_list;    /* tiles whose fragments did not fit, dcount[7] of them */
+  int *ord_list;    /* tiles whose fragments did not fit (dcount[7] from the front) and tiles with
+                       listed fragments (dcount[8] from the back) */
   int ord_cap;      /* jobs slots */
   unsigned long long *counters; /* [0] fs invocations, [1] samples written, [2] clip overflows,
                                    [3] bin items, [4] primitives binned, [5] clipped triangles,

@@ -647,6 +647,7 @@ __device__ __forceinline__ void scanFansBody(const DrawParams &P) {
     P.dcount[2] = 0;
     P.dcount[3] = 0;
     P.dcount[7] = 0;
+    P.dcount[8] = 0;
   }
 }
 
@@ -1606,18 +1607,11 @@ __device__ __forceinline__ void shadeJob(const DrawParams &P, int j, int need_de
   }
 }
 
-/* Adds a block's fragment-shader and sample-write counts to the draw counters. */
+/* Adds the lanes' fragment-shader and sample-write counts to the draw counters
+ * (one atomic each per wavefront; every lane of the wavefront calls it). */
 __device__ __forceinline__ void addShadeCounts(const DrawParams &P, unsigned long long nfs, unsigned long long nwr) {
-  __shared__ unsigned long long s_fs, s_wr;
-  if (threadIdx.x == 0) s_fs = s_wr = 0;
-  __syncthreads();
-  if (nfs) atomicAdd(&s_fs, nfs);
-  if (nwr) atomicAdd(&s_wr, nwr);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (s_fs) atomicAdd(counterAt(P, 0), s_fs);
-    if (s_wr) atomicAdd(counterAt(P, 1), s_wr);
-  }
+  waveAddCounter(P, 0, nfs);
+  waveAddCounter(P, 1, nwr);
 }
 
 template <int S, int SHADING>
@@ -2010,6 +2004,7 @@ __device__ __forceinline__ void listTile(const DrawParams &P, int tile) {
   __shared__ uint32_t win[kOrdWin];
   __shared__ float4 s_stage[kListStage][5];
   __shared__ uint2 s_frag[4][kListCap];
+  __shared__ uint32_t s_rank[4][kListCap]; /* rank among its pixel's fragments << 8 | pixel in the tile */
   __shared__ float s_dep[kTile * kTile * S];
   __shared__ int s_cnt[kTile * kTile];
   __shared__ int4 s_run[4][64]; /* per triangle of a wavefront's batch: first pair, x0, y0, width */
@@ -2242,11 +2237,26 @@ __device__ __forceinline__ void listTile(const DrawParams &P, int tile) {
           full = true;
           break;
         }
+        /* ranks within each pixel, triangle by triangle: a triangle's pairs
+         * are distinct pixels, and triangles come in primitive order */
+        const int lpx = (y - ty0) * kTile + (x - tx0);
+        int rank = 0;
+        const int o0 = __builtin_amdgcn_readlane(owner, 0);
+        const int o1 = __builtin_amdgcn_readlane(owner, min(total - r0, 64) - 1);
+        for (int o = o0; o <= o1; o++) {
+          if (owner == o && nI) {
+            rank = s_cnt[lpx];
+            s_cnt[lpx] = rank + nI;
+          }
+          waveSync();
+        }
         if (nI) {
           int q = wn + pos - nI;
           for (int f = 0; f < 4; f++)
-            if ((qm >> f) & 1u) s_frag[wave][q++] = make_uint2(word, ((uint32_t)id << 2) | (uint32_t)f);
-          atomicAdd(&s_cnt[(y - ty0) * kTile + (x - tx0)], nI);
+            if ((qm >> f) & 1u) {
+              s_frag[wave][q] = make_uint2(word, ((uint32_t)id << 2) | (uint32_t)f);
+              s_rank[wave][q++] = ((uint32_t)rank++ << 8) | (uint32_t)lpx;
+            }
         }
         wn += tot;
         waveSync(); /* marks and starts are read before the next round resets them */
@@ -2282,14 +2292,17 @@ __device__ __forceinline__ void listTile(const DrawParams &P, int tile) {
   }
   int base = gb + inc - c;
   for (int v = 0; v < wave; v++) base += s_wn[v];
-  const uint32_t me = (uint32_t)pix;
-  int r = 0;
-  for (int e = 0; e < wn; e++) {
-    const uint2 f = s_frag[wave][e];
-    if (c > 0 && (f.x & 0x0fffffffu) == me) P.jobs[base + r++] = f;
+  s_cnt[pl] = base; /* every pixel of this quarter: where its fragments start */
+  waveSync();
+  for (int e = lane; e < wn; e += 64) {
+    const uint32_t rk = s_rank[wave][e];
+    P.jobs[s_cnt[rk & 255u] + (int)(rk >> 8)] = s_frag[wave][e];
   }
   if (inFb) P.ord_span[pix] = make_int2(base, c);
-  if (lp == 0) P.ord_state[tile] = kOrdListed;
+  if (lp == 0) {
+    P.ord_state[tile] = kOrdListed;
+    if (all) P.ord_list[P.ntiles - 1 - atomicAdd(&P.dcount[8], 1)] = tile;
+  }
   waveAddCounter(P, 6, nculled);
 }
 
@@ -2356,16 +2369,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 }
 
 /* Each pixel's listed fragments written to its samples in order, blended when
- * blending is on; the 4x target is resolved. Tiles left to k_ord_rest are skipped. */
+ * blending is on: one workgroup per tile with fragments, one lane per pixel. */
 template <int S>
 __global__ __launch_bounds__(256) void k_ord_apply(DrawParams P) {
-  const uint32_t npx = (uint32_t)P.fb_w * (uint32_t)P.fb_h, w = (uint32_t)P.fb_w;
+  const int n = P.dcount[8];
   unsigned long long nfs = 0, nwr = 0;
-  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npx; p += gridDim.x * blockDim.x) {
-    const uint32_t y = p / w, x = p - y * w;
-    const int st = P.ord_state[(y / kTile) * P.tiles_x + x / kTile];
-    if (st == kOrdOverflow) continue;
-    if (S == 1 && st != kOrdListed) continue;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const int tile = P.ord_list[P.ntiles - 1 - i];
+    const int x = (tile % P.tiles_x) * kTile + (int)(threadIdx.x & (kTile - 1));
+    const int y = (tile / P.tiles_x) * kTile + (int)(threadIdx.x / kTile);
+    if (x >= P.fb_w || y >= P.fb_h) continue;
+    const size_t p = (size_t)y * P.fb_w + x;
+    const int2 sp = P.ord_span[p];
+    if (sp.y == 0) continue;
     uint32_t col[S];
     if (S == 4) {
       const uint4 c4 = ((const uint4 *)P.color)[p];
@@ -2373,29 +2389,23 @@ __global__ __launch_bounds__(256) void k_ord_apply(DrawParams P) {
     } else {
       col[0] = P.color[p];
     }
-    if (st == kOrdListed) {
-      const int2 sp = P.ord_span[p];
-      for (int j = sp.x; j < sp.x + sp.y; j++) {
-        const uint32_t m = P.jobs[j].x >> 28;
-        const float4 c = P.ord_col[j];
-        nfs++;
+    for (int j = sp.x; j < sp.x + sp.y; j++) {
+      const uint32_t m = P.jobs[j].x >> 28;
+      const float4 c = P.ord_col[j];
+      nfs++;
 #pragma unroll
-        for (int s = 0; s < S; s++) {
-          if (!((m >> s) & 1u)) continue;
-          RGBAf o = {c.x, c.y, c.z, c.w};
-          if (P.st.blend) o = blendColor(o, unpack01(col[s]), P.st);
-          col[s] = pack4(u8t(o.r * 255.f), u8t(o.g * 255.f), u8t(o.b * 255.f), u8t(o.a * 255.f));
-          nwr++;
-        }
-      }
-      if (sp.y) {
-        if (S == 4)
-          ((uint4 *)P.color)[p] = make_uint4(col[0], col[1], col[2], col[3]);
-        else
-          P.color[p] = col[0];
+      for (int s = 0; s < S; s++) {
+        if (!((m >> s) & 1u)) continue;
+        RGBAf o = {c.x, c.y, c.z, c.w};
+        if (P.st.blend) o = blendColor(o, unpack01(col[s]), P.st);
+        col[s] = pack4(u8t(o.r * 255.f), u8t(o.g * 255.f), u8t(o.b * 255.f), u8t(o.a * 255.f));
+        nwr++;
       }
     }
-    if (S == 4) P.color_resolved[p] = resolve4(col);
+    if (S == 4)
+      ((uint4 *)P.color)[p] = make_uint4(col[0], col[1], col[2], col[3]);
+    else
+      P.color[p] = col[0];
   }
   addShadeCounts(P, nfs, nwr);
 }

@@ -414,9 +414,6 @@ __device__ __forceinline__ void nnJobRow(const DrawParams &P, int j, int need_de
 template <int S>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_nn_shade(DrawParams P, const NnPacked *W, int need_deriv) {
   __shared__ __bf16 hbuf[4][64 * kHs]; /* per wavefront: input rows, then hidden activations, then outputs */
-  __shared__ unsigned long long s_fs, s_wr;
-  if (threadIdx.x == 0) s_fs = s_wr = 0;
-  __syncthreads();
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 4, c = l & 15;
   __bf16 *H = hbuf[wave];
   const int n = P.dcount[2];
@@ -506,13 +503,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     }
     waveSync();
   }
-  if (nfs) atomicAdd(&s_fs, nfs);
-  if (nwr) atomicAdd(&s_wr, nwr);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (s_fs) atomicAdd(counterAt(P, 0), s_fs);
-    if (s_wr) atomicAdd(counterAt(P, 1), s_wr);
-  }
+  addShadeCounts(P, nfs, nwr);
 }
 
 /* --------------------------------------------------------------------------- evaluation */
