@@ -81,10 +81,18 @@ def frame_draws(data):
             color, depth = fbo[cur]
             w, h, smp = tex[color if color >= 0 else depth]
             two_phase = (not st[0]) and depth >= 0 and st[7] and st[8] and st[9] in (1, 3, 4, 6)
-            kern = ("k_raster_vis" if two_phase else "k_raster_ord") + ("_msaa4" if smp == 4 else "")
+            writes_depth = depth >= 0 and st[7] and st[8]
+            sfx = "_msaa4" if smp == 4 else ""
+            # the fine raster of the draw, and the kernel that runs its fragment shader
+            if two_phase:
+                kern, shade = "k_raster_vis" + sfx, ("k_shade" + sfx if color >= 0 else None)
+            elif not writes_depth:
+                kern, shade = "k_ord_list" + sfx, ("k_ord_shade" + sfx if color >= 0 else None)
+            else:
+                kern, shade = "k_raster_ord" + sfx, None
             t, v = vao[vid]
-            draws.append(dict(kernel=kern, tris=t, verts=v, w=w, h=h, samples=smp, color=color >= 0,
-                              vary=_vary_floats(shading, defines) if color >= 0 else 0))
+            draws.append(dict(kernel=kern, shade_kernel=shade, tris=t, verts=v, w=w, h=h, samples=smp,
+                              color=color >= 0, vary=_vary_floats(shading, defines) if color >= 0 else 0))
     return draws
 
 
@@ -97,6 +105,13 @@ def raster_algorithmic_bytes(d):
     if d["color"] and d["samples"] == 4:
         fb += px * 4
     return fb + 12 * d["tris"] + d["verts"] * (16 + 4 * d["vary"])
+
+
+def shade_algorithmic_bytes(d):
+    """Compulsory HBM bytes of one shading launch: each queued fragment's 8-byte record read once, each sample
+    it writes (4 bytes) written once, and per vertex its screen position and the varyings the shader reads,
+    once (texel reads are not counted). d["fs"] and d["samples_written"] are the draw's own counts."""
+    return 8 * d["fs"] + 4 * d["samples_written"] + d["verts"] * (16 + 4 * d["vary"])
 
 
 class _HipEvents:
@@ -421,17 +436,36 @@ def main():
     fs_per_frame = stats["fs_invocations"] / args.steps
     print(f"[bench] {args.steps} frames: {ms_per_frame:.3f} ms/frame; kernels {prof}", file=sys.stderr, flush=True)
 
+    # one more frame, draw by draw, for each draw's own fragment-shader and sample-write counts
+    draws = frame_draws(data)
+    k = 0
+    r.resetStats()
+    prev = r.stats()
+    for op, p in frame:
+        rp.exec(op, p, sink)
+        if op == S.OP_DRAW:
+            r.waitIdle()
+            cur = r.stats()
+            draws[k]["fs"] = cur["fs_invocations"] - prev["fs_invocations"]
+            draws[k]["samples_written"] = cur["samples_written"] - prev["samples_written"]
+            prev, k = cur, k + 1
+    r.waitIdle()
+
     # roofline of the dominant kernel (largest summed time over the timed frames); its algorithmic bytes are
     # those of the draws it ran for, averaged over its launches
-    draws = frame_draws(data)
     dom = max(prof.items(), key=lambda kv: kv[1][0]) if prof else None
     roofline = None
     if dom:
         name, (tot_ms, launches) = dom
         avg_ms = tot_ms / launches
         mine = [d for d in draws if d["kernel"] == name]
+        shading = [d for d in draws if d.get("shade_kernel") == name]
+        algo = None
         if mine:
             algo = sum(raster_algorithmic_bytes(d) for d in mine) / len(mine)
+        elif shading:
+            algo = sum(shade_algorithmic_bytes(d) for d in shading) / len(shading)
+        if algo:
             achieved = algo / (avg_ms * 1e-3) / 1e9
             roofline = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
