@@ -753,6 +753,31 @@ __device__ __forceinline__ bool bary(const Tri &T, float px, float py, float4 &b
   return !(bc.x < 0 || bc.y < 0 || bc.z < 0);
 }
 
+/* True only for a sample bary() finds outside, decided without its divisions:
+ * from the same numerators, ux = nx/uz or uy = ny/uz is negative and too large
+ * to round to zero (a live primitive's |uz| is below 2^31 and nonzero), or
+ * their sum exceeds 1 by far more than the rounding of the two quotients, their
+ * sum and 1 - sum can take back. Samples near an edge and NaNs give false. */
+__device__ __forceinline__ bool sampleSurelyOut(const Tri &T, float px, float py) {
+  const float a2 = T.v0x - px, b2 = T.v0y - py;
+  const float nx = fmaf(T.a1, b2, -(a2 * T.b1));
+  const float ny = fmaf(a2, T.b0, -(T.a0 * b2));
+  const float sg = T.uz < 0.f ? -1.f : 1.f;
+  const float sx = nx * sg, sy = ny * sg, au = fabsf(T.uz);
+  if (sx <= -0x1p-90f || sy <= -0x1p-90f) return true;
+  return (sx + sy) - au > 0x1p-18f * (fabsf(sx) + fabsf(sy) + au);
+}
+
+/* A pixel none of whose samples (the centre without MSAA) any triangle test can cover. */
+template <int S>
+__device__ __forceinline__ bool pixelSurelyOut(const Tri &T, int x, int y) {
+  const float fx = (float)x, fy = (float)y;
+  if (S == 4)
+    return sampleSurelyOut(T, 0.375f + fx, 0.875f + fy) && sampleSurelyOut(T, 0.875f + fx, 0.625f + fy) &&
+           sampleSurelyOut(T, 0.125f + fx, 0.375f + fy) && sampleSurelyOut(T, 0.625f + fx, 0.125f + fy);
+  return sampleSurelyOut(T, fx + 0.5f, fy + 0.5f);
+}
+
 /* Position of MSAA-4x sample s in its pixel (RendererInternal.h:59-186) */
 __device__ __forceinline__ float sampleX(int s) { return s == 0 ? 0.375f : s == 1 ? 0.875f : s == 2 ? 0.125f : 0.625f; }
 __device__ __forceinline__ float sampleY(int s) { return s == 0 ? 0.875f : s == 1 ? 0.625f : s == 2 ? 0.375f : 0.125f; }
@@ -1423,12 +1448,31 @@ __device__ void visTile(const DrawParams &P, int tile) {
   /* phase 1b: the tile's own triangles. A wavefront takes 64 at a time, stages
    * what visPixel needs of each in LDS and spreads their (triangle, pixel) pairs
    * evenly over its lanes, 64 pairs per round, instead of each lane walking its
-   * own triangle's pixels (the keys' atomics make the order irrelevant). */
+   * own triangle's pixels (the keys' atomics make the order irrelevant). Pairs
+   * that the division-free test does not rule out are queued and tested exactly
+   * 64 at a time: most pixels of a small triangle's quad range are uncovered. */
   {
     __shared__ float4 s_pr[WAVES][64][6];
     __shared__ int s_start[WAVES][64];
     __shared__ unsigned long long s_marks[WAVES];
+    __shared__ uint32_t s_q[WAVES][128]; /* queued pairs: triangle | x << 8 | y << 16, tile-relative */
     const int wv = lp >> 6, lane = lp & 63;
+    /* the exact test of queued pairs [0, cnt) */
+    auto exactPairs = [&](int cnt) {
+      if (lane < cnt) {
+        const uint32_t e = s_q[wv][lane];
+        const int x = tx0 + (int)((e >> 8) & 255u), y = ty0 + (int)(e >> 16);
+        const float4 *d = s_pr[wv][e & 255u];
+        const float4 d0 = d[0], d1 = d[1], d2 = d[2], d3 = d[3], d4 = d[4];
+        PrimSetup ps;
+        ps.min_x = d0.x, ps.min_y = d0.y, ps.max_x = d0.z, ps.max_y = d0.w;
+        ps.bcx = __float_as_int(d1.x), ps.bcy = __float_as_int(d1.y);
+        Tri T;
+        T.v0x = d2.z, T.v0y = d2.w, T.a0 = d3.x, T.a1 = d3.y, T.b0 = d3.z, T.b1 = d3.w;
+        T.uz = d4.x, T.z[0] = d4.y, T.z[1] = d4.z, T.z[2] = d4.w;
+        visPixel<S, MODE>(ps, T, __float_as_int(d1.z), x, y, &keys[((y - ty0) * TILE + (x - tx0)) * S]);
+      }
+    };
     for (int b0 = wv * 64; b0 < tb.n[0]; b0 += NT) {
       int n = 0;
       if (b0 + lane < tb.n[0]) {
@@ -1455,7 +1499,7 @@ __device__ void visTile(const DrawParams &P, int tile) {
       const int inc = waveInclusiveScan(n);
       const int off = inc - n, total = __builtin_amdgcn_readlane(inc, 63);
       s_pr[wv][lane][5].x = __int_as_float(off);
-      int carry = 0;
+      int carry = 0, qn = 0;
       for (int r0 = 0; r0 < total; r0 += 64) {
         if (lane == 0) s_marks[wv] = 0ull;
         waveSync();
@@ -1469,22 +1513,33 @@ __device__ void visTile(const DrawParams &P, int tile) {
         const int owner = m ? s_start[wv][63 - __clzll(m)] : carry;
         carry = __builtin_amdgcn_readlane(owner, 63);
         const int t = r0 + lane;
+        bool keep = false;
+        uint32_t e = 0;
         if (t < total) {
           const float4 *d = s_pr[wv][owner];
-          const float4 d0 = d[0], d1 = d[1], d2 = d[2], d3 = d[3], d4 = d[4];
-          PrimSetup ps;
-          ps.min_x = d0.x, ps.min_y = d0.y, ps.max_x = d0.z, ps.max_y = d0.w;
-          ps.bcx = __float_as_int(d1.x), ps.bcy = __float_as_int(d1.y);
+          const float4 d1 = d[1], d2 = d[2], d3 = d[3];
           Tri T;
           T.v0x = d2.z, T.v0y = d2.w, T.a0 = d3.x, T.a1 = d3.y, T.b0 = d3.z, T.b1 = d3.w;
-          T.uz = d4.x, T.z[0] = d4.y, T.z[1] = d4.z, T.z[2] = d4.w;
+          T.uz = d[4].x;
           const int w = __float_as_int(d2.y), r = t - __float_as_int(d[5].x);
           const int ry = r / w, rx = r - ry * w;
           const int x = __float_as_int(d1.w) + rx, y = __float_as_int(d2.x) + ry;
-          visPixel<S, MODE>(ps, T, __float_as_int(d1.z), x, y, &keys[((y - ty0) * TILE + (x - tx0)) * S]);
+          keep = !pixelSurelyOut<S>(T, x, y);
+          e = (uint32_t)owner | ((uint32_t)(x - tx0) << 8) | ((uint32_t)(y - ty0) << 16);
         }
-        waveSync(); /* marks and starts are read before the next round resets them */
+        const unsigned long long kb = __ballot(keep);
+        if (keep) s_q[wv][qn + __popcll(kb & ((1ull << lane) - 1ull))] = e;
+        qn += __popcll(kb);
+        waveSync();
+        if (qn >= 64) {
+          exactPairs(64);
+          waveSync();
+          if (lane < qn - 64) s_q[wv][lane] = s_q[wv][64 + lane];
+          qn -= 64;
+        }
+        waveSync(); /* marks, starts and the queue are read before the next round changes them */
       }
+      if (qn > 0) exactPairs(qn);
       waveSync(); /* staged triangles are read before the next batch overwrites them */
     }
   }
