@@ -2048,7 +2048,7 @@ constexpr int kOrdEmpty = 0, kOrdListed = 1, kOrdOverflow = 2;
 constexpr int kListCap = 512;              /* fragments one wavefront's 8x8 quarter can list */
 constexpr uint32_t kListVoid = 0xffffffffu; /* a jobs slot reserved by a tile that overflowed */
 
-constexpr int kListStage = 128; /* triangles of a window whose walk data is staged in LDS */
+constexpr int kListStage = 96; /* triangles of a window whose walk data is staged in LDS */
 
 /* One tile by one workgroup, one wavefront per 8x8 quarter. A wavefront takes
  * the window's triangles 64 at a time and spreads their (triangle, pixel) pairs
@@ -2060,6 +2060,7 @@ __device__ __forceinline__ void listTile(const DrawParams &P, int tile) {
   __shared__ float4 s_stage[kListStage][5];
   __shared__ uint2 s_frag[4][kListCap];
   __shared__ uint32_t s_rank[4][kListCap]; /* rank among its pixel's fragments << 8 | pixel in the tile */
+  __shared__ uint32_t s_lq[4][128];        /* queued pairs: triangle | x << 8 | y << 16, tile-relative */
   __shared__ float s_dep[kTile * kTile * S];
   __shared__ int s_cnt[kTile * kTile];
   __shared__ int4 s_run[4][64]; /* per triangle of a wavefront's batch: first pair, x0, y0, width */
@@ -2191,29 +2192,18 @@ __device__ __forceinline__ void listTile(const DrawParams &P, int tile) {
       const int inc = waveInclusiveScan(np);
       const int off = inc - np, total = __builtin_amdgcn_readlane(inc, 63);
       s_run[wave][lane] = make_int4(off, x0, y0, w);
-      int carry = 0;
-      for (int r0 = 0; r0 < total; r0 += 64) {
-        if (lane == 0) s_marks[wave] = 0ull;
-        waveSync();
-        if (np > 0 && off >= r0 && off < r0 + 64) {
-          s_start[wave][off - r0] = lane;
-          atomicOr(&s_marks[wave], 1ull << (off - r0));
-        }
-        waveSync();
-        /* the owner of pair r0 + lane: the last triangle starting at or before it */
-        const unsigned long long mk = s_marks[wave] & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
-        const int owner = mk ? s_start[wave][63 - __clzll(mk)] : carry;
-        carry = __builtin_amdgcn_readlane(owner, 63);
-        const int t = r0 + lane;
-        int nI = 0, id = 0, x = 0, y = 0;
+      int carry = 0, qn = 0;
+      /* the exact test of queued pairs [0, cnt); the queue keeps primitive order */
+      auto exactPairs = [&](int cnt) {
+        int nI = 0, id = 0, x = 0, y = 0, owner = 0;
         uint32_t word = 0, qm = 0;
-        if (t < total) {
+        if (lane < cnt) {
+          const uint32_t e = s_lq[wave][lane];
+          owner = (int)(e & 255u);
+          x = tx0 + (int)((e >> 8) & 255u);
+          y = ty0 + (int)(e >> 16);
           const int kk = c0 + owner;
           id = (int)win[kk];
-          const int4 run = s_run[wave][owner];
-          const int r = t - run.x, ry = r / run.w, rx = r - ry * run.w;
-          x = run.y + rx;
-          y = run.z + ry;
           float mnx, mxx, mny, mxy;
           int bcx, bcy;
           Tri T;
@@ -2290,14 +2280,14 @@ __device__ __forceinline__ void listTile(const DrawParams &P, int tile) {
         const int tot = __builtin_amdgcn_readlane(pos, 63);
         if (wn + tot > kListCap) {
           full = true;
-          break;
+          return;
         }
         /* ranks within each pixel, triangle by triangle: a triangle's pairs
          * are distinct pixels, and triangles come in primitive order */
         const int lpx = (y - ty0) * kTile + (x - tx0);
         int rank = 0;
         const int o0 = __builtin_amdgcn_readlane(owner, 0);
-        const int o1 = __builtin_amdgcn_readlane(owner, min(total - r0, 64) - 1);
+        const int o1 = __builtin_amdgcn_readlane(owner, cnt - 1);
         for (int o = o0; o <= o1; o++) {
           if (owner == o && nI) {
             rank = s_cnt[lpx];
@@ -2314,8 +2304,50 @@ __device__ __forceinline__ void listTile(const DrawParams &P, int tile) {
             }
         }
         wn += tot;
-        waveSync(); /* marks and starts are read before the next round resets them */
+      };
+      for (int r0 = 0; r0 < total && !full; r0 += 64) {
+        if (lane == 0) s_marks[wave] = 0ull;
+        waveSync();
+        if (np > 0 && off >= r0 && off < r0 + 64) {
+          s_start[wave][off - r0] = lane;
+          atomicOr(&s_marks[wave], 1ull << (off - r0));
+        }
+        waveSync();
+        /* the owner of pair r0 + lane: the last triangle starting at or before it */
+        const unsigned long long mk = s_marks[wave] & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+        const int owner = mk ? s_start[wave][63 - __clzll(mk)] : carry;
+        carry = __builtin_amdgcn_readlane(owner, 63);
+        const int t = r0 + lane;
+        bool keep = false;
+        uint32_t e = 0;
+        if (t < total) {
+          const int kk = c0 + owner;
+          const int4 run = s_run[wave][owner];
+          const int r = t - run.x, ry = r / run.w, rx = r - ry * run.w;
+          const int x = run.y + rx, y = run.z + ry;
+          keep = true;
+          if (kk < ms) { /* pairs the division-free test rules out never reach the exact test */
+            const float4 d2 = s_stage[kk][2], d3 = s_stage[kk][3];
+            Tri T;
+            T.v0x = d2.z, T.v0y = d2.w, T.a0 = d3.x, T.a1 = d3.y, T.b0 = d3.z, T.b1 = d3.w;
+            T.uz = s_stage[kk][4].x;
+            keep = !pixelSurelyOut<S>(T, x, y);
+          }
+          e = (uint32_t)owner | ((uint32_t)(x - tx0) << 8) | ((uint32_t)(y - ty0) << 16);
+        }
+        const unsigned long long kb = __ballot(keep);
+        if (keep) s_lq[wave][qn + __popcll(kb & ((1ull << lane) - 1ull))] = e;
+        qn += __popcll(kb);
+        waveSync();
+        if (qn >= 64) {
+          exactPairs(64);
+          waveSync();
+          if (lane < qn - 64) s_lq[wave][lane] = s_lq[wave][64 + lane];
+          qn -= 64;
+        }
+        waveSync(); /* marks, starts and the queue are read before the next round changes them */
       }
+      if (qn > 0 && !full) exactPairs(qn);
       waveSync(); /* the batch's runs are read before the next batch overwrites them */
     }
     if (full && lane == 0) s_over = 1;
