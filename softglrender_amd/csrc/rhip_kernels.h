@@ -187,20 +187,25 @@ __device__ __forceinline__ void waveSync() {
 }
 
 /* Exclusive prefix sum of in[0..n) into out[0..n) by one block of 1024 threads,
- * 8192 entries at a time with coalesced loads and stores; also clears clear[0..n)
- * when given. Returns the total (to every thread). */
+ * 8192 entries at a time with coalesced loads and stores (the next chunk's loads
+ * issued before this chunk's scan); also clears clear[0..n) when given. Returns
+ * the total (to every thread). */
 __device__ int blockScanArray(const int *in, int *out, int n, int *clear) {
   __shared__ int buf[8192];
   __shared__ int wsum[16];
   __shared__ int s_carry;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   if (t == 0) s_carry = 0;
+  int nxt[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) nxt[k] = k * 1024 + t < n ? in[k * 1024 + t] : 0;
   __syncthreads();
   for (int c0 = 0; c0 < n; c0 += 8192) {
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-      const int i = c0 + k * 1024 + t;
-      buf[k * 1024 + t] = i < n ? in[i] : 0;
+      buf[k * 1024 + t] = nxt[k];
+      const int i = c0 + 8192 + k * 1024 + t;
+      nxt[k] = i < n ? in[i] : 0;
     }
     __syncthreads();
     int v[8], sum = 0;
@@ -250,6 +255,26 @@ __device__ void blockSortU32(uint32_t *a, int n) {
       }
       __syncthreads();
     }
+}
+
+/* Ascending order of the m distinct ids in win[0..m) (LDS, at least a power of
+ * two above m long), by the whole block: each id's rank is the count of smaller
+ * ones when every id has a thread, a bitonic sort beyond. */
+__device__ void sortIds(uint32_t *win, int m) {
+  if (m <= (int)blockDim.x) {
+    const uint32_t v = (int)threadIdx.x < m ? win[threadIdx.x] : 0u;
+    int rk = 0;
+    for (int i = 0; i < m; i++) rk += win[i] < v ? 1 : 0;
+    __syncthreads();
+    if ((int)threadIdx.x < m) win[rk] = v;
+    __syncthreads();
+    return;
+  }
+  int mp = 1;
+  while (mp < m) mp <<= 1;
+  for (int i = m + (int)threadIdx.x; i < mp; i += blockDim.x) win[i] = 0xffffffffu;
+  __syncthreads();
+  blockSortU32(win, mp);
 }
 
 /* --------------------------------------------------------------------------- triangle setup */
@@ -496,6 +521,8 @@ __device__ __forceinline__ bool validTri(const DrawParams &P, int i0, int i1, in
 /* processPolygonAssembly (RendererSoft.cpp:428-437) + the clip test: triangles
  * with a vertex outside the frustum are set up later by k_clip. */
 __device__ __forceinline__ void setupBody(const DrawParams &P, int blk) {
+  /* the PBR shader's linear base colour, rpowf(base colour, 2.2), once per draw */
+  if (blk == 0 && threadIdx.x < 3) P.dcount[4 + threadIdx.x] = __float_as_int(rpowf(P.u.base_color[threadIdx.x], 2.2f));
   const int t = blk * blockDim.x + threadIdx.x;
   int needs = 0;
   BinRect r = {-1, 0, 0, 0, 0};
@@ -675,8 +702,6 @@ static __global__ __launch_bounds__(256) void k_clip_fans(DrawParams P) {
 
 /* One block of 1024: bin starts from the counts setup made; cursors cleared. */
 __device__ __forceinline__ void binScanBody(const DrawParams &P) {
-  /* the PBR shader's linear base colour, rpowf(base colour, 2.2), once per draw */
-  if (threadIdx.x < 3) P.dcount[4 + threadIdx.x] = __float_as_int(rpowf(P.u.base_color[threadIdx.x], 2.2f));
   const int total = blockScanArray(P.bin_count, P.bin_start, P.nbins, P.bin_cursor);
   if (threadIdx.x == 0) {
     P.bin_start[P.nbins] = total;
@@ -1807,11 +1832,7 @@ __device__ __forceinline__ void ordTile(const DrawParams &P, int tile, int need_
     }
     __syncthreads();
     const int m = s_n;
-    int mp = 1;
-    while (mp < m) mp <<= 1;
-    for (int i = m + lp; i < mp; i += 256) win[i] = 0xffffffffu;
-    __syncthreads();
-    blockSortU32(win, mp);
+    sortIds(win, m);
     /* what the walk reads of the window's first triangles, staged once per window
      * instead of two dependent global loads per triangle and wavefront */
     const int ms = min(m, kOrdStage);
@@ -2150,11 +2171,7 @@ __device__ __forceinline__ void listTile(const DrawParams &P, int tile) {
     }
     __syncthreads();
     const int m = s_n;
-    int mp = 1;
-    while (mp < m) mp <<= 1;
-    for (int i = m + lp; i < mp; i += 256) win[i] = 0xffffffffu;
-    __syncthreads();
-    blockSortU32(win, mp);
+    sortIds(win, m);
     const int ms = min(m, kListStage);
     for (int i = lp; i < ms; i += 256) {
       const PrimSetup ps = P.prims[win[i]];
