@@ -107,7 +107,15 @@ __device__ __forceinline__ void vertexBody(const DrawParams &P, int blk) {
   if (i >= P.nverts) return;
   float a[16];
   const float *src = P.verts + (size_t)i * P.stride_f;
-  for (int k = 0; k < 16; k++) a[k] = k < P.stride_f ? src[k] : 0.f;
+  if ((P.stride_f & 3) == 0) { /* 16-B aligned records: four-float loads */
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const float4 q = 4 * k < P.stride_f ? ((const float4 *)src)[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+      a[4 * k] = q.x, a[4 * k + 1] = q.y, a[4 * k + 2] = q.z, a[4 * k + 3] = q.w;
+    }
+  } else {
+    for (int k = 0; k < 16; k++) a[k] = k < P.stride_f ? src[k] : 0.f;
+  }
   float v[kVaryStride];
 #pragma unroll
   for (int k = 0; k < kVaryStride; k++) v[k] = 0.f;
