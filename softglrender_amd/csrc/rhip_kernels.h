@@ -12,7 +12,10 @@
  *   k_bin_scan    start of every bin
  *   k_bin_emit    primitive ids into their bins
  *   k_raster_vis  draws whose result follows from depths and order: visibility, then one
- *                 shading per visible fragment
+ *                 shading per visible fragment (k_shade)
+ *   k_ord_list    draws that write no depth: each pixel's fragments listed in API order,
+ *                 then shaded all at once (k_ord_shade) and written in order (k_ord_apply);
+ *                 tiles that do not fit take the ordered raster (k_ord_rest)
  *   k_raster_ord  every other draw: fragments in API order per pixel, shading deferred and
  *                 batched per wavefront, per-sample operations in order (:720-912)
  */
