@@ -28,6 +28,50 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def _host_cpus():
+    """CPUs this process may run on, and the count std::thread::hardware_concurrency() reports (the
+    reference's thread pool size, ThreadPool.h:22)."""
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return allowed, os.cpu_count() or 1
+
+
+def _time_reference(oracle, data, out_bytes, frames, threaded):
+    """Median ms/frame of SoftGLRender's RendererSoft over `frames` whole frames after one warm-up."""
+    _, times = oracle.ref_replay(data, out_bytes, repeats=1 + frames, threaded=threaded)
+    return statistics.median(times[1:]), times[1:]
+
+
+def _reference_baseline(oracle, data, out_bytes, frames, scale, what, ms_per_frame):
+    """cpu_baseline: the shipped multithreaded RendererSoft (RASTER_MULTI_THREAD, RendererSoft.cpp:20; a pool of
+    hardware_concurrency() workers, ThreadPool.h:22) on this host's cores, with the serial build beside it."""
+    allowed, online = _host_cpus()
+    res = None
+    if oracle.have_reference(threaded=True):
+        med, times = _time_reference(oracle, data, out_bytes, frames, True)
+        res = {"value": round(med * scale, 2), "unit": "ms/frame", "cores": allowed, "kind": "reference",
+               "threads": online,
+               "sample": f"{what}: median of {frames} frames after 1 warm-up ({', '.join(f'{t:.0f}' for t in times)} "
+                         f"ms) of SoftGLRender RendererSoft as shipped (multithreaded, {online}-worker pool) built "
+                         f"from its sources" + (f", times {scale} (the area ratio)" if scale != 1 else "") +
+                         f"; host CPU '{_cpu_model()}', {allowed} CPUs available, nproc {online}",
+               "gpu_speedup": round(med * scale / ms_per_frame, 1)}
+    if oracle.have_reference(threaded=False):
+        sframes = max(1, min(frames, 3))
+        med, times = _time_reference(oracle, data, out_bytes, sframes, False)
+        serial = {"value": round(med * scale, 2), "unit": "ms/frame", "cores": 1, "kind": "reference",
+                  "sample": f"{what}: median of {sframes} frames after 1 warm-up of the serial build (the parity "
+                            f"oracle)" + (f", times {scale}" if scale != 1 else ""),
+                  "gpu_speedup": round(med * scale / ms_per_frame, 1)}
+        if res is None:
+            res = serial
+        else:
+            res["serial"] = serial
+    return res
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -109,9 +153,11 @@ def raster_algorithmic_bytes(d):
 
 def shade_algorithmic_bytes(d):
     """Compulsory HBM bytes of one shading launch: each queued fragment's 8-byte record read once, each sample
-    it writes (4 bytes) written once, and per vertex its screen position and the varyings the shader reads,
-    once (texel reads are not counted). d["fs"] and d["samples_written"] are the draw's own counts."""
-    return 8 * d["fs"] + 4 * d["samples_written"] + d["verts"] * (16 + 4 * d["vary"])
+    it writes (4 bytes) written once, per vertex its screen position and the varyings the shader reads, once,
+    and each distinct texel the shader reads, once (d["texel_bytes"], counted by the checker; absent without
+    the CPU leg). d["fs"] and d["samples_written"] are the draw's own counts."""
+    return (8 * d["fs"] + 4 * d["samples_written"] + d["verts"] * (16 + 4 * d["vary"]) +
+            d.get("texel_bytes", 0))
 
 
 class _HipEvents:
@@ -151,6 +197,58 @@ def _measured_traffic(kernel):
             return json.load(f)["kernels"][kernel]["hbm_bytes"]
     except (OSError, KeyError, ValueError):
         return None
+
+
+def _sq_counters(kernel):
+    """Per-dispatch SQ counters of `kernel` on the C2 frame from rocprofv3 --pmc runs, or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "c2_sq_counters.json")) as f:
+            return json.load(f)["kernels"][kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+# MI355X: 256 CUs at 2.4 GHz; a CU issues one wave64 VALU instruction per cycle (4 SIMD16 units, 4 cycles each)
+VALU_PEAK_GINSTR = 256 * 2.4
+
+
+def dominant_roofline(prof, draws, config):
+    """Roofline of the dominant kernel (largest summed time over the event-timed frames). Its algorithmic bytes
+    are those of the draws it ran for, averaged over its launches. A shading kernel is bound by its VALU issue:
+    then `bound` is "valu" with the wave64 VALU instructions per launch (SQ_INSTS_VALU) over the CU issue peak, and
+    the HBM line (texel bytes included where the checker counted them) sits beside it."""
+    if not prof:
+        return None
+    name, (tot_ms, launches) = max(prof.items(), key=lambda kv: kv[1][0])
+    avg_ms = tot_ms / launches
+    mine = [d for d in draws if d["kernel"] == name]
+    shading = [d for d in draws if d.get("shade_kernel") == name]
+    algo = None
+    if mine:
+        algo = sum(raster_algorithmic_bytes(d) for d in mine) / len(mine)
+    elif shading:
+        algo = sum(shade_algorithmic_bytes(d) for d in shading) / len(shading)
+    if not algo:
+        return None
+    achieved = algo / (avg_ms * 1e-3) / 1e9
+    hbm = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 5),
+           "traffic": _measured_traffic(name) if config == "c2" else None,
+           "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE runs of this command, scaled "
+                           "on k_resolve (profiles/c2_hbm_traffic.md)",
+           "algorithmic_bytes_per_launch": int(algo),
+           "texel_bytes_counted": bool(shading) and all("texel_bytes" in d for d in shading)}
+    sq = _sq_counters(name) if (shading and config == "c2") else None
+    if sq and sq.get("valu_insts"):
+        ginstr = sq["valu_insts"] / (avg_ms * 1e-3) / 1e9
+        return {"kernel": name, "bound": "valu", "achieved": round(ginstr, 2), "peak": VALU_PEAK_GINSTR,
+                "unit": "G wave64 VALU instructions/s", "frac": round(ginstr / VALU_PEAK_GINSTR, 5),
+                "traffic": hbm["traffic"], "avg_launch_ms": round(avg_ms, 4),
+                "valu_insts_per_launch": sq["valu_insts"],
+                "counter_note": "SQ_INSTS_VALU per dispatch from rocprofv3 --pmc on this command "
+                                "(profiles/c2_sq_counters.md)",
+                "hbm": hbm}
+    return {"kernel": name, **hbm, "avg_launch_ms": round(avg_ms, 4)}
 
 
 def _pct(x, q):
@@ -207,8 +305,9 @@ def run_batched(r, quick=False, reps=5, log=lambda m: None):
     own = [ev.ms(x, y) for x, y in marks]
     since = [ev.ms(marks[0][0], y) for _, y in marks]
     ev.close()
-    out_tex = sorted(t for t in rp.tex if t > len(sc.meta["scene_triangles"]))[0::2][:8]
-    digest = lambda: hashlib.sha256(b"".join(rp.tex[t].readImage().tobytes() for t in out_tex)).hexdigest()
+    # every view's colour and depth target (the textures after the scenes' own)
+    out_tex = sorted(t for t in rp.tex if t > len(sc.meta["scene_triangles"]))
+    digest = lambda: [hashlib.sha256(rp.tex[t].readImage().tobytes()).hexdigest() for t in out_tex]
     ref_digest = digest()
     log(f"[bench] separate draws: {sep_ms:.3f} ms per {nv} views")
 
@@ -230,8 +329,11 @@ def run_batched(r, quick=False, reps=5, log=lambda m: None):
     ev.close()
     batch.run()
     lat = batch.view_times()
-    same = digest() == ref_digest
-    log(f"[bench] batch: {bat_ms:.3f} ms per {nv} views; identical to separate draws: {same}")
+    got_digest = digest()
+    same_targets = sum(a == b for a, b in zip(got_digest, ref_digest))
+    same = same_targets == len(ref_digest)
+    log(f"[bench] batch: {bat_ms:.3f} ms per {nv} views; identical to separate draws: {same} "
+        f"({same_targets} of {len(ref_digest)} colour and depth targets)")
     draws = frame_draws(data)
     algo = sum(raster_algorithmic_bytes(d) for d in draws)
     res = {
@@ -253,6 +355,8 @@ def run_batched(r, quick=False, reps=5, log=lambda m: None):
                       "seconds": round(t_tune, 1)},
         "algorithmic_bytes_per_batch": int(algo),
         "identical_to_separate_draws": same,
+        "targets_compared": len(ref_digest),
+        "targets_identical": same_targets,
     }
     batch.destroy()
     return res
@@ -344,7 +448,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-neural", action="store_true", help="skip the learned-shader (c5) measurement")
     ap.add_argument("--neural-quick", action="store_true", help="reduced c5 run (fewer cameras and steps)")
-    ap.add_argument("--cpu-frames", type=int, default=3, help="timed reference frames (after one warm-up)")
+    ap.add_argument("--cpu-frames", type=int, default=5, help="timed reference frames (after one warm-up)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-views (c4) measurement")
     ap.add_argument("--batched-quick", action="store_true", help="reduced c4 run (64 views over 16 scenes)")
     args = ap.parse_args()
@@ -451,29 +555,6 @@ def main():
             prev, k = cur, k + 1
     r.waitIdle()
 
-    # roofline of the dominant kernel (largest summed time over the timed frames); its algorithmic bytes are
-    # those of the draws it ran for, averaged over its launches
-    dom = max(prof.items(), key=lambda kv: kv[1][0]) if prof else None
-    roofline = None
-    if dom:
-        name, (tot_ms, launches) = dom
-        avg_ms = tot_ms / launches
-        mine = [d for d in draws if d["kernel"] == name]
-        shading = [d for d in draws if d.get("shade_kernel") == name]
-        algo = None
-        if mine:
-            algo = sum(raster_algorithmic_bytes(d) for d in mine) / len(mine)
-        elif shading:
-            algo = sum(shade_algorithmic_bytes(d) for d in shading) / len(shading)
-        if algo:
-            achieved = algo / (avg_ms * 1e-3) / 1e9
-            roofline = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                        "traffic": _measured_traffic(name) if args.config == "c2" else None,
-                        "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE runs of this "
-                                        "command, scaled on k_resolve (profiles/c2_hbm_traffic.md)",
-                        "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(algo)}
-
     cpu = None
     ref_frags = None
     if not args.no_cpu_baseline and args.config == "c3":
@@ -486,15 +567,9 @@ def main():
             oracle.cpu_ref_replay(crop.stream.bytes(), crop.stream.output_bytes(), stats=st,
                                   threads=min(16, os.cpu_count() or 1))
             ref_frags = st["fs_invocations"] * 256
-            if oracle.have_reference(threaded=False):
-                _, times = oracle.ref_replay(crop.stream.bytes(), crop.stream.output_bytes(), repeats=1)
-                est = times[0] * 256
-                cpu = {"value": round(est, 1), "unit": "ms/frame", "cores": 1, "kind": "reference",
-                       "sample": f"one 240x135 frame with {2_000_000 // 256} random triangles and the 2000 "
-                                 f"screen-covering ones ({times[0]:.0f} ms) of SoftGLRender RendererSoft built "
-                                 f"serially from its sources, times 256 (the area ratio); host CPU "
-                                 f"'{_cpu_model()}', nproc {os.cpu_count()}",
-                       "gpu_speedup": round(est / ms_per_frame, 1)}
+            cpu = _reference_baseline(oracle, crop.stream.bytes(), crop.stream.output_bytes(), 1, 256,
+                                      f"one 240x135 crop with {2_000_000 // 256} random triangles and the 2000 "
+                                      f"screen-covering ones", ms_per_frame)
         except Exception as e:  # report, do not hide
             cpu = {"error": repr(e)}
     if not args.no_cpu_baseline and args.config != "c3":
@@ -502,19 +577,18 @@ def main():
             import oracle
             # fragments the reference shades per frame (its early-Z survivors), counted by the
             # checker on the same frame: the work Gfrag/s is quoted against
-            st = {}
-            oracle.cpu_ref_replay(data, sc.stream.output_bytes(), stats=st)
+            st, tb = {}, []
+            oracle.cpu_ref_replay(data, sc.stream.output_bytes(), stats=st, texel_bytes=tb)
             ref_frags = st["fs_invocations"]
-            if oracle.have_reference(threaded=False):
-                _, times = oracle.ref_replay(data, sc.stream.output_bytes(), repeats=1 + args.cpu_frames)
-                med = statistics.median(times[1:])
-                cpu = {"value": round(med, 2), "unit": "ms/frame", "cores": 1, "kind": "reference",
-                       "sample": f"{args.cpu_frames} whole {sc.name} frames (median, after 1 warm-up) of "
-                                 f"SoftGLRender RendererSoft built serially from its sources; host CPU "
-                                 f"'{_cpu_model()}', nproc {os.cpu_count()}",
-                       "gpu_speedup": round(med / ms_per_frame, 1)}
+            # the distinct texels each draw's shader reads: the texture part of its algorithmic bytes
+            for d, t in zip(draws, tb):
+                d["texel_bytes"] = t
+            cpu = _reference_baseline(oracle, data, sc.stream.output_bytes(), args.cpu_frames, 1,
+                                      f"whole {sc.name} frames", ms_per_frame)
         except Exception as e:  # report, do not hide
             cpu = {"error": repr(e)}
+
+    roofline = dominant_roofline(prof, draws, args.config)
 
     neural_res = None
     if args.config == "c2" and not args.no_neural:

@@ -145,7 +145,14 @@ int rhip_begin_render_pass(rhip_device dev, int fbo, const rhip_clear_states *cl
 int rhip_set_viewport(rhip_device dev, int x, int y, int width, int height);          /* setViewPort (Renderer.h:51) */
 int rhip_draw(rhip_device dev, const rhip_draw_desc *draw);                           /* set*(...) + draw (Renderer.h:52-56) */
 int rhip_end_render_pass(rhip_device dev);                                            /* endRenderPass (Renderer.h:57) */
-int rhip_wait_idle(rhip_device dev);                                                  /* waitIdle (Renderer.h:58) */
+/* waitIdle (Renderer.h:58). Also returns RHIP_E_DEVICE, naming the stage, when a
+ * kernel dropped work since the last wait or readback (a clipped polygon over
+ * 9 vertices); rhip_texture_read reports the same. Reporting clears it. */
+int rhip_wait_idle(rhip_device dev);
+/* Test hook, no reference counterpart: a kernel marks device error `stage`
+ * (0 = clipping overflow) as a dropping kernel would, so the reporting path
+ * can be tested without provoking a real overflow. */
+int rhip_debug_raise_device_error(rhip_device dev, int stage);
 
 /* ---------------------------------------------------------------- measurement */
 /* ---------------------------------------------------------------- learned PBR shader
@@ -215,6 +222,23 @@ int rhip_batch_destroy(rhip_device dev, int batch);
 
 int rhip_stats_get(rhip_device dev, rhip_stats *out); /* waits for queued work */
 int rhip_stats_reset(rhip_device dev);
+/* The host libm's single-precision functions on the device (no reference
+ * counterpart; for tests). The shaders call glibc's log2f, exp2f and powf
+ * (ShaderSoft.h:132, PbrSoft.h:201-235,313, BlinnPhongSoft.h:227); the device
+ * reproduces them bit for bit from a table of the inputs where its own
+ * rounding differs, built from the host libm at rhip_device_create.
+ * fn: 0 log2f(x), 1 exp2f(-x), 2 powf(x, 5), 3 powf(x, 128), 4 powf(x, 1/2.2f).
+ * rhip_libm_info (no device needed) builds fn's table on the host and gives
+ * its listed inputs and gate; rhip_libm_eval evaluates fn on the device. */
+int rhip_libm_info(int fn, long long *entries, double *gate);
+int rhip_libm_eval(rhip_device dev, int fn, const float *x, long long n, float *out);
+/* The same evaluation on the host, from the host's copy of the table. */
+int rhip_libm_eval_host(int fn, const float *x, long long n, float *out);
+/* Hierarchical Z on (1, the default) or off (0) for later draws of the device.
+ * No reference counterpart: the reference tests every triangle at every sample;
+ * with it off the two-phase raster does too (no depth-bound culling), so tests
+ * can check that culling never changes an output. */
+int rhip_set_hierarchical_z(rhip_device dev, int enabled);
 /* The HIP stream the device queues on (hipStream_t), for event timing. */
 void *rhip_device_stream(rhip_device dev);
 /* Per-kernel timing with HIP events recorded on the device's stream around every

@@ -65,18 +65,28 @@ def ref_replay(stream_bytes: bytes, out_nbytes: int, repeats: int = 1, threaded:
 
 
 def cpu_ref_replay(stream_bytes: bytes, out_nbytes: int, repeats: int = 1, threads: int = 1, stats=None,
-                   exact_math: bool = False):
+                   exact_math: bool = False, texel_bytes=None):
     """Runs the stream through the restatement. Returns (output bytes, per-frame ms).
 
     exact_math=False calls the host libm's powf/log2f/exp2f like the reference;
     exact_math=True uses correctly rounded versions, the arithmetic of the GPU path.
 
     `threads` > 1 splits fragment work over rows of 32-pixel blocks (same result).
-    `stats`, if a dict, receives counters of the last frame (fragments shaded etc.)."""
+    `stats`, if a dict, receives counters of the last frame (fragments shaded etc.).
+    `texel_bytes`, if a list, receives per draw of the last frame the bytes of the distinct texels its shaders
+    read (4 per texel); counting runs the raster on one thread."""
     lib = _load(os.path.join(BUILD_DIR, "libcpu_ref.so"))
     lib.cpu_ref_set_threads(ctypes.c_int(threads))
     lib.cpu_ref_set_math(ctypes.c_int(1 if exact_math else 0))
-    out = _replay(lib.cpu_ref_replay, stream_bytes, out_nbytes, repeats)
+    lib.cpu_ref_count_texels(ctypes.c_int(1 if texel_bytes is not None else 0))
+    try:
+        out = _replay(lib.cpu_ref_replay, stream_bytes, out_nbytes, repeats)
+    finally:
+        lib.cpu_ref_count_texels(ctypes.c_int(0))
+    if texel_bytes is not None:
+        arr = (ctypes.c_longlong * 4096)()
+        n = lib.cpu_ref_draw_texel_bytes(arr, 4096)
+        texel_bytes[:] = [int(arr[i]) for i in range(max(n, 0))]
     if stats is not None:
         arr = (ctypes.c_longlong * 8)()
         lib.cpu_ref_stats(arr)

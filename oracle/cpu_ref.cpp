@@ -41,6 +41,10 @@ static inline float m_powf(float x, float y) { return g_cr_math ? (float)pow((do
 static inline float m_log2f(float x) { return g_cr_math ? (float)log2((double)x) : log2f(x); }
 static inline float m_exp2f(float x) { return g_cr_math ? (float)exp2((double)x) : exp2f(x); }
 static long long g_stats[8];
+/* unique-texel counting: while on (single-threaded raster), every texel a draw's shaders read is marked, and
+ * the draw's count of distinct texels read, times 4 bytes, is appended to g_draw_texel_bytes */
+static bool g_touch = false;
+static std::vector<long long> g_draw_texel_bytes;
 
 /* ------------------------------------------------------------------ small vector helpers */
 
@@ -130,6 +134,7 @@ struct Level {
   std::vector<uint32_t> rgbaMs; /* 4 samples per pixel, contiguous (TextureSoft.h:33) */
   std::vector<float> fMs;
   bool hasSingle = false, hasMs = false;
+  mutable std::vector<uint8_t> touched; /* texels read by the current draw (unique-texel counting) */
 };
 
 struct Texture {
@@ -206,6 +211,10 @@ static T pixelWrap(const Level &l, int x, int y, int wrap, T border) {
   }
   if ((size_t)x >= (size_t)w || (size_t)y >= (size_t)h) return TexelOps<T>::zero(); /* Buffer::get -> nullptr */
   g_stats[6]++;
+  if (g_touch) {
+    if (l.touched.size() != (size_t)w * h) l.touched.assign((size_t)w * h, 0);
+    l.touched[(size_t)y * w + x] = 1;
+  }
   return TexelOps<T>::data(l)[(size_t)y * w + x];
 }
 
@@ -1170,7 +1179,12 @@ struct Renderer {
     }
     /* processRasterization :301-340 */
     int H = (int)vpH;
-    int nt = std::max(1, std::min(g_threads, H / 32));
+    int nt = g_touch ? 1 : std::max(1, std::min(g_threads, H / 32));
+    if (g_touch)
+      for (int s = 0; s < RHS_SLOT_COUNT; s++)
+        if (ctx.tex[s])
+          for (auto &layer : ctx.tex[s]->layers)
+            for (auto &l : layer) l.touched.clear();
     std::vector<long long> st((size_t)nt * 8, 0);
     auto work = [&](int w) {
       int lo = (H * w / nt) & ~1, hi = w == nt - 1 ? 1 << 30 : (H * (w + 1) / nt) & ~1;
@@ -1190,6 +1204,19 @@ struct Renderer {
     }
     for (int w = 0; w < nt; w++)
       for (int k = 0; k < 6; k++) g_stats[k] += st[w * 8 + k];
+    if (g_touch) {
+      long long texels = 0;
+      std::vector<const Texture *> seen;
+      for (int s = 0; s < RHS_SLOT_COUNT; s++) {
+        const Texture *t = ctx.tex[s];
+        if (!t || std::find(seen.begin(), seen.end(), t) != seen.end()) continue;
+        seen.push_back(t);
+        for (auto &layer : t->layers)
+          for (auto &l : layer)
+            for (uint8_t b : l.touched) texels += b;
+      }
+      g_draw_texel_bytes.push_back(4 * texels);
+    }
     if (cl && cl->hasMs) resolve();
     return 0;
   }
@@ -1355,6 +1382,18 @@ void cpu_ref_set_threads(int n) {
 
 void cpu_ref_stats(long long *out8) { memcpy(out8, cref::g_stats, sizeof(cref::g_stats)); }
 
+/* Unique-texel counting for the next replays (the raster then runs on one thread). */
+void cpu_ref_count_texels(int on) { cref::g_touch = on != 0; }
+
+/* Bytes of distinct texels each draw of the last frame read (4 per texel), in draw order; returns the number
+ * of draws, or -1 when `cap` is too small. */
+int cpu_ref_draw_texel_bytes(long long *out, int cap) {
+  const int n = (int)cref::g_draw_texel_bytes.size();
+  if (n > cap) return -1;
+  for (int i = 0; i < n; i++) out[i] = cref::g_draw_texel_bytes[i];
+  return n;
+}
+
 long long cpu_ref_replay(const void *stream, size_t bytes, void *out, size_t out_cap, int repeats, double *frame_ms) {
   RhsReader rd;
   if (rhs_open(&rd, stream, bytes)) return -1;
@@ -1388,6 +1427,7 @@ long long cpu_ref_replay(const void *stream, size_t bytes, void *out, size_t out
   if (repeats < 1) repeats = 1;
   for (int i = 0; i < repeats; i++) {
     memset(cref::g_stats, 0, sizeof(cref::g_stats));
+    cref::g_draw_texel_bytes.clear();
     auto t0 = std::chrono::steady_clock::now();
     for (auto &c : frame)
       if (r->exec(c.op, c.p, c.n, result)) return -1;

@@ -102,6 +102,11 @@ SIGNATURES = {
     "rhip_draw": [ctypes.c_void_p, ctypes.POINTER(DrawDesc)],
     "rhip_end_render_pass": [ctypes.c_void_p],
     "rhip_wait_idle": [ctypes.c_void_p],
+    "rhip_debug_raise_device_error": [ctypes.c_void_p, ctypes.c_int],
+    "rhip_set_hierarchical_z": [ctypes.c_void_p, ctypes.c_int],
+    "rhip_libm_info": [ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_double)],
+    "rhip_libm_eval": [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p],
+    "rhip_libm_eval_host": [ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p],
     "rhip_stats_get": [ctypes.c_void_p, ctypes.POINTER(Stats)],
     "rhip_stats_reset": [ctypes.c_void_p],
     "rhip_device_stream": [ctypes.c_void_p],

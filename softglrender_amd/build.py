@@ -22,7 +22,7 @@ HIP_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contr
              "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 
 TARGETS = {
-    "librhip.so": ["rhip_api.hip", "rhip_batch.hip", "rhip_batch_vis1.hip", "rhip_batch_vis4.hip"],
+    "librhip.so": ["rhip_api.hip", "rhip_batch.hip", "rhip_batch_vis1.hip", "rhip_batch_vis4.hip", "rhip_libm.hip"],
 }
 OBJ_DIR = os.path.join(LIB_DIR, "obj")
 

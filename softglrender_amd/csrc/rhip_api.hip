@@ -163,6 +163,22 @@ void rhip::decodeUniforms(const rhip_draw_desc &dd, Uniforms &u) {
 
 namespace {
 
+/* After a synchronisation: reports (RHIP_E_DEVICE, naming the stage) and
+ * clears what the kernels' error words recorded since the last check. */
+int takeDeviceErrors(rhip_device d) {
+  uint32_t e[kErrSlots];
+  HIPCHK(hipMemcpy(e, d->errors, sizeof e, hipMemcpyDeviceToHost));
+  if (!e[kErrClip]) return RHIP_OK;
+  HIPCHK(hipMemsetAsync(d->errors, 0, sizeof e, d->stream));
+  HIPCHK(hipStreamSynchronize(d->stream));
+  return fail(RHIP_E_DEVICE,
+              "clipping: a clipped polygon exceeded %d vertices and its triangle was dropped; the output differs "
+              "from the reference's",
+              kMaxPoly);
+}
+
+__global__ void k_set_error(uint32_t *errors, int slot) { errors[slot] = 1u; }
+
 hipEvent_t takeEvent(rhip_device d) {
   if (!d->event_pool.empty()) {
     hipEvent_t e = d->event_pool.back();
@@ -435,7 +451,11 @@ int drawSetup(rhip_device d, int fbo, const int vp[4], const rhip_draw_desc &dd,
   P.idx = va.idx;
   P.ntris = va.nidx / 3;
   P.counters = d->counters;
+  P.errors = d->errors;
+  P.hiz_off = d->hiz_off ? 1 : 0;
   P.gamma_lut = d->gamma_lut;
+  P.libm = d->libm;
+  for (int c = 0; c < 3; c++) P.base_lin[c] = hostPowf(P.u.base_color[c], 2.2f); /* PbrSoft.h:235 */
   sh.V = (size_t)va.nverts;
   sh.T = (size_t)P.ntris;
   sh.stride = va.stride;
@@ -477,10 +497,11 @@ size_t carveArenas(DrawParams &P, const DrawShape &sh, char *base) {
   if (sh.two_phase) {
     const int ncoarse = P.coarse_x * P.coarse_y;
     P.jobs = (uint2 *)take((size_t)P.fb_w * P.fb_h * P.samples * 8);
-    P.hiz_bound = (uint32_t *)take((size_t)ncoarse * (3 + kHizCap) * 4);
+    P.hiz_bound = (uint32_t *)take((size_t)ncoarse * (4 + kHizCap) * 4);
     P.hiz_count = P.hiz_bound ? (int *)P.hiz_bound + ncoarse : nullptr;
     P.hiz_seed = P.hiz_count ? P.hiz_count + ncoarse : nullptr;
-    P.hiz_surv = P.hiz_seed ? P.hiz_seed + ncoarse : nullptr;
+    P.hiz_poly = P.hiz_seed ? P.hiz_seed + ncoarse : nullptr;
+    P.hiz_surv = P.hiz_poly ? P.hiz_poly + ncoarse : nullptr;
   }
   if (sh.listed) {
     const size_t px = (size_t)P.fb_w * P.fb_h;
@@ -513,8 +534,22 @@ int rhip_device_create(int hip_device, rhip_device *out) {
   HIPCHK(hipMalloc(&d->counters, kCounterSlots * kCounterStride * sizeof(unsigned long long)));
   HIPCHK(hipMemset(d->counters, 0, kCounterSlots * kCounterStride * sizeof(unsigned long long)));
   HIPCHK(hipMalloc(&d->gamma_lut, 512 * sizeof(float)));
-  RHIP_LAUNCH(k_gamma_tables, 1, 256, d->stream, d->gamma_lut);
-  HIPCHK(hipGetLastError());
+  HIPCHK(hipMalloc(&d->errors, kErrSlots * sizeof(uint32_t)));
+  HIPCHK(hipMemset(d->errors, 0, kErrSlots * sizeof(uint32_t)));
+  {
+    float gt[512];
+    gammaTablesHost(gt);
+    HIPCHK(hipMemcpy(d->gamma_lut, gt, sizeof gt, hipMemcpyHostToDevice));
+  }
+  /* the host libm on the device: every table, built once per process */
+  {
+    int rc = libmEnsure(d.get(), (1u << kLmCount) - 1);
+    if (rc) {
+      if (d->libm) hipFree(d->libm);
+      for (void *p : d->libm_bufs) hipFree(p);
+      return rc;
+    }
+  }
   *out = d.release();
   return RHIP_OK;
 }
@@ -528,6 +563,9 @@ int rhip_device_destroy(rhip_device d) {
   d->vao.clear();
   if (d->counters) hipFree(d->counters);
   if (d->gamma_lut) hipFree(d->gamma_lut);
+  if (d->libm) hipFree(d->libm);
+  for (void *p : d->libm_bufs) hipFree(p);
+  if (d->errors) hipFree(d->errors);
   for (void *p : {(void *)d->nn.params, (void *)d->nn.grad, (void *)d->nn.m1, (void *)d->nn.m2, (void *)d->nn.packed,
                   (void *)d->nn.scratch, (void *)d->nn.hist, (void *)d->nn.rec_count})
     if (p) hipFree(p);
@@ -642,7 +680,7 @@ int rhip_texture_read(rhip_device d, int id, int layer, int level, int which, vo
     if (bytes < n) return fail(RHIP_E_INVALID, "output too small");
     HIPCHK(hipMemcpyAsync(out, t->data + lv.offset, n, hipMemcpyDeviceToHost, d->stream));
     HIPCHK(hipStreamSynchronize(d->stream));
-    return RHIP_OK;
+    return takeDeviceErrors(d);
   }
   size_t n = (size_t)lv.w * lv.h * 4;
   if (bytes < n) return fail(RHIP_E_INVALID, "output too small");
@@ -650,7 +688,7 @@ int rhip_texture_read(rhip_device d, int id, int layer, int level, int which, vo
     if (!t->resolved) return fail(RHIP_E_INVALID, "a multisampled depth texture has no single-sample image");
     HIPCHK(hipMemcpyAsync(out, t->resolved, n, hipMemcpyDeviceToHost, d->stream));
     HIPCHK(hipStreamSynchronize(d->stream));
-    return RHIP_OK;
+    return takeDeviceErrors(d);
   }
   std::vector<uint32_t> tmp((size_t)lv.inner_w * lv.inner_h);
   HIPCHK(hipMemcpyAsync(tmp.data(), t->data + lv.offset, tmp.size() * 4, hipMemcpyDeviceToHost, d->stream));
@@ -658,7 +696,7 @@ int rhip_texture_read(rhip_device d, int id, int layer, int level, int which, vo
   uint32_t *o = (uint32_t *)out;
   for (int y = 0; y < lv.h; y++)
     for (int x = 0; x < lv.w; x++) o[(size_t)y * lv.w + x] = tmp[hostTexelIndex(lv, t->hd.layout, x, y)];
-  return RHIP_OK;
+  return takeDeviceErrors(d);
 }
 
 int rhip_texture_destroy(rhip_device d, int id) {
@@ -820,10 +858,31 @@ int rhip_wait_idle(rhip_device d) {
   int rc = checkDev(d);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(d->stream));
+  return takeDeviceErrors(d);
+}
+
+int rhip_set_hierarchical_z(rhip_device d, int enabled) {
+  if (!d) return fail(RHIP_E_INVALID, "null device");
+  d->hiz_off = enabled == 0;
   return RHIP_OK;
 }
 
-int rhip_draw(rhip_device d, const rhip_draw_desc *dd) {
+int rhip_debug_raise_device_error(rhip_device d, int stage) {
+  int rc = checkDev(d);
+  if (rc) return rc;
+  if (stage != kErrClip) return fail(RHIP_E_INVALID, "no device error stage %d", stage);
+  RHIP_LAUNCH(k_set_error, 1, 1, d->stream, d->errors, stage);
+  HIPCHK(hipGetLastError());
+  return RHIP_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+/* One draw (RendererSoft::draw, RendererSoft.cpp:136-164) with its arena in
+ * the device's grow-only buffer. */
+int drawOne(rhip_device d, const rhip_draw_desc *dd) {
   int rc = checkDev(d);
   if (rc) return rc;
   if (!dd) return fail(RHIP_E_INVALID, "null draw");
@@ -835,7 +894,16 @@ int rhip_draw(rhip_device d, const rhip_draw_desc *dd) {
   if (sh.empty) return RHIP_OK;
   if ((rc = d->arena.ensure(carveArenas(P, sh, nullptr)))) return rc;
   carveArenas(P, sh, d->arena.as<char>());
-  if (sh.T == 0) return RHIP_OK;
+  if (sh.T == 0) {
+    /* no triangles: the reference still resolves a multisampled colour target (:161-163) */
+    if (P.color && P.samples == 4) {
+      Timer tm(d, "k_resolve");
+      RHIP_LAUNCH(k_resolve, (unsigned)std::min<size_t>(blocks((size_t)P.fb_w * P.fb_h, 256), 4096), 256, d->stream, P);
+      HIPCHK(hipGetLastError());
+    }
+    d->host_stats.draws++;
+    return RHIP_OK;
+  }
   const RhsStates &st = dd->states;
   const int ntiles = P.ntiles, nblk = P.nblk, fbw = P.fb_w, fbh = P.fb_h, samples = P.samples;
   const bool need_deriv = P.need_deriv != 0;
@@ -909,6 +977,12 @@ int rhip_draw(rhip_device d, const rhip_draw_desc *dd) {
   d->host_stats.draws++;
   return RHIP_OK;
 }
+
+}  // namespace
+
+extern "C" {
+
+int rhip_draw(rhip_device d, const rhip_draw_desc *dd) { return drawOne(d, dd); }
 
 int rhip_profile_enable(rhip_device d, int on) {
   int rc = checkDev(d);

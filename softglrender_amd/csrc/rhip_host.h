@@ -101,8 +101,13 @@ struct rhip_device_s {
   int vp_x = 0, vp_y = 0, vp_w = 0, vp_h = 0;
   /* per-draw arenas */
   rhip::DevBuf arena;
+  bool hiz_off = false;       /* rhip_set_hierarchical_z(dev, 0): no depth-bound culling */
   unsigned long long *counters = nullptr; /* device */
-  float *gamma_lut = nullptr;             /* device: k_gamma_tables */
+  uint32_t *errors = nullptr;             /* device: kErrSlots words set by kernels that drop work */
+  float *gamma_lut = nullptr;             /* device: gamma tables (rhip_libm.hip) */
+  rhip::LibmTables *libm = nullptr;       /* device: host libm tables (rhip_libm.h) */
+  uint32_t libm_ready = 0;                /* functions whose table is on this device */
+  std::vector<void *> libm_bufs;
   /* optional per-kernel timing: an event pair around every launch */
   bool profiling = false;
   struct Timed {
@@ -165,5 +170,16 @@ int drawSetup(rhip_device d, int fbo, const int vp[4], const rhip_draw_desc &dd,
 size_t carveArenas(DrawParams &P, const DrawShape &sh, char *base);
 
 void decodeUniforms(const rhip_draw_desc &dd, Uniforms &u);
+
+/* Puts the host libm tables of the functions in `fns` (bit per LibmFn) on the
+ * device, building them first if this process has not (rhip_libm.hip). */
+int libmEnsure(rhip_device d, uint32_t fns);
+/* L with function fn's host-side table (the others empty; builds it if needed). */
+int libmHostTables(int fn, LibmTables *L);
+/* Listed inputs and gate of function fn's table (builds it if needed). */
+int libmTableStats(int fn, long long *entries, double *gate);
+/* t[0, 256): powf(k / 255, 2.2); t[256 + k]: smallest c >= 0 whose output byte is at least k. */
+void gammaTablesHost(float *t);
+float hostPowf(float x, float y);
 
 }  // namespace rhip

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "rhip_libm.h"
 #include "rhip_stream.h"
 
 namespace rhip {
@@ -103,6 +104,11 @@ __host__ __device__ inline uint32_t varyGroups(int shading, uint32_t defines, in
   return g;
 }
 
+/* Device-side error words (DrawParams.errors): a kernel that has to drop work
+ * sets its word to 1; rhip_wait_idle and rhip_texture_read report and clear them. */
+constexpr int kErrClip = 0; /* a clipped polygon exceeded kMaxPoly vertices: the triangle was dropped */
+constexpr int kErrSlots = 4;
+
 /* PrimSetup.flags */
 constexpr int kPrimLive = 1, kPrimFront = 2, kPrimCoarse = 4, kPrimGlobal = 8;
 
@@ -113,7 +119,10 @@ struct DrawParams {
   int shading;
   uint32_t defines;
   int need_deriv; /* a sampler of the draw needs dFdx/dFdy (mipmapped minification) */
-  const float *gamma_lut; /* [0, 256): rpowf(k / 255, 2.2); [256, 512): the PBR output-byte thresholds */
+  const float *gamma_lut; /* [0, 256): powf(k / 255, 2.2); [256, 512): the PBR output-byte thresholds
+                             (host libm, rhip_libm.hip) */
+  const LibmTables *libm; /* the host libm's log2f / exp2f / powf on the device (rhip_libm.h) */
+  float base_lin[3];      /* powf(base colour, 2.2) by the host libm (PbrSoft.h:235) */
   uint32_t vary_groups; /* stored groups of 4 varying slots (varyGroups) */
   int vary_stride;      /* floats per vertex in `vary`: 4 per stored group */
   /* viewport (RendererSoft.cpp:92-113) */
@@ -163,6 +172,8 @@ struct DrawParams {
   int *hiz_count;      /* per coarse bin: big triangles that bound leaves, -1 if over kHizCap */
   int *hiz_surv;       /* per coarse bin: those triangles (kHizCap slots) */
   int *hiz_seed;       /* per coarse bin: nearest clipped polygon covering it, or -1 */
+  int *hiz_poly;       /* per coarse bin: 1 when that polygon's depth range tightened hiz_bound */
+  int hiz_off;         /* 1: no depth-bound culling at all (every triangle is rasterized; for tests) */
   /* listed ordered raster (draws that write no depth): fragments in jobs[0, dcount[3]) */
   float4 *ord_col;  /* shaded colour per listed fragment */
   int2 *ord_span;   /* per pixel of a listed tile: its fragments jobs[x, x + y), in API order */
@@ -170,6 +181,7 @@ struct DrawParams {
   int *ord_list;    /* tiles whose fragments did not fit (dcount[7] from the front) and tiles with
                        listed fragments (dcount[8] from the back) */
   int ord_cap;      /* jobs slots */
+  uint32_t *errors; /* kErrSlots words, see kErrClip */
   unsigned long long *counters; /* [0] fs invocations, [1] samples written, [2] clip overflows,
                                    [3] bin items, [4] primitives binned, [5] clipped triangles,
                                    [6] (tile, triangle) pairs skipped by the hierarchical-Z test */

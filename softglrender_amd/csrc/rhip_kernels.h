@@ -26,26 +26,6 @@ namespace rhip {
 
 /* --------------------------------------------------------------------------- shading tables */
 
-/* The output byte of a PBR colour channel c in the two-phase raster (gamma 1/2.2, clamp, truncation). */
-__device__ __forceinline__ int gammaOutExact(float c) { return u8t(gclamp(rpowf(c, 1.0f / 2.2f), 0.f, 1.f) * 255.f); }
-
-/* One block of 256: t[k] = rpowf(k / 255, 2.2) (input gamma of RGBA8 albedo), and
- * t[256 + k] = the smallest non-negative float c with gammaOutExact(c) >= k, by
- * bisection over float bit patterns (order-preserving for non-negative floats). */
-static __global__ void k_gamma_tables(float *t) {
-  const int k = threadIdx.x;
-  t[k] = rpowf((float)k / 255.f, 2.2f);
-  uint32_t lo = 0, hi = 0x7f800000u; /* +inf gives byte 255 */
-  while (lo < hi) {
-    const uint32_t mid = lo + (hi - lo) / 2;
-    if (gammaOutExact(__uint_as_float(mid)) >= k)
-      hi = mid;
-    else
-      lo = mid + 1;
-  }
-  t[256 + k] = __uint_as_float(lo);
-}
-
 /* --------------------------------------------------------------------------- XCD placement
  *
  * The MI355X deals a launch's workgroups to its eight XCDs in turn (workgroup i
@@ -532,8 +512,6 @@ __device__ __forceinline__ bool validTri(const DrawParams &P, int i0, int i1, in
 /* processPolygonAssembly (RendererSoft.cpp:428-437) + the clip test: triangles
  * with a vertex outside the frustum are set up later by k_clip. */
 __device__ __forceinline__ void setupBody(const DrawParams &P, int blk) {
-  /* the PBR shader's linear base colour, rpowf(base colour, 2.2), once per draw */
-  if (blk == 0 && threadIdx.x < 3) P.dcount[4 + threadIdx.x] = __float_as_int(rpowf(P.u.base_color[threadIdx.x], 2.2f));
   const int t = blk * blockDim.x + threadIdx.x;
   int needs = 0;
   BinRect r = {-1, 0, 0, 0, 0};
@@ -642,6 +620,7 @@ __device__ int clipTriangle(const DrawParams &P, int t) {
       /* more vertices than a convex polygon can have: rounding produced extra
        * plane crossings; the triangle is dropped and the overflow counted */
       atomicAdd(counterAt(P, 2), 1ull);
+      P.errors[kErrClip] = 1u;
       break;
     }
     for (int q = 0; q < nout; q++) in[q] = out[q];
@@ -1206,6 +1185,15 @@ __device__ __forceinline__ void hizCoarseBody(const DrawParams &P, int c) {
   const int gs = P.bin_start[gb], gn = P.bin_start[gb + 1] - gs;
   const int n = cn + gn;
   const int lp = threadIdx.x;
+  if (P.hiz_off) { /* no bound: every tile reads the whole coarse and global bins */
+    if (lp == 0) {
+      P.hiz_bound[c] = MX ? 0u : 0xffffffffu;
+      P.hiz_count[c] = -1;
+      P.hiz_seed[c] = -1;
+      P.hiz_poly[c] = 0;
+    }
+    return;
+  }
   if (lp == 0) {
     s_bound = MX ? 0u : 0xffffffffu;
     s_cnt = 0;
@@ -1223,9 +1211,12 @@ __device__ __forceinline__ void hizCoarseBody(const DrawParams &P, int c) {
     }
   }
   __syncthreads();
-  /* the nearest clipped polygon covering the whole bin: its tiles rasterize it
-   * first, and since its live fans leave a sample depth no farther than its
-   * depth range over the bin, that range bounds the bin too */
+  /* the nearest clipped polygon covering the whole bin: its depth range over
+   * the bin bounds the bin too when its live fans leave every sample of the bin
+   * covered. Fans rasterize with float barycentrics and may be culled as
+   * back-facing, so that is not certain: a tile checks it against the depths
+   * the big triangles actually leave and, where it fails, rasterizes what this
+   * bound ruled out (visTile, hiz_poly) */
   uint32_t my_pz = MX ? 0u : 0xffffffffu;
   int my_pt = -1;
   for (int i = lp; i < n; i += blockDim.x) {
@@ -1267,6 +1258,7 @@ __device__ __forceinline__ void hizCoarseBody(const DrawParams &P, int c) {
     P.hiz_bound[c] = bound;
     P.hiz_count[c] = s_cnt <= kHizCap ? s_cnt : -1;
     P.hiz_seed[c] = s_pt;
+    P.hiz_poly[c] = bound != s_bound ? 1 : 0;
   }
 }
 
@@ -1320,18 +1312,20 @@ __device__ void visTile(const DrawParams &P, int tile) {
     s_n = 0;
   }
   __syncthreads();
+  const bool hiz = !P.hiz_off;
+  const uint32_t coarse_bound = P.hiz_bound[coarse];
   /* the tile's own depths bound it too */
 #pragma unroll
   for (int q = 0; q < PPL; q++) {
     const int pl = lp + q * NT;
-    if (tx0 + pl % TILE < P.fb_w && ty0 + pl / TILE < P.fb_h) hizFromDepth<S>(dep[q], MX, &s_bound2);
+    if (hiz && tx0 + pl % TILE < P.fb_w && ty0 + pl / TILE < P.fb_h) hizFromDepth<S>(dep[q], MX, &s_bound2);
   }
   __syncthreads();
-  if (lp == 0) s_bound = MX ? max(s_bound, s_bound2) : min(s_bound, s_bound2);
+  if (lp == 0 && hiz) s_bound = MX ? max(s_bound, s_bound2) : min(s_bound, s_bound2);
   __syncthreads();
 
   /* big triangles that cover the tile tighten the bound */
-  for (int i = lp; i < nbig; i += NT) {
+  for (int i = lp; i < (hiz ? nbig : 0); i += NT) {
     const int id = bigAt(i);
     const PrimSetup ps = P.prims[id];
     float zmin, zmax;
@@ -1353,7 +1347,7 @@ __device__ void visTile(const DrawParams &P, int tile) {
    * bin without one, whose big list overflowed, look for their own. */
   const int cseed = P.hiz_seed[coarse];
   int seed = -1; /* block-uniform */
-  if (cseed < 0 && hc < 0 && nbig > 0) {
+  if (hiz && cseed < 0 && hc < 0 && nbig > 0) {
     if (lp == 0) {
       s_pz = MX ? 0u : 0xffffffffu;
       s_pt = -1;
@@ -1454,7 +1448,7 @@ __device__ void visTile(const DrawParams &P, int tile) {
     const int nb = s_n;
     for (int j = 0; j < nb; j++) {
       const uint32_t zk = big_z[j];
-      if (MX ? zk < wave_lim : zk > wave_lim) continue; /* wave-uniform */
+      if (hiz && (MX ? zk < wave_lim : zk > wave_lim)) continue; /* wave-uniform */
       const int id = big[j];
       const PrimSetup ps = P.prims[id];
       const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
@@ -1472,14 +1466,45 @@ __device__ void visTile(const DrawParams &P, int tile) {
     __syncthreads();
   }
   /* the depths after the big triangles bound the rest */
-  if (lp == 0) s_bound2 = MX ? 0xffffffffu : 0u;
-  __syncthreads();
-  if (MX)
-    atomicMin(&s_bound2, mine);
-  else
-    atomicMax(&s_bound2, mine);
-  __syncthreads();
-  const uint32_t bound2 = MX ? max(bound, s_bound2) : min(bound, s_bound2);
+  auto tileLimit = [&]() {
+    __syncthreads();
+    if (lp == 0) s_bound2 = MX ? 0xffffffffu : 0u;
+    __syncthreads();
+    if (MX)
+      atomicMin(&s_bound2, mine);
+    else
+      atomicMax(&s_bound2, mine);
+    __syncthreads();
+  };
+  tileLimit();
+  uint32_t bound2 = MX ? max(bound, s_bound2) : min(bound, s_bound2);
+  /* The coarse bound may rest on the bin's nearest covering polygon
+   * (hizCoarseBody). It holds for this tile when every sample now holds a depth
+   * at least as near as that bound: a triangle it ruled out is then farther
+   * than every sample. Where a sample is farther (a gap between the polygon's
+   * fans, a sliver fan culled as back-facing, rounding), every big triangle of
+   * the bins that the tile's actual depths do not rule out is rasterized too;
+   * those already rasterized leave their keys unchanged. */
+  if (hiz && P.hiz_poly[coarse] && (MX ? s_bound2 < coarse_bound : s_bound2 > coarse_bound)) {
+    const uint32_t actual = s_bound2;
+    const int nfull = tb.n[1] + tb.n[2];
+    for (int i = 0; i < nfull; i++) {
+      const int id = tb.at(P, tb.n[0] + i);
+      const PrimSetup ps = P.prims[id];
+      if (ps.qx0 > X1 || ps.qx1 < tx0 || ps.qy0 > Y1 || ps.qy1 < ty0 || zCulled(ps, actual, MX)) continue;
+      const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
+#pragma unroll
+      for (int q = 0; q < PPL; q++) {
+        const int pl = lp + q * NT, px = tx0 + pl % TILE, py = ty0 + pl / TILE;
+        if (px < P.fb_w && py < P.fb_h && px >= ps.qx0 && px <= ps.qx1 && py >= ps.qy0 && py <= ps.qy1)
+          visPixel<S, MODE>(ps, T, id, px, py, &keys[pl * S]);
+      }
+    }
+    mine = laneLimit();
+    tileLimit();
+    bound2 = s_bound2;
+  }
+  if (!hiz) bound2 = MX ? 0u : 0xffffffffu;
 
   /* phase 1b: the tile's own triangles. A wavefront takes 64 at a time, stages
    * what visPixel needs of each in LDS and spreads their (triangle, pixel) pairs

@@ -393,7 +393,7 @@ __device__ __forceinline__ void nnJobRow(const DrawParams &P, int j, int need_de
   const f2 uv{fr.v[0], fr.v[1]};
   const TexDesc *ta = P.tex[RHS_SLOT_ALBEDO];
   const RGBAf alb = (P.defines & RHS_DEF_ALBEDO_MAP)
-                        ? tex2D(ta, uv, lodFor(ta, fr))
+                        ? tex2D(ta, uv, lodFor(P, ta, fr))
                         : RGBAf{P.u.base_color[0], P.u.base_color[1], P.u.base_color[2], P.u.base_color[3]};
   float metal = 0.f, rough = 1.f;
   if (P.defines & RHS_DEF_METALROUGHNESS_MAP) {

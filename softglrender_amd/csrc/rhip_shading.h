@@ -30,37 +30,10 @@ __device__ __forceinline__ f3 nrm3(f3 v) { return scl3(v, 1.0f / sqrtf(dot3(v, v
 __device__ __forceinline__ f3 cross3(f3 a, f3 b) {
   return {fmaf(a.y, b.z, -(a.z * b.y)), fmaf(a.z, b.x, -(a.x * b.z)), fmaf(a.x, b.y, -(a.y * b.x))};
 }
-/* powf / log2f / exp2f as glibc's single-precision versions round them: those
- * work in double precision internally and are within ~0.5 ulp of the exact
- * value, so the double-precision result rounded to float reproduces them. */
+/* Correctly rounded powf (double precision, rounded once): the learned shader's
+ * feature encoding and inputs outside the host-libm tables (rhip_libm.h). */
 __device__ __forceinline__ float rpowf(float x, float y) { return (float)pow((double)x, (double)y); }
-__device__ __forceinline__ float rlog2f(float x) { return (float)log2((double)x); }
-__device__ __forceinline__ float rexp2f(float x) { return (float)exp2((double)x); }
 
-/* rpowf(x, N) for the integer exponents the shaders use, without the general pow
- * when it can be avoided (Ziv's strategy): the product in double carries a relative
- * error below 2^-47 (x^5: two roundings; x^128: x^2 is exact, then six squarings,
- * at most 63 double ulps). If both ends of that error interval round to the same
- * float, so does the exact power, and that float is rpowf's; otherwise (about one
- * call in a million) the general pow decides. Zero, subnormal and overflowing results
- * are covered by the same test. */
-template <int N>
-__device__ __forceinline__ float rpowi(float x) {
-  static_assert(N == 5 || N == 128, "exponent");
-  const double a = x;
-  double r;
-  if (N == 5) {
-    const double a2 = a * a;
-    r = a2 * a2 * a;
-  } else {
-    r = a;
-#pragma unroll
-    for (int i = 0; i < 7; i++) r = r * r;
-  }
-  const float f = (float)r;
-  if ((float)(r * (1.0 - 0x1p-44)) == f && (float)(r * (1.0 + 0x1p-44)) == f) return f;
-  return rpowf(x, (float)N);
-}
 __device__ __forceinline__ float gmax(float a, float b) { return a < b ? b : a; } /* glm::max */
 __device__ __forceinline__ float gmin(float a, float b) { return b < a ? b : a; } /* glm::min */
 /* glm::clamp = min(max(x, lo), hi) with glm's comparisons (keeps -0 and NaN like the reference) */
@@ -307,7 +280,7 @@ struct Frag {
 };
 
 /* ShaderSoft::getSampler2DLod (ShaderSoft.h:117-133), used only when the sampler filters with mipmaps */
-__device__ __forceinline__ float lodFor(const TexDesc *t, const Frag &f) {
+__device__ __forceinline__ float lodFor(const DrawParams &P, const TexDesc *t, const Frag &f) {
   if (!t || t->filter_min <= 1) return 0.f;
   float w = (float)t->width, h = (float)t->height;
   float dxx = (f.tc[1].x - f.tc[0].x) * w, dyx = (f.tc[2].x - f.tc[0].x) * w;
@@ -315,7 +288,7 @@ __device__ __forceinline__ float lodFor(const TexDesc *t, const Frag &f) {
   float ddx = fmaf(dxy, dxy, dxx * dxx);
   float ddy = fmaf(dyy, dyy, dyx * dyx);
   float d = ddy < ddx ? ddx : ddy;
-  float lod = 0.5f * rlog2f(d);
+  float lod = 0.5f * hlog2f(P.libm, d);
   return 0.f < lod ? lod : 0.f;
 }
 
@@ -340,7 +313,7 @@ __device__ __forceinline__ f3 normalFromMap(const DrawParams &P, const Frag &f, 
     T = nrm3(sub3(T, scl3(N, dot3(T, N))));
     f3 B = cross3(T, N);
     const TexDesc *nt = P.tex[RHS_SLOT_NORMAL];
-    RGBAf t4 = tex2D(nt, {v[0], v[1]}, lodFor(nt, f));
+    RGBAf t4 = tex2D(nt, {v[0], v[1]}, lodFor(P, nt, f));
     f3 tn{t4.r * 2.0f - 1.0f, t4.g * 2.0f - 1.0f, t4.b * 2.0f - 1.0f};
     f3 r{fmaf(N.x, tn.z, fmaf(T.x, tn.x, B.x * tn.y)), fmaf(N.y, tn.z, fmaf(T.y, tn.x, B.y * tn.y)),
          fmaf(N.z, tn.z, fmaf(T.z, tn.x, B.z * tn.y))};
@@ -377,11 +350,11 @@ __device__ __forceinline__ RGBAf fsBlinnPhong(const DrawParams &P, const Frag &f
   f2 uv{v[0], v[1]};
   const TexDesc *ta = P.tex[RHS_SLOT_ALBEDO];
   RGBAf base = (P.defines & RHS_DEF_ALBEDO_MAP)
-                   ? tex2D(ta, uv, lodFor(ta, f))
+                   ? tex2D(ta, uv, lodFor(P, ta, f))
                    : RGBAf{u.base_color[0], u.base_color[1], u.base_color[2], u.base_color[3]};
   f3 N = normalFromMap(P, f, 24, 28);
   const TexDesc *tao = P.tex[RHS_SLOT_AO];
-  float ao = (P.defines & RHS_DEF_AO_MAP) ? tex2D(tao, uv, lodFor(tao, f)).r : 1.f;
+  float ao = (P.defines & RHS_DEF_AO_MAP) ? tex2D(tao, uv, lodFor(P, tao, f)).r : 1.f;
   f3 b3{base.r, base.g, base.b};
   f3 amb = scl3(mul3(b3, {u.ambient[0], u.ambient[1], u.ambient[2]}), ao);
   f3 dif{0, 0, 0}, spe{0, 0, 0}, emi{0, 0, 0};
@@ -400,7 +373,7 @@ __device__ __forceinline__ RGBAf fsBlinnPhong(const DrawParams &P, const Frag &f
     f3 C = nrm3({v[12], v[13], v[14]});
     f3 H = nrm3(add3(L, C));
     float sa = gmax(dot3(N, H), 0.0f);
-    float sp = u.k_specular * rpowi<128>(sa);
+    float sp = u.k_specular * hpowi<128>(P.libm, sa);
     spe = {sp, sp, sp};
     if (u.enable_shadow) {
       float s = 1.0f - shadowCalc(P, make_float4(v[20], v[21], v[22], v[23]), N, ldv);
@@ -410,7 +383,7 @@ __device__ __forceinline__ RGBAf fsBlinnPhong(const DrawParams &P, const Frag &f
   }
   if (P.defines & RHS_DEF_EMISSIVE_MAP) {
     const TexDesc *te = P.tex[RHS_SLOT_EMISSIVE];
-    RGBAf e = tex2D(te, uv, lodFor(te, f));
+    RGBAf e = tex2D(te, uv, lodFor(P, te, f));
     emi = {e.r, e.g, e.b};
   }
   f3 col = add3(add3(add3(amb, dif), spe), emi);
@@ -431,27 +404,28 @@ __device__ __forceinline__ float schlickGGX(float nv, float r) {
   float k = ((r + 1.0f) * (r + 1.0f)) / 8.0f;
   return nv / (nv * (1.0f - k) + k);
 }
-__device__ __forceinline__ f3 envBRDFApprox(f3 spec, float r, float nv) {
+__device__ __forceinline__ f3 envBRDFApprox(const DrawParams &P, f3 spec, float r, float nv) {
   float rx = r * -1.f + 1.f, ry = r * -0.0275f + 0.0425f, rz = r * -0.572f + 1.04f, rw = r * 0.022f + -0.04f;
-  float a004 = gmin(rx * rx, rexp2f(-9.28f * nv)) * rx + ry;
+  float a004 = gmin(rx * rx, hexp2f(P.libm, -9.28f * nv)) * rx + ry;
   float ax = -1.04f * a004 + rz, ay = 1.04f * a004 + rw;
   ay *= gmax(0.f, gmin(1.f, 50.0f * spec.y));
   return {spec.x * ax + ay, spec.y * ax + ay, spec.z * ax + ay};
 }
 
-/* rpowf(x, 2.2) of a channel sampled from an RGBA8 texture: x is k / 255 for a
- * byte k, and the device computed rpowf of all 256 once (k_gamma_tables). */
+/* powf(x, 2.2) of a channel sampled from an RGBA8 texture: x is k / 255 for a
+ * byte k, and the host libm's values of all 256 are in the gamma table. */
 __device__ __forceinline__ float gammaIn(const DrawParams &P, float x) {
   return P.gamma_lut[(int)(x * 255.f + 0.5f)];
 }
 
 /* The byte the two-phase raster writes for a PBR colour channel c without an
- * emissive map: u8(clamp(rpowf(c, 1/2.2), 0, 1) * 255), monotonic in c. The
- * device found, with rpowf itself, the smallest c giving each byte k
- * (k_gamma_tables); a hardware log2/exp2 estimate picks the starting byte and
- * the thresholds correct it. Returned as a float that truncates back to k. */
+ * emissive map: u8(clamp(powf(c, 1/2.2), 0, 1) * 255), which the host libm makes
+ * monotonic in c (checked over every float of (0, 1] when the table is built).
+ * The host found the smallest c giving each byte k (gammaTablesHost); a
+ * hardware log2/exp2 estimate picks the starting byte and the thresholds
+ * correct it. Returned as a float that truncates back to k. */
 __device__ __forceinline__ float gammaOutByte(const DrawParams &P, float c) {
-  if (!(c > 0.f) || !(c < INFINITY)) return rpowf(c, 1.0f / 2.2f);
+  if (!(c > 0.f) || !(c < INFINITY)) return hpowGamma(P.libm, c);
   const float *T = P.gamma_lut + 256;
   const float e = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(c) * (1.0f / 2.2f));
   int k = (int)(gmin(e, 1.f) * 255.f);
@@ -476,12 +450,12 @@ __device__ __forceinline__ PbrInputs pbrInputs(const DrawParams &P, const Frag &
   f2 uv{v[0], v[1]};
   const TexDesc *ta = P.tex[RHS_SLOT_ALBEDO];
   if (P.defines & RHS_DEF_ALBEDO_MAP) {
-    in.albedo_rgba = tex2D(ta, uv, lodFor(ta, f));
+    in.albedo_rgba = tex2D(ta, uv, lodFor(P, ta, f));
     in.albedo = {gammaIn(P, in.albedo_rgba.r), gammaIn(P, in.albedo_rgba.g), gammaIn(P, in.albedo_rgba.b)};
   } else {
-    /* rpowf(base colour, 2.2), computed once per draw by k_bin_scan */
+    /* powf(base colour, 2.2), by the host libm once per draw */
     in.albedo_rgba = RGBAf{u.base_color[0], u.base_color[1], u.base_color[2], u.base_color[3]};
-    in.albedo = {__int_as_float(P.dcount[4]), __int_as_float(P.dcount[5]), __int_as_float(P.dcount[6])};
+    in.albedo = {P.base_lin[0], P.base_lin[1], P.base_lin[2]};
   }
   in.metallic = 0.0f;
   in.rough = 1.0f;
@@ -492,7 +466,7 @@ __device__ __forceinline__ PbrInputs pbrInputs(const DrawParams &P, const Frag &
     in.rough = mr.g;
   }
   const TexDesc *tao = P.tex[RHS_SLOT_AO];
-  in.ao = (P.defines & RHS_DEF_AO_MAP) ? tex2D(tao, uv, lodFor(tao, f)).r : 1.f;
+  in.ao = (P.defines & RHS_DEF_AO_MAP) ? tex2D(tao, uv, lodFor(P, tao, f)).r : 1.f;
   in.N = normalFromMap(P, f, 20, 24);
   in.V = nrm3({v[12], v[13], v[14]});
   f3 I{-in.V.x, -in.V.y, -in.V.z};
@@ -523,7 +497,7 @@ __device__ __forceinline__ RGBAf fsPbr(const DrawParams &P, const Frag &f) {
     float nv = gmax(dot3(N, V), 0.0f), nl = gmax(dot3(N, L), 0.0f);
     float G = schlickGGX(nl, rough) * schlickGGX(nv, rough);
     float ct = gmax(dot3(H, V), 0.0f);
-    float p5 = rpowi<5>(gclamp(1.0f - ct, 0.0f, 1.0f));
+    float p5 = hpowi<5>(P.libm, gclamp(1.0f - ct, 0.0f, 1.0f));
     f3 F{F0.x + (1.0f - F0.x) * p5, F0.y + (1.0f - F0.y) * p5, F0.z + (1.0f - F0.z) * p5};
     float ng = NDF * G;
     f3 num = scl3(F, ng);
@@ -536,7 +510,7 @@ __device__ __forceinline__ RGBAf fsPbr(const DrawParams &P, const Frag &f) {
   f3 ambient;
   if (u.enable_ibl) {
     float nv = gmax(dot3(N, V), 0.0f);
-    float p5 = rpowi<5>(gclamp(1.0f - nv, 0.0f, 1.0f));
+    float p5 = hpowi<5>(P.libm, gclamp(1.0f - nv, 0.0f, 1.0f));
     float omr = 1.0f - rough;
     f3 F{F0.x + (gmax(omr, F0.x) - F0.x) * p5, F0.y + (gmax(omr, F0.y) - F0.y) * p5,
          F0.z + (gmax(omr, F0.z) - F0.z) * p5};
@@ -544,7 +518,7 @@ __device__ __forceinline__ RGBAf fsPbr(const DrawParams &P, const Frag &f) {
     RGBAf irr = texCube(P.tex[RHS_SLOT_IRRADIANCE], N, 0.f);
     f3 diffuse{irr.r * albedo.x, irr.g * albedo.y, irr.b * albedo.z};
     RGBAf pf = texCube(P.tex[RHS_SLOT_PREFILTER], in.R, rough * 4.0f);
-    f3 brdf = envBRDFApprox(F, rough, nv);
+    f3 brdf = envBRDFApprox(P, F, rough, nv);
     f3 spec{pf.r * brdf.x, pf.g * brdf.y, pf.b * brdf.z};
     ambient = scl3(add3(mul3(kD, diffuse), spec), in.ao);
   } else {
@@ -554,10 +528,10 @@ __device__ __forceinline__ RGBAf fsPbr(const DrawParams &P, const Frag &f) {
   if (U8OUT && !(P.defines & RHS_DEF_EMISSIVE_MAP))
     col = {gammaOutByte(P, col.x), gammaOutByte(P, col.y), gammaOutByte(P, col.z)};
   else
-    col = {rpowf(col.x, 1.0f / 2.2f), rpowf(col.y, 1.0f / 2.2f), rpowf(col.z, 1.0f / 2.2f)};
+    col = {hpowGamma(P.libm, col.x), hpowGamma(P.libm, col.y), hpowGamma(P.libm, col.z)};
   if (P.defines & RHS_DEF_EMISSIVE_MAP) {
     const TexDesc *te = P.tex[RHS_SLOT_EMISSIVE];
-    RGBAf e = tex2D(te, {v[0], v[1]}, lodFor(te, f));
+    RGBAf e = tex2D(te, {v[0], v[1]}, lodFor(P, te, f));
     col = add3(col, {e.r, e.g, e.b});
   }
   return {col.x, col.y, col.z, in.albedo_rgba.a};

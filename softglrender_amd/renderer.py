@@ -382,6 +382,10 @@ class RendererHIP:
     def waitIdle(self):
         _lib.check(self.lib.rhip_wait_idle(self.dev))
 
+    def setHierarchicalZ(self, enabled: bool):
+        """Depth-bound culling on (default) or off for later draws (rhip_set_hierarchical_z)."""
+        _lib.check(self.lib.rhip_set_hierarchical_z(self.dev, 1 if enabled else 0))
+
     # measurement helpers (not part of the reference interface)
     def stats(self) -> dict:
         s = _lib.Stats()

@@ -269,6 +269,34 @@ def scene_c1(width=640, height=480, seed=1, msaa=False, lon=64, lat=32, read_msa
     return Scene("c1", st, width, height, msaa, dict(triangles=int(idx.shape[0]), vertices=int(verts.shape[0])))
 
 
+def scene_msaa_empty_draw(width=96, height=64):
+    """A 4x-MSAA frame of a shaded sphere, then a second pass that clears to a colour and draws a vertex array
+    with no indices: the reference resolves after every draw to a multisampled colour target, even one with no
+    triangles (RendererSoft.cpp:161-163), so the resolved image read last is the clear colour."""
+    st = S.Stream()
+    verts, idx = uv_sphere(16, 8)
+    st.vao(1, verts, idx)
+    st.vao(2, verts, np.zeros(0, np.int32))
+    st.texture(2, width, height, usage=S.USAGE_COLOR | S.USAGE_SAMPLER, msaa=True)
+    st.tex_init(2)
+    st.texture(3, width, height, format=S.FMT_FLOAT32, usage=S.USAGE_DEPTH | S.USAGE_SAMPLER, msaa=True)
+    st.tex_init(3)
+    st.fbo(1, color=2, depth=3)
+    st.frame()
+    eye = (0.0, 0.0, 3.0)
+    mvp = projection(math.radians(60.0), width / height, 0.1) @ look_at(eye, (0, 0, 0), (0, 1, 0))
+    args = (_common_states(S.DEPTH_LESS), S.uniforms_model(np.eye(4), mvp),
+            S.uniforms_scene((0.2, 0.2, 0.2), eye, (1.0, 1.0, 1.0), (0.8, 0.8, 0.8)),
+            S.uniforms_material(1, 0, 0, 1.0, 0.5, (1, 1, 1, 1)))
+    for vao, clear in ((1, (0, 0, 0, 0)), (2, (0.25, 0.5, 0.75, 1.0))):
+        st.begin(1, True, True, clear, 1.0)
+        st.viewport(0, 0, width, height)
+        st.draw(vao, S.SHADING_BLINN_PHONG_DIR, 0, *args)
+        st.end()
+    st.read(2, which=0)
+    return Scene("msaa_empty_draw", st, width, height, True)
+
+
 def scene_c4(n_scenes=64, n_views=256, sizes=(256, 512, 1024), tri_lo=1_000, tri_hi=500_000, tex_size=1024,
              seed=9, reads=False):
     """C4: `n_views` views of `n_scenes` seeded scenes. Scene i is a displaced icosphere with a
@@ -519,21 +547,25 @@ def random_triangles(n, width, height, seed, area_lo=0.25, area_hi=16.0, z_lo=0.
     return v, np.arange(m * 3, dtype=np.int32).reshape(-1, 3)
 
 
-def scene_c3(width=3840, height=2160, n=2_000_000, big=2000, seed=5, msaa=True, read_samples=False):
+def scene_c3(width=3840, height=2160, n=2_000_000, big=2000, seed=5, msaa=True, read_samples=False, crop=None):
     """C3 binning/raster stress: `n` random triangles (screen area log-uniform in 0.25-16 px, centres uniform,
     NDC z uniform in [0, 1]) and `big` screen-covering triangles at random depths after them; per-vertex RGBA32F
-    colours, depth LESS, no culling, 3840x2160 MSAA-4x."""
+    colours, depth LESS, no culling, 3840x2160 MSAA-4x.
+
+    crop=(x0, y0, w, h) renders the same triangles into a w x h framebuffer through the viewport
+    (-x0, -y0, width, height): the window of the full frame at (x0, y0), every triangle still drawn."""
     v, i = random_triangles(n, width, height, seed, 0.25, 16.0, 0.0, 1.0, big=big)
+    fx, fy, fw, fh = crop if crop else (0, 0, width, height)
     st = S.Stream()
-    st.texture(2, width, height, usage=S.USAGE_COLOR, msaa=msaa)
+    st.texture(2, fw, fh, usage=S.USAGE_COLOR, msaa=msaa)
     st.tex_init(2)
-    st.texture(3, width, height, format=S.FMT_FLOAT32, usage=S.USAGE_DEPTH, msaa=msaa)
+    st.texture(3, fw, fh, format=S.FMT_FLOAT32, usage=S.USAGE_DEPTH, msaa=msaa)
     st.tex_init(3)
     st.fbo(1, color=2, depth=3)
     st.vao(1, v, i)
     st.frame()
     st.begin(1, True, True, (0, 0, 0, 0), 1.0)
-    st.viewport(0, 0, width, height)
+    st.viewport(-fx, -fy, width, height)
     rs = S.RenderStates(depth_test=True, depth_func=S.DEPTH_LESS)
     st.draw(1, S.SHADING_VERTEX_COLOR, 0, rs, S.uniforms_model(np.eye(4), np.eye(4)), bytes(64), bytes(48))
     st.end()
@@ -541,7 +573,7 @@ def scene_c3(width=3840, height=2160, n=2_000_000, big=2000, seed=5, msaa=True, 
     st.read(3, which=1)
     if read_samples:
         st.read(2, which=1)
-    return Scene("c3", st, width, height, msaa, {"triangles": n + big})
+    return Scene("c3", st, fw, fh, msaa, {"triangles": n + big})
 
 
 VARIANTS = ["blend_alpha", "blend_add_one", "blend_minmax", "blend_sub", "depth_lequal", "depth_gequal_rz",

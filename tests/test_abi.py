@@ -13,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared():
     text = open(os.path.join(ROOT, "include", "rhip.h")).read()
-    return sorted(set(re.findall(r"^(?:int|const char \*|void \*)\s*(rhip_\w+)\(", text, re.M)))
+    return sorted(set(re.findall(r"^(?:int|size_t|const char \*|void \*)\s*(rhip_\w+)\(", text, re.M)))
 
 
 def test_header_declares_the_renderer_interface():
@@ -39,4 +39,5 @@ def test_invalid_arguments_fail_cleanly_without_a_device():
     assert lib.rhip_device_create(0, None) == -1
     assert b"null" in lib.rhip_last_error()
     assert lib.rhip_draw(None, None) == -1
+    assert lib.rhip_debug_raise_device_error(None, 0) == -1
     assert lib.rhip_texture_create(None, None, None) == -1

@@ -1,10 +1,13 @@
 """RendererHIP on the GPU against the CPU checker and the reference's golden
 outputs, through librhip's C-ABI (softglrender_amd.renderer drives it).
 
-Against the checker with correctly rounded powf/log2f/exp2f (the GPU path's
-arithmetic) every output must be identical bit for bit. Against the
-reference's fixtures (which used the host libm's ~0.8-ulp versions) depth must
-be identical and colour within 1/255."""
+The GPU reproduces the host libm's powf/log2f/exp2f (rhip_libm.h), so against
+the checker calling the host libm like the reference every output must be
+identical bit for bit. Against the reference's own outputs (fixtures, and the
+full-size C2 frame rendered live where the reference is built) depth must be
+identical and colour within 1/255, with at most a handful of values off by one
+(contraction choices of the reference's compiler that the checker does not
+replicate)."""
 import os
 
 import numpy as np
@@ -14,7 +17,7 @@ import oracle
 from softglrender_amd import scenes
 from softglrender_amd.replay import replay
 from tests.golden_scenes import GOLDEN
-from tests.util import assert_parity, compare, load_golden
+from tests.util import assert_parity, compare, differing_values, load_golden
 
 GOLDEN_DIR = os.path.join(os.path.dirname(__file__), "golden")
 pytestmark = pytest.mark.gpu
@@ -26,7 +29,7 @@ def _hip(renderer, sc, repeats=1):
 
 
 def _oracle(sc):
-    out, _ = oracle.cpu_ref_replay(sc.stream.bytes(), sc.stream.output_bytes(), exact_math=True)
+    out, _ = oracle.cpu_ref_replay(sc.stream.bytes(), sc.stream.output_bytes(), exact_math=False)
     return sc.stream.split_outputs(out)
 
 
@@ -42,13 +45,23 @@ def test_hip_matches_reference_fixture(renderer, name):
     sc = GOLDEN[name]()
     print(f"[gpu] fixture {name}", flush=True)
     got = _hip(renderer, sc)
-    assert_parity(got, load_golden(os.path.join(GOLDEN_DIR, f"{name}.npz")), name)
+    want = load_golden(os.path.join(GOLDEN_DIR, f"{name}.npz"))
+    assert_parity(got, want, name)
+    assert differing_values(got, want) <= 1, name
 
 
 @pytest.mark.parametrize("name", sorted(GOLDEN))
 def test_hip_identical_to_checker(renderer, name):
     sc = GOLDEN[name]()
     assert_identical(_hip(renderer, sc), _oracle(sc), name)
+
+
+def test_hip_empty_draw_resolves(renderer):
+    # after a frame with a sphere, a clear and a draw with no indices: the resolved image is the clear colour
+    sc = scenes.scene_msaa_empty_draw()
+    got = _hip(renderer, sc)
+    assert_identical(got, _oracle(sc), "msaa_empty_draw")
+    assert np.all(got[0].reshape(-1, 4) == np.array([63, 127, 191, 255], np.uint8))
 
 
 @pytest.mark.parametrize("seed", [5, 6])
@@ -83,6 +96,21 @@ def test_hip_c2_full_size(renderer):
     dmis, cmax, cfrac = compare(got, want, "c2")
     print(f"[gpu] c2: depth mismatches {dmis}, colour max {cmax}, colour diff fraction {cfrac:.2e}", flush=True)
     assert_identical(got, want, "c2")
+
+
+@pytest.mark.skipif(not oracle.have_reference(), reason="the reference RendererSoft (oracle/_ref) is not built here")
+def test_hip_c2_full_size_against_reference(renderer):
+    """The full C2 frame against SoftGLRender's own RendererSoft (serial build) on the same inputs: the
+    north star's bar, depth identical, colour within 1/255, at most one value in a million off by one."""
+    print("[gpu] c2 full size against the reference", flush=True)
+    sc = scenes.scene_c2(read_shadow=True, read_samples=True)
+    got = _hip(renderer, sc)
+    ref, _ = oracle.ref_replay(sc.stream.bytes(), sc.stream.output_bytes())
+    want = sc.stream.split_outputs(ref)
+    dmis, cmax, cfrac = compare(got, want, "c2")
+    n = differing_values(got, want)
+    print(f"[gpu] c2 vs reference: depth mismatches {dmis}, colour max {cmax}, differing values {n}", flush=True)
+    assert dmis == 0 and cmax <= 1 and cfrac <= 1e-6, (dmis, cmax, cfrac)
 
 
 def test_random_triangles_with_screen_covering(renderer):
@@ -120,7 +148,7 @@ def test_c3_shape_reduced_with_hierarchical_z(renderer):
     got = _hip(renderer, sc)
     st = renderer.stats()
     out, _ = oracle.cpu_ref_replay(sc.stream.bytes(), sc.stream.output_bytes(), threads=min(16, os.cpu_count()),
-                                   exact_math=True)
+                                   exact_math=False)
     assert_identical(got, sc.stream.split_outputs(out), "c3_reduced")
     assert st["hiz_culled"] > st["tile_pairs"] // 2, st
 

@@ -12,7 +12,7 @@ import pytest
 import oracle
 from softglrender_amd import scenes
 from tests.golden_scenes import GOLDEN
-from tests.util import assert_parity, load_golden
+from tests.util import assert_parity, compare, load_golden
 
 GOLDEN_DIR = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -41,6 +41,29 @@ def test_oracle_matches_live_reference_new_seed(name):
     sc = scenes.scene_variant(name, width=136, height=96, seed=97)
     ref, _ = oracle.ref_replay(sc.stream.bytes(), sc.stream.output_bytes())
     assert_parity(_oracle(sc), sc.stream.split_outputs(ref), name)
+
+
+def test_empty_draw_resolves_to_the_clear_colour():
+    # a draw with no triangles still resolves a multisampled colour target (RendererSoft.cpp:161-163)
+    sc = scenes.scene_msaa_empty_draw()
+    (color,) = _oracle(sc)
+    assert np.all(color.reshape(-1, 4) == np.array([63, 127, 191, 255], np.uint8))
+    if oracle.have_reference():
+        ref, _ = oracle.ref_replay(sc.stream.bytes(), sc.stream.output_bytes())
+        assert np.array_equal(sc.stream.split_outputs(ref)[0], color)
+
+
+@pytest.mark.skipif(not oracle.have_reference(), reason="the reference RendererSoft (oracle/_ref) is not built here")
+def test_oracle_c2_full_size_against_reference():
+    """The full C2 frame (1080p MSAA-4x, 1M-triangle PBR+IBL, translucent sphere, 2048^2 shadow pass): the
+    checker calling the host libm against RendererSoft itself. Depth and the shadow map identical; colour within
+    1/255, off by one in at most one value in a million (the reference compiler's contraction choices)."""
+    sc = scenes.scene_c2(read_shadow=True, read_samples=True)
+    ref, _ = oracle.ref_replay(sc.stream.bytes(), sc.stream.output_bytes())
+    got = _oracle(sc, threads=min(8, os.cpu_count() or 1))
+    want = sc.stream.split_outputs(ref)
+    dmis, cmax, cfrac = compare(got, want, "c2")
+    assert dmis == 0 and cmax <= 1 and cfrac <= 1e-6, (dmis, cmax, cfrac)
 
 
 def test_threaded_oracle_equals_serial():

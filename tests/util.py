@@ -16,6 +16,11 @@ def compare(got, want, label=""):
     return dmis, cmax, cfrac
 
 
+def differing_values(got, want):
+    """Colour values (bytes of RGBA8 outputs) that differ."""
+    return sum(int((g != w).sum()) for g, w in zip(got, want) if g.dtype == np.uint8)
+
+
 def assert_parity(got, want, label="", colour_frac=1e-4):
     """The parity bar: depth bit-exact, colour within 1/255 per channel on every
     value and exact on all but `colour_frac` of them."""
