@@ -144,6 +144,16 @@ int rhip_program_create(rhip_device dev, int shading, uint32_t defines, int *out
 int rhip_begin_render_pass(rhip_device dev, int fbo, const rhip_clear_states *clear); /* beginRenderPass (Renderer.h:50) */
 int rhip_set_viewport(rhip_device dev, int x, int y, int width, int height);          /* setViewPort (Renderer.h:51) */
 int rhip_draw(rhip_device dev, const rhip_draw_desc *draw);                           /* set*(...) + draw (Renderer.h:52-56) */
+/* rhip_draw without any device allocation. No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). The draw's arena lives in a
+ * static device buffer of rhip_draw_noalloc_capacity() bytes (64 MiB) that
+ * every call on one GPU shares and that keeps the last call's contents: calls
+ * that overlap on one GPU (two devices' streams, or another thread) race. A
+ * draw whose worst-case arena exceeds it fails with RHIP_E_UNSUPPORTED (about
+ * 4k triangles at 640x480 with 4x MSAA fit). Results equal rhip_draw's. */
+int rhip_draw_noalloc(rhip_device dev, const rhip_draw_desc *draw);
+size_t rhip_draw_noalloc_capacity(void);
 int rhip_end_render_pass(rhip_device dev);                                            /* endRenderPass (Renderer.h:57) */
 /* waitIdle (Renderer.h:58). Also returns RHIP_E_DEVICE, naming the stage, when a
  * kernel dropped work since the last wait or readback (a clipped polygon over

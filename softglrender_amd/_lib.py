@@ -100,6 +100,8 @@ SIGNATURES = {
     "rhip_begin_render_pass": [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ClearStates)],
     "rhip_set_viewport": [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "rhip_draw": [ctypes.c_void_p, ctypes.POINTER(DrawDesc)],
+    "rhip_draw_noalloc": [ctypes.c_void_p, ctypes.POINTER(DrawDesc)],
+    "rhip_draw_noalloc_capacity": [],
     "rhip_end_render_pass": [ctypes.c_void_p],
     "rhip_wait_idle": [ctypes.c_void_p],
     "rhip_debug_raise_device_error": [ctypes.c_void_p, ctypes.c_int],
@@ -156,6 +158,7 @@ def load(path: str = LIB_PATH):
         fn.restype = ctypes.c_int
     lib.rhip_last_error.restype = ctypes.c_char_p
     lib.rhip_device_stream.restype = ctypes.c_void_p
+    lib.rhip_draw_noalloc_capacity.restype = ctypes.c_size_t
     _lib = lib
     return lib
 

@@ -163,6 +163,14 @@ void rhip::decodeUniforms(const rhip_draw_desc &dd, Uniforms &u) {
 
 namespace {
 
+/* Scratch of rhip_draw_noalloc: the arena of a draw that may not allocate
+ * (vertex slots, setup records, bins, fragment jobs), one copy per GPU, held
+ * until the process ends. Host code never uses the name itself for device
+ * memory: each device's address comes from hipGetSymbolAddress at
+ * rhip_device_create (rhip_device_s::scratch). */
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
 /* After a synchronisation: reports (RHIP_E_DEVICE, naming the stage) and
  * clears what the kernels' error words recorded since the last check. */
 int takeDeviceErrors(rhip_device d) {
@@ -196,7 +204,7 @@ struct Timer {
   const char *name;
   hipEvent_t a = nullptr;
   Timer(rhip_device dev, const char *n) : d(dev), name(n) {
-    if (d->profiling) {
+    if (d->profiling && !d->no_alloc_call) {
       a = takeEvent(d);
       hipEventRecord(a, d->stream);
     }
@@ -531,6 +539,7 @@ int rhip_device_create(int hip_device, rhip_device *out) {
   std::unique_ptr<rhip_device_s> d(new rhip_device_s());
   d->hip_device = hip_device;
   HIPCHK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+  HIPCHK(hipGetSymbolAddress((void **)&d->scratch, HIP_SYMBOL(scratch)));
   HIPCHK(hipMalloc(&d->counters, kCounterSlots * kCounterStride * sizeof(unsigned long long)));
   HIPCHK(hipMemset(d->counters, 0, kCounterSlots * kCounterStride * sizeof(unsigned long long)));
   HIPCHK(hipMalloc(&d->gamma_lut, 512 * sizeof(float)));
@@ -881,8 +890,9 @@ int rhip_debug_raise_device_error(rhip_device d, int stage) {
 namespace {
 
 /* One draw (RendererSoft::draw, RendererSoft.cpp:136-164) with its arena in
- * the device's grow-only buffer. */
-int drawOne(rhip_device d, const rhip_draw_desc *dd) {
+ * the device's grow-only buffer, or, for no_alloc, in the library's static
+ * scratch of kScratchBytes. */
+int drawOne(rhip_device d, const rhip_draw_desc *dd, bool no_alloc) {
   int rc = checkDev(d);
   if (rc) return rc;
   if (!dd) return fail(RHIP_E_INVALID, "null draw");
@@ -892,8 +902,19 @@ int drawOne(rhip_device d, const rhip_draw_desc *dd) {
   DrawShape sh;
   if ((rc = drawSetup(d, d->cur_fbo, vp, *dd, kTile, P, sh))) return rc;
   if (sh.empty) return RHIP_OK;
-  if ((rc = d->arena.ensure(carveArenas(P, sh, nullptr)))) return rc;
-  carveArenas(P, sh, d->arena.as<char>());
+  const size_t need = carveArenas(P, sh, nullptr);
+  if (no_alloc) {
+    if (need > kScratchBytes)
+      return fail(RHIP_E_UNSUPPORTED,
+                  "rhip_draw_noalloc: this draw needs %zu bytes of scratch, more than the %zu the library holds "
+                  "(%zu vertices, %zu triangles, %dx%d x%d samples); use rhip_draw",
+                  need, kScratchBytes, sh.V, sh.T, P.fb_w, P.fb_h, P.samples);
+    if (!d->scratch) return fail(RHIP_E_HIP, "rhip_draw_noalloc: no scratch address for this device");
+    carveArenas(P, sh, d->scratch);
+  } else {
+    if ((rc = d->arena.ensure(need))) return rc;
+    carveArenas(P, sh, d->arena.as<char>());
+  }
   if (sh.T == 0) {
     /* no triangles: the reference still resolves a multisampled colour target (:161-163) */
     if (P.color && P.samples == 4) {
@@ -982,7 +1003,17 @@ int drawOne(rhip_device d, const rhip_draw_desc *dd) {
 
 extern "C" {
 
-int rhip_draw(rhip_device d, const rhip_draw_desc *dd) { return drawOne(d, dd); }
+int rhip_draw(rhip_device d, const rhip_draw_desc *dd) { return drawOne(d, dd, false); }
+
+int rhip_draw_noalloc(rhip_device d, const rhip_draw_desc *dd) {
+  if (!d) return fail(RHIP_E_INVALID, "null device");
+  d->no_alloc_call = true;
+  const int rc = drawOne(d, dd, true);
+  d->no_alloc_call = false;
+  return rc;
+}
+
+size_t rhip_draw_noalloc_capacity(void) { return kScratchBytes; }
 
 int rhip_profile_enable(rhip_device d, int on) {
   int rc = checkDev(d);

@@ -101,6 +101,10 @@ struct rhip_device_s {
   int vp_x = 0, vp_y = 0, vp_w = 0, vp_h = 0;
   /* per-draw arenas */
   rhip::DevBuf arena;
+  /* device address of this GPU's copy of the library's static draw scratch
+   * (rhip_draw_noalloc), from hipGetSymbolAddress at creation */
+  char *scratch = nullptr;
+  bool no_alloc_call = false; /* inside rhip_draw_noalloc: no event timers */
   bool hiz_off = false;       /* rhip_set_hierarchical_z(dev, 0): no depth-bound culling */
   unsigned long long *counters = nullptr; /* device */
   uint32_t *errors = nullptr;             /* device: kErrSlots words set by kernels that drop work */

@@ -238,6 +238,8 @@ class RendererHIP:
         self._batch = None  # views being recorded by beginBatch / endBatch
         self._pass = None
         self._vp = (0, 0, 0, 0)
+        # draw() through rhip_draw_noalloc (the arena in the library's static scratch) instead of rhip_draw
+        self.no_alloc = False
 
     def type(self):
         return RENDERER_HIP
@@ -333,7 +335,8 @@ class RendererHIP:
             self._batch.append((_lib.BatchView(fbo, (ctypes.c_int * 4)(*self._vp), cs, dd), keep))
             self._pass[1] = _lib.ClearStates()  # later draws of the pass clear nothing
             return
-        _lib.check(self.lib.rhip_draw(self.dev, ctypes.byref(dd)))
+        fn = self.lib.rhip_draw_noalloc if self.no_alloc else self.lib.rhip_draw
+        _lib.check(fn(self.dev, ctypes.byref(dd)))
 
     def _draw_desc(self):
         rs = self._ps.renderStates

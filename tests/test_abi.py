@@ -39,5 +39,7 @@ def test_invalid_arguments_fail_cleanly_without_a_device():
     assert lib.rhip_device_create(0, None) == -1
     assert b"null" in lib.rhip_last_error()
     assert lib.rhip_draw(None, None) == -1
+    assert lib.rhip_draw_noalloc(None, None) == -1
     assert lib.rhip_debug_raise_device_error(None, 0) == -1
+    assert lib.rhip_draw_noalloc_capacity() == 64 << 20
     assert lib.rhip_texture_create(None, None, None) == -1
