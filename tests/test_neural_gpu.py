@@ -48,7 +48,8 @@ def test_mfma_forward_matches_numpy_network(renderer):
     want, _ = nn.forward(p.astype(np.float64), nn.features(rec), bf=True)
     err = np.abs(out - want)
     print(f"[gpu] forward: max |gpu - numpy| {err.max():.2e}, mean {err.mean():.2e}", flush=True)
-    assert err.max() < 2e-2 and err.mean() < 1e-3
+    # bf16 operands, FP32 accumulation on both sides: only summation order differs
+    assert err.max() < 1e-4 and err.mean() < 1e-5
 
 
 def test_mfma_gradient_matches_numpy_network(renderer):
@@ -61,11 +62,11 @@ def test_mfma_gradient_matches_numpy_network(renderer):
     gw, lw = nn.gradient(p.astype(np.float64), nn.features(rec), rec[:, nn.REC_TARGET:nn.REC_TARGET + 3], bf=True)
     rel = np.linalg.norm(g - gw) / np.linalg.norm(gw)
     print(f"[gpu] gradient: relative L2 error {rel:.2e}, loss {loss:.6f} vs {lw:.6f}", flush=True)
-    assert abs(loss - lw) <= 1e-3 * lw
-    assert rel < 1e-2
+    assert abs(loss - lw) <= 1e-5 * lw
+    assert rel < 1e-5
     for a, b in ((nn.OFF_W1, nn.OFF_B1), (nn.OFF_B1, nn.OFF_W2), (nn.OFF_W2, nn.OFF_B2), (nn.OFF_B2, nn.OFF_W3),
                  (nn.OFF_W3, nn.OFF_B3), (nn.OFF_B3, nn.N_PARAMS)):
-        assert np.linalg.norm(g[a:b] - gw[a:b]) <= 2e-2 * np.linalg.norm(gw[a:b]) + 1e-6, (a, b)
+        assert np.linalg.norm(g[a:b] - gw[a:b]) <= 1e-4 * np.linalg.norm(gw[a:b]) + 1e-9, (a, b)
 
 
 def _small_views(n_cams, seed, materials, normal, w=320, h=180):
@@ -94,7 +95,8 @@ def test_capture_targets_match_numpy_fp64_shader(renderer, small_setup):
     print(f"[gpu] capture: {count} records, FP64 target vs numpy max {d.max():.2e}; plain GPU vs FP64 mean "
           f"{plain.mean():.3f}/255 max {plain.max():.2f}/255", flush=True)
     assert d.max() < 1e-6
-    assert plain.mean() < 1.0
+    # the plain shader truncates to bytes (texels and output): within about a byte of the FP64 value
+    assert plain.mean() < 1.0 and np.percentile(plain, 99) < 2.0 and plain.max() < 8.0
 
 
 def test_training_gains_over_untrained_network(renderer, small_setup):
