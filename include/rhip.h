@@ -237,9 +237,11 @@ int rhip_stats_reset(rhip_device dev);
  * (ShaderSoft.h:132, PbrSoft.h:201-235,313, BlinnPhongSoft.h:227); the device
  * reproduces them bit for bit from a table of the inputs where its own
  * rounding differs, built from the host libm at rhip_device_create.
- * fn: 0 log2f(x), 1 exp2f(-x), 2 powf(x, 5), 3 powf(x, 128), 4 powf(x, 1/2.2f).
+ * fn: 0 log2f(x), 1 exp2f(-x), 2 powf(x, 5), 3 powf(x, 128), 4 powf(x, 1/2.2f),
+ * 5 atan2f(y, x) (SkyboxSoft.h:84; no table: the algorithm glibc uses,
+ * step for step), where x holds n (y, x) pairs, 2n floats.
  * rhip_libm_info (no device needed) builds fn's table on the host and gives
- * its listed inputs and gate; rhip_libm_eval evaluates fn on the device. */
+ * its listed inputs and gate (fn 0-4); rhip_libm_eval evaluates fn on the device. */
 int rhip_libm_info(int fn, long long *entries, double *gate);
 int rhip_libm_eval(rhip_device dev, int fn, const float *x, long long n, float *out);
 /* The same evaluation on the host, from the host's copy of the table. */

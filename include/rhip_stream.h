@@ -40,7 +40,10 @@ enum RhsOp {
 
 /* Shading models. 0..2 are the reference's Shading_BaseColor / BlinnPhong / PBR
  * (Material.h:24-33); 3..5 are workloads with no reference shader:
- * a Blinn-Phong directional light, flat per-vertex colour, and the learned PBR. */
+ * a Blinn-Phong directional light, flat per-vertex colour, and the learned PBR;
+ * 6..9 are the reference's Shading_Skybox, Shading_IBL_Irradiance,
+ * Shading_IBL_Prefilter and Shading_FXAA (SkyboxSoft.h, IBLIrradianceSoft.h,
+ * IBLPrefilterSoft.h, FxaaSoft.h), drawn into targets without depth writes. */
 enum RhsShading {
   RHS_SHADING_BASIC = 0,
   RHS_SHADING_BLINN_PHONG = 1,
@@ -48,6 +51,10 @@ enum RhsShading {
   RHS_SHADING_BLINN_PHONG_DIR = 3,
   RHS_SHADING_VERTEX_COLOR = 4,
   RHS_SHADING_NEURAL_PBR = 5,
+  RHS_SHADING_SKYBOX = 6,         /* u_cubeMap, or u_equirectangularMap with RHS_DEF_EQUIRECTANGULAR_MAP */
+  RHS_SHADING_IBL_IRRADIANCE = 7, /* u_cubeMap */
+  RHS_SHADING_IBL_PREFILTER = 8,  /* u_cubeMap; material block = UniformsIBLPrefilter {srcResolution, roughness} */
+  RHS_SHADING_FXAA = 9,           /* u_screenTexture; material block = UniformsQuadFilter {screenSize} */
 };
 
 /* Shader define bits (BlinnPhongSoft.h:14-19, PbrSoft.h:14-20). */
@@ -57,6 +64,7 @@ enum RhsDefine {
   RHS_DEF_EMISSIVE_MAP = 4,
   RHS_DEF_AO_MAP = 8,
   RHS_DEF_METALROUGHNESS_MAP = 16,
+  RHS_DEF_EQUIRECTANGULAR_MAP = 32, /* SkyboxSoft.h:14-16 */
 };
 
 /* Sampler slots, in the order the reference shaders declare them
@@ -71,6 +79,11 @@ enum RhsSlot {
   RHS_SLOT_IRRADIANCE = 6,
   RHS_SLOT_PREFILTER = 7,
   RHS_SLOT_COUNT = 8,
+  /* the skybox, IBL and FXAA shaders' samplers share the slots above:
+   * u_equirectangularMap and u_screenTexture in slot 0, u_cubeMap in slot 6 */
+  RHS_SLOT_EQUIRECT = RHS_SLOT_ALBEDO,
+  RHS_SLOT_SCREEN = RHS_SLOT_ALBEDO,
+  RHS_SLOT_CUBE = RHS_SLOT_IRRADIANCE,
 };
 
 /* RenderStates (RenderStates.h:52-63) flattened to 14 words. */

@@ -40,6 +40,9 @@ static bool g_cr_math = false;
 static inline float m_powf(float x, float y) { return g_cr_math ? (float)pow((double)x, (double)y) : powf(x, y); }
 static inline float m_log2f(float x) { return g_cr_math ? (float)log2((double)x) : log2f(x); }
 static inline float m_exp2f(float x) { return g_cr_math ? (float)exp2((double)x) : exp2f(x); }
+static inline float m_sinf(float x) { return g_cr_math ? (float)sin((double)x) : sinf(x); }
+static inline float m_cosf(float x) { return g_cr_math ? (float)cos((double)x) : cosf(x); }
+static inline float m_atan2f(float y, float x) { return g_cr_math ? (float)atan2((double)y, (double)x) : atan2f(y, x); }
 static long long g_stats[8];
 /* unique-texel counting: while on (single-threaded raster), every texel a draw's shaders read is marked, and
  * the draw's count of distinct texels read, times 4 bytes, is appended to g_draw_texel_bytes */
@@ -105,6 +108,7 @@ struct Uniforms {
   int enableLight, enableIBL, enableShadow;
   float pointSize, kSpecular;
   V4 baseColor;
+  float aux[4]; /* UniformsIBLPrefilter {srcResolution, roughness} or UniformsQuadFilter {screenSize} */
 };
 
 static void parseUniforms(const RhsDraw &d, Uniforms &u) {
@@ -123,6 +127,7 @@ static void parseUniforms(const RhsDraw &d, Uniforms &u) {
   memcpy(&u.pointSize, d.material + 12, 4);
   memcpy(&u.kSpecular, d.material + 16, 4);
   memcpy(&u.baseColor, d.material + 32, 16);
+  memcpy(u.aux, d.material, 16);
 }
 
 /* ------------------------------------------------------------------ textures (TextureSoft.h, SamplerSoft.h) */
@@ -353,6 +358,12 @@ static int varyingCount(int shading) {
       return 28;
     case RHS_SHADING_VERTEX_COLOR:
       return 4;
+    case RHS_SHADING_SKYBOX:
+    case RHS_SHADING_IBL_IRRADIANCE:
+    case RHS_SHADING_IBL_PREFILTER:
+      return 3; /* v_worldPos */
+    case RHS_SHADING_FXAA:
+      return 2; /* v_texCoord */
     default:
       return 0;
   }
@@ -403,6 +414,19 @@ static void runVS(const DrawCtx &c, const float *a, VertexHolder &vh) {
     }
   } else if (c.shading == RHS_SHADING_VERTEX_COLOR) {
     v[0] = a[4], v[1] = a[5], v[2] = a[6], v[3] = a[7];
+  } else if (c.shading == RHS_SHADING_SKYBOX) {
+    /* SkyboxSoft.h:66-75 */
+    vh.clip = {vh.clip.x, vh.clip.y, vh.clip.w, vh.clip.w};
+    if (u.reverseZ) vh.clip.z = 0.f;
+    v[0] = a[0], v[1] = a[1], v[2] = a[2];
+  } else if (c.shading == RHS_SHADING_IBL_IRRADIANCE || c.shading == RHS_SHADING_IBL_PREFILTER) {
+    /* IBLIrradianceSoft.h:62-67, IBLPrefilterSoft.h:66-71 */
+    vh.clip.z = vh.clip.w;
+    v[0] = a[0], v[1] = a[1], v[2] = a[2];
+  } else if (c.shading == RHS_SHADING_FXAA) {
+    /* FxaaSoft.h:72-75 */
+    vh.clip = {a[0], a[1], a[2], 1.0f};
+    v[0] = a[4], v[1] = a[5];
   }
 }
 
@@ -644,6 +668,232 @@ static V4 fsPbr(const DrawCtx &c, const FragIn &f) {
   return {col.x, col.y, col.z, albedoRGBA.w};
 }
 
+/* ------------------------------------------------------------------ skybox, IBL precompute, FXAA */
+
+/* textureLodOffset, SamplerSoft.h:216-238 (offset after scaling: to the texel-space coordinate for bilinear,
+ * to the floored texel for nearest). The reference build fuses the bilinear `uv * size + offset` into one
+ * multiply-add, so a coordinate that rounds to a texel centre unshifted may land just past it shifted. */
+static uint32_t sampleOff(const Level &l, V2 uv, bool linear, int wrap, uint32_t border, int ox, int oy) {
+  if (linear)
+    return bilinearAt<uint32_t>(l, fmaf(uv.x, (float)l.w, (float)ox), fmaf(uv.y, (float)l.h, (float)oy), wrap,
+                                border);
+  float tx = uv.x * (float)l.w, ty = uv.y * (float)l.h;
+  return pixelWrap<uint32_t>(l, (int)floorf(tx) + ox, (int)floorf(ty) + oy, wrap, border);
+}
+static V4 tex2DOffset(const DrawCtx &c, int slot, V2 uv, float lod, int ox, int oy) {
+  const Texture *t = c.tex[slot];
+  if (!t || t->layers[0].empty()) return {0, 0, 0, 0};
+  const std::vector<Level> &lv = t->layers[0];
+  const uint32_t b = borderRGBA(*t);
+  const int fm = t->filterMin, wrap = t->wrapS, maxLevel = (int)lv.size() - 1;
+  uint32_t p;
+  if (fm <= 1) {
+    p = sampleOff(lv[0], uv, fm == 1, wrap, b, ox, oy);
+  } else if (fm == 2 || fm == 3) {
+    int level = std::min(std::max((int)ceilf(lod + 0.5f) - 1, 0), maxLevel);
+    p = sampleOff(lv[level], uv, fm == 3, wrap, b, ox, oy);
+  } else {
+    int hi = std::min(std::max((int)floorf(lod), 0), maxLevel), lo = std::min(std::max(hi + 1, 0), maxLevel);
+    p = sampleOff(lv[hi], uv, fm == 5, wrap, b, ox, oy);
+    if (hi != lo) p = mixRGBA(p, sampleOff(lv[lo], uv, fm == 5, wrap, b, ox, oy), lod - floorf(lod));
+  }
+  return {(float)chan(p, 0) / 255.f, (float)chan(p, 1) / 255.f, (float)chan(p, 2) / 255.f, (float)chan(p, 3) / 255.f};
+}
+
+/* glm::dot / glm::normalize as the reference build compiles them in the skybox, prefilter and FXAA
+ * shaders: fma(z, z', fma(y, y', x x')) */
+static inline float dot3b(V3 a, V3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
+static inline V3 normalize3b(V3 v) { return scale3(v, 1.0f / sqrtf(dot3b(v, v))); }
+
+/* SkyboxSoft.h:78-96: the host's atan2f, asin in double as the reference calls it; the reference build fuses
+ * `uv * scale + 0.5`. */
+static V4 fsSkybox(const DrawCtx &c, const FragIn &f) {
+  const float *v = f.v[f.self];
+  V3 wp{v[0], v[1], v[2]};
+  if (c.defines & RHS_DEF_EQUIRECTANGULAR_MAP) {
+    V3 d = normalize3b(wp);
+    float ax = m_atan2f(d.z, d.x), ay = (float)asin((double)-d.y);
+    V2 uv{fmaf(ax, 0.1591f, 0.5f), fmaf(ay, 0.3183f, 0.5f)};
+    return tex2D(c, RHS_SLOT_EQUIRECT, uv, 0.f); /* the skybox sampler has no LOD callback */
+  }
+  return texCube(c, RHS_SLOT_CUBE, wp, 0.f);
+}
+
+/* IBLIrradianceSoft.h:74-101 */
+static V4 fsIrradiance(const DrawCtx &c, const FragIn &f) {
+  const float *v = f.v[f.self];
+  V3 N = normalize3({v[0], v[1], v[2]});
+  V3 irr{0.f, 0.f, 0.f};
+  V3 up{0.0f, 1.0f, 0.0f};
+  V3 right = normalize3(cross3(up, N));
+  up = normalize3(cross3(N, right));
+  const float sampleDelta = 0.025f;
+  float nr = 0.0f;
+  for (float phi = 0.0f; phi < 2.0f * kPI; phi += sampleDelta) {
+    for (float theta = 0.0f; theta < 0.5f * kPI; theta += sampleDelta) {
+      V3 ts{m_sinf(theta) * m_cosf(phi), m_sinf(theta) * m_sinf(phi), m_cosf(theta)};
+      V3 sv = add3(add3(scale3(right, ts.x), scale3(up, ts.y)), scale3(N, ts.z));
+      V4 t = texCube(c, RHS_SLOT_CUBE, sv, 0.f);
+      float ct = m_cosf(theta), st = m_sinf(theta);
+      irr = add3(irr, {t.x * ct * st, t.y * ct * st, t.z * ct * st});
+      nr++;
+    }
+  }
+  irr = scale3(scale3(irr, kPI), 1.0f / nr);
+  return {irr.x, irr.y, irr.z, 1.0f};
+}
+
+/* IBLPrefilterSoft.h:88-168 */
+static uint32_t radicalInverseBits(uint32_t bits) {
+  bits = (bits << 16u) | (bits >> 16u);
+  bits = ((bits & 0x55555555u) << 1u) | ((bits & 0xAAAAAAAAu) >> 1u);
+  bits = ((bits & 0x33333333u) << 2u) | ((bits & 0xCCCCCCCCu) >> 2u);
+  bits = ((bits & 0x0F0F0F0Fu) << 4u) | ((bits & 0xF0F0F0F0u) >> 4u);
+  bits = ((bits & 0x00FF00FFu) << 8u) | ((bits & 0xFF00FF00u) >> 8u);
+  return bits;
+}
+static V4 fsPrefilter(const DrawCtx &c, const FragIn &f) {
+  const float *v = f.v[f.self];
+  V3 N = normalize3b({v[0], v[1], v[2]});
+  V3 R = N, V = R;
+  const float roughness = c.u.aux[1], resolution = c.u.aux[0];
+  const uint32_t SAMPLE_COUNT = 1024u;
+  V3 pre{0.f, 0.f, 0.f};
+  float totalWeight = 0.0f;
+  for (uint32_t i = 0u; i < SAMPLE_COUNT; ++i) {
+    V2 Xi{(float)i / (float)SAMPLE_COUNT, (float)((double)radicalInverseBits(i) * 2.3283064365386963e-10)};
+    float a = roughness * roughness;
+    float phi = 2.0f * kPI * Xi.x;
+    /* the reference build contracts: (a^2 - 1) Xi.y + 1, 1 - cos^2, the tangent-space sum, 2 (V.H) H - V,
+     * the GGX denominator, 1024 pdf + 1e-4 and the colour sum into fused multiply-adds */
+    float cosTheta = sqrtf((1.0f - Xi.y) / fmaf(a * a - 1.0f, Xi.y, 1.0f));
+    float sinTheta = sqrtf(fmaf(-cosTheta, cosTheta, 1.0f));
+    V3 Ht{m_cosf(phi) * sinTheta, m_sinf(phi) * sinTheta, cosTheta};
+    V3 upv = (double)fabsf(N.z) < 0.999 ? V3{0.0f, 0.0f, 1.0f} : V3{1.0f, 0.0f, 0.0f};
+    V3 tangent = normalize3b(cross3(upv, N));
+    V3 bitangent = cross3(N, tangent);
+    V3 sv{fmaf(N.x, Ht.z, fmaf(bitangent.x, Ht.y, tangent.x * Ht.x)),
+          fmaf(N.y, Ht.z, fmaf(bitangent.y, Ht.y, tangent.y * Ht.x)),
+          fmaf(N.z, Ht.z, fmaf(bitangent.z, Ht.y, tangent.z * Ht.x))};
+    V3 H = normalize3b(sv);
+    float vh2 = 2.0f * dot3b(V, H);
+    V3 L = normalize3b({fmaf(H.x, vh2, -V.x), fmaf(H.y, vh2, -V.y), fmaf(H.z, vh2, -V.z)});
+    float NdotL = maxf(dot3b(N, L), 0.0f);
+    if (NdotL > 0.0f) {
+      float ra = roughness * roughness, ra2 = ra * ra;
+      float nh = maxf(dot3b(N, H), 0.0f);
+      float den = fmaf(nh * nh, ra2 - 1.0f, 1.0f);
+      float D = ra2 / (kPI * den * den);
+      float NdotH = maxf(dot3b(N, H), 0.0f);
+      float HdotV = maxf(dot3b(H, V), 0.0f);
+      float pdf = D * NdotH / (4.0f * HdotV) + 0.0001f;
+      float saTexel = 4.0f * kPI / (6.0f * resolution * resolution);
+      float saSample = 1.0f / fmaf((float)SAMPLE_COUNT, pdf, 0.0001f);
+      float mipLevel = roughness == 0.0f ? 0.0f : 0.5f * m_log2f(saSample / saTexel);
+      V4 t = texCube(c, RHS_SLOT_CUBE, L, mipLevel);
+      pre = {fmaf(t.x, NdotL, pre.x), fmaf(t.y, NdotL, pre.y), fmaf(t.z, NdotL, pre.z)};
+      totalWeight += NdotL;
+    }
+  }
+  return {pre.x / totalWeight, pre.y / totalWeight, pre.z / totalWeight, 1.0f};
+}
+
+/* FxaaSoft.h:81-266 */
+/* the reference build contracts: the luma dot product red first, -2 l + corners, |.| * 2 + |.|,
+ * 2 (DU + LR) + LC and (average - centre) into fused multiply-adds */
+static float rgb2luma(V4 c) { return dot3b({c.x, c.y, c.z}, {0.299f, 0.587f, 0.114f}); }
+static float fxaaQuality(float q) { return q < 5.f ? 1.0f : (q > 5.f ? (q < 10.f ? 2.0f : (q < 11.f ? 4.0f : 8.0f)) : 1.5f); }
+static V4 fsFxaa(const DrawCtx &c, const FragIn &f) {
+  const float *v = f.v[f.self];
+  const int S = RHS_SLOT_SCREEN;
+  const float lod = 0.f; /* the FXAA sampler has no LOD callback */
+  V2 tc{v[0], v[1]};
+  V2 inv{1.0f / c.u.aux[0], 1.0f / c.u.aux[1]};
+  V4 colorCenter = tex2D(c, S, tc, lod);
+  float lumaCenter = rgb2luma(colorCenter);
+  float lumaDown = rgb2luma(tex2DOffset(c, S, tc, 0.f, 0, -1));
+  float lumaUp = rgb2luma(tex2DOffset(c, S, tc, 0.f, 0, 1));
+  float lumaLeft = rgb2luma(tex2DOffset(c, S, tc, 0.f, -1, 0));
+  float lumaRight = rgb2luma(tex2DOffset(c, S, tc, 0.f, 1, 0));
+  float lumaMin = minf(lumaCenter, minf(minf(lumaDown, lumaUp), minf(lumaLeft, lumaRight)));
+  float lumaMax = maxf(lumaCenter, maxf(maxf(lumaDown, lumaUp), maxf(lumaLeft, lumaRight)));
+  float lumaRange = lumaMax - lumaMin;
+  if (lumaRange < maxf(0.0312f, lumaMax * 0.125f)) return {colorCenter.x, colorCenter.y, colorCenter.z, 1.0f};
+  float lumaDownLeft = rgb2luma(tex2DOffset(c, S, tc, 0.f, -1, -1));
+  float lumaUpRight = rgb2luma(tex2DOffset(c, S, tc, 0.f, 1, 1));
+  float lumaUpLeft = rgb2luma(tex2DOffset(c, S, tc, 0.f, -1, 1));
+  float lumaDownRight = rgb2luma(tex2DOffset(c, S, tc, 0.f, 1, -1));
+  float lumaDownUp = lumaDown + lumaUp, lumaLeftRight = lumaLeft + lumaRight;
+  float lumaLeftCorners = lumaDownLeft + lumaUpLeft, lumaDownCorners = lumaDownLeft + lumaDownRight;
+  float lumaRightCorners = lumaDownRight + lumaUpRight, lumaUpCorners = lumaUpRight + lumaUpLeft;
+  float edgeHorizontal = fmaf(fabsf(fmaf(-2.0f, lumaCenter, lumaDownUp)), 2.0f,
+                              fabsf(fmaf(-2.0f, lumaLeft, lumaLeftCorners))) +
+                         fabsf(fmaf(-2.0f, lumaRight, lumaRightCorners));
+  float edgeVertical = fmaf(fabsf(fmaf(-2.0f, lumaCenter, lumaLeftRight)), 2.0f,
+                            fabsf(fmaf(-2.0f, lumaUp, lumaUpCorners))) +
+                       fabsf(fmaf(-2.0f, lumaDown, lumaDownCorners));
+  bool isHorizontal = edgeHorizontal >= edgeVertical;
+  float stepLength = isHorizontal ? inv.y : inv.x;
+  float luma1 = isHorizontal ? lumaDown : lumaLeft, luma2 = isHorizontal ? lumaUp : lumaRight;
+  float gradient1 = luma1 - lumaCenter, gradient2 = luma2 - lumaCenter;
+  bool is1Steepest = fabsf(gradient1) >= fabsf(gradient2);
+  float gradientScaled = 0.25f * maxf(fabsf(gradient1), fabsf(gradient2));
+  float lumaLocalAverage;
+  if (is1Steepest) {
+    stepLength = -stepLength;
+    lumaLocalAverage = 0.5f * (luma1 + lumaCenter);
+  } else {
+    lumaLocalAverage = 0.5f * (luma2 + lumaCenter);
+  }
+  V2 currentUv = tc;
+  if (isHorizontal)
+    currentUv.y += stepLength * 0.5f;
+  else
+    currentUv.x += stepLength * 0.5f;
+  V2 offset = isHorizontal ? V2{inv.x, 0.0f} : V2{0.0f, inv.y};
+  V2 uv1{currentUv.x - offset.x * fxaaQuality(0.f), currentUv.y - offset.y * fxaaQuality(0.f)};
+  V2 uv2{currentUv.x + offset.x * fxaaQuality(0.f), currentUv.y + offset.y * fxaaQuality(0.f)};
+  float lumaEnd1 = 0.f, lumaEnd2 = 0.f;
+  bool reached1 = false, reached2 = false, reachedBoth = false;
+  for (int i = 1; i < 12; i++) {
+    if (!reached1) {
+      lumaEnd1 = rgb2luma(tex2D(c, S, uv1, lod)) - lumaLocalAverage;
+      reached1 = fabsf(lumaEnd1) >= gradientScaled;
+    }
+    if (!reached2) {
+      lumaEnd2 = rgb2luma(tex2D(c, S, uv2, lod)) - lumaLocalAverage;
+      reached2 = fabsf(lumaEnd2) >= gradientScaled;
+    }
+    reachedBoth = reached1 && reached2;
+    if (!reached1) uv1 = {uv1.x - offset.x * fxaaQuality((float)i), uv1.y - offset.y * fxaaQuality((float)i)};
+    if (!reached2) uv2 = {uv2.x + offset.x * fxaaQuality((float)i), uv2.y + offset.y * fxaaQuality((float)i)};
+    if (reachedBoth) break;
+  }
+  float distance1 = isHorizontal ? (tc.x - uv1.x) : (tc.y - uv1.y);
+  float distance2 = isHorizontal ? (uv2.x - tc.x) : (uv2.y - tc.y);
+  bool isDirection1 = distance1 < distance2;
+  float distanceFinal = minf(distance1, distance2);
+  bool isLumaCenterSmaller = lumaCenter < lumaLocalAverage;
+  bool correctVariation1 = (lumaEnd1 < 0.0f) != isLumaCenterSmaller;
+  bool correctVariation2 = (lumaEnd2 < 0.0f) != isLumaCenterSmaller;
+  bool correctVariation = isDirection1 ? correctVariation1 : correctVariation2;
+  float edgeLength = distance1 + distance2;
+  float pixelOffset = -distanceFinal / edgeLength + 0.5f;
+  float finalOffset = correctVariation ? pixelOffset : 0.0f;
+  float lumaSum = fmaf(2.0f, lumaDownUp + lumaLeftRight, lumaLeftCorners) + lumaRightCorners;
+  float subPixelOffset1 = clampf(fabsf(fmaf(lumaSum, 1.0f / 12.0f, -lumaCenter)) / lumaRange, 0.0f, 1.0f);
+  float subPixelOffset2 = fmaf(-2.0f, subPixelOffset1, 3.0f) * subPixelOffset1 * subPixelOffset1;
+  float subPixelOffsetFinal = subPixelOffset2 * subPixelOffset2 * 0.75f;
+  finalOffset = maxf(finalOffset, subPixelOffsetFinal);
+  V2 finalUv = tc;
+  if (isHorizontal)
+    finalUv.y += finalOffset * stepLength;
+  else
+    finalUv.x += finalOffset * stepLength;
+  V4 fc = tex2D(c, S, finalUv, lod);
+  return {fc.x, fc.y, fc.z, 1.0f};
+}
+
 static V4 runFS(const DrawCtx &c, const FragIn &f) {
   switch (c.shading) {
     case RHS_SHADING_BASIC:
@@ -658,6 +908,14 @@ static V4 runFS(const DrawCtx &c, const FragIn &f) {
       const float *v = f.v[f.self];
       return {v[0], v[1], v[2], v[3]};
     }
+    case RHS_SHADING_SKYBOX:
+      return fsSkybox(c, f);
+    case RHS_SHADING_IBL_IRRADIANCE:
+      return fsIrradiance(c, f);
+    case RHS_SHADING_IBL_PREFILTER:
+      return fsPrefilter(c, f);
+    case RHS_SHADING_FXAA:
+      return fsFxaa(c, f);
   }
   return {0, 0, 0, 0};
 }
@@ -1108,7 +1366,7 @@ struct Renderer {
     vao = &vit->second;
     ctx = DrawCtx();
     ctx.shading = d.shading;
-    if (d.shading < 0 || d.shading > RHS_SHADING_VERTEX_COLOR) return -1;
+    if (d.shading < 0 || d.shading > RHS_SHADING_FXAA || d.shading == RHS_SHADING_NEURAL_PBR) return -1;
     ctx.defines = d.defines;
     ctx.st = d.states;
     parseUniforms(d, ctx.u);

@@ -23,6 +23,10 @@
 #include "Viewer/Shader/Software/BasicSoft.h"
 #include "Viewer/Shader/Software/BlinnPhongSoft.h"
 #include "Viewer/Shader/Software/PbrSoft.h"
+#include "Viewer/Shader/Software/SkyboxSoft.h"
+#include "Viewer/Shader/Software/IBLIrradianceSoft.h"
+#include "Viewer/Shader/Software/IBLPrefilterSoft.h"
+#include "Viewer/Shader/Software/FxaaSoft.h"
 
 #include "rhip_stream.h"
 
@@ -122,14 +126,24 @@ namespace {
 const char *kSlotNames[RHS_SLOT_COUNT] = {"u_albedoMap", "u_normalMap", "u_emissiveMap", "u_aoMap",
                                           "u_shadowMap", "u_metalRoughnessMap", "u_irradianceMap", "u_prefilterMap"};
 
+/* The sampler name a shading model's shader declares for a slot. */
+const char *slotName(int shading, int slot) {
+  if (shading == RHS_SHADING_FXAA && slot == RHS_SLOT_SCREEN) return "u_screenTexture";
+  if (shading == RHS_SHADING_SKYBOX || shading == RHS_SHADING_IBL_IRRADIANCE || shading == RHS_SHADING_IBL_PREFILTER) {
+    if (slot == RHS_SLOT_CUBE) return "u_cubeMap";
+    if (slot == RHS_SLOT_EQUIRECT) return "u_equirectangularMap";
+  }
+  return kSlotNames[slot];
+}
+
 struct Harness {
   RendererSoft r;
   std::map<int, std::shared_ptr<Texture>> tex;
   std::map<int, std::shared_ptr<VertexArrayObject>> vao;
   std::map<int, std::shared_ptr<FrameBuffer>> fbo;
   std::map<uint64_t, std::shared_ptr<ShaderProgram>> programs;
-  std::shared_ptr<UniformBlock> ubModel, ubScene, ubMaterial;
-  std::shared_ptr<UniformSampler> samplers[RHS_SLOT_COUNT];
+  std::shared_ptr<UniformBlock> ubModel, ubScene, ubMaterial, ubPrefilter, ubQuadFilter;
+  std::map<std::string, std::shared_ptr<UniformSampler>> samplers;
   std::vector<std::shared_ptr<PipelineStates>> keepStates;
   std::string err;
 
@@ -137,6 +151,9 @@ struct Harness {
     ubModel = r.createUniformBlock("UniformsModel", RHS_MODEL_BYTES);
     ubScene = r.createUniformBlock("UniformsScene", RHS_SCENE_BYTES);
     ubMaterial = r.createUniformBlock("UniformsMaterial", RHS_MATERIAL_BYTES);
+    /* the IBL prefilter's and the quad filter's blocks (Material.h:88-95) come from the stream's material bytes */
+    ubPrefilter = r.createUniformBlock("UniformsPrefilter", 8);
+    ubQuadFilter = r.createUniformBlock("UniformsQuadFilter", 8);
   }
 
   std::shared_ptr<ShaderProgram> program(int shading, uint32_t defines) {
@@ -144,8 +161,9 @@ struct Harness {
     auto it = programs.find(key);
     if (it != programs.end()) return it->second;
     auto prog = r.createShaderProgram();
-    static const char *defNames[5] = {"ALBEDO_MAP", "NORMAL_MAP", "EMISSIVE_MAP", "AO_MAP", "METALROUGHNESS_MAP"};
-    for (int i = 0; i < 5; i++)
+    static const char *defNames[6] = {"ALBEDO_MAP", "NORMAL_MAP",         "EMISSIVE_MAP",
+                                      "AO_MAP",     "METALROUGHNESS_MAP", "EQUIRECTANGULAR_MAP"};
+    for (int i = 0; i < 6; i++)
       if (defines & (1u << i)) prog->addDefine(defNames[i]);
     auto *ps = dynamic_cast<ShaderProgramSoft *>(prog.get());
     bool ok = false;
@@ -165,6 +183,18 @@ struct Harness {
       case RHS_SHADING_VERTEX_COLOR:
         ok = ps->SetShaders(std::make_shared<HarnessShaders::VColor::VS>(),
                             std::make_shared<HarnessShaders::VColor::FS>());
+        break;
+      case RHS_SHADING_SKYBOX:
+        ok = ps->SetShaders(std::make_shared<ShaderSkybox::VS>(), std::make_shared<ShaderSkybox::FS>());
+        break;
+      case RHS_SHADING_IBL_IRRADIANCE:
+        ok = ps->SetShaders(std::make_shared<ShaderIBLIrradiance::VS>(), std::make_shared<ShaderIBLIrradiance::FS>());
+        break;
+      case RHS_SHADING_IBL_PREFILTER:
+        ok = ps->SetShaders(std::make_shared<ShaderIBLPrefilter::VS>(), std::make_shared<ShaderIBLPrefilter::FS>());
+        break;
+      case RHS_SHADING_FXAA:
+        ok = ps->SetShaders(std::make_shared<ShaderFXAA::VS>(), std::make_shared<ShaderFXAA::FS>());
         break;
       default:
         break;
@@ -318,17 +348,26 @@ struct Harness {
         res->blocks[0] = ubModel;
         res->blocks[1] = ubScene;
         res->blocks[2] = ubMaterial;
+        if (d.shading == RHS_SHADING_IBL_PREFILTER) {
+          ubPrefilter->setData(d.material, 8);
+          res->blocks[3] = ubPrefilter;
+        } else if (d.shading == RHS_SHADING_FXAA) {
+          ubQuadFilter->setData(d.material, 8);
+          res->blocks[4] = ubQuadFilter;
+        }
         for (int i = 0; i < d.nsamplers; i++) {
           int slot = d.slot[i];
           auto &t = tex.at(d.tex[i]);
-          if (!samplers[slot]) {
+          const std::string name = slotName(d.shading, slot);
+          auto &smp = samplers[name];
+          if (!smp) {
             TextureDesc td;
             td.type = t->type;
             td.format = t->format;
-            samplers[slot] = r.createUniformSampler(kSlotNames[slot], td);
+            smp = r.createUniformSampler(name, td);
           }
-          samplers[slot]->setTexture(t);
-          res->samplers[slot] = samplers[slot];
+          smp->setTexture(t);
+          res->samplers[slot] = smp;
         }
         RenderStates rs;
         rs.blend = d.states.blend != 0;

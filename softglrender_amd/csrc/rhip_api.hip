@@ -157,6 +157,7 @@ void rhip::decodeUniforms(const rhip_draw_desc &dd, Uniforms &u) {
     memcpy(&u.point_size, t + 12, 4);
     memcpy(&u.k_specular, t + 16, 4);
     memcpy(u.base_color, t + 32, 16);
+    memcpy(u.aux, t, 16);
   }
 }
 
@@ -275,6 +276,10 @@ void launchOrdShade(const DrawParams &P, int grid, int need_deriv, hipStream_t s
       RHIP_LAUNCH(k_ord_shade<S RHIP_COMMA RHS_SHADING_BLINN_PHONG_DIR>, grid, 256, st, P, need_deriv);
       break;
     case RHS_SHADING_PBR: RHIP_LAUNCH(k_ord_shade<S RHIP_COMMA RHS_SHADING_PBR>, grid, 256, st, P, need_deriv); break;
+    case RHS_SHADING_SKYBOX:
+    case RHS_SHADING_IBL_IRRADIANCE:
+    case RHS_SHADING_IBL_PREFILTER:
+    case RHS_SHADING_FXAA: RHIP_LAUNCH(k_ord_shade<S RHIP_COMMA kShadingEnv>, grid, 256, st, P, need_deriv); break;
     default: RHIP_LAUNCH(k_ord_shade<S RHIP_COMMA RHS_SHADING_VERTEX_COLOR>, grid, 256, st, P, need_deriv); break;
   }
 }
@@ -290,6 +295,10 @@ void launchOrdRest(const DrawParams &P, int grid, int need_deriv, hipStream_t st
       RHIP_LAUNCH(k_ord_rest<S RHIP_COMMA RHS_SHADING_BLINN_PHONG_DIR>, grid, 256, st, P, need_deriv);
       break;
     case RHS_SHADING_PBR: RHIP_LAUNCH(k_ord_rest<S RHIP_COMMA RHS_SHADING_PBR>, grid, 256, st, P, need_deriv); break;
+    case RHS_SHADING_SKYBOX:
+    case RHS_SHADING_IBL_IRRADIANCE:
+    case RHS_SHADING_IBL_PREFILTER:
+    case RHS_SHADING_FXAA: RHIP_LAUNCH(k_ord_rest<S RHIP_COMMA kShadingEnv>, grid, 256, st, P, need_deriv); break;
     default: RHIP_LAUNCH(k_ord_rest<S RHIP_COMMA RHS_SHADING_VERTEX_COLOR>, grid, 256, st, P, need_deriv); break;
   }
 }
@@ -392,6 +401,10 @@ int drawSetup(rhip_device d, int fbo, const int vp[4], const rhip_draw_desc &dd,
   /* a draw that writes no depth: every fragment's tests see the depths the draw started with */
   sh.listed = !sh.two_phase && !(dt && st.depthTest && st.depthMask);
 
+  if (isEnvShading(pg.shading) && !sh.listed)
+    return fail(RHIP_E_UNSUPPORTED,
+                "skybox, IBL and FXAA programs draw without depth writes (the reference's states for them); "
+                "this draw writes depth");
   if (pg.shading == RHS_SHADING_NEURAL_PBR) {
     uint32_t enable_light = 0;
     if (dd.uniform_material) memcpy(&enable_light, dd.uniform_material, 4);
@@ -414,7 +427,9 @@ int drawSetup(rhip_device d, int fbo, const int vp[4], const rhip_draw_desc &dd,
     if (!t) return fail(RHIP_E_INVALID, "sampler slot %d: invalid texture %d", s, id);
     if (!t->initialised) return fail(RHIP_E_INVALID, "sampler slot %d: texture %d has no image", s, id);
     P.tex[s] = t->dd;
-    if (t->samp.filter_min > 1 && s != RHS_SLOT_METAL_ROUGHNESS && s <= RHS_SLOT_AO) P.need_deriv = 1;
+    /* only the Blinn-Phong and PBR shaders set a derivative LOD on their samplers (PbrSoft.h:138-151) */
+    if (t->samp.filter_min > 1 && s != RHS_SLOT_METAL_ROUGHNESS && s <= RHS_SLOT_AO && !isEnvShading(pg.shading))
+      P.need_deriv = 1;
   }
   P.vp_w = (float)vp[2];
   P.vp_h = (float)vp[3];
@@ -463,6 +478,7 @@ int drawSetup(rhip_device d, int fbo, const int vp[4], const rhip_draw_desc &dd,
   P.hiz_off = d->hiz_off ? 1 : 0;
   P.gamma_lut = d->gamma_lut;
   P.libm = d->libm;
+  P.ibl = d->ibl;
   for (int c = 0; c < 3; c++) P.base_lin[c] = hostPowf(P.u.base_color[c], 2.2f); /* PbrSoft.h:235 */
   sh.V = (size_t)va.nverts;
   sh.T = (size_t)P.ntris;
@@ -550,6 +566,12 @@ int rhip_device_create(int hip_device, rhip_device *out) {
     gammaTablesHost(gt);
     HIPCHK(hipMemcpy(d->gamma_lut, gt, sizeof gt, hipMemcpyHostToDevice));
   }
+  HIPCHK(hipMalloc(&d->ibl, sizeof(IblTables)));
+  {
+    std::unique_ptr<IblTables> it(new IblTables());
+    iblTablesHost(it.get());
+    HIPCHK(hipMemcpy(d->ibl, it.get(), sizeof(IblTables), hipMemcpyHostToDevice));
+  }
   /* the host libm on the device: every table, built once per process */
   {
     int rc = libmEnsure(d.get(), (1u << kLmCount) - 1);
@@ -572,6 +594,7 @@ int rhip_device_destroy(rhip_device d) {
   d->vao.clear();
   if (d->counters) hipFree(d->counters);
   if (d->gamma_lut) hipFree(d->gamma_lut);
+  if (d->ibl) hipFree(d->ibl);
   if (d->libm) hipFree(d->libm);
   for (void *p : d->libm_bufs) hipFree(p);
   if (d->errors) hipFree(d->errors);
@@ -805,7 +828,7 @@ int rhip_fbo_destroy(rhip_device d, int id) {
 
 int rhip_program_create(rhip_device d, int shading, uint32_t defines, int *out) {
   if (!d || !out) return fail(RHIP_E_INVALID, "null argument");
-  if (shading < RHS_SHADING_BASIC || shading > RHS_SHADING_NEURAL_PBR)
+  if (shading < RHS_SHADING_BASIC || shading > RHS_SHADING_FXAA)
     return fail(RHIP_E_UNSUPPORTED, "shading model %d", shading);
   if (defines & ~31u) return fail(RHIP_E_INVALID, "unknown define bits 0x%x", defines);
   int id = d->next_id++;

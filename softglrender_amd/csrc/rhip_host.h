@@ -110,6 +110,7 @@ struct rhip_device_s {
   uint32_t *errors = nullptr;             /* device: kErrSlots words set by kernels that drop work */
   float *gamma_lut = nullptr;             /* device: gamma tables (rhip_libm.hip) */
   rhip::LibmTables *libm = nullptr;       /* device: host libm tables (rhip_libm.h) */
+  rhip::IblTables *ibl = nullptr;         /* device: sin / cos tables of the IBL shaders */
   uint32_t libm_ready = 0;                /* functions whose table is on this device */
   std::vector<void *> libm_bufs;
   /* optional per-kernel timing: an event pair around every launch */
@@ -185,5 +186,7 @@ int libmTableStats(int fn, long long *entries, double *gate);
 /* t[0, 256): powf(k / 255, 2.2); t[256 + k]: smallest c >= 0 whose output byte is at least k. */
 void gammaTablesHost(float *t);
 float hostPowf(float x, float y);
+/* The IBL shaders' loop tables from the host libm's sinf / cosf. */
+void iblTablesHost(IblTables *t);
 
 }  // namespace rhip

@@ -62,6 +62,27 @@ struct Uniforms {
   int enable_light, enable_ibl, enable_shadow;
   float point_size, k_specular;
   float base_color[4];
+  float aux[4]; /* the material block's first 16 bytes as floats: UniformsIBLPrefilter {srcResolution,
+                   roughness} or UniformsQuadFilter {screenSize} for those shaders */
+};
+
+/* Shading models drawn by one kernel specialisation that switches on
+ * DrawParams::shading (skybox, IBL irradiance and prefilter, FXAA). */
+constexpr int kShadingEnv = 100;
+__host__ __device__ inline bool isEnvShading(int s) {
+  return s == RHS_SHADING_SKYBOX || s == RHS_SHADING_IBL_IRRADIANCE || s == RHS_SHADING_IBL_PREFILTER ||
+         s == RHS_SHADING_FXAA;
+}
+
+/* Loop tables of the image-based-lighting shaders, from the host libm
+ * (rhip_libm.hip): IBLIrradianceSoft.h:84-85 steps phi over [0, 2 PI) and
+ * theta over [0, PI / 2) by 0.025 in float; IBLPrefilterSoft.h:106-112 takes
+ * cos and sin of 2 PI i / 1024. */
+struct IblTables {
+  int nphi, ntheta;
+  float sphi[256], cphi[256];
+  float stheta[64], ctheta[64];
+  float hcos[1024], hsin[1024];
 };
 
 /* Per-primitive setup record (after clipping / culling). */
@@ -83,6 +104,8 @@ struct PrimSetup {
  * feeds the vertex shader's own outputs. */
 __host__ __device__ inline bool varySlotUsed(int shading, uint32_t defines, int enable_shadow, int j) {
   if (shading == RHS_SHADING_VERTEX_COLOR) return j < 4;
+  if (shading == RHS_SHADING_FXAA) return j < 2;  /* v_texCoord */
+  if (isEnvShading(shading)) return j < 3;        /* v_worldPos */
   if (shading == RHS_SHADING_BASIC) return false;
   const bool nm = (defines & RHS_DEF_NORMAL_MAP) != 0;
   if (j < 2) return true;
@@ -122,6 +145,7 @@ struct DrawParams {
   const float *gamma_lut; /* [0, 256): powf(k / 255, 2.2); [256, 512): the PBR output-byte thresholds
                              (host libm, rhip_libm.hip) */
   const LibmTables *libm; /* the host libm's log2f / exp2f / powf on the device (rhip_libm.h) */
+  const IblTables *ibl;   /* sin / cos tables of the IBL shaders */
   float base_lin[3];      /* powf(base colour, 2.2) by the host libm (PbrSoft.h:235) */
   uint32_t vary_groups; /* stored groups of 4 varying slots (varyGroups) */
   int vary_stride;      /* floats per vertex in `vary`: 4 per stored group */

@@ -883,6 +883,8 @@ __device__ __forceinline__ RGBAf runFS(const DrawParams &P, const Frag &f) {
     return fsBlinnPhong(P, f, true);
   } else if (SHADING == RHS_SHADING_PBR) {
     return fsPbr<U8OUT>(P, f);
+  } else if (SHADING == kShadingEnv) {
+    return fsEnv(P, f);
   } else {
     return {f.v[0], f.v[1], f.v[2], f.v[3]};
   }
@@ -891,7 +893,7 @@ __device__ __forceinline__ RGBAf runFS(const DrawParams &P, const Frag &f) {
 /* Varyings a fragment shader reads (the others are never interpolated). */
 template <int SHADING>
 __device__ __forceinline__ bool varyUsed(const DrawParams &P, int j) {
-  return varySlotUsed(SHADING, P.defines, P.u.enable_shadow, j);
+  return varySlotUsed(SHADING == kShadingEnv ? P.shading : SHADING, P.defines, P.u.enable_shadow, j);
 }
 
 /* Shading of one fragment: `b` is the raw (screen-space) barycentric of its shading

@@ -14,7 +14,8 @@
  * made of IEEE basic operations only (+, -, *, /, rint, ldexp: correctly
  * rounded, so host and device compute the same double), rounded to float, is
  * within one ulp of the host libm's result everywhere in the function's input
- * range. At first use the library evaluates both on every float of that range
+ * range. For log2f the approximation is glibc's own algorithm (glibcLog2),
+ * which leaves its table empty. At first use the library evaluates both on every float of that range
  * on the host (rhip_libm.hip) and keeps the inputs where they differ, bucketed
  * by the top 16 bits of the input, with the direction of the one-ulp step. The
  * device evaluates the approximation and consults the table only when the
@@ -30,7 +31,7 @@
 namespace rhip {
 
 enum LibmFn {
-  kLmLog2 = 0,   /* log2f(x), x in [1, FLT_MAX] (the LOD: below 1 the LOD is clamped to 0 anyway) */
+  kLmLog2 = 0,   /* log2f(x), x in [2^-149, FLT_MAX] (the LOD, the prefilter's source level) */
   kLmExp2Neg,    /* exp2f(-u), u in [2^-27, 16] (EnvBRDFApprox, u = 9.28 NdotV; below: both are 1) */
   kLmPow5,       /* powf(x, 5), x in [2^-31, 1 + 2^-10] (Fresnel; below: both are 0) */
   kLmPow128,     /* powf(x, 128), x in [0.4, 1 + 2^-10] (Blinn-Phong specular; below: both are 0) */
@@ -122,11 +123,50 @@ __host__ __device__ inline double lmExp2(double t) {
   return ldexp(p, (int)k);
 }
 
+/* glibc's log2f before its final rounding (glibc 2.35 sysdeps/ieee754/flt-32/
+ * e_log2f.c, e_log2f_data.c, from Arm's optimized-routines): x = 2^k z with z
+ * in [0x3f330000, 2 * that) split into 16 intervals, each with a double 1/c
+ * and log2(c) for c near its centre; log2 x = k + log2 c + log1p(z/c - 1)/ln 2,
+ * the last by a cubic in r = z/c - 1. The result rounded to float equals the
+ * host's log2f on every positive float, with or without fused multiply-adds
+ * in the double steps (both checked exhaustively): the function needs no table
+ * of exceptions and costs a few double operations, which gfx950 runs at full
+ * rate. Positive finite x only. */
+__host__ __device__ inline double glibcLog2(float x) {
+  /* {1/c, log2 c} per interval */
+  constexpr double kInvc[16] = {0x1.661ec79f8f3bep+0, 0x1.571ed4aaf883dp+0, 0x1.49539f0f010bp+0,
+                                0x1.3c995b0b80385p+0, 0x1.30d190c8864a5p+0, 0x1.25e227b0b8eap+0,
+                                0x1.1bb4a4a1a343fp+0, 0x1.12358f08ae5bap+0, 0x1.0953f419900a7p+0,
+                                0x1p+0,               0x1.e608cfd9a47acp-1, 0x1.ca4b31f026aap-1,
+                                0x1.b2036576afce6p-1, 0x1.9c2d163a1aa2dp-1, 0x1.886e6037841edp-1,
+                                0x1.767dcf5534862p-1};
+  constexpr double kLogc[16] = {-0x1.efec65b963019p-2, -0x1.b0b6832d4fca4p-2, -0x1.7418b0a1fb77bp-2,
+                                -0x1.39de91a6dcf7bp-2, -0x1.01d9bf3f2b631p-2, -0x1.97c1d1b3b7afp-3,
+                                -0x1.2f9e393af3c9fp-3, -0x1.960cbbf788d5cp-4, -0x1.a6f9db6475fcep-5,
+                                0x0p+0,                0x1.338ca9f24f53dp-4,  0x1.476a9543891bap-3,
+                                0x1.e840b4ac4e4d2p-3,  0x1.40645f0c6651cp-2,  0x1.88e9c2c1b9ff8p-2,
+                                0x1.ce0a44eb17bccp-2};
+  uint32_t ix = lmBits(x);
+  if (ix == 0x3f800000u) return 0.0;
+  if (ix < 0x00800000u) ix = lmBits(x * 0x1p23f) - (23u << 23); /* subnormal: scale exactly */
+  const uint32_t tmp = ix - 0x3f330000u;
+  const int i = (int)((tmp >> 19) & 15u);
+  const uint32_t iz = ix - (tmp & 0xff800000u);
+  const int k = (int32_t)tmp >> 23;
+  const double r = (double)lmAsFloat(iz) * kInvc[i] - 1.0;
+  const double y0 = kLogc[i] + (double)k;
+  const double r2 = r * r;
+  double y = 0x1.ecabf496832ep-2 * r + -0x1.715479ffae3dep-1;
+  y = -0x1.712b6f70a7e4dp-2 * r2 + y;
+  const double p = 0x1.715475f35c8b8p0 * r + y0;
+  return y * r2 + p;
+}
+
 /* The approximation of function fn at x, in double (x inside the function's range). */
 __host__ __device__ inline double lmApprox(int fn, float x) {
   switch (fn) {
     case kLmLog2:
-      return lmLog2(x);
+      return glibcLog2(x);
     case kLmExp2Neg:
       return lmExp2(-(double)x);
     case kLmPow5: {
@@ -144,8 +184,10 @@ __host__ __device__ inline double lmApprox(int fn, float x) {
 }
 
 /* Distance in ulps of d from the nearest midpoint between floats, measured
- * against the neighbour on d's side of its rounding (for d >= 0). */
+ * against the neighbour on d's side of its rounding (rounding to nearest is
+ * symmetric in sign: a negative d is measured as -d). */
 __host__ __device__ inline double lmMidDist(double d) {
+  if (d < 0) d = -d;
   const float f = (float)d;
   const double fl = (double)f, err = d - fl;
   const uint32_t b = lmBits(f);
@@ -161,7 +203,7 @@ __host__ __device__ inline double lmMidDist(double d) {
 __host__ __device__ inline void lmRange(int fn, uint32_t *lo, uint32_t *hi) {
   switch (fn) {
     case kLmLog2:
-      *lo = 0x3f800000u; /* 1 */
+      *lo = 0x00000001u; /* 2^-149 */
       *hi = 0x7f7fffffu; /* FLT_MAX */
       break;
     case kLmExp2Neg:
@@ -190,7 +232,7 @@ __host__ __device__ inline void lmRange(int fn, uint32_t *lo, uint32_t *hi) {
 __host__ __device__ inline float lmEval(const LibmTab &T, int fn, float x) {
   const double d = lmApprox(fn, x);
   float f = (float)d;
-  if (lmMidDist(d) <= T.gate) {
+  if (T.gate >= 0.0 && lmMidDist(d) <= T.gate) {
     const uint32_t xb = lmBits(x), key = xb & 0xffffu;
     const uint32_t bk = (xb >> 16) - (T.lo >> 16);
     int lo = (int)T.bucket[bk], hi = (int)T.bucket[bk + 1] - 1;
@@ -215,7 +257,7 @@ __host__ __device__ inline bool lmInRange(const LibmTab &T, float x) {
   return x > 0.f && b >= T.lo && b <= T.hi;
 }
 
-/* log2f as the host libm computes it (x >= 1; smaller x: correctly rounded). */
+/* log2f as the host libm computes it (x > 0; zero, infinities and NaN: correctly rounded, as glibc). */
 __host__ __device__ inline float hlog2f(const LibmTables *L, float x) {
   if (lmInRange(L->t[kLmLog2], x)) return lmEval(L->t[kLmLog2], kLmLog2, x);
   return (float)log2((double)x);
@@ -242,5 +284,97 @@ __host__ __device__ inline float hpowGamma(const LibmTables *L, float x) {
   if (lmInRange(L->t[kLmPowGamma], x)) return lmEval(L->t[kLmPowGamma], kLmPowGamma, x);
   return (float)pow((double)x, (double)(1.0f / 2.2f));
 }
+
+/* ------------------------------------------------------------- atan2f (no table) */
+
+/* The skybox's equirectangular lookup calls glibc's atan2f (SkyboxSoft.h:84).
+ * glibc 2.35 computes it with the fdlibm single-precision algorithm
+ * (sysdeps/ieee754/flt-32/e_atan2f.c, s_atanf.c): |y/x| reduced to one of four
+ * breakpoints (atan 0.5, 1, 1.5, inf) and an 11-term odd polynomial in two
+ * halves, every step a correctly rounded single-precision operation. The same
+ * steps here give the same float on host and device (the library is built
+ * without contraction): checked bit for bit against the host's atan2f on
+ * tens of millions of pairs. It is about 0.7 ulp from the true value, which is
+ * why a correctly rounded atan2 differs from it on about one pair in six. */
+__host__ __device__ inline float fdAtanf(float x) {
+  /* atan(b) split hi + lo at b = 0.5, 1, 1.5, inf */
+  const float hi[4] = {4.6364760399e-01f, 7.8539812565e-01f, 9.8279368877e-01f, 1.5707962513e+00f};
+  const float lo[4] = {5.0121582440e-09f, 3.7748947079e-08f, 3.4473217170e-08f, 7.5497894159e-08f};
+  const uint32_t hx = lmBits(x), ix = hx & 0x7fffffffu;
+  const bool neg = (hx >> 31) != 0;
+  if (ix >= 0x4c000000u) { /* |x| >= 2^25 or NaN */
+    if (ix > 0x7f800000u) return x + x;
+    return neg ? -hi[3] - lo[3] : hi[3] + lo[3];
+  }
+  int id;
+  if (ix < 0x3ee00000u) { /* |x| < 7/16 */
+    if (ix < 0x31000000u) return x; /* |x| < 2^-29 */
+    id = -1;
+  } else {
+    x = fabsf(x);
+    if (ix < 0x3f300000u) { /* < 11/16 */
+      id = 0;
+      x = (2.0f * x - 1.0f) / (2.0f + x);
+    } else if (ix < 0x3f980000u) { /* < 19/16 */
+      id = 1;
+      x = (x - 1.0f) / (x + 1.0f);
+    } else if (ix < 0x401c0000u) { /* < 39/16 */
+      id = 2;
+      x = (x - 1.5f) / (1.0f + 1.5f * x);
+    } else {
+      id = 3;
+      x = -1.0f / x;
+    }
+  }
+  /* odd polynomial: z (c0 + w (c2 + ...)) + w (c1 + w (c3 + ...)), z = x^2, w = z^2 */
+  const float z = x * x, w = z * z;
+  const float s1 = z * (3.3333334327e-01f +
+                        w * (1.4285714924e-01f +
+                             w * (9.0908870101e-02f +
+                                  w * (6.6610731184e-02f + w * (4.9768779427e-02f + w * 1.6285819933e-02f)))));
+  const float s2 = w * (-2.0000000298e-01f +
+                        w * (-1.1111110449e-01f +
+                             w * (-7.6918758452e-02f + w * (-5.8335702866e-02f + w * -3.6531571299e-02f))));
+  if (id < 0) return x - x * (s1 + s2);
+  const float r = hi[id] - ((x * (s1 + s2) - lo[id]) - x);
+  return neg ? -r : r;
+}
+
+__host__ __device__ inline float fdAtan2f(float y, float x) {
+  const float pi = 3.1415927410e+00f, pi_lo = -8.7422776573e-08f, pi_2 = 1.5707963705e+00f,
+              pi_4 = 7.8539818525e-01f, tiny = 1.0e-30f;
+  const uint32_t hx = lmBits(x), ix = hx & 0x7fffffffu, hy = lmBits(y), iy = hy & 0x7fffffffu;
+  if (ix > 0x7f800000u || iy > 0x7f800000u) return x + y; /* NaN */
+  if (hx == 0x3f800000u) return fdAtanf(y);                /* x = 1 */
+  const int m = (int)((hy >> 31) | ((hx >> 30) & 2u));     /* 2 sign(x) + sign(y) */
+  if (iy == 0) return m <= 1 ? y : (m == 2 ? pi + tiny : -pi - tiny);
+  if (ix == 0) return (hy >> 31) ? -pi_2 - tiny : pi_2 + tiny;
+  if (ix == 0x7f800000u) {
+    if (iy == 0x7f800000u) {
+      const float q[4] = {pi_4, -pi_4, 3.0f * pi_4, -3.0f * pi_4};
+      return q[m];
+    }
+    const float q[4] = {0.0f, -0.0f, pi, -pi};
+    return q[m];
+  }
+  if (iy == 0x7f800000u) return (hy >> 31) ? -pi_2 - tiny : pi_2 + tiny;
+  const int k = ((int)iy - (int)ix) >> 23;
+  float z;
+  if (k > 60)
+    z = pi_2 + 0.5f * pi_lo;
+  else if ((hx >> 31) && k < -60)
+    z = 0.0f;
+  else
+    z = fdAtanf(fabsf(y / x));
+  switch (m) {
+    case 0: return z;
+    case 1: return -z;
+    case 2: return pi - (z - pi_lo);
+    default: return (z - pi_lo) - pi;
+  }
+}
+
+/* rhip_libm_eval's id for atan2f over (y, x) pairs; it has no table. */
+constexpr int kLmAtan2 = kLmCount;
 
 }  // namespace rhip

@@ -219,6 +219,34 @@ void gammaTablesHost(float *t) {
 
 float hostPowf(float x, float y) { return g_powf(x, y); }
 
+float (*volatile g_sinf)(float) = sinf;
+float (*volatile g_cosf)(float) = cosf;
+
+void iblTablesHost(IblTables *t) {
+  memset(t, 0, sizeof *t);
+  /* IBLIrradianceSoft.h:84-85: float loop variables stepped by 0.025f while below 2 PI and PI / 2 */
+  const float pi = 3.14159265359f, delta = 0.025f;
+  int n = 0;
+  for (float phi = 0.0f; phi < 2.0f * pi && n < 256; phi += delta, n++) {
+    t->sphi[n] = g_sinf(phi);
+    t->cphi[n] = g_cosf(phi);
+  }
+  t->nphi = n;
+  n = 0;
+  for (float theta = 0.0f; theta < 0.5f * pi && n < 64; theta += delta, n++) {
+    t->stheta[n] = g_sinf(theta);
+    t->ctheta[n] = g_cosf(theta);
+  }
+  t->ntheta = n;
+  /* IBLPrefilterSoft.h:102-112: phi = 2 PI * (i / 1024) */
+  for (int i = 0; i < 1024; i++) {
+    const float xi = (float)i / (float)1024u;
+    const float phi = 2.0f * pi * xi;
+    t->hcos[i] = g_cosf(phi);
+    t->hsin[i] = g_sinf(phi);
+  }
+}
+
 }  // namespace rhip
 
 namespace {
@@ -241,7 +269,7 @@ __host__ __device__ inline float evalFn(const rhip::LibmTables *L, int fn, float
 
 __global__ void k_libm_eval(const rhip::LibmTables *L, int fn, const float *x, long long n, float *out) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    out[i] = evalFn(L, fn, x[i]);
+    out[i] = fn == rhip::kLmAtan2 ? rhip::fdAtan2f(x[2 * i], x[2 * i + 1]) : evalFn(L, fn, x[i]);
 }
 
 }  // namespace
@@ -252,7 +280,11 @@ int rhip_libm_info(int fn, long long *entries, double *gate) { return rhip::libm
 
 int rhip_libm_eval_host(int fn, const float *x, long long n, float *out) {
   using namespace rhip;
-  if (fn < 0 || fn >= kLmCount || n < 0 || (n && (!x || !out))) return fail(RHIP_E_INVALID, "libm eval arguments");
+  if (fn < 0 || fn > kLmAtan2 || n < 0 || (n && (!x || !out))) return fail(RHIP_E_INVALID, "libm eval arguments");
+  if (fn == kLmAtan2) {
+    for (long long i = 0; i < n; i++) out[i] = fdAtan2f(x[2 * i], x[2 * i + 1]);
+    return RHIP_OK;
+  }
   LibmTables L;
   int rc = libmHostTables(fn, &L);
   if (rc) return rc;
@@ -264,16 +296,17 @@ int rhip_libm_eval(rhip_device d, int fn, const float *x, long long n, float *ou
   using namespace rhip;
   int rc = checkDev(d);
   if (rc) return rc;
-  if (fn < 0 || fn >= kLmCount || n < 0 || (n && (!x || !out))) return fail(RHIP_E_INVALID, "libm eval arguments");
+  if (fn < 0 || fn > kLmAtan2 || n < 0 || (n && (!x || !out))) return fail(RHIP_E_INVALID, "libm eval arguments");
   if (!n) return RHIP_OK;
-  if ((rc = libmEnsure(d, 1u << fn))) return rc;
+  if (fn < kLmCount && (rc = libmEnsure(d, 1u << fn))) return rc;
+  const long long nin = fn == kLmAtan2 ? 2 * n : n;
   float *dx = nullptr, *dout = nullptr;
-  HIPCHK(hipMalloc(&dx, n * 4));
+  HIPCHK(hipMalloc(&dx, nin * 4));
   if (hipMalloc(&dout, n * 4) != hipSuccess) {
     hipFree(dx);
     return fail(RHIP_E_OOM, "libm eval buffers");
   }
-  hipError_t e = hipMemcpyAsync(dx, x, n * 4, hipMemcpyHostToDevice, d->stream);
+  hipError_t e = hipMemcpyAsync(dx, x, nin * 4, hipMemcpyHostToDevice, d->stream);
   if (e == hipSuccess) {
     const long long blocksNeeded = (n + 255) / 256;
     k_libm_eval<<<(unsigned)std::min<long long>(blocksNeeded, 8192), 256, 0, d->stream>>>(d->libm, fn, dx, n, dout);

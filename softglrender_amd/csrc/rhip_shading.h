@@ -246,6 +246,19 @@ __device__ __forceinline__ float4 runVS(const DrawParams &P, const float *a, flo
     }
   } else if (sh == RHS_SHADING_VERTEX_COLOR) {
     v[0] = a[4], v[1] = a[5], v[2] = a[6], v[3] = a[7];
+  } else if (sh == RHS_SHADING_SKYBOX) {
+    /* SkyboxSoft.h:66-75: depth at the far plane (w/w = 1), or 0 with reversed Z */
+    clip = make_float4(clip.x, clip.y, clip.w, clip.w);
+    if (u.reverse_z) clip.z = 0.f;
+    v[0] = a[0], v[1] = a[1], v[2] = a[2];
+  } else if (sh == RHS_SHADING_IBL_IRRADIANCE || sh == RHS_SHADING_IBL_PREFILTER) {
+    /* IBLIrradianceSoft.h:62-67, IBLPrefilterSoft.h:66-71 */
+    clip.z = clip.w;
+    v[0] = a[0], v[1] = a[1], v[2] = a[2];
+  } else if (sh == RHS_SHADING_FXAA) {
+    /* FxaaSoft.h:72-75: the quad is already in clip space */
+    clip = make_float4(a[0], a[1], a[2], 1.0f);
+    v[0] = a[4], v[1] = a[5];
   }
   return clip;
 }
@@ -535,6 +548,228 @@ __device__ __forceinline__ RGBAf fsPbr(const DrawParams &P, const Frag &f) {
     col = add3(col, {e.r, e.g, e.b});
   }
   return {col.x, col.y, col.z, in.albedo_rgba.a};
+}
+
+/* --------------------------------------------------------------------------- environment shaders */
+
+/* textureLodOffset (ShaderSoft.h:105-110, SamplerSoft.h:216-238): the texel
+ * offset is added after scaling to the level (bilinear: to the texel-space
+ * coordinate; nearest: to the floored texel). The reference build fuses the
+ * bilinear `uv * size + offset` into one multiply-add: a coordinate that
+ * rounds to a texel centre unshifted can land just past it shifted. */
+__device__ __forceinline__ uint32_t sampleLevelOff(const TexDesc &d, int level, f2 uv, bool linear, int ox, int oy) {
+  const TexLevel &l = d.lv[0][level];
+  if (linear)
+    return bilinear<uint32_t>(d, l, fmaf(uv.x, (float)l.w, (float)ox), fmaf(uv.y, (float)l.h, (float)oy), d.wrap);
+  const float tx = uv.x * (float)l.w, ty = uv.y * (float)l.h;
+  return fetchWrap<uint32_t>(d, l, (int)floorf(tx) + ox, (int)floorf(ty) + oy, d.wrap);
+}
+__device__ RGBAf tex2DOffset(const TexDesc *t, f2 uv, float lod, int ox, int oy) {
+  if (!t) return {0, 0, 0, 0};
+  const TexDesc &d = *t;
+  const int fm = d.filter_min;
+  if (fm <= 1) return unpack01(sampleLevelOff(d, 0, uv, fm == 1, ox, oy));
+  const int maxLevel = d.levels - 1;
+  if (fm == 2 || fm == 3) {
+    const int level = min(max((int)ceilf(lod + 0.5f) - 1, 0), maxLevel);
+    return unpack01(sampleLevelOff(d, level, uv, fm == 3, ox, oy));
+  }
+  const int hi = min(max((int)floorf(lod), 0), maxLevel), lo = min(max(hi + 1, 0), maxLevel);
+  const uint32_t th = sampleLevelOff(d, hi, uv, fm == 5, ox, oy);
+  if (hi == lo) return unpack01(th);
+  return unpack01(mixRGBA(th, sampleLevelOff(d, lo, uv, fm == 5, ox, oy), lod - floorf(lod)));
+}
+
+/* glm::dot and glm::normalize as the reference build compiles them in the
+ * skybox, prefilter and FXAA shaders: fma(z, z', fma(y, y', x x')), x first (the
+ * lighting shaders' order, dot3, starts from y). */
+__device__ __forceinline__ float dot3b(f3 a, f3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
+__device__ __forceinline__ f3 nrm3b(f3 v) { return scl3(v, 1.0f / sqrtf(dot3b(v, v))); }
+
+/* SkyboxSoft.h:78-96. The equirectangular lookup calls glibc's atan2f
+ * (reproduced step for step, fdAtan2f) and the double-precision asin, rounded
+ * to float; the reference build fuses `uv * scale + 0.5`. Only the Blinn-Phong
+ * and PBR shaders give their samplers a derivative LOD (BlinnPhongSoft.h:135-148,
+ * PbrSoft.h:138-151): the skybox's and FXAA's samplers read at LOD 0. */
+__device__ __forceinline__ RGBAf fsSkybox(const DrawParams &P, const Frag &f) {
+  const f3 wp{f.v[0], f.v[1], f.v[2]};
+  if (P.defines & RHS_DEF_EQUIRECTANGULAR_MAP) {
+    const f3 d = nrm3b(wp);
+    const float ax = fdAtan2f(d.z, d.x), ay = (float)asin((double)-d.y);
+    const f2 uv{fmaf(ax, 0.1591f, 0.5f), fmaf(ay, 0.3183f, 0.5f)};
+    return tex2D(P.tex[RHS_SLOT_EQUIRECT], uv, 0.f);
+  }
+  return texCube(P.tex[RHS_SLOT_CUBE], wp, 0.f);
+}
+
+/* IBLIrradianceSoft.h:74-101: the cosine-weighted hemisphere sum in the
+ * reference's loop order; sin and cos of the loop angles come from the host
+ * libm (IblTables). */
+__device__ RGBAf fsIrradiance(const DrawParams &P, const Frag &f) {
+  const f3 N = nrm3({f.v[0], f.v[1], f.v[2]});
+  f3 up{0.f, 1.f, 0.f};
+  const f3 right = nrm3(cross3(up, N));
+  up = nrm3(cross3(N, right));
+  const IblTables &T = *P.ibl;
+  const TexDesc *cube = P.tex[RHS_SLOT_CUBE];
+  f3 irr{0.f, 0.f, 0.f};
+  float nr = 0.f;
+  for (int i = 0; i < T.nphi; i++) {
+    const float sp = T.sphi[i], cp = T.cphi[i];
+    for (int j = 0; j < T.ntheta; j++) {
+      const float st = T.stheta[j], ct = T.ctheta[j];
+      const f3 ts{st * cp, st * sp, ct};
+      const f3 sv = add3(add3(scl3(right, ts.x), scl3(up, ts.y)), scl3(N, ts.z));
+      const RGBAf c = texCube(cube, sv, 0.f);
+      irr = add3(irr, {c.r * ct * st, c.g * ct * st, c.b * ct * st});
+      nr += 1.f;
+    }
+  }
+  irr = scl3(scl3(irr, kPI), 1.0f / nr);
+  return {irr.x, irr.y, irr.z, 1.f};
+}
+
+/* IBLPrefilterSoft.h:88-168: GGX importance sampling over 1024 Hammersley
+ * points, each sample read from the source mip level its pdf selects. The
+ * reference build contracts (a^2 - 1) Xi.y + 1, 1 - cos^2, the tangent-space
+ * sum, 2 (V.H) H - V, the GGX denominator, 1024 pdf + 1e-4 and the colour
+ * sum into fused multiply-adds; so does this one, term for term. */
+__device__ RGBAf fsPrefilter(const DrawParams &P, const Frag &f) {
+  const f3 N = nrm3b({f.v[0], f.v[1], f.v[2]});
+  const f3 V = N;
+  const float rough = P.u.aux[1], res = P.u.aux[0];
+  const IblTables &T = *P.ibl;
+  const TexDesc *cube = P.tex[RHS_SLOT_CUBE];
+  const float a = rough * rough;
+  const f3 upv = (double)fabsf(N.z) < 0.999 ? f3{0.f, 0.f, 1.f} : f3{1.f, 0.f, 0.f};
+  const f3 tangent = nrm3b(cross3(upv, N));
+  const f3 bitangent = cross3(N, tangent);
+  const float saTexel = 4.0f * kPI / (6.0f * res * res);
+  const float ra2 = a * a;
+  f3 acc{0.f, 0.f, 0.f};
+  float tw = 0.f;
+  for (uint32_t i = 0; i < 1024u; i++) {
+    const float xiy = (float)((double)__builtin_bitreverse32(i) * 2.3283064365386963e-10);
+    const float cosT = sqrtf((1.0f - xiy) / fmaf(ra2 - 1.0f, xiy, 1.0f));
+    const float sinT = sqrtf(fmaf(-cosT, cosT, 1.0f));
+    const f3 Ht{T.hcos[i] * sinT, T.hsin[i] * sinT, cosT};
+    const f3 H = nrm3b({fmaf(N.x, Ht.z, fmaf(bitangent.x, Ht.y, tangent.x * Ht.x)),
+                        fmaf(N.y, Ht.z, fmaf(bitangent.y, Ht.y, tangent.y * Ht.x)),
+                        fmaf(N.z, Ht.z, fmaf(bitangent.z, Ht.y, tangent.z * Ht.x))});
+    const float vh2 = 2.0f * dot3b(V, H);
+    const f3 L = nrm3b({fmaf(H.x, vh2, -V.x), fmaf(H.y, vh2, -V.y), fmaf(H.z, vh2, -V.z)});
+    const float NdotL = gmax(dot3b(N, L), 0.0f);
+    if (NdotL > 0.0f) {
+      const float NdotH = gmax(dot3b(N, H), 0.0f), HdotV = gmax(dot3b(H, V), 0.0f);
+      const float den = fmaf(NdotH * NdotH, ra2 - 1.0f, 1.0f);
+      const float D = ra2 / (kPI * den * den);
+      const float pdf = D * NdotH / (4.0f * HdotV) + 0.0001f;
+      const float saSample = 1.0f / fmaf(1024.0f, pdf, 0.0001f);
+      const float mip = rough == 0.0f ? 0.0f : 0.5f * hlog2f(P.libm, saSample / saTexel);
+      const RGBAf c = texCube(cube, L, mip);
+      acc = {fmaf(c.r, NdotL, acc.x), fmaf(c.g, NdotL, acc.y), fmaf(c.b, NdotL, acc.z)};
+      tw += NdotL;
+    }
+  }
+  return {acc.x / tw, acc.y / tw, acc.z / tw, 1.f};
+}
+
+/* FxaaSoft.h:81-266. The reference build contracts the luma dot product (red
+ * first), -2 l + corners, |.| 2 + |.|, 2 (DU + LR) + LC, (average - centre)
+ * and the sub-pixel smoothstep into fused multiply-adds; so does this one. */
+__device__ __forceinline__ float lumaOf(RGBAf c) { return dot3b({c.r, c.g, c.b}, {0.299f, 0.587f, 0.114f}); }
+__device__ __forceinline__ float fxaaQuality(float q) {
+  return q < 5.f ? 1.0f : (q > 5.f ? (q < 10.f ? 2.0f : (q < 11.f ? 4.0f : 8.0f)) : 1.5f);
+}
+__device__ RGBAf fsFxaa(const DrawParams &P, const Frag &f) {
+  const TexDesc *t = P.tex[RHS_SLOT_SCREEN];
+  const f2 tc{f.v[0], f.v[1]};
+  const float lod = 0.f;
+  const f2 inv{1.0f / P.u.aux[0], 1.0f / P.u.aux[1]};
+  const RGBAf cc = tex2D(t, tc, lod);
+  const float lC = lumaOf(cc);
+  const float lD = lumaOf(tex2DOffset(t, tc, 0.f, 0, -1)), lU = lumaOf(tex2DOffset(t, tc, 0.f, 0, 1));
+  const float lL = lumaOf(tex2DOffset(t, tc, 0.f, -1, 0)), lR = lumaOf(tex2DOffset(t, tc, 0.f, 1, 0));
+  const float lMin = gmin(lC, gmin(gmin(lD, lU), gmin(lL, lR)));
+  const float lMax = gmax(lC, gmax(gmax(lD, lU), gmax(lL, lR)));
+  const float range = lMax - lMin;
+  if (range < gmax(0.0312f, lMax * 0.125f)) return {cc.r, cc.g, cc.b, 1.f};
+  const float lDL = lumaOf(tex2DOffset(t, tc, 0.f, -1, -1)), lUR = lumaOf(tex2DOffset(t, tc, 0.f, 1, 1));
+  const float lUL = lumaOf(tex2DOffset(t, tc, 0.f, -1, 1)), lDR = lumaOf(tex2DOffset(t, tc, 0.f, 1, -1));
+  const float lDU = lD + lU, lLR = lL + lR;
+  const float lLC = lDL + lUL, lDC = lDL + lDR, lRC = lDR + lUR, lUC = lUR + lUL;
+  const float eH = fmaf(fabsf(fmaf(-2.0f, lC, lDU)), 2.0f, fabsf(fmaf(-2.0f, lL, lLC))) + fabsf(fmaf(-2.0f, lR, lRC));
+  const float eV = fmaf(fabsf(fmaf(-2.0f, lC, lLR)), 2.0f, fabsf(fmaf(-2.0f, lU, lUC))) + fabsf(fmaf(-2.0f, lD, lDC));
+  const bool hor = eH >= eV;
+  float step = hor ? inv.y : inv.x;
+  const float l1 = hor ? lD : lL, l2 = hor ? lU : lR;
+  const float g1 = l1 - lC, g2 = l2 - lC;
+  const bool steep1 = fabsf(g1) >= fabsf(g2);
+  const float gs = 0.25f * gmax(fabsf(g1), fabsf(g2));
+  float avg;
+  if (steep1) {
+    step = -step;
+    avg = 0.5f * (l1 + lC);
+  } else {
+    avg = 0.5f * (l2 + lC);
+  }
+  f2 cur = tc;
+  if (hor)
+    cur.y += step * 0.5f;
+  else
+    cur.x += step * 0.5f;
+  const f2 off = hor ? f2{inv.x, 0.0f} : f2{0.0f, inv.y};
+  f2 uv1{cur.x - off.x * fxaaQuality(0.f), cur.y - off.y * fxaaQuality(0.f)};
+  f2 uv2{cur.x + off.x * fxaaQuality(0.f), cur.y + off.y * fxaaQuality(0.f)};
+  float e1 = 0.f, e2 = 0.f;
+  bool r1 = false, r2 = false;
+  for (int i = 1; i < 12; i++) {
+    if (!r1) {
+      e1 = lumaOf(tex2D(t, uv1, lod)) - avg;
+      r1 = fabsf(e1) >= gs;
+    }
+    if (!r2) {
+      e2 = lumaOf(tex2D(t, uv2, lod)) - avg;
+      r2 = fabsf(e2) >= gs;
+    }
+    const bool both = r1 && r2;
+    const float q = fxaaQuality((float)i);
+    if (!r1) uv1 = {uv1.x - off.x * q, uv1.y - off.y * q};
+    if (!r2) uv2 = {uv2.x + off.x * q, uv2.y + off.y * q};
+    if (both) break;
+  }
+  const float d1 = hor ? (tc.x - uv1.x) : (tc.y - uv1.y);
+  const float d2 = hor ? (uv2.x - tc.x) : (uv2.y - tc.y);
+  const bool dir1 = d1 < d2;
+  const float dmin = gmin(d1, d2);
+  const bool smaller = lC < avg;
+  const bool ok1 = (e1 < 0.0f) != smaller, ok2 = (e2 < 0.0f) != smaller;
+  const bool ok = dir1 ? ok1 : ok2;
+  const float edgeLen = d1 + d2;
+  const float pxOff = -dmin / edgeLen + 0.5f;
+  float fin = ok ? pxOff : 0.0f;
+  const float lSum = fmaf(2.0f, lDU + lLR, lLC) + lRC;
+  const float sp1 = gclamp(fabsf(fmaf(lSum, 1.0f / 12.0f, -lC)) / range, 0.0f, 1.0f);
+  const float sp2 = fmaf(-2.0f, sp1, 3.0f) * sp1 * sp1;
+  const float spF = sp2 * sp2 * 0.75f;
+  fin = gmax(fin, spF);
+  f2 fuv = tc;
+  if (hor)
+    fuv.y += fin * step;
+  else
+    fuv.x += fin * step;
+  const RGBAf fc = tex2D(t, fuv, lod);
+  return {fc.r, fc.g, fc.b, 1.f};
+}
+
+/* The environment shaders, one specialisation switching on the draw's model. */
+__device__ __forceinline__ RGBAf fsEnv(const DrawParams &P, const Frag &f) {
+  switch (P.shading) {
+    case RHS_SHADING_SKYBOX: return fsSkybox(P, f);
+    case RHS_SHADING_IBL_IRRADIANCE: return fsIrradiance(P, f);
+    case RHS_SHADING_IBL_PREFILTER: return fsPrefilter(P, f);
+    default: return fsFxaa(P, f);
+  }
 }
 
 /* --------------------------------------------------------------------------- per-sample operations */

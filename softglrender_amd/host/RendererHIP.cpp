@@ -25,6 +25,8 @@ int blockLocation(const std::string &name) {
   if (name == "UniformsModel") return 0;
   if (name == "UniformsScene") return 1;
   if (name == "UniformsMaterial") return 2;
+  /* the IBL prefilter's and the quad filter's blocks take the material block's place (Material.h:88-95) */
+  if (name == "UniformsPrefilter" || name == "UniformsQuadFilter") return 2;
   return -1;
 }
 
@@ -34,6 +36,10 @@ int samplerSlot(const std::string &name) {
                                               "u_prefilterMap"};
   for (int i = 0; i < RHS_SLOT_COUNT; i++)
     if (name == names[i]) return i;
+  /* the skybox, IBL and FXAA shaders (SkyboxSoft.h, IBLIrradianceSoft.h, FxaaSoft.h) */
+  if (name == "u_equirectangularMap") return RHS_SLOT_EQUIRECT;
+  if (name == "u_screenTexture") return RHS_SLOT_SCREEN;
+  if (name == "u_cubeMap") return RHS_SLOT_CUBE;
   return -1;
 }
 
@@ -157,8 +163,9 @@ void FrameBufferHIP::setDepthAttachment(std::shared_ptr<Texture> &depth) {
 /* ------------------------------------------------------------------ programs and uniforms */
 
 void ShaderProgramHIP::addDefine(const std::string &def) {
-  static const char *names[5] = {"ALBEDO_MAP", "NORMAL_MAP", "EMISSIVE_MAP", "AO_MAP", "METALROUGHNESS_MAP"};
-  for (int i = 0; i < 5; i++)
+  static const char *names[6] = {"ALBEDO_MAP", "NORMAL_MAP",         "EMISSIVE_MAP",
+                                 "AO_MAP",     "METALROUGHNESS_MAP", "EQUIRECTANGULAR_MAP"};
+  for (int i = 0; i < 6; i++)
     if (def == names[i]) defines_ |= 1u << i;
 }
 

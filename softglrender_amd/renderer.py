@@ -23,10 +23,13 @@ RENDERER_HIP = "HIP"  # next to Renderer_SOFT / _OPENGL / _Vulkan (Renderer.h:18
 
 _SLOT_BY_NAME = {"u_albedoMap": S.SLOT_ALBEDO, "u_normalMap": S.SLOT_NORMAL, "u_emissiveMap": S.SLOT_EMISSIVE,
                  "u_aoMap": S.SLOT_AO, "u_shadowMap": S.SLOT_SHADOW, "u_metalRoughnessMap": S.SLOT_METALROUGH,
-                 "u_irradianceMap": S.SLOT_IRRADIANCE, "u_prefilterMap": S.SLOT_PREFILTER}
+                 "u_irradianceMap": S.SLOT_IRRADIANCE, "u_prefilterMap": S.SLOT_PREFILTER,
+                 "u_equirectangularMap": S.SLOT_EQUIRECT, "u_screenTexture": S.SLOT_SCREEN, "u_cubeMap": S.SLOT_CUBE}
 _BLOCK_SIZES = {"UniformsModel": S.MODEL_BYTES, "UniformsScene": S.SCENE_BYTES, "UniformsMaterial": S.MATERIAL_BYTES}
+# the IBL prefilter's and the quad filter's blocks take the material block's place (Material.h:88-95)
+_BLOCK_ALIASES = {"UniformsPrefilter": "UniformsMaterial", "UniformsQuadFilter": "UniformsMaterial"}
 _DEFINES = {"ALBEDO_MAP": S.DEF_ALBEDO, "NORMAL_MAP": S.DEF_NORMAL, "EMISSIVE_MAP": S.DEF_EMISSIVE, "AO_MAP": S.DEF_AO,
-            "METALROUGHNESS_MAP": S.DEF_METALROUGH}
+            "METALROUGHNESS_MAP": S.DEF_METALROUGH, "EQUIRECTANGULAR_MAP": S.DEF_EQUIRECT}
 
 
 @dataclass
@@ -348,7 +351,7 @@ class RendererHIP:
         samplers = [0] * 8
         if self._res is not None:
             for b in self._res.blocks.values():
-                blocks[b.name] = bytes(b.buffer)
+                blocks[_BLOCK_ALIASES.get(b.name, b.name)] = bytes(b.buffer)
             for s in self._res.samplers.values():
                 if s.texture is not None and s.name in _SLOT_BY_NAME:
                     samplers[_SLOT_BY_NAME[s.name]] = s.texture.handle

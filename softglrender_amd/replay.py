@@ -13,7 +13,19 @@ from .renderer import (ClearStates, RendererHIP, SamplerDesc, ShaderResources, T
 
 _SLOT_NAMES = ["u_albedoMap", "u_normalMap", "u_emissiveMap", "u_aoMap", "u_shadowMap", "u_metalRoughnessMap",
                "u_irradianceMap", "u_prefilterMap"]
-_DEF_NAMES = ["ALBEDO_MAP", "NORMAL_MAP", "EMISSIVE_MAP", "AO_MAP", "METALROUGHNESS_MAP"]
+_DEF_NAMES = ["ALBEDO_MAP", "NORMAL_MAP", "EMISSIVE_MAP", "AO_MAP", "METALROUGHNESS_MAP", "EQUIRECTANGULAR_MAP"]
+
+
+def _slot_name(shading, slot):
+    """The sampler name a shader declares for a slot (SkyboxSoft.h, IBL*Soft.h and FxaaSoft.h name theirs)."""
+    if shading == S.SHADING_FXAA and slot == S.SLOT_SCREEN:
+        return "u_screenTexture"
+    if shading in (S.SHADING_SKYBOX, S.SHADING_IBL_IRRADIANCE, S.SHADING_IBL_PREFILTER):
+        if slot == S.SLOT_CUBE:
+            return "u_cubeMap"
+        if slot == S.SLOT_EQUIRECT:
+            return "u_equirectangularMap"
+    return _SLOT_NAMES[slot]
 
 
 def parse(data: bytes):
@@ -106,7 +118,7 @@ class Replayer:
             res = ShaderResources(blocks=dict(self.ub))
             for i in range(ns):
                 slot, tid = struct.unpack_from("<2i", p, 440 + 8 * i)
-                name = _SLOT_NAMES[slot]
+                name = _slot_name(shading, slot)
                 if name not in self.samplers:
                     self.samplers[name] = r.createUniformSampler(name, TextureDesc())
                 self.samplers[name].setTexture(self.tex[tid])

@@ -9,6 +9,7 @@ follow src/Viewer/Camera.cpp:26-69 (infinite-far projection, optional reverse Z)
 from __future__ import annotations
 
 import math
+import struct
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -406,6 +407,152 @@ def ibl_maps(seed=7, irr_size=32, pref_size=128, pref_levels=5):
         faces = [_to_u8(np.clip(_sh_basis(d) @ (coef * w[:, None]), 0, 1) * 0.8) for d in _cube_dirs(s)]
         levels.append([np.concatenate([f, np.full(f.shape[:2] + (1,), 255, np.uint8)], -1) for f in faces])
     return irr, levels
+
+
+def env_cube_faces(size, seed=7):
+    """The six RGBA8 faces of a seeded smooth low-frequency radiance cube (SH radiance, quantised to bytes:
+    the reference's cube maps are RGBA8, Texture.h:63-66), face order and orientation as the cube sampler's."""
+    coef = ibl_coefficients(seed)
+    out = []
+    for d in _cube_dirs(size):
+        f = _to_u8(np.clip(_sh_basis(d) @ coef, 0, 1) * 0.8)
+        out.append(np.concatenate([f, np.full(f.shape[:2] + (1,), 255, np.uint8)], -1))
+    return out
+
+
+def cube_mesh():
+    """A cube of half-size 1 around the origin, 12 triangles of unshared vertices (the IBL and skybox mesh,
+    ModelLoader.cpp:30-46), in the 64-byte Vertex layout."""
+    q = [((1, 1, 1), (1, -1, 1), (1, -1, -1), (1, 1, -1)), ((-1, 1, -1), (-1, -1, -1), (-1, -1, 1), (-1, 1, 1)),
+         ((-1, 1, -1), (-1, 1, 1), (1, 1, 1), (1, 1, -1)), ((-1, -1, 1), (-1, -1, -1), (1, -1, -1), (1, -1, 1)),
+         ((-1, 1, 1), (-1, -1, 1), (1, -1, 1), (1, 1, 1)), ((1, 1, -1), (1, -1, -1), (-1, -1, -1), (-1, 1, -1))]
+    pos = []
+    for a, b, c, d in q:
+        pos += [a, b, c, a, c, d]
+    pos = np.array(pos, np.float32)
+    z2 = np.zeros((len(pos), 2), np.float32)
+    z3 = np.zeros((len(pos), 3), np.float32)
+    return _pack_vertices(pos, z2, z3, z3), np.arange(len(pos), dtype=np.int32).reshape(-1, 3)
+
+
+# IBLGenerator::drawCubeFaces (Environment.cpp:152-195): look-at targets and up vectors of the six faces
+_CUBE_VIEWS = [((1, 0, 0), (0, -1, 0)), ((-1, 0, 0), (0, -1, 0)), ((0, 1, 0), (0, 0, 1)),
+               ((0, -1, 0), (0, 0, -1)), ((0, 0, 1), (0, -1, 0)), ((0, 0, -1), (0, -1, 0))]
+
+
+def ibl_precompute(st, vao, src_tex, irr_tex, pref_tex, src_size, irr_size, pref_size, pref_levels=5, first_fbo=100):
+    """Appends the reference's IBL precompute (Environment.cpp:48-103, 152-195) to a stream: the irradiance
+    convolution of `src_tex` into the faces of `irr_tex`, then GGX prefiltering into `pref_size`-px faces of
+    `pref_levels` mip levels of `pref_tex`, each face one render pass of the cube mesh with a 90-degree
+    camera at the origin (rotation only). Returns the next free framebuffer id."""
+    proj = projection(math.radians(90.0), 1.0, 0.1)
+    fid = first_fbo
+    ident = np.eye(4)
+    jobs = [(S.SHADING_IBL_IRRADIANCE, irr_tex, 0, irr_size, bytes(48))]
+    for lev in range(pref_levels):
+        rough = lev / (pref_levels - 1)
+        mat = struct.pack("<2f", float(src_size), rough) + bytes(40)
+        jobs.append((S.SHADING_IBL_PREFILTER, pref_tex, lev, max(1, pref_size >> lev), mat))
+    for shading, tex, level, size, mat in jobs:
+        for face, (center, up) in enumerate(_CUBE_VIEWS):
+            view = look_at((0, 0, 0), center, up)
+            view3 = np.eye(4)
+            view3[:3, :3] = view[:3, :3]
+            st.fbo(fid, color=tex, color_layer=face, color_level=level)
+            st.begin(fid, True, False, (0, 0, 0, 0), 1.0)
+            st.viewport(0, 0, size, size)
+            st.draw(vao, shading, 0, S.RenderStates(), S.uniforms_model(ident, proj @ view3 @ ident),
+                    bytes(64), mat, samplers=[(S.SLOT_CUBE, src_tex)])
+            st.end()
+            fid += 1
+    return fid
+
+
+def scene_ibl(src_size=32, irr_size=8, pref_size=16, pref_levels=5, seed=7, width=96, height=64, equirect=False,
+              fxaa=False, flat=None):
+    """IBL precompute (irradiance and prefiltered maps drawn from a mipmapped source cube), then a skybox pass
+    over a shaded sphere (the cube map, or an equirectangular image with `equirect`), optionally followed by
+    an FXAA pass over the result; every map and the final image are read back. `flat` (RGB bytes) makes
+    every source texel that one colour."""
+    st = S.Stream()
+    cv, ci = cube_mesh()
+    st.vao(1, cv, ci)
+    trilinear = dict(filter_min=S.FILTER_LINEAR_MIPMAP_LINEAR, filter_mag=S.FILTER_LINEAR, wrap=S.WRAP_CLAMP_TO_EDGE)
+    st.texture(10, src_size, src_size, type=S.TEX_CUBE, mipmaps=True, usage=S.USAGE_SAMPLER | S.USAGE_UPLOAD,
+               **trilinear)
+    faces = env_cube_faces(src_size, seed)
+    if flat is not None:
+        for f in faces:
+            f[..., :3] = np.asarray(flat, np.uint8)
+    st.tex_data(10, faces)
+    st.texture(11, irr_size, irr_size, type=S.TEX_CUBE, usage=S.USAGE_COLOR | S.USAGE_SAMPLER,
+               filter_min=S.FILTER_LINEAR, filter_mag=S.FILTER_LINEAR, wrap=S.WRAP_CLAMP_TO_EDGE)
+    st.tex_init(11)
+    st.texture(12, pref_size, pref_size, type=S.TEX_CUBE, mipmaps=True, usage=S.USAGE_COLOR | S.USAGE_SAMPLER,
+               **trilinear)
+    st.tex_init(12)
+    if equirect:
+        rng = np.random.default_rng(seed)
+        eq = np.zeros((src_size, 2 * src_size, 4), np.uint8)
+        eq[..., :3] = _to_u8(0.5 + 0.45 * np.sin(np.linspace(0, 6 * np.pi, 2 * src_size))[None, :, None]
+                             * np.cos(np.linspace(0, 3 * np.pi, src_size))[:, None, None]
+                             * rng.uniform(0.6, 1.0, (1, 1, 3)))
+        eq[..., 3] = 255
+        if flat is not None:
+            eq[..., :3] = np.asarray(flat, np.uint8)
+        st.texture(13, 2 * src_size, src_size, mipmaps=True, usage=S.USAGE_SAMPLER | S.USAGE_UPLOAD,
+                   filter_min=S.FILTER_LINEAR_MIPMAP_LINEAR, filter_mag=S.FILTER_LINEAR, wrap=S.WRAP_REPEAT)
+        st.tex_data(13, [eq])
+    st.texture(2, width, height, usage=S.USAGE_COLOR | S.USAGE_SAMPLER, filter_min=S.FILTER_LINEAR,
+               filter_mag=S.FILTER_LINEAR, wrap=S.WRAP_CLAMP_TO_EDGE)
+    st.tex_init(2)
+    st.texture(3, width, height, format=S.FMT_FLOAT32, usage=S.USAGE_DEPTH)
+    st.tex_init(3)
+    st.fbo(1, color=2, depth=3)
+    sv, si = uv_sphere(24, 12, 0.6)
+    st.vao(2, sv, si)
+    if fxaa:
+        st.texture(4, width, height, usage=S.USAGE_COLOR)
+        st.tex_init(4)
+        st.fbo(2, color=4)
+        qv = np.zeros((4, 16), np.float32)
+        qv[:, 0:2] = [[1, -1], [-1, -1], [1, 1], [-1, 1]]
+        qv[:, 4:6] = [[1, 0], [0, 0], [1, 1], [0, 1]]
+        st.vao(3, qv, np.array([[0, 1, 2], [1, 2, 3]], np.int32))
+    ibl_precompute(st, 1, 10, 11, 12, src_size, irr_size, pref_size, pref_levels)
+    st.frame()
+    eye = (0.0, 0.4, 2.2)
+    view = look_at(eye, (0, 0, 0), (0, 1, 0))
+    proj = projection(math.radians(60.0), width / height, 0.1)
+    st.begin(1, True, True, (0, 0, 0, 0), 1.0)
+    st.viewport(0, 0, width, height)
+    st.draw(2, S.SHADING_PBR, 0, _common_states(S.DEPTH_LESS), S.uniforms_model(np.eye(4), proj @ view),
+            S.uniforms_scene((0.1, 0.1, 0.1), eye, (2.0, 2.0, 2.0), (1.0, 1.0, 1.0)),
+            S.uniforms_material(0, 1, 0, 1.0, 0.5, (0.8, 0.6, 0.4, 1.0)),
+            samplers=[(S.SLOT_IRRADIANCE, 11), (S.SLOT_PREFILTER, 12)])
+    sky = S.RenderStates(depth_test=True, depth_func=S.DEPTH_LEQUAL)
+    sky.depth_mask = False
+    view3 = np.eye(4)
+    view3[:3, :3] = view[:3, :3]
+    st.draw(1, S.SHADING_SKYBOX, S.DEF_EQUIRECT if equirect else 0, sky, S.uniforms_model(np.eye(4), proj @ view3),
+            bytes(64), bytes(48), samplers=[(S.SLOT_EQUIRECT, 13)] if equirect else [(S.SLOT_CUBE, 10)])
+    st.end()
+    if fxaa:
+        st.begin(2, True, False, (0, 0, 0, 0), 1.0)
+        st.viewport(0, 0, width, height)
+        st.draw(3, S.SHADING_FXAA, 0, S.RenderStates(), S.uniforms_model(np.eye(4), np.eye(4)), bytes(64),
+                struct.pack("<2f", float(width), float(height)) + bytes(40), samplers=[(S.SLOT_SCREEN, 2)])
+        st.end()
+    for face in range(6):
+        st.read(11, layer=face)
+    for lev in range(pref_levels):
+        for face in range(6):
+            st.read(12, layer=face, level=lev)
+    st.read(2)
+    st.read(3)
+    if fxaa:
+        st.read(4)
+    return Scene("ibl", st, width, height, False, {})
 
 
 def pbr_textures(size, seed):

@@ -38,10 +38,13 @@ POLY_POINT, POLY_LINE, POLY_FILL = range(3)
 
 # Shading models and defines (rhip_stream.h)
 SHADING_BASIC, SHADING_BLINN_PHONG, SHADING_PBR, SHADING_BLINN_PHONG_DIR, SHADING_VERTEX_COLOR, \
-    SHADING_NEURAL_PBR = range(6)
-DEF_ALBEDO, DEF_NORMAL, DEF_EMISSIVE, DEF_AO, DEF_METALROUGH = 1, 2, 4, 8, 16
+    SHADING_NEURAL_PBR, SHADING_SKYBOX, SHADING_IBL_IRRADIANCE, SHADING_IBL_PREFILTER, SHADING_FXAA = range(10)
+DEF_ALBEDO, DEF_NORMAL, DEF_EMISSIVE, DEF_AO, DEF_METALROUGH, DEF_EQUIRECT = 1, 2, 4, 8, 16, 32
 SLOT_ALBEDO, SLOT_NORMAL, SLOT_EMISSIVE, SLOT_AO, SLOT_SHADOW, SLOT_METALROUGH, SLOT_IRRADIANCE, \
     SLOT_PREFILTER = range(8)
+# the skybox, IBL and FXAA shaders' samplers (rhip_stream.h RHS_SLOT_EQUIRECT / _SCREEN / _CUBE)
+SLOT_EQUIRECT = SLOT_SCREEN = SLOT_ALBEDO
+SLOT_CUBE = SLOT_IRRADIANCE
 
 MODEL_BYTES, SCENE_BYTES, MATERIAL_BYTES = 256, 64, 48
 

@@ -2,7 +2,7 @@
 RendererSoft (the serial build compiled from the reference sources by
 oracle/Makefile). Run in a container where /root/reference is present:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [name ...]   (default: every fixture)
 
 Each fixture holds the outputs (RHS_READ order) of one seeded scene from
 softglrender_amd/scenes.py; the scene is rebuilt from its name and parameters
@@ -27,7 +27,9 @@ from tests.golden_scenes import GOLDEN  # noqa: E402
 def main():
     if not oracle.have_reference():
         oracle.build(with_reference=True)
-    for name, make in GOLDEN.items():
+    names = sys.argv[1:] or list(GOLDEN)
+    for name in names:
+        make = GOLDEN[name]
         sc = make()
         out, _ = oracle.ref_replay(sc.stream.bytes(), sc.stream.output_bytes())
         arrays = sc.stream.split_outputs(out)

@@ -7,5 +7,8 @@ GOLDEN = {
     "c2_small": lambda: scenes.scene_c2(480, 270, freq=40, tex_size=256, shadow_size=512, read_shadow=True,
                                         read_samples=True),
 }
+# image-based lighting precompute (irradiance, prefiltered levels), skybox (cube and equirectangular) and FXAA
+GOLDEN["ibl_cube"] = lambda: scenes.scene_ibl()
+GOLDEN["ibl_equirect_fxaa"] = lambda: scenes.scene_ibl(equirect=True, fxaa=True)
 for _v in scenes.VARIANTS:
     GOLDEN[f"v_{_v}"] = (lambda v=_v: scenes.scene_variant(v))

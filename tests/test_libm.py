@@ -1,5 +1,6 @@
 """The host libm's log2f, exp2f and powf as the shaders call them (ShaderSoft.h:132, PbrSoft.h:201-235,313,
-BlinnPhongSoft.h:227), reproduced from librhip's tables (rhip_libm.h).
+BlinnPhongSoft.h:227), reproduced from librhip's tables (rhip_libm.h; log2f by glibc's own algorithm, with an
+empty table), and glibc's atan2f (SkyboxSoft.h:84) restated step for step.
 
 CPU: the table lookup run on the host (rhip_libm_eval_host) against glibc itself, called through ctypes, on
 dense runs of consecutive floats and on random inputs of every function's range; the runs hold inputs where
@@ -22,7 +23,8 @@ INV_GAMMA = float(np.float32(1.0) / np.float32(2.2))
 
 # fn: (name, glibc call as the shaders make it, input range (inclusive bits), dense runs (start, count))
 FNS = {
-    0: ("log2f(x)", lambda x: LIBM.log2f(x), (0x3F800000, 0x7F7FFFFF), [(0x3F800000, 1 << 17), (0x4B3A0000, 1 << 17)]),
+    0: ("log2f(x)", lambda x: LIBM.log2f(x), (0x00000001, 0x7F7FFFFF),
+        [(0x3F780000, 1 << 18), (0x4B3A0000, 1 << 17), (0x00000001, 1 << 12), (0x00800000 - 4096, 8192)]),
     1: ("exp2f(-x)", lambda x: LIBM.exp2f(-x), (0x32000000, 0x41800000), [(0x3F000000, 1 << 17), (0x41000000, 1 << 16)]),
     2: ("powf(x, 5)", lambda x: LIBM.powf(x, 5.0), (0x30000000, 0x3F802000), [(0x3F400000, 1 << 17), (0x3E000000, 1 << 16)]),
     3: ("powf(x, 128)", lambda x: LIBM.powf(x, 128.0), (0x3ECCCCCD, 0x3F802000), [(0x3F7E0000, 1 << 17)]),
@@ -57,7 +59,10 @@ def test_table_lookup_reproduces_glibc(fn):
     lib = _lib.load()
     entries, gate = ctypes.c_longlong(), ctypes.c_double()
     _lib.check(lib.rhip_libm_info(fn, ctypes.byref(entries), ctypes.byref(gate)))
-    assert entries.value > 1000 and 0.0 <= gate.value < 0.5
+    if fn == 0:  # log2f: glibc's own algorithm, restated, equals the host's on every float: no exceptions
+        assert entries.value == 0 and gate.value < 0.0
+    else:
+        assert entries.value > 1000 and 0.0 <= gate.value < 0.5
     x = inputs(fn)
     want = glibc(fn, x)
     got = host_eval(fn, x)
@@ -95,3 +100,58 @@ def test_device_reproduces_glibc(renderer, fn):
     dev = np.empty_like(y)
     _lib.check(lib.rhip_libm_eval(renderer.dev, fn, y.ctypes.data, y.size, dev.ctypes.data))
     assert np.array_equal(dev.view(np.uint32), host_eval(fn, y).view(np.uint32)), FNS[fn][0]
+
+
+# ---------------------------------------------------------------- atan2f (SkyboxSoft.h:84), fn 5, no table
+
+LIBM.atan2f.restype = ctypes.c_float
+LIBM.atan2f.argtypes = [ctypes.c_float, ctypes.c_float]
+ATAN2 = 5
+
+
+def atan2_pairs(seed, n):
+    """(y, x) pairs: unit-vector components as the skybox makes them, arbitrary bit patterns (every exponent,
+    subnormals, infinities, NaNs) and the special cases fdlibm branches on."""
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    unit = d[:, [2, 0]].astype(np.float32)
+    bits = rng.integers(0, 2 ** 32, (n, 2), dtype=np.uint64).astype(np.uint32).view(np.float32)
+    vals = np.array([0.0, -0.0, 1.0, -1.0, np.inf, -np.inf, np.nan, 1e-30, -1e30, 2.0 ** -149, 0.4375, 1.1875,
+                     2.4375, 2.0 ** 25, 2.0 ** -29], np.float32)
+    special = np.array([(a, b) for a in vals for b in vals], np.float32)
+    return np.ascontiguousarray(np.concatenate([unit, bits, special]))
+
+
+def glibc_atan2(yx):
+    return np.array([LIBM.atan2f(float(a), float(b)) for a, b in yx], np.float32)
+
+
+def same_float(a, b):
+    return (a.view(np.uint32) == b.view(np.uint32)) | (np.isnan(a) & np.isnan(b))
+
+
+def test_atan2f_restatement_reproduces_glibc():
+    lib = _lib.load()
+    yx = atan2_pairs(3, 100_000)
+    got = np.empty(len(yx), np.float32)
+    _lib.check(lib.rhip_libm_eval_host(ATAN2, yx.ctypes.data, len(yx), got.ctypes.data))
+    want = glibc_atan2(yx)
+    bad = np.flatnonzero(~same_float(got, want))
+    assert bad.size == 0, (yx[bad[:5]], got[bad[:5]], want[bad[:5]])
+    # glibc's atan2f is not correctly rounded: the restatement is checked where it matters
+    with np.errstate(invalid="ignore"):
+        cr = np.arctan2(yx[:, 0].astype(np.float64), yx[:, 1].astype(np.float64)).astype(np.float32)
+    assert int((~same_float(cr, want)).sum()) > 1000
+
+
+@pytest.mark.gpu
+def test_device_atan2f_reproduces_glibc(renderer):
+    lib = _lib.load()
+    print("[gpu] libm atan2f", flush=True)
+    yx = atan2_pairs(11, 100_000)
+    out = np.empty(len(yx), np.float32)
+    _lib.check(lib.rhip_libm_eval(renderer.dev, ATAN2, yx.ctypes.data, len(yx), out.ctypes.data))
+    want = glibc_atan2(yx)
+    bad = np.flatnonzero(~same_float(out, want))
+    assert bad.size == 0, (yx[bad[:5]], out[bad[:5]], want[bad[:5]])
