@@ -830,7 +830,8 @@ int rhip_program_create(rhip_device d, int shading, uint32_t defines, int *out) 
   if (!d || !out) return fail(RHIP_E_INVALID, "null argument");
   if (shading < RHS_SHADING_BASIC || shading > RHS_SHADING_FXAA)
     return fail(RHIP_E_UNSUPPORTED, "shading model %d", shading);
-  if (defines & ~31u) return fail(RHIP_E_INVALID, "unknown define bits 0x%x", defines);
+  if (defines & ~(uint32_t)(2 * RHS_DEF_EQUIRECTANGULAR_MAP - 1))
+    return fail(RHIP_E_INVALID, "unknown define bits 0x%x", defines);
   int id = d->next_id++;
   d->prog[id] = Program{shading, defines};
   *out = id;
