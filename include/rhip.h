@@ -123,7 +123,9 @@ int rhip_texture_destroy(rhip_device dev, int tex);
 
 /* ---------------------------------------------------------------- vertex arrays */
 /* createVertexArrayObject (Renderer.h:37): float attributes, `stride` bytes per
- * vertex, int32 triangle indices. */
+ * vertex, int32 indices (read as points, lines or triangles by the draw's
+ * primitive type; a trailing partial primitive is ignored, as the reference's
+ * assembly does: RendererSoft.cpp:410-437). */
 int rhip_vao_create(rhip_device dev, const void *vertices, int stride, int nverts, const int32_t *indices,
                     int nindices, int *out_vao);
 int rhip_vao_update(rhip_device dev, int vao, const void *vertices, size_t bytes); /* VertexArrayObject::updateVertexData (Vertex.h:18) */
@@ -143,7 +145,12 @@ int rhip_program_create(rhip_device dev, int shading, uint32_t defines, int *out
 /* ---------------------------------------------------------------- pipeline */
 int rhip_begin_render_pass(rhip_device dev, int fbo, const rhip_clear_states *clear); /* beginRenderPass (Renderer.h:50) */
 int rhip_set_viewport(rhip_device dev, int x, int y, int width, int height);          /* setViewPort (Renderer.h:51) */
-int rhip_draw(rhip_device dev, const rhip_draw_desc *draw);                           /* set*(...) + draw (Renderer.h:52-56) */
+/* set*(...) + draw (Renderer.h:52-56). Filled triangles draw with every
+ * shading model; points, lines and triangles in point or line mode
+ * (RendererSoft.cpp:560-718) with the base colour and vertex colour models
+ * (others: RHIP_E_UNSUPPORTED), at point sizes and line widths up to 1024;
+ * such a draw waits once for its fragment count before it sizes the list. */
+int rhip_draw(rhip_device dev, const rhip_draw_desc *draw);
 /* rhip_draw without any device allocation. No counterpart in the reference:
  * it is for callers that cannot allocate (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc). The draw's arena lives in a
@@ -151,7 +158,8 @@ int rhip_draw(rhip_device dev, const rhip_draw_desc *draw);                     
  * every call on one GPU shares and that keeps the last call's contents: calls
  * that overlap on one GPU (two devices' streams, or another thread) race. A
  * draw whose worst-case arena exceeds it fails with RHIP_E_UNSUPPORTED (about
- * 4k triangles at 640x480 with 4x MSAA fit). Results equal rhip_draw's. */
+ * 4k triangles at 640x480 with 4x MSAA fit), and so does a draw of points or
+ * lines (their fragment list is sized at draw time). Results equal rhip_draw's. */
 int rhip_draw_noalloc(rhip_device dev, const rhip_draw_desc *draw);
 size_t rhip_draw_noalloc_capacity(void);
 int rhip_end_render_pass(rhip_device dev);                                            /* endRenderPass (Renderer.h:57) */

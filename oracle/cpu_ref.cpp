@@ -1338,6 +1338,178 @@ struct Renderer {
     }
   }
 
+  /* ---------------------------------------------------------------- points and lines (RendererSoft.cpp:575-718) */
+
+  /* gl->PointSize after the vertex shader: BasicSoft.h:68 sets u_pointSize; the others leave the default 1 */
+  float pointSize() const { return ctx.shading == RHS_SHADING_BASIC ? ctx.u.pointSize : 1.0f; }
+
+  /* the fragment shader of a point or line fragment (BasicSoft.h:78-80; the vertex colour shader: its varying) */
+  V4 shadeFlat(const std::vector<float> &vary) const {
+    if (ctx.shading == RHS_SHADING_BASIC) return ctx.u.baseColor;
+    return {vary[0], vary[1], vary[2], vary[3]};
+  }
+
+  /* perspectiveDivideImpl + viewportTransformImpl, RendererSoft.cpp:981-992 */
+  void toScreen(VertexHolder &v) {
+    float iw = 1.f / v.clip.w;
+    V4 f{v.clip.x * iw, v.clip.y * iw, v.clip.z * iw, v.clip.w * iw};
+    f.w = iw;
+    V4 g{f.x * pX, f.y * pY, f.z * 1.f, f.w * 1.f};
+    v.frag = {g.x + oX, g.y + oY, g.z + 0.f, g.w + 0.f};
+  }
+
+  /* clippingLine, RendererSoft.cpp:451-487. The second endpoint is interpolated from the line's first index
+   * after that index may already have been replaced by the first clip vertex, with t1 of the original line. */
+  bool clipLine(size_t &i0, size_t &i1, bool post) {
+    const int m0 = verts[i0].mask, m1 = verts[i1].mask;
+    const V4 c0 = verts[i0].clip, c1 = verts[i1].clip;
+    float t0 = 0.f, t1 = 1.f;
+    const int mask = m0 | m1;
+    if (mask != 0) {
+      for (int pl = 0; pl < 6; pl++) {
+        if (!(mask & (1 << pl))) continue;
+        float d0 = planeDist(pl, c0), d1 = planeDist(pl, c1);
+        if (d0 < 0 && d1 < 0) return false;
+        if (d0 < 0) {
+          float t = -d0 / (d1 - d0);
+          t0 = t0 < t ? t : t0;
+        } else {
+          float t = d0 / (d0 - d1);
+          t1 = t < t1 ? t : t1;
+        }
+      }
+    }
+    if (m0) {
+      i0 = newClipVertex(i0, i1, t0);
+      if (post) toScreen(verts[i0]);
+    }
+    if (m1) {
+      i1 = newClipVertex(i0, i1, t1);
+      if (post) toScreen(verts[i1]);
+    }
+    return true;
+  }
+
+  /* rasterizationPoint, RendererSoft.cpp:637-661. FragCoord is the integer pixel (x, y), the depth the
+   * point's z; every sample gets the fragment. The loop writes (x, y) into the point's own screen position, so
+   * a vertex rasterized again starts from where the last one ended. No colour attachment: nothing at all. */
+  /* screen coordinates a point or line may start from (the device's rule): finite and below 2^24; a vertex
+   * inside the frustum can still sit at w = 0, and the reference's int conversion of infinity is undefined */
+  static bool sane(V4 p) { return fabsf(p.x) < 16777216.f && fabsf(p.y) < 16777216.f; }
+
+  void rasterPoint(V4 &frag, const std::vector<float> &vary, float size) {
+    if (!cl) return;
+    float left = frag.x - size / 2.f + 0.5f, right = left + size;
+    float top = frag.y - size / 2.f + 0.5f, bottom = top + size;
+    const V4 col = shadeFlat(vary);
+    long long st[8] = {};
+    for (int x = (int)left; x < (int)right; x++)
+      for (int y = (int)top; y < (int)bottom; y++) {
+        frag.x = (float)x;
+        frag.y = (float)y;
+        for (int smp = 0; smp < samples; smp++) perSample(x, y, frag.z, col, smp, st);
+      }
+  }
+
+  /* rasterizationLine, RendererSoft.cpp:663-718: Bresenham over the major axis of the truncated endpoints,
+   * z, w and the varyings interpolated linearly in t = (x - x0) / dx (NaN for a line within one column),
+   * every step a point of the line width. */
+  void rasterLine(size_t ia, size_t ib, float width) {
+    const VertexHolder *va = &verts[ia], *vb = &verts[ib];
+    int x0 = (int)va->frag.x, y0 = (int)va->frag.y, x1 = (int)vb->frag.x, y1 = (int)vb->frag.y;
+    float z0 = va->frag.z, z1 = vb->frag.z, w0 = va->frag.w, w1 = vb->frag.w;
+    bool steep = false;
+    if (std::abs(x0 - x1) < std::abs(y0 - y1)) {
+      std::swap(x0, y0);
+      std::swap(x1, y1);
+      steep = true;
+    }
+    const float *vin[2] = {va->vary.data(), vb->vary.data()};
+    if (x0 > x1) {
+      std::swap(x0, x1);
+      std::swap(y0, y1);
+      std::swap(z0, z1);
+      std::swap(w0, w1);
+      std::swap(vin[0], vin[1]);
+    }
+    const int dx = x1 - x0, dy = y1 - y0;
+    int error = 0;
+    const int dError = 2 * std::abs(dy);
+    int y = y0;
+    std::vector<float> vary(std::max(ctx.nvary, 1), 0.f);
+    for (int x = x0; x <= x1; x++) {
+      const float t = (float)(x - x0) / (float)dx;
+      const float omt = 1.0f - t;
+      /* glm::mix as compiled there and in interpolateLinear: fma(a, 1 - t, b t) */
+      V4 frag{(float)x, (float)y, fmaf(z0, omt, z1 * t), fmaf(w0, omt, w1 * t)};
+      if (steep) std::swap(frag.x, frag.y);
+      for (int k = 0; k < ctx.nvary; k++) vary[k] = fmaf(vin[0][k], omt, vin[1][k] * t);
+      rasterPoint(frag, vary, width);
+      error += dError;
+      if (error > dx) {
+        y += (y1 > y0 ? 1 : -1);
+        error -= 2 * dx;
+      }
+    }
+  }
+
+  /* RendererSoft::draw for points, lines and triangles in point or line mode (RendererSoft.cpp:136-164:
+   * processPrimitiveAssembly :192-204, processClipping :206-257, processPerspectiveDivide/ViewportTransform
+   * :259-275, processFaceCulling :277-299, processRasterization :301-321 and rasterizationPolygons :560-626) */
+  void drawPointsLines() {
+    const int type = ctx.st.primitiveType, mode = ctx.st.polygonMode;
+    const int per = type == 0 ? 1 : (type == 1 ? 2 : 3);
+    const size_t np = vao->idx.size() / per;
+    prims.assign(np, Prim());
+    for (size_t t = 0; t < np; t++)
+      for (int k = 0; k < per; k++) prims[t].i[k] = (size_t)vao->idx[t * per + k];
+    for (auto &pr : prims) {
+      if (type == 0)
+        pr.discard = verts[pr.i[0]].mask != 0;
+      else if (type == 1)
+        pr.discard = !clipLine(pr.i[0], pr.i[1], false);
+      /* triangles in point or line mode are not clipped here */
+    }
+    for (auto &v : verts) v.discard = true;
+    for (auto &pr : prims)
+      if (!pr.discard)
+        for (int k = 0; k < per; k++) verts[pr.i[k]].discard = false;
+    for (auto &v : verts)
+      if (!v.discard) toScreen(v);
+    if (type == 2)
+      for (auto &pr : prims) {
+        if (pr.discard) continue;
+        V4 a = verts[pr.i[0]].frag, b = verts[pr.i[1]].frag, c = verts[pr.i[2]].frag;
+        float e0x = b.x - a.x, e0y = b.y - a.y, e1x = c.x - a.x, e1y = c.y - a.y;
+        float nz = fmaf(e0x, e1y, -(e0y * e1x));
+        pr.front = nz > 0;
+        if (ctx.st.cullFace) pr.discard = !pr.front;
+      }
+    const float psize = pointSize(), lwidth = ctx.st.lineWidth;
+    for (size_t t = 0; t < prims.size(); t++) {
+      if (prims[t].discard) continue;
+      if (type == 0) {
+        VertexHolder &v = verts[prims[t].i[0]];
+        if (sane(v.frag)) rasterPoint(v.frag, v.vary, psize);
+      } else if (type == 1) {
+        if (sane(verts[prims[t].i[0]].frag) && sane(verts[prims[t].i[1]].frag))
+          rasterLine(prims[t].i[0], prims[t].i[1], lwidth);
+      } else if (mode == 0) {
+        for (int k = 0; k < 3; k++) {
+          const size_t vi = prims[t].i[k];
+          if (verts[vi].mask != 0) continue; /* clippingPoint */
+          if (sane(verts[vi].frag)) rasterPoint(verts[vi].frag, verts[vi].vary, psize);
+        }
+      } else {
+        for (int k = 0; k < 3; k++) {
+          size_t a = prims[t].i[k], b = prims[t].i[(k + 1) % 3];
+          if (!clipLine(a, b, true)) continue;
+          if (sane(verts[a].frag) && sane(verts[b].frag)) rasterLine(a, b, lwidth);
+        }
+      }
+    }
+  }
+
   /* multiSampleResolve, RendererSoft.cpp:880-912 */
   void resolve() {
     Level *c = cl;
@@ -1375,7 +1547,12 @@ struct Renderer {
       if (it == tex.end() || d.slot[i] < 0 || d.slot[i] >= RHS_SLOT_COUNT) return -1;
       ctx.tex[d.slot[i]] = &it->second;
     }
-    if (ctx.st.primitiveType != 2 || ctx.st.polygonMode != 2) return -1;
+    const bool fill = ctx.st.primitiveType == 2 && ctx.st.polygonMode == 2;
+    /* points and lines shade without a derivative context; the reference's Viewer draws them with the base
+     * colour shader only (Viewer.cpp:264-275, ModelLoader.cpp:60-85) */
+    if (!fill && d.shading != RHS_SHADING_BASIC && d.shading != RHS_SHADING_VERTEX_COLOR) return -1;
+    if (ctx.st.primitiveType < 0 || ctx.st.primitiveType > 2 || ctx.st.polygonMode < 0 || ctx.st.polygonMode > 2)
+      return -1;
     ctx.nvary = varyingCount(d.shading);
     g_stats[7]++;
     cl = colorLevel();
@@ -1397,6 +1574,11 @@ struct Renderer {
       vh.vary.assign(std::max(ctx.nvary, 1), 0.f);
       runVS(ctx, vh.rawPtr, vh);
       vh.mask = clipMask(vh.clip);
+    }
+    if (!fill) {
+      drawPointsLines();
+      if (cl && cl->hasMs) resolve();
+      return 0;
     }
     /* processPolygonAssembly :428-437 */
     size_t np = vao->idx.size() / 3;

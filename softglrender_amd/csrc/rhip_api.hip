@@ -20,6 +20,9 @@
 #include "rhip.h"
 #include "rhip_host.h"
 #include "rhip_neural.h"
+#include "rhip_lines.h"
+
+#include <hipcub/hipcub.hpp>
 
 using namespace rhip;
 
@@ -381,9 +384,17 @@ int drawSetup(rhip_device d, int fbo, const int vp[4], const rhip_draw_desc &dd,
   const Vao &va = *vit->second;
   const Program &pg = pit->second;
   const RhsStates &st = dd.states;
-  if (st.primitiveType != 2 || st.polygonMode != 2)
-    return fail(RHIP_E_UNSUPPORTED, "only filled triangles are implemented (primitive %d, polygon mode %d)",
-                st.primitiveType, st.polygonMode);
+  if (st.primitiveType < 0 || st.primitiveType > 2 || st.polygonMode < 0 || st.polygonMode > 2)
+    return fail(RHIP_E_INVALID, "primitive type %d, polygon mode %d", st.primitiveType, st.polygonMode);
+  sh.lines = st.primitiveType != 2 || st.polygonMode != 2;
+  if (sh.lines) {
+    /* points and lines shade without a derivative context (RendererSoft.cpp:637-661); the reference's
+     * Viewer draws them with the base colour shader only (Viewer.cpp:264-275, ModelLoader.cpp:60-85) */
+    if (pg.shading != RHS_SHADING_BASIC && pg.shading != RHS_SHADING_VERTEX_COLOR)
+      return fail(RHIP_E_UNSUPPORTED, "points and lines draw with the base colour or vertex colour shader only");
+    if (!(st.lineWidth >= 0.f && st.lineWidth <= 1024.f))
+      return fail(RHIP_E_UNSUPPORTED, "line width %g (0 to 1024)", (double)st.lineWidth);
+  }
   if (st.depthFunc < 0 || st.depthFunc > 7 || st.blendFuncRgb < 0 || st.blendFuncRgb > 4)
     return fail(RHIP_E_INVALID, "invalid render states");
   auto fit = d->fbo.find(fbo);
@@ -396,12 +407,12 @@ int drawSetup(rhip_device d, int fbo, const int vp[4], const rhip_draw_desc &dd,
     return RHIP_OK;
   }
   if (vp[2] <= 0 || vp[3] <= 0) return fail(RHIP_E_INVALID, "viewport not set");
-  sh.two_phase = !st.blend && dt && st.depthTest && st.depthMask &&
+  sh.two_phase = !sh.lines && !st.blend && dt && st.depthTest && st.depthMask &&
                  (st.depthFunc == 1 || st.depthFunc == 3 || st.depthFunc == 4 || st.depthFunc == 6);
   /* a draw that writes no depth: every fragment's tests see the depths the draw started with */
-  sh.listed = !sh.two_phase && !(dt && st.depthTest && st.depthMask);
+  sh.listed = !sh.lines && !sh.two_phase && !(dt && st.depthTest && st.depthMask);
 
-  if (isEnvShading(pg.shading) && !sh.listed)
+  if (isEnvShading(pg.shading) && !sh.listed && !sh.lines)
     return fail(RHIP_E_UNSUPPORTED,
                 "skybox, IBL and FXAA programs draw without depth writes (the reference's states for them); "
                 "this draw writes depth");
@@ -416,6 +427,9 @@ int drawSetup(rhip_device d, int fbo, const int vp[4], const rhip_draw_desc &dd,
   }
   decodeUniforms(dd, P.u);
   P.st = st;
+  P.pl_point_size = pg.shading == RHS_SHADING_BASIC ? P.u.point_size : 1.0f; /* BasicSoft.h:68 */
+  if (sh.lines && !(P.pl_point_size >= 0.f && P.pl_point_size <= 1024.f))
+    return fail(RHIP_E_UNSUPPORTED, "point size %g (0 to 1024)", (double)P.pl_point_size);
   P.shading = pg.shading;
   P.defines = pg.defines;
   P.vary_groups = varyGroups(P.shading, P.defines, P.u.enable_shadow);
@@ -482,6 +496,13 @@ int drawSetup(rhip_device d, int fbo, const int vp[4], const rhip_draw_desc &dd,
   for (int c = 0; c < 3; c++) P.base_lin[c] = hostPowf(P.u.base_color[c], 2.2f); /* PbrSoft.h:235 */
   sh.V = (size_t)va.nverts;
   sh.T = (size_t)P.ntris;
+  if (sh.lines) {
+    sh.units = plUnitCount(st.primitiveType, va.nidx);
+    sh.T = 0; /* no triangle pipeline */
+    P.nblk = 0;
+    P.nbins = 0;
+    return RHIP_OK;
+  }
   sh.stride = va.stride;
   P.nblk = (int)blocks(sh.T, 256);
   const size_t vcap = sh.V + sh.T * kMaxClipVerts, pcap = sh.T * (1 + kMaxFans);
@@ -748,7 +769,7 @@ int rhip_vao_create(rhip_device d, const void *vertices, int stride, int nverts,
   if (rc) return rc;
   if (!out || !vertices || !indices) return fail(RHIP_E_INVALID, "null argument");
   if (stride <= 0 || stride % 4 || stride > 64) return fail(RHIP_E_UNSUPPORTED, "vertex stride %d (need 4..64, x4)", stride);
-  if (nverts <= 0 || nindices < 0 || nindices % 3) return fail(RHIP_E_INVALID, "bad vertex/index count");
+  if (nverts <= 0 || nindices < 0) return fail(RHIP_E_INVALID, "bad vertex/index count");
   for (int i = 0; i < nindices; i++)
     if (indices[i] < 0 || indices[i] >= nverts) return fail(RHIP_E_INVALID, "index %d out of range at %d", indices[i], i);
   std::unique_ptr<Vao> v(new Vao());
@@ -913,6 +934,132 @@ int rhip_debug_raise_device_error(rhip_device d, int stage) {
 
 namespace {
 
+/* A draw of points, lines, or triangles in point or line mode (rhip_lines.h). The fragment list is
+ * sized from the count the units report, which is read back once per draw. */
+int drawPointsLines(rhip_device d, DrawParams &P, const DrawShape &sh, bool no_alloc) {
+  if (no_alloc)
+    return fail(RHIP_E_UNSUPPORTED,
+                "rhip_draw_noalloc: points and lines size their fragment list at draw time; use rhip_draw");
+  hipStream_t s = d->stream;
+  const long long U = sh.units, V = (long long)sh.V;
+  const bool has_colour = P.color != nullptr;
+  const bool points = P.st.primitiveType == 0 || (P.st.primitiveType == 2 && P.st.polygonMode == 0);
+  if (U > (long long)INT32_MAX) return fail(RHIP_E_UNSUPPORTED, "draw too large (%lld points or lines)", U);
+  /* phase 1: vertices, units, per-unit counts and offsets, the point sort */
+  size_t sort_tmp = 0, scan_tmp = 0;
+  if (points)
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_tmp, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                              (uint32_t *)nullptr, (uint32_t *)nullptr, (int)U, 0, 32, s));
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_tmp, (unsigned long long *)nullptr,
+                                          (unsigned long long *)nullptr, (int)U, s));
+  size_t off = 0;
+  char *base = nullptr;
+  auto take = [&](size_t bytes) {
+    char *q = base ? base + off : nullptr;
+    off += (bytes + 255) & ~(size_t)255;
+    return q;
+  };
+  PlUnit *units = nullptr;
+  unsigned long long *count = nullptr, *offset = nullptr;
+  uint32_t *pkey = nullptr, *pval = nullptr, *skey = nullptr, *sval = nullptr;
+  void *tmp = nullptr;
+  for (int pass = 0; pass < 2; pass++) {
+    off = 0;
+    P.clip = (float4 *)take((size_t)V * 16);
+    P.frag = (float4 *)take((size_t)V * 16);
+    P.mask = (int *)take((size_t)V * 4);
+    P.vary = (float *)take((size_t)V * P.vary_stride * 4);
+    units = (PlUnit *)take((size_t)U * sizeof(PlUnit));
+    count = (unsigned long long *)take((size_t)U * 8);
+    offset = (unsigned long long *)take((size_t)U * 8);
+    pkey = (uint32_t *)take((size_t)U * 4);
+    pval = (uint32_t *)take((size_t)U * 4);
+    skey = (uint32_t *)take((size_t)U * 4);
+    sval = (uint32_t *)take((size_t)U * 4);
+    tmp = take(std::max(sort_tmp, scan_tmp));
+    if (pass == 0) {
+      int rc = d->arena.ensure(off);
+      if (rc) return rc;
+      base = d->arena.as<char>();
+    }
+  }
+  { Timer tm(d, "k_vertex"); RHIP_LAUNCH(k_vertex, std::max(blocks((size_t)V, 256), 1u), 256, s, P); }
+  unsigned long long total = 0;
+  if (U > 0) {
+    const unsigned g = blocks((size_t)U, 256);
+    { Timer tm(d, "k_pl_units"); RHIP_LAUNCH(k_pl_units, g, 256, s, P, units, U, count, pkey, pval, has_colour ? 1 : 0); }
+    if (points) {
+      size_t t = sort_tmp;
+      HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp, t, pkey, skey, pval, sval, (int)U, 0, 32, s));
+      Timer tm(d, "k_pl_points");
+      RHIP_LAUNCH(k_pl_points, g, 256, s, P, units, skey, sval, U, count, has_colour ? 1 : 0);
+    }
+    size_t t = scan_tmp;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(tmp, t, count, offset, (int)U, s));
+    unsigned long long last[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(&last[0], offset + (U - 1), 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&last[1], count + (U - 1), 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    total = last[0] + last[1];
+  }
+  HIPCHK(hipGetLastError());
+  if (total > (unsigned long long)INT32_MAX)
+    return fail(RHIP_E_UNSUPPORTED, "draw too large (%llu point and line fragments)", total);
+  if (total > 0) {
+    /* phase 2: the fragments, sorted by pixel, then applied pixel by pixel */
+    const long long F = (long long)total;
+    const uint32_t npix = (uint32_t)P.fb_w * (uint32_t)P.fb_h;
+    int bits = 1;
+    while (bits < 32 && (1ull << bits) <= (unsigned long long)npix) bits++;
+    size_t fsort_tmp = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, fsort_tmp, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                              (uint32_t *)nullptr, (uint32_t *)nullptr, (int)F, 0, bits, s));
+    char *fb = nullptr;
+    size_t foff = 0;
+    auto ftake = [&](size_t bytes) {
+      char *q = fb ? fb + foff : nullptr;
+      foff += (bytes + 255) & ~(size_t)255;
+      return q;
+    };
+    uint32_t *key = nullptr, *val = nullptr, *key2 = nullptr, *val2 = nullptr;
+    float *zf = nullptr;
+    float4 *cf = nullptr;
+    void *ftmp = nullptr;
+    for (int pass = 0; pass < 2; pass++) {
+      foff = 0;
+      key = (uint32_t *)ftake((size_t)F * 4);
+      val = (uint32_t *)ftake((size_t)F * 4);
+      key2 = (uint32_t *)ftake((size_t)F * 4);
+      val2 = (uint32_t *)ftake((size_t)F * 4);
+      zf = (float *)ftake((size_t)F * 4);
+      cf = (float4 *)ftake((size_t)F * 16);
+      ftmp = ftake(fsort_tmp);
+      if (pass == 0) {
+        int rc = d->lines.ensure(foff);
+        if (rc) return rc;
+        fb = d->lines.as<char>();
+      }
+    }
+    const unsigned g = blocks((size_t)U, 256), gf = blocks((size_t)F, 256);
+    { Timer tm(d, "k_pl_emit"); RHIP_LAUNCH(k_pl_emit, g, 256, s, P, units, U, offset, key, val, zf, cf); }
+    size_t t = fsort_tmp;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(ftmp, t, key, key2, val, val2, (int)F, 0, bits, s));
+    Timer tm(d, "k_pl_apply");
+    if (P.samples == 4)
+      RHIP_LAUNCH(k_pl_apply<4>, gf, 256, s, P, key2, val2, zf, cf, F);
+    else
+      RHIP_LAUNCH(k_pl_apply<1>, gf, 256, s, P, key2, val2, zf, cf, F);
+  }
+  /* RendererSoft::draw resolves a multisampled colour target after every draw (:161-163) */
+  if (has_colour && P.samples == 4) {
+    Timer tm(d, "k_resolve");
+    RHIP_LAUNCH(k_resolve, (unsigned)std::min<size_t>(blocks((size_t)P.fb_w * P.fb_h, 256), 4096), 256, s, P);
+  }
+  HIPCHK(hipGetLastError());
+  d->host_stats.draws++;
+  return RHIP_OK;
+}
+
 /* One draw (RendererSoft::draw, RendererSoft.cpp:136-164) with its arena in
  * the device's grow-only buffer, or, for no_alloc, in the library's static
  * scratch of kScratchBytes. */
@@ -926,6 +1073,7 @@ int drawOne(rhip_device d, const rhip_draw_desc *dd, bool no_alloc) {
   DrawShape sh;
   if ((rc = drawSetup(d, d->cur_fbo, vp, *dd, kTile, P, sh))) return rc;
   if (sh.empty) return RHIP_OK;
+  if (sh.lines) return drawPointsLines(d, P, sh, no_alloc);
   const size_t need = carveArenas(P, sh, nullptr);
   if (no_alloc) {
     if (need > kScratchBytes)

@@ -101,6 +101,7 @@ struct rhip_device_s {
   int vp_x = 0, vp_y = 0, vp_w = 0, vp_h = 0;
   /* per-draw arenas */
   rhip::DevBuf arena;
+  rhip::DevBuf lines; /* fragment lists of point and line draws, sized once their count is known */
   /* device address of this GPU's copy of the library's static draw scratch
    * (rhip_draw_noalloc), from hipGetSymbolAddress at creation */
   char *scratch = nullptr;
@@ -163,6 +164,8 @@ struct DrawShape {
   bool empty = false;  /* no attachment: nothing to do */
   bool two_phase = false; /* visibility first, then shading (opaque draws that write depth) */
   bool listed = false;    /* ordered draws that write no depth: list, shade, then blend in order */
+  bool lines = false;     /* points, lines, or triangles in point or line mode (rhip_lines.h) */
+  long long units = 0;    /* lines: the points and lines the reference rasterizes */
 };
 
 /* Validates a draw into framebuffer `fbo` through viewport vp = (x, y, w, h)

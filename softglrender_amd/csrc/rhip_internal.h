@@ -151,6 +151,7 @@ struct DrawParams {
   int vary_stride;      /* floats per vertex in `vary`: 4 per stored group */
   /* viewport (RendererSoft.cpp:92-113) */
   float vp_w, vp_h, o_x, o_y, p_x, p_y;
+  float pl_point_size; /* gl->PointSize after the vertex shader: u_pointSize for the base colour shader, else 1 */
   /* framebuffer */
   uint32_t *color;          /* RGBA8, `samples` per pixel */
   uint32_t *color_resolved; /* RGBA8 single sample, when samples == 4 */

@@ -893,3 +893,107 @@ def scene_variant(name, width=160, height=120, seed=11):
             st.read(2, which=1)
     st.read(3, which=1 if msaa else 0)
     return Scene(name, st, width, height, msaa, dict(layouts=layouts))
+
+
+def _colour_vertices(pos, col):
+    """Vertices {vec3 a_position; vec4 a_color} (8 floats, the vertex colour shader's layout)."""
+    v = np.zeros((len(pos), 8), np.float32)
+    v[:, 0:3] = pos
+    v[:, 4:8] = col
+    return v
+
+
+def scene_points_lines(width=160, height=120, seed=17, msaa=False, read_samples=False):
+    """Points, lines and triangles in point and line mode (RendererSoft.cpp:575-718), in the ways the Viewer
+    draws them (wireframe meshes, the world axis, the point light: Viewer.cpp:264-275, ModelLoader.cpp:60-85)
+    and past them: over a shaded sphere with depth testing; lines and points crossing the near plane and the
+    frustum sides (clipped, or dropped), wide lines and big points running off the framebuffer, points on
+    repeated indices and on the shared vertices of a mesh in point mode (each rasterization starts where the
+    last one of that vertex ended), a line within one pixel column, blended and vertex-coloured lines, a
+    culled wireframe, and a depth-only pass that draws nothing."""
+    rng = np.random.default_rng(seed)
+    st = S.Stream()
+    sv, si = uv_sphere(16, 8, 0.8)
+    st.vao(1, sv, si)
+    wv, wi = uv_sphere(12, 6, 0.95)
+    st.vao(2, wv, wi)
+    # world axis: three lines from the origin, and lines through the near plane and past the sides
+    ax = np.array([[0, 0, 0], [1.5, 0, 0], [0, 0, 0], [0, 1.5, 0], [0, 0, 0], [0, 0, 1.5],
+                   [-0.3, -0.2, 3.5], [0.4, 0.3, -1.0], [-4.0, 0.5, 0.2], [4.0, -0.6, 0.1],
+                   [0.2, -3.0, 2.9], [0.1, 3.0, 2.95], [5.0, 5.0, 5.0], [6.0, 6.0, -5.0],
+                   [0.71, -0.4, 0.2], [0.71, 0.6, 0.2]], np.float32)
+    axc = rng.uniform(0, 1, (len(ax), 4)).astype(np.float32)
+    axc[:, 3] = rng.uniform(0.3, 1.0, len(ax))
+    st.vao(3, _colour_vertices(ax, axc), np.arange(len(ax), dtype=np.int32).reshape(-1, 2))
+    # points: inside, at the framebuffer edge, behind the camera, outside the sides, and repeated indices
+    pts = np.concatenate([rng.uniform(-1.2, 1.2, (24, 3)), [[0, 0, 4.0], [3.5, 0, 0], [-1.45, 1.05, 0.0],
+                                                            [1.5, -1.1, 0.3]]]).astype(np.float32)
+    ptc = rng.uniform(0, 1, (len(pts), 4)).astype(np.float32)
+    pidx = np.concatenate([np.arange(len(pts)), [3, 3, 5, 3, 26, 26]]).astype(np.int32)
+    st.vao(4, _colour_vertices(pts, ptc), pidx)
+    st.texture(2, width, height, usage=S.USAGE_COLOR, msaa=msaa)
+    st.tex_init(2)
+    st.texture(3, width, height, format=S.FMT_FLOAT32, usage=S.USAGE_DEPTH, msaa=msaa)
+    st.tex_init(3)
+    st.fbo(1, color=2, depth=3)
+    st.texture(5, width, height, format=S.FMT_FLOAT32, usage=S.USAGE_DEPTH)
+    st.tex_init(5)
+    st.fbo(2, depth=5)
+    st.frame()
+    eye = (0.6, 0.8, 3.0)
+    view = look_at(eye, (0, 0, 0), (0, 1, 0))
+    proj = projection(math.radians(60.0), width / height, 0.1)
+    mvp = proj @ view
+    scene_u = S.uniforms_scene((0.2, 0.2, 0.2), eye, (1.0, 1.2, 1.5), (0.9, 0.9, 0.9))
+    st.begin(1, True, True, (0.05, 0.1, 0.15, 1.0), 1.0)
+    st.viewport(0, 0, width, height)
+    st.draw(1, S.SHADING_BLINN_PHONG, 0, _common_states(S.DEPTH_LESS, cull=True), S.uniforms_model(np.eye(4), mvp),
+            scene_u, S.uniforms_material(1, 0, 0, 1.0, 0.5, (0.8, 0.7, 0.6, 1.0)))
+
+    def basic(vao, prim, mode, colour, point_size=1.0, width_=1.0, depth_func=S.DEPTH_LEQUAL, cull=False,
+              blend=False, depth_test=True, model=np.eye(4)):
+        rs = _common_states(depth_func, cull=cull, blend=blend)
+        rs.depth_test = depth_test
+        rs.primitive_type, rs.polygon_mode, rs.line_width = prim, mode, width_
+        st.draw(vao, S.SHADING_BASIC, 0, rs, S.uniforms_model(model, mvp @ model), bytes(S.SCENE_BYTES),
+                S.uniforms_material(0, 0, 0, point_size, 0.5, colour))
+
+    # wireframe of a larger sphere: back faces culled, then all edges on top without depth testing, thick
+    basic(2, S.PRIM_TRIANGLE, S.POLY_LINE, (0.9, 0.9, 0.2, 1.0), cull=True)
+    basic(2, S.PRIM_TRIANGLE, S.POLY_LINE, (0.2, 0.9, 0.9, 0.5), width_=2.0, blend=True, depth_test=False,
+          model=_translate((0.9, -0.5, -0.6)))
+    # the axis and the clipped lines, width 1 then 3
+    basic(3, S.PRIM_LINE, S.POLY_FILL, (1.0, 0.3, 0.3, 1.0))
+    basic(3, S.PRIM_LINE, S.POLY_FILL, (0.3, 1.0, 0.3, 1.0), width_=3.0, depth_func=S.DEPTH_LESS,
+          model=_translate((0.12, 0.2, 0.0)))
+    # vertex-coloured lines, blended
+    rs = _common_states(S.DEPTH_LEQUAL, cull=False, blend=True)
+    rs.primitive_type = S.PRIM_LINE
+    tl = _translate((-0.25, -0.3, 0.2))
+    st.draw(3, S.SHADING_VERTEX_COLOR, 0, rs, S.uniforms_model(tl, mvp @ tl), bytes(64), bytes(48))
+    # points of size 10 (the point light), 1 and 4.5, with repeated indices
+    basic(4, S.PRIM_POINT, S.POLY_FILL, (1.0, 1.0, 1.0, 1.0), point_size=10.0, depth_func=S.DEPTH_LESS)
+    basic(4, S.PRIM_POINT, S.POLY_FILL, (0.0, 0.0, 1.0, 1.0), point_size=1.0, depth_test=False)
+    basic(4, S.PRIM_POINT, S.POLY_FILL, (1.0, 0.5, 0.0, 0.6), point_size=4.5, blend=True, depth_test=False,
+          model=_translate((0.15, 0.05, 0.0)))
+    # a mesh in point mode: shared vertices are rasterized several times
+    basic(1, S.PRIM_TRIANGLE, S.POLY_POINT, (0.6, 0.1, 0.9, 1.0), point_size=3.0, depth_test=False, cull=True,
+          model=_translate((-0.7, 0.4, 0.0)))
+    # vertex-coloured points
+    rs = _common_states(S.DEPTH_ALWAYS, cull=False)
+    rs.primitive_type = S.PRIM_POINT
+    tp = _translate((0.0, -0.12, 0.05))
+    st.draw(4, S.SHADING_VERTEX_COLOR, 0, rs, S.uniforms_model(tp, mvp @ tp), bytes(64), bytes(48))
+    st.end()
+    # a depth-only pass: points and lines draw nothing without a colour attachment
+    st.begin(2, False, True, (0, 0, 0, 0), 1.0)
+    st.viewport(0, 0, width, height)
+    basic(3, S.PRIM_LINE, S.POLY_FILL, (1, 1, 1, 1), depth_func=S.DEPTH_ALWAYS)
+    basic(2, S.PRIM_TRIANGLE, S.POLY_LINE, (1, 1, 1, 1), depth_func=S.DEPTH_ALWAYS)
+    st.end()
+    st.read(2)
+    st.read(3, which=1 if msaa else 0)  # a multisampled depth buffer has no single-sample image
+    if read_samples and msaa:
+        st.read(2, which=1)
+    st.read(5)
+    return Scene("points_lines", st, width, height, msaa, {})

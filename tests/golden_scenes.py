@@ -10,5 +10,8 @@ GOLDEN = {
 # image-based lighting precompute (irradiance, prefiltered levels), skybox (cube and equirectangular) and FXAA
 GOLDEN["ibl_cube"] = lambda: scenes.scene_ibl()
 GOLDEN["ibl_equirect_fxaa"] = lambda: scenes.scene_ibl(equirect=True, fxaa=True)
+# points, lines and triangles in point and line mode, single-sampled and 4x
+GOLDEN["points_lines"] = lambda: scenes.scene_points_lines()
+GOLDEN["points_lines_msaa"] = lambda: scenes.scene_points_lines(msaa=True, read_samples=True)
 for _v in scenes.VARIANTS:
     GOLDEN[f"v_{_v}"] = (lambda v=_v: scenes.scene_variant(v))
