@@ -106,7 +106,33 @@ struct Texel<float> {
 };
 
 /* pixelWithWrapMode (SamplerSoft.h:170-213). REPEAT masks with (n-1): the
- * reference's CoordMod macro parses as i & (2n-1) & (n-1). */
+ * reference's CoordMod macro parses as i & (2n-1) & (n-1). Where the read at
+ * (x, y) lands: 0 the texel at `idx`, 1 the border colour, 2 zero. */
+__device__ __forceinline__ int wrapTexel(const TexDesc &d, const TexLevel &l, int x, int y, int wrap, size_t &idx) {
+  int w = l.w, h = l.h;
+  if (wrap == 0) {
+    x &= w - 1;
+    y &= h - 1;
+  } else if (wrap == 1) {
+    x &= 2 * w - 1;
+    y &= 2 * h - 1;
+    x -= w;
+    y -= h;
+    x = x >= 0 ? x : (-1 - x);
+    y = y >= 0 ? y : (-1 - y);
+    x = w - 1 - x;
+    y = h - 1 - y;
+  } else if (wrap == 2) {
+    x = x < 0 ? 0 : (x >= w ? w - 1 : x);
+    y = y < 0 ? 0 : (y >= h ? h - 1 : y);
+  } else {
+    if (x < 0 || x >= w || y < 0 || y >= h) return 1;
+  }
+  if ((unsigned)x >= (unsigned)w || (unsigned)y >= (unsigned)h) return 2;
+  idx = texelIndex(l, d.layout, x, y);
+  return 0;
+}
+
 template <typename T>
 __device__ __forceinline__ T fetchWrap(const TexDesc &d, const TexLevel &l, int x, int y, int wrap) {
   int w = l.w, h = l.h;
@@ -174,6 +200,132 @@ __device__ T textureImpl(const TexDesc &d, int layer, f2 uv, float lod) {
   return Texel<T>::mix(th, tl, lod - floorf(lod));
 }
 
+/* ------------------------------------------------------------------ RGBA8 sampling on byte-valued floats
+ *
+ * The same arithmetic as textureImpl<uint32_t>, with each channel kept as a
+ * float holding its byte instead of being repacked after every mix: the
+ * reference's u8 conversion of a mix result in [0, 256) is its truncation, and
+ * a NaN weight (which makes every mix of that bilinear NaN) converts to 0 at
+ * the end of the bilinear, as the byte chain gives. */
+__device__ __forceinline__ float4 bytesF(uint32_t p) {
+  return make_float4((float)(p & 255u), (float)((p >> 8) & 255u), (float)((p >> 16) & 255u), (float)(p >> 24));
+}
+__device__ __forceinline__ float4 mixB(float4 x, float4 y, float a) {
+  const float na = 1.0f - a;
+  return make_float4(truncf(fmaf(y.x, a, x.x * na)), truncf(fmaf(y.y, a, x.y * na)), truncf(fmaf(y.z, a, x.z * na)),
+                     truncf(fmaf(y.w, a, x.w * na)));
+}
+__device__ __forceinline__ float4 nanToZero(float4 v) {
+  return make_float4(v.x == v.x ? v.x : 0.f, v.y == v.y ? v.y : 0.f, v.z == v.z ? v.z : 0.f, v.w == v.w ? v.w : 0.f);
+}
+
+__device__ __forceinline__ float4 bilinear4(const TexDesc &d, const TexLevel &l, float ux, float uy, int wrap) {
+  const float fx0 = ux - 0.5f, fy0 = uy - 0.5f;
+  const float flx = floorf(fx0), fly = floorf(fy0);
+  const int x = (int)flx, y = (int)fly;
+  const float4 s1 = bytesF(fetchWrap<uint32_t>(d, l, x, y, wrap));
+  const float4 s2 = bytesF(fetchWrap<uint32_t>(d, l, x + 1, y, wrap));
+  const float4 s3 = bytesF(fetchWrap<uint32_t>(d, l, x, y + 1, wrap));
+  const float4 s4 = bytesF(fetchWrap<uint32_t>(d, l, x + 1, y + 1, wrap));
+  const float fx = fx0 - flx, fy = fy0 - fly;
+  return nanToZero(mixB(mixB(s1, s2, fx), mixB(s3, s4, fx), fy));
+}
+
+__device__ __forceinline__ float4 sampleLevel4(const TexDesc &d, int layer, int level, f2 uv, bool linear) {
+  const TexLevel &l = d.lv[layer][level];
+  const float tx = uv.x * (float)l.w, ty = uv.y * (float)l.h;
+  if (linear) return bilinear4(d, l, tx, ty, d.wrap);
+  return bytesF(fetchWrap<uint32_t>(d, l, (int)floorf(tx), (int)floorf(ty), d.wrap));
+}
+
+/* textureImpl<uint32_t> with byte-valued float channels */
+__device__ __forceinline__ float4 textureImpl4(const TexDesc &d, int layer, f2 uv, float lod) {
+  const int fm = d.filter_min;
+  if (fm == 0) return sampleLevel4(d, layer, 0, uv, false);
+  if (fm == 1) return sampleLevel4(d, layer, 0, uv, true);
+  const int maxLevel = d.levels - 1;
+  if (fm == 2 || fm == 3) {
+    const int level = min(max((int)ceilf(lod + 0.5f) - 1, 0), maxLevel);
+    return sampleLevel4(d, layer, level, uv, fm == 3);
+  }
+  const int hi = min(max((int)floorf(lod), 0), maxLevel);
+  const int lo = min(max(hi + 1, 0), maxLevel);
+  const float4 th = sampleLevel4(d, layer, hi, uv, fm == 5);
+  if (hi == lo) return th;
+  const float4 tl = sampleLevel4(d, layer, lo, uv, fm == 5);
+  return nanToZero(mixB(th, tl, lod - floorf(lod)));
+}
+
+/* Two textures of one shape (size, levels, layout, wrap and filter) read at the
+ * same coordinate and LOD: the levels, weights and texel indices are the same,
+ * so they are computed once and both images are read with them. */
+__device__ __forceinline__ bool sameShape(const TexDesc &a, const TexDesc &b) {
+  return a.width == b.width && a.height == b.height && a.levels == b.levels && a.layout == b.layout &&
+         a.wrap == b.wrap && a.filter_min == b.filter_min && a.layers == b.layers && a.type == b.type &&
+         a.format == b.format;
+}
+
+/* One texel read of a shape: where the wrap mode sends it (wrapTexel) and the index there. */
+struct TexelAt {
+  int kind; /* 0 the texel at idx, 1 the border colour, 2 zero */
+  uint32_t idx;
+};
+__device__ __forceinline__ TexelAt texelAt(const TexDesc &a, const TexLevel &l, int x, int y) {
+  size_t idx = 0;
+  const int k = wrapTexel(a, l, x, y, a.wrap, idx);
+  return {k, (uint32_t)idx};
+}
+__device__ __forceinline__ float4 readAt(const TexDesc &d, const TexLevel &l, TexelAt t) {
+  if (t.kind == 0) return bytesF(((const uint32_t *)(d.base + l.offset))[t.idx]);
+  return bytesF(t.kind == 1 ? Texel<uint32_t>::border(d) : 0u);
+}
+
+__device__ __forceinline__ void sampleLevelPair(const TexDesc &a, const TexDesc &b, int layer, int level, f2 uv,
+                                                bool linear, float4 &ra, float4 &rb) {
+  const TexLevel &la = a.lv[layer][level], &lb = b.lv[layer][level];
+  const float tx = uv.x * (float)la.w, ty = uv.y * (float)la.h;
+  if (!linear) {
+    const TexelAt t = texelAt(a, la, (int)floorf(tx), (int)floorf(ty));
+    ra = readAt(a, la, t);
+    rb = readAt(b, lb, t);
+    return;
+  }
+  const float fx0 = tx - 0.5f, fy0 = ty - 0.5f;
+  const float flx = floorf(fx0), fly = floorf(fy0);
+  const int x = (int)flx, y = (int)fly;
+  const TexelAt t1 = texelAt(a, la, x, y), t2 = texelAt(a, la, x + 1, y);
+  const TexelAt t3 = texelAt(a, la, x, y + 1), t4 = texelAt(a, la, x + 1, y + 1);
+  const float fx = fx0 - flx, fy = fy0 - fly;
+  /* one image at a time: the second's texels are read after the first is mixed */
+  ra = nanToZero(mixB(mixB(readAt(a, la, t1), readAt(a, la, t2), fx), mixB(readAt(a, la, t3), readAt(a, la, t4), fx), fy));
+  rb = nanToZero(mixB(mixB(readAt(b, lb, t1), readAt(b, lb, t2), fx), mixB(readAt(b, lb, t3), readAt(b, lb, t4), fx), fy));
+}
+
+/* textureImpl of two textures of one shape at one coordinate and LOD */
+__device__ __forceinline__ void textureImplPair(const TexDesc &a, const TexDesc &b, int layer, f2 uv, float lod,
+                                                float4 &ra, float4 &rb) {
+  const int fm = a.filter_min;
+  if (fm <= 1) {
+    sampleLevelPair(a, b, layer, 0, uv, fm == 1, ra, rb);
+    return;
+  }
+  const int maxLevel = a.levels - 1;
+  if (fm == 2 || fm == 3) {
+    const int level = min(max((int)ceilf(lod + 0.5f) - 1, 0), maxLevel);
+    sampleLevelPair(a, b, layer, level, uv, fm == 3, ra, rb);
+    return;
+  }
+  const int hi = min(max((int)floorf(lod), 0), maxLevel);
+  const int lo = min(max(hi + 1, 0), maxLevel);
+  sampleLevelPair(a, b, layer, hi, uv, fm == 5, ra, rb);
+  if (hi == lo) return;
+  float4 la, lb;
+  sampleLevelPair(a, b, layer, lo, uv, fm == 5, la, lb);
+  const float t = lod - floorf(lod);
+  ra = nanToZero(mixB(ra, la, t));
+  rb = nanToZero(mixB(rb, lb, t));
+}
+
 /* convertXYZ2UV (SamplerSoft.h:311-373); later faces win ties */
 __device__ __forceinline__ void cubeFace(f3 c, int *face, f2 *uv) {
   float ax = fabsf(c.x), ay = fabsf(c.y), az = fabsf(c.z);
@@ -195,9 +347,18 @@ __device__ __forceinline__ void cubeFace(f3 c, int *face, f2 *uv) {
 struct RGBAf {
   float r, g, b, a;
 };
-__device__ __forceinline__ RGBAf unpack01(uint32_t p) {
-  return {(float)chan(p, 0) / 255.f, (float)chan(p, 1) / 255.f, (float)chan(p, 2) / 255.f, (float)chan(p, 3) / 255.f};
+/* k / 255.f for a byte k, correctly rounded like the division (ShaderSoft.h:80-83 divides the texel by 255):
+ * q = k c with c = 1/255 rounded, then one Newton step with fused multiply-adds; equal to the division for
+ * every byte (all 256 checked exactly). */
+__device__ __forceinline__ float div255(float k) {
+  const float c = 1.0f / 255.0f;
+  const float q = k * c;
+  return fmaf(fmaf(-q, 255.0f, k), c, q);
 }
+__device__ __forceinline__ RGBAf unpack01(uint32_t p) {
+  return {div255((float)chan(p, 0)), div255((float)chan(p, 1)), div255((float)chan(p, 2)), div255((float)chan(p, 3))};
+}
+__device__ __forceinline__ RGBAf unpack01(float4 b) { return {div255(b.x), div255(b.y), div255(b.z), div255(b.w)}; }
 
 /* --------------------------------------------------------------------------- vertex shaders */
 
@@ -307,7 +468,7 @@ __device__ __forceinline__ float lodFor(const DrawParams &P, const TexDesc *t, c
 
 __device__ __forceinline__ RGBAf tex2D(const TexDesc *t, f2 uv, float lod) {
   if (!t) return {0, 0, 0, 0};
-  return unpack01(textureImpl<uint32_t>(*t, 0, uv, lod));
+  return unpack01(textureImpl4(*t, 0, uv, lod));
 }
 
 __device__ __forceinline__ RGBAf texCube(const TexDesc *t, f3 dir, float lod) {
@@ -315,22 +476,30 @@ __device__ __forceinline__ RGBAf texCube(const TexDesc *t, f3 dir, float lod) {
   int face;
   f2 uv;
   cubeFace(dir, &face, &uv);
-  return unpack01(textureImpl<uint32_t>(*t, face, uv, lod));
+  return unpack01(textureImpl4(*t, face, uv, lod));
+}
+
+/* The normal from the normal map's texel t4 and the interpolated TBN frame (BlinnPhongSoft.h:152-163,
+ * PbrSoft.h:154-166). */
+__device__ __forceinline__ f3 normalFromTexel(const Frag &f, int offN, int offT, RGBAf t4) {
+  const float *v = f.v;
+  {
+    f3 N = nrm3({v[offN], v[offN + 1], v[offN + 2]});
+    f3 T = nrm3({v[offT], v[offT + 1], v[offT + 2]});
+    T = nrm3(sub3(T, scl3(N, dot3(T, N))));
+    f3 B = cross3(T, N);
+    f3 tn{t4.r * 2.0f - 1.0f, t4.g * 2.0f - 1.0f, t4.b * 2.0f - 1.0f};
+    f3 r{fmaf(N.x, tn.z, fmaf(T.x, tn.x, B.x * tn.y)), fmaf(N.y, tn.z, fmaf(T.y, tn.x, B.y * tn.y)),
+         fmaf(N.z, tn.z, fmaf(T.z, tn.x, B.z * tn.y))};
+    return nrm3(r);
+  }
 }
 
 __device__ __forceinline__ f3 normalFromMap(const DrawParams &P, const Frag &f, int offN, int offT) {
   const float *v = f.v;
   if (P.defines & RHS_DEF_NORMAL_MAP) {
-    f3 N = nrm3({v[offN], v[offN + 1], v[offN + 2]});
-    f3 T = nrm3({v[offT], v[offT + 1], v[offT + 2]});
-    T = nrm3(sub3(T, scl3(N, dot3(T, N))));
-    f3 B = cross3(T, N);
     const TexDesc *nt = P.tex[RHS_SLOT_NORMAL];
-    RGBAf t4 = tex2D(nt, {v[0], v[1]}, lodFor(P, nt, f));
-    f3 tn{t4.r * 2.0f - 1.0f, t4.g * 2.0f - 1.0f, t4.b * 2.0f - 1.0f};
-    f3 r{fmaf(N.x, tn.z, fmaf(T.x, tn.x, B.x * tn.y)), fmaf(N.y, tn.z, fmaf(T.y, tn.x, B.y * tn.y)),
-         fmaf(N.z, tn.z, fmaf(T.z, tn.x, B.z * tn.y))};
-    return nrm3(r);
+    return normalFromTexel(f, offN, offT, tex2D(nt, {v[0], v[1]}, lodFor(P, nt, f)));
   }
   return nrm3({v[4], v[5], v[6]});
 }
@@ -461,8 +630,19 @@ __device__ __forceinline__ PbrInputs pbrInputs(const DrawParams &P, const Frag &
   const Uniforms &u = P.u;
   PbrInputs in;
   f2 uv{v[0], v[1]};
-  const TexDesc *ta = P.tex[RHS_SLOT_ALBEDO];
-  if (P.defines & RHS_DEF_ALBEDO_MAP) {
+  const TexDesc *ta = P.tex[RHS_SLOT_ALBEDO], *tn = P.tex[RHS_SLOT_NORMAL];
+  /* albedo and normal maps of one shape share their LOD, levels, weights and texel indices */
+  const bool pair = (P.defines & RHS_DEF_ALBEDO_MAP) && (P.defines & RHS_DEF_NORMAL_MAP) && ta && tn &&
+                    sameShape(*ta, *tn);
+  RGBAf ntexel{0.f, 0.f, 0.f, 0.f};
+  if (pair) {
+    float4 ba, bn;
+    textureImplPair(*ta, *tn, 0, uv, lodFor(P, ta, f), ba, bn);
+    in.albedo_rgba = unpack01(ba);
+    ntexel = unpack01(bn);
+    /* gammaIn's table index is the byte itself */
+    in.albedo = {P.gamma_lut[(int)ba.x], P.gamma_lut[(int)ba.y], P.gamma_lut[(int)ba.z]};
+  } else if (P.defines & RHS_DEF_ALBEDO_MAP) {
     in.albedo_rgba = tex2D(ta, uv, lodFor(P, ta, f));
     in.albedo = {gammaIn(P, in.albedo_rgba.r), gammaIn(P, in.albedo_rgba.g), gammaIn(P, in.albedo_rgba.b)};
   } else {
@@ -480,7 +660,7 @@ __device__ __forceinline__ PbrInputs pbrInputs(const DrawParams &P, const Frag &
   }
   const TexDesc *tao = P.tex[RHS_SLOT_AO];
   in.ao = (P.defines & RHS_DEF_AO_MAP) ? tex2D(tao, uv, lodFor(P, tao, f)).r : 1.f;
-  in.N = normalFromMap(P, f, 20, 24);
+  in.N = pair ? normalFromTexel(f, 20, 24, ntexel) : normalFromMap(P, f, 20, 24);
   in.V = nrm3({v[12], v[13], v[14]});
   f3 I{-in.V.x, -in.V.y, -in.V.z};
   float dNI = dot3(in.N, I);
