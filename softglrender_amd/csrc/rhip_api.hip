@@ -595,7 +595,8 @@ int rhip_device_create(int hip_device, rhip_device *out) {
   }
   /* the host libm on the device: every table, built once per process */
   {
-    int rc = libmEnsure(d.get(), (1u << kLmCount) - 1);
+    /* log2f and exp2f need no table (glibc's own algorithms): only the powf tables are built here */
+    int rc = libmEnsure(d.get(), (1u << kLmPow5) | (1u << kLmPow128) | (1u << kLmPowGamma));
     if (rc) {
       if (d->libm) hipFree(d->libm);
       for (void *p : d->libm_bufs) hipFree(p);

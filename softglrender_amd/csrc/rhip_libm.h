@@ -14,8 +14,8 @@
  * made of IEEE basic operations only (+, -, *, /, rint, ldexp: correctly
  * rounded, so host and device compute the same double), rounded to float, is
  * within one ulp of the host libm's result everywhere in the function's input
- * range. For log2f the approximation is glibc's own algorithm (glibcLog2),
- * which leaves its table empty. At first use the library evaluates both on every float of that range
+ * range. For log2f and exp2f the approximation is glibc's own algorithm
+ * (glibcLog2, glibcExp2), which leaves their tables empty. At first use the library evaluates both on every float of that range
  * on the host (rhip_libm.hip) and keeps the inputs where they differ, bucketed
  * by the top 16 bits of the input, with the direction of the one-ulp step. The
  * device evaluates the approximation and consults the table only when the
@@ -131,21 +131,21 @@ __host__ __device__ inline double lmExp2(double t) {
  * host's log2f on every positive float, with or without fused multiply-adds
  * in the double steps (both checked exhaustively): the function needs no table
  * of exceptions and costs a few double operations, which gfx950 runs at full
- * rate. Positive finite x only. */
+ * rate. Positive finite x only. The tables live in constant memory. */
+constexpr double kLog2Invc[16] = {0x1.661ec79f8f3bep+0, 0x1.571ed4aaf883dp+0, 0x1.49539f0f010bp+0,
+                                  0x1.3c995b0b80385p+0, 0x1.30d190c8864a5p+0, 0x1.25e227b0b8eap+0,
+                                  0x1.1bb4a4a1a343fp+0, 0x1.12358f08ae5bap+0, 0x1.0953f419900a7p+0,
+                                  0x1p+0,               0x1.e608cfd9a47acp-1, 0x1.ca4b31f026aap-1,
+                                  0x1.b2036576afce6p-1, 0x1.9c2d163a1aa2dp-1, 0x1.886e6037841edp-1,
+                                  0x1.767dcf5534862p-1};
+constexpr double kLog2Logc[16] = {-0x1.efec65b963019p-2, -0x1.b0b6832d4fca4p-2, -0x1.7418b0a1fb77bp-2,
+                                  -0x1.39de91a6dcf7bp-2, -0x1.01d9bf3f2b631p-2, -0x1.97c1d1b3b7afp-3,
+                                  -0x1.2f9e393af3c9fp-3, -0x1.960cbbf788d5cp-4, -0x1.a6f9db6475fcep-5,
+                                  0x0p+0,                0x1.338ca9f24f53dp-4,  0x1.476a9543891bap-3,
+                                  0x1.e840b4ac4e4d2p-3,  0x1.40645f0c6651cp-2,  0x1.88e9c2c1b9ff8p-2,
+                                  0x1.ce0a44eb17bccp-2};
+
 __host__ __device__ inline double glibcLog2(float x) {
-  /* {1/c, log2 c} per interval */
-  constexpr double kInvc[16] = {0x1.661ec79f8f3bep+0, 0x1.571ed4aaf883dp+0, 0x1.49539f0f010bp+0,
-                                0x1.3c995b0b80385p+0, 0x1.30d190c8864a5p+0, 0x1.25e227b0b8eap+0,
-                                0x1.1bb4a4a1a343fp+0, 0x1.12358f08ae5bap+0, 0x1.0953f419900a7p+0,
-                                0x1p+0,               0x1.e608cfd9a47acp-1, 0x1.ca4b31f026aap-1,
-                                0x1.b2036576afce6p-1, 0x1.9c2d163a1aa2dp-1, 0x1.886e6037841edp-1,
-                                0x1.767dcf5534862p-1};
-  constexpr double kLogc[16] = {-0x1.efec65b963019p-2, -0x1.b0b6832d4fca4p-2, -0x1.7418b0a1fb77bp-2,
-                                -0x1.39de91a6dcf7bp-2, -0x1.01d9bf3f2b631p-2, -0x1.97c1d1b3b7afp-3,
-                                -0x1.2f9e393af3c9fp-3, -0x1.960cbbf788d5cp-4, -0x1.a6f9db6475fcep-5,
-                                0x0p+0,                0x1.338ca9f24f53dp-4,  0x1.476a9543891bap-3,
-                                0x1.e840b4ac4e4d2p-3,  0x1.40645f0c6651cp-2,  0x1.88e9c2c1b9ff8p-2,
-                                0x1.ce0a44eb17bccp-2};
   uint32_t ix = lmBits(x);
   if (ix == 0x3f800000u) return 0.0;
   if (ix < 0x00800000u) ix = lmBits(x * 0x1p23f) - (23u << 23); /* subnormal: scale exactly */
@@ -153,13 +153,46 @@ __host__ __device__ inline double glibcLog2(float x) {
   const int i = (int)((tmp >> 19) & 15u);
   const uint32_t iz = ix - (tmp & 0xff800000u);
   const int k = (int32_t)tmp >> 23;
-  const double r = (double)lmAsFloat(iz) * kInvc[i] - 1.0;
-  const double y0 = kLogc[i] + (double)k;
+  const double r = (double)lmAsFloat(iz) * kLog2Invc[i] - 1.0;
+  const double y0 = kLog2Logc[i] + (double)k;
   const double r2 = r * r;
   double y = 0x1.ecabf496832ep-2 * r + -0x1.715479ffae3dep-1;
   y = -0x1.712b6f70a7e4dp-2 * r2 + y;
   const double p = 0x1.715475f35c8b8p0 * r + y0;
   return y * r2 + p;
+}
+
+/* glibc's exp2f before its final rounding (glibc 2.35 sysdeps/ieee754/flt-32/
+ * e_exp2f.c, e_exp2f_data.c): x = k/32 + r with |r| <= 1/64, 2^x = 2^(k/32) 2^r
+ * with 2^(k/32) from a table (the double nearest 2^(i/32), its exponent field
+ * lowered by i/32 so that adding k << 47 yields 2^(k/32)) and 2^r by a cubic.
+ * Rounded to float it equals the host's exp2f on every float (checked
+ * exhaustively). Finite x in (-150, 128) only. */
+constexpr uint64_t kExp2Tab[32] = {
+    0x3ff0000000000000ull, 0x3fefd9b0d3158574ull, 0x3fefb5586cf9890full, 0x3fef9301d0125b51ull,
+    0x3fef72b83c7d517bull, 0x3fef54873168b9aaull, 0x3fef387a6e756238ull, 0x3fef1e9df51fdee1ull,
+    0x3fef06fe0a31b715ull, 0x3feef1a7373aa9cbull, 0x3feedea64c123422ull, 0x3feece086061892dull,
+    0x3feebfdad5362a27ull, 0x3feeb42b569d4f82ull, 0x3feeab07dd485429ull, 0x3feea47eb03a5585ull,
+    0x3feea09e667f3bcdull, 0x3fee9f75e8ec5f74ull, 0x3feea11473eb0187ull, 0x3feea589994cce13ull,
+    0x3feeace5422aa0dbull, 0x3feeb737b0cdc5e5ull, 0x3feec49182a3f090ull, 0x3feed503b23e255dull,
+    0x3feee89f995ad3adull, 0x3feeff76f2fb5e47ull, 0x3fef199bdd85529cull, 0x3fef3720dcef9069ull,
+    0x3fef5818dcfba487ull, 0x3fef7c97337b9b5full, 0x3fefa4afa2a490daull, 0x3fefd0765b6e4540ull};
+
+__host__ __device__ inline double glibcExp2(float x) {
+  const double shift = 0x1.8p+52 / 32, xd = (double)x;
+  double kd = xd + shift;
+  union {
+    double d;
+    uint64_t u;
+  } ki{kd}, sv{0.0};
+  kd -= shift; /* k / 32 */
+  const double r = xd - kd;
+  sv.u = kExp2Tab[ki.u % 32] + (ki.u << 47);
+  const double z = 0x1.c6af84b912394p-5 * r + 0x1.ebfce50fac4f3p-3;
+  const double r2 = r * r;
+  double y = 0x1.62e42ff0c52d6p-1 * r + 1.0;
+  y = z * r2 + y;
+  return y * sv.d;
 }
 
 /* The approximation of function fn at x, in double (x inside the function's range). */
@@ -168,7 +201,7 @@ __host__ __device__ inline double lmApprox(int fn, float x) {
     case kLmLog2:
       return glibcLog2(x);
     case kLmExp2Neg:
-      return lmExp2(-(double)x);
+      return glibcExp2(-x);
     case kLmPow5: {
       const double a = x, a2 = a * a;
       return a2 * a2 * a;
@@ -257,32 +290,44 @@ __host__ __device__ inline bool lmInRange(const LibmTab &T, float x) {
   return x > 0.f && b >= T.lo && b <= T.hi;
 }
 
-/* log2f as the host libm computes it (x > 0; zero, infinities and NaN: correctly rounded, as glibc). */
-__host__ __device__ inline float hlog2f(const LibmTables *L, float x) {
-  if (lmInRange(L->t[kLmLog2], x)) return lmEval(L->t[kLmLog2], kLmLog2, x);
-  return (float)log2((double)x);
+/* pow in double precision, rounded once: for the arguments outside a table's
+ * range. Out of line, so that its constants and registers stay out of the
+ * shaders that call it on their rare paths. */
+__host__ __device__ __attribute__((noinline)) inline float lmPowSlow(float x, float y) {
+  return (float)pow((double)x, (double)y);
 }
 
-/* exp2f(t) as the host libm computes it for t in [-16, -2^-27]; elsewhere correctly rounded. */
-__host__ __device__ inline float hexp2f(const LibmTables *L, float t) {
-  const float u = -t;
-  if (lmInRange(L->t[kLmExp2Neg], u)) return lmEval(L->t[kLmExp2Neg], kLmExp2Neg, u);
-  return (float)exp2((double)t);
+/* log2f as the host libm computes it: glibc's algorithm on positive finite x;
+ * -inf at zero, NaN below zero and for NaN, +inf at +inf (as glibc). */
+__host__ __device__ inline float hlog2f(const LibmTables *, float x) {
+  if (x > 0.f && x < INFINITY) return (float)glibcLog2(x);
+  return x == 0.f ? -INFINITY : (x == INFINITY ? INFINITY : __builtin_nanf(""));
 }
 
-/* powf(x, N), N = 5 or 128, as the host libm computes it. */
+/* exp2f as the host libm computes it: glibc's algorithm on (-150, 128); 0 at
+ * and below -150, +inf from 128 on, NaN for NaN (as glibc). */
+__host__ __device__ inline float hexp2f(const LibmTables *, float t) {
+  if (t > -150.f && t < 128.f) return (float)glibcExp2(t);
+  if (t != t) return t;
+  return t >= 128.f ? INFINITY : 0.f;
+}
+
+/* powf(x, N), N = 5 or 128, as the host libm computes it; below the table's
+ * range (x >= 0) the result is +0 for glibc too. */
 template <int N>
 __host__ __device__ inline float hpowi(const LibmTables *L, float x) {
   static_assert(N == 5 || N == 128, "exponent");
   constexpr int fn = N == 5 ? kLmPow5 : kLmPow128;
   if (lmInRange(L->t[fn], x)) return lmEval(L->t[fn], fn, x);
-  return (float)pow((double)x, (double)N);
+  if (x >= 0.f && lmBits(x) < L->t[fn].lo) return 0.f;
+  return lmPowSlow(x, (float)N);
 }
 
 /* powf(x, 1/2.2f) as the host libm computes it for x in (0, 1]; elsewhere correctly rounded. */
 __host__ __device__ inline float hpowGamma(const LibmTables *L, float x) {
   if (lmInRange(L->t[kLmPowGamma], x)) return lmEval(L->t[kLmPowGamma], kLmPowGamma, x);
-  return (float)pow((double)x, (double)(1.0f / 2.2f));
+  if (x == 0.f) return 0.f;
+  return lmPowSlow(x, 1.0f / 2.2f);
 }
 
 /* ------------------------------------------------------------- atan2f (no table) */

@@ -219,14 +219,75 @@ __device__ __forceinline__ float4 nanToZero(float4 v) {
   return make_float4(v.x == v.x ? v.x : 0.f, v.y == v.y ? v.y : 0.f, v.z == v.z ? v.z : 0.f, v.w == v.w ? v.w : 0.f);
 }
 
+/* A texel index (texelIndex) is the sum of a column part and a row part: linear
+ * x + y w; 4x4 tiles ((x >> 2) << 4) + (x & 3) and ((y >> 2) tiles_x << 4) +
+ * ((y & 3) << 2); 32x32 tiles with the Morton order inside, whose two halves
+ * interleave the low five bits of x and of y. So the four texels of a bilinear
+ * footprint need the wrap and the parts of two columns and two rows only. */
+__device__ __forceinline__ uint32_t spread5(uint32_t v) {
+  v &= 31u;
+  v = (v | (v << 4)) & 0x0f0fu;
+  v = (v | (v << 2)) & 0x3333u;
+  return (v | (v << 1)) & 0x5555u;
+}
+__device__ __forceinline__ uint32_t colPart(const TexLevel &l, int layout, uint32_t x) {
+  if (layout == LAYOUT_TILED) return ((x >> 2) << 4) + (x & 3u);
+  if (layout == LAYOUT_MORTON) return ((x >> 5) << 10) + spread5(x);
+  return x;
+}
+__device__ __forceinline__ uint32_t rowPart(const TexLevel &l, int layout, uint32_t y) {
+  if (layout == LAYOUT_TILED) return (((y >> 2) * (uint32_t)l.tiles_x) << 4) + ((y & 3u) << 2);
+  if (layout == LAYOUT_MORTON) return (((y >> 5) * (uint32_t)l.tiles_x) << 10) + (spread5(y) << 1);
+  return y * (uint32_t)l.w;
+}
+
+/* One axis of pixelWithWrapMode (SamplerSoft.h:170-213): the wrapped
+ * coordinate; `out` when CLAMP_TO_BORDER finds it outside (border colour),
+ * `bad` when the wrapped coordinate is still outside (zero). */
+struct AxisTexel {
+  uint32_t v;
+  bool out, bad;
+};
+__device__ __forceinline__ AxisTexel wrapAxis(int v, int n, int wrap) {
+  AxisTexel a{0u, false, false};
+  if (wrap == 0) {
+    v &= n - 1;
+  } else if (wrap == 1) {
+    v &= 2 * n - 1;
+    v -= n;
+    v = v >= 0 ? v : (-1 - v);
+    v = n - 1 - v;
+  } else if (wrap == 2) {
+    v = v < 0 ? 0 : (v >= n ? n - 1 : v);
+  } else {
+    a.out = v < 0 || v >= n;
+  }
+  a.bad = (unsigned)v >= (unsigned)n;
+  a.v = (uint32_t)v;
+  return a;
+}
+
+/* The texel of column c and row r: border colour, zero or the image's texel (fetchWrap's order of tests). */
+__device__ __forceinline__ float4 readTexel(const TexDesc &d, const uint32_t *img, int wrap, const AxisTexel &c,
+                                            uint32_t cp, const AxisTexel &r, uint32_t rp) {
+  if (wrap == 3 && (c.out || r.out)) return bytesF(Texel<uint32_t>::border(d));
+  if (c.bad || r.bad) return make_float4(0.f, 0.f, 0.f, 0.f);
+  return bytesF(img[cp + rp]);
+}
+
 __device__ __forceinline__ float4 bilinear4(const TexDesc &d, const TexLevel &l, float ux, float uy, int wrap) {
   const float fx0 = ux - 0.5f, fy0 = uy - 0.5f;
   const float flx = floorf(fx0), fly = floorf(fy0);
   const int x = (int)flx, y = (int)fly;
-  const float4 s1 = bytesF(fetchWrap<uint32_t>(d, l, x, y, wrap));
-  const float4 s2 = bytesF(fetchWrap<uint32_t>(d, l, x + 1, y, wrap));
-  const float4 s3 = bytesF(fetchWrap<uint32_t>(d, l, x, y + 1, wrap));
-  const float4 s4 = bytesF(fetchWrap<uint32_t>(d, l, x + 1, y + 1, wrap));
+  const AxisTexel c0 = wrapAxis(x, l.w, wrap), c1 = wrapAxis(x + 1, l.w, wrap);
+  const AxisTexel r0 = wrapAxis(y, l.h, wrap), r1 = wrapAxis(y + 1, l.h, wrap);
+  const uint32_t cp0 = colPart(l, d.layout, c0.v), cp1 = colPart(l, d.layout, c1.v);
+  const uint32_t rp0 = rowPart(l, d.layout, r0.v), rp1 = rowPart(l, d.layout, r1.v);
+  const uint32_t *img = (const uint32_t *)(d.base + l.offset);
+  const float4 s1 = readTexel(d, img, wrap, c0, cp0, r0, rp0);
+  const float4 s2 = readTexel(d, img, wrap, c1, cp1, r0, rp0);
+  const float4 s3 = readTexel(d, img, wrap, c0, cp0, r1, rp1);
+  const float4 s4 = readTexel(d, img, wrap, c1, cp1, r1, rp1);
   const float fx = fx0 - flx, fy = fy0 - fly;
   return nanToZero(mixB(mixB(s1, s2, fx), mixB(s3, s4, fx), fy));
 }
@@ -293,12 +354,18 @@ __device__ __forceinline__ void sampleLevelPair(const TexDesc &a, const TexDesc 
   const float fx0 = tx - 0.5f, fy0 = ty - 0.5f;
   const float flx = floorf(fx0), fly = floorf(fy0);
   const int x = (int)flx, y = (int)fly;
-  const TexelAt t1 = texelAt(a, la, x, y), t2 = texelAt(a, la, x + 1, y);
-  const TexelAt t3 = texelAt(a, la, x, y + 1), t4 = texelAt(a, la, x + 1, y + 1);
+  const int wrap = a.wrap;
+  const AxisTexel c0 = wrapAxis(x, la.w, wrap), c1 = wrapAxis(x + 1, la.w, wrap);
+  const AxisTexel r0 = wrapAxis(y, la.h, wrap), r1 = wrapAxis(y + 1, la.h, wrap);
+  const uint32_t cp0 = colPart(la, a.layout, c0.v), cp1 = colPart(la, a.layout, c1.v);
+  const uint32_t rp0 = rowPart(la, a.layout, r0.v), rp1 = rowPart(la, a.layout, r1.v);
   const float fx = fx0 - flx, fy = fy0 - fly;
   /* one image at a time: the second's texels are read after the first is mixed */
-  ra = nanToZero(mixB(mixB(readAt(a, la, t1), readAt(a, la, t2), fx), mixB(readAt(a, la, t3), readAt(a, la, t4), fx), fy));
-  rb = nanToZero(mixB(mixB(readAt(b, lb, t1), readAt(b, lb, t2), fx), mixB(readAt(b, lb, t3), readAt(b, lb, t4), fx), fy));
+  const uint32_t *ia = (const uint32_t *)(a.base + la.offset), *ib = (const uint32_t *)(b.base + lb.offset);
+  ra = nanToZero(mixB(mixB(readTexel(a, ia, wrap, c0, cp0, r0, rp0), readTexel(a, ia, wrap, c1, cp1, r0, rp0), fx),
+                      mixB(readTexel(a, ia, wrap, c0, cp0, r1, rp1), readTexel(a, ia, wrap, c1, cp1, r1, rp1), fx), fy));
+  rb = nanToZero(mixB(mixB(readTexel(b, ib, wrap, c0, cp0, r0, rp0), readTexel(b, ib, wrap, c1, cp1, r0, rp0), fx),
+                      mixB(readTexel(b, ib, wrap, c0, cp0, r1, rp1), readTexel(b, ib, wrap, c1, cp1, r1, rp1), fx), fy));
 }
 
 /* textureImpl of two textures of one shape at one coordinate and LOD */
@@ -607,7 +674,9 @@ __device__ __forceinline__ float gammaIn(const DrawParams &P, float x) {
  * hardware log2/exp2 estimate picks the starting byte and the thresholds
  * correct it. Returned as a float that truncates back to k. */
 __device__ __forceinline__ float gammaOutByte(const DrawParams &P, float c) {
-  if (!(c > 0.f) || !(c < INFINITY)) return hpowGamma(P.libm, c);
+  /* powf(c, 1/2.2) off (0, inf): +0 at zero (either sign), NaN below zero and for NaN, inf at inf */
+  if (!(c > 0.f)) return c == 0.f ? 0.f : __builtin_nanf("");
+  if (!(c < INFINITY)) return INFINITY;
   const float *T = P.gamma_lut + 256;
   const float e = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(c) * (1.0f / 2.2f));
   int k = (int)(gmin(e, 1.f) * 255.f);
