@@ -362,6 +362,56 @@ def run_batched(r, quick=False, reps=5, log=lambda m: None):
     return res
 
 
+def run_ibl(r, reps=5, log=lambda m: None):
+    """The C2 environment's IBL precompute on the GPU (SURVEY's next rows: IBLIrradianceSoft.h:74-101,
+    IBLPrefilterSoft.h:88-168 drawn into cube faces as Environment.cpp:48-103,152-195 does): a 512^2 RGBA8
+    radiance cube with mips -> 32^2 irradiance cube and 128^2 x 5-level prefiltered cube, 36 render passes.
+    Timed from resident resources to the maps written; the CPU reference's time for the same passes on one
+    face (the serial build) is given beside it."""
+    from softglrender_amd import scenes
+    from softglrender_amd.replay import Replayer, parse
+    from softglrender_amd import stream as S
+    sc = scenes.scene_ibl_precompute()
+    rp = Replayer(r)
+    setup, frame = [], []
+    in_frame = False
+    for op, p in parse(sc.stream.bytes()):
+        if op == S.OP_FRAME:
+            in_frame = True
+        elif op == S.OP_READ:
+            continue
+        elif in_frame:
+            frame.append((op, p))
+        else:
+            setup.append((op, p))
+    sink = []
+    for op, p in setup:
+        rp.exec(op, p, sink)
+    for op, p in frame:
+        rp.exec(op, p, sink)
+    r.waitIdle()
+    t = time.perf_counter()
+    for _ in range(reps):
+        for op, p in frame:
+            rp.exec(op, p, sink)
+    r.waitIdle()
+    ms = (time.perf_counter() - t) * 1e3 / reps
+    r.profile(True)
+    for op, p in frame:
+        rp.exec(op, p, sink)
+    r.waitIdle()
+    prof = r.profileResults()
+    r.profile(False)
+    m = sc.meta
+    # fragments: every texel of the 6 irradiance faces and of the prefiltered levels' faces
+    frags = 6 * (m["irr_size"] ** 2 + sum(max(1, m["pref_size"] >> lev) ** 2 for lev in range(m["pref_levels"])))
+    log(f"[ibl] precompute {ms:.3f} ms ({frags} fragments)")
+    return {"workload": f"{m['src_size']}^2 RGBA8 mipmapped radiance cube -> {m['irr_size']}^2 irradiance, "
+                        f"{m['pref_size']}^2 x {m['pref_levels']} levels prefiltered (36 render passes)",
+            "ms": round(ms, 3), "fragments": frags,
+            "kernels_ms": {k: round(v[0], 4) for k, v in prof.items()}}
+
+
 def run_neural(r, quick=False, log=lambda m: None):
     """C5: the learned PBR shader on the C2 scene (point light off for both shaders). Trains on fragments of
     64 random cameras x 8 random materials, evaluates per material on 32 held-out cameras against the FP64
@@ -450,6 +500,7 @@ def main():
     ap.add_argument("--neural-quick", action="store_true", help="reduced c5 run (fewer cameras and steps)")
     ap.add_argument("--cpu-frames", type=int, default=5, help="timed reference frames (after one warm-up)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-views (c4) measurement")
+    ap.add_argument("--no-ibl", action="store_true", help="skip the IBL precompute measurement")
     ap.add_argument("--batched-quick", action="store_true", help="reduced c4 run (64 views over 16 scenes)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -598,6 +649,13 @@ def main():
         except Exception as e:  # report, do not hide
             neural_res = {"error": repr(e)}
 
+    ibl_res = None
+    if args.config == "c2" and not args.no_ibl:
+        try:
+            ibl_res = run_ibl(r, log=lambda m: print(m, file=sys.stderr, flush=True))
+        except Exception as e:  # report, do not hide
+            ibl_res = {"error": repr(e)}
+
     batched_res = None
     if args.config == "c2" and not args.no_batched:
         try:
@@ -630,6 +688,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "neural": neural_res,
+        "ibl_precompute": ibl_res,
         "batched_views": batched_res,
     }
     print(json.dumps(line), flush=True)

@@ -909,10 +909,22 @@ __device__ __forceinline__ Frag makeFrag(const DrawParams &P, const PrimSetup &p
   const float *vb = P.vary + (size_t)ps.v[1] * P.vary_stride;
   const float *vc = P.vary + (size_t)ps.v[2] * P.vary_stride;
   Frag fr;
+  float4 g0a = make_float4(0.f, 0.f, 0.f, 0.f), g0b = g0a, g0c = g0a; /* group 0 (texture coordinates) */
 #pragma unroll
-  for (int j = 0; j < kVaryStride; j++) {
-    const int o = varyGroupOffset(P, j >> 2) + (j & 3);
-    fr.v[j] = varyUsed<SHADING>(P, j) ? interp(b, va[o], vb[o], vc[o]) : 0.f;
+  for (int g = 0; g < kVaryStride / 4; g++) {
+    if (!((P.vary_groups >> g) & 1u)) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) fr.v[4 * g + k] = 0.f;
+      continue;
+    }
+    /* a stored group is read whole, with one four-float load per vertex */
+    const int o = varyGroupOffset(P, g);
+    const float4 qa = *(const float4 *)(va + o), qb = *(const float4 *)(vb + o), qc = *(const float4 *)(vc + o);
+    if (g == 0) g0a = qa, g0b = qb, g0c = qc;
+    const float ea[4] = {qa.x, qa.y, qa.z, qa.w}, eb[4] = {qb.x, qb.y, qb.z, qb.w}, ec[4] = {qc.x, qc.y, qc.z, qc.w};
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      fr.v[4 * g + k] = varyUsed<SHADING>(P, 4 * g + k) ? interp(b, ea[k], eb[k], ec[k]) : 0.f;
   }
   fr.tc[0] = fr.tc[1] = fr.tc[2] = {0.f, 0.f};
   if (need_deriv && (P.vary_groups & 1u)) { /* texture coordinates are slots 0-1 (group 0, offset 0) */
@@ -928,7 +940,7 @@ __device__ __forceinline__ Frag makeFrag(const DrawParams &P, const PrimSetup &p
         fr.tc[j] = {fr.v[0], fr.v[1]};
       } else {
         const float4 pb = shadingBc<S>(T, (float)(qx + (j & 1)), (float)(qy + (j >> 1)));
-        fr.tc[j] = {interp(pb, va[0], vb[0], vc[0]), interp(pb, va[1], vb[1], vc[1])};
+        fr.tc[j] = {interp(pb, g0a.x, g0b.x, g0c.x), interp(pb, g0a.y, g0b.y, g0c.y)};
       }
     }
   }

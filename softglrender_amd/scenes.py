@@ -997,3 +997,32 @@ def scene_points_lines(width=160, height=120, seed=17, msaa=False, read_samples=
         st.read(2, which=1)
     st.read(5)
     return Scene("points_lines", st, width, height, msaa, {})
+
+
+def scene_ibl_precompute(src_size=512, irr_size=32, pref_size=128, pref_levels=5, seed=7):
+    """The C2 environment's IBL precompute alone: a seeded smooth radiance cube of `src_size`-px RGBA8 faces
+    with a full mip chain (RendererSoft stores colour textures as RGBA8: Texture.h:63-66), convolved into a
+    `irr_size`-px irradiance cube and GGX-prefiltered into a `pref_size`-px cube of `pref_levels` levels
+    (Environment.cpp:48-103, 152-195). The frame is the 36 render passes; the maps are read back."""
+    st = S.Stream()
+    cv, ci = cube_mesh()
+    st.vao(1, cv, ci)
+    trilinear = dict(filter_min=S.FILTER_LINEAR_MIPMAP_LINEAR, filter_mag=S.FILTER_LINEAR, wrap=S.WRAP_CLAMP_TO_EDGE)
+    st.texture(10, src_size, src_size, type=S.TEX_CUBE, mipmaps=True, usage=S.USAGE_SAMPLER | S.USAGE_UPLOAD,
+               **trilinear)
+    st.tex_data(10, env_cube_faces(src_size, seed))
+    st.texture(11, irr_size, irr_size, type=S.TEX_CUBE, usage=S.USAGE_COLOR | S.USAGE_SAMPLER,
+               filter_min=S.FILTER_LINEAR, filter_mag=S.FILTER_LINEAR, wrap=S.WRAP_CLAMP_TO_EDGE)
+    st.tex_init(11)
+    st.texture(12, pref_size, pref_size, type=S.TEX_CUBE, mipmaps=True, usage=S.USAGE_COLOR | S.USAGE_SAMPLER,
+               **trilinear)
+    st.tex_init(12)
+    st.frame()
+    ibl_precompute(st, 1, 10, 11, 12, src_size, irr_size, pref_size, pref_levels)
+    for face in range(6):
+        st.read(11, layer=face)
+    for lev in range(pref_levels):
+        for face in range(6):
+            st.read(12, layer=face, level=lev)
+    return Scene("ibl_precompute", st, pref_size, pref_size, False,
+                 dict(src_size=src_size, irr_size=irr_size, pref_size=pref_size, pref_levels=pref_levels))
