@@ -279,9 +279,12 @@ void launchOrdShade(const DrawParams &P, int grid, int need_deriv, hipStream_t s
       RHIP_LAUNCH(k_ord_shade<S RHIP_COMMA RHS_SHADING_BLINN_PHONG_DIR>, grid, 256, st, P, need_deriv);
       break;
     case RHS_SHADING_PBR: RHIP_LAUNCH(k_ord_shade<S RHIP_COMMA RHS_SHADING_PBR>, grid, 256, st, P, need_deriv); break;
-    case RHS_SHADING_SKYBOX:
     case RHS_SHADING_IBL_IRRADIANCE:
     case RHS_SHADING_IBL_PREFILTER:
+      /* one wavefront per fragment: `grid` blocks of 256 lanes cover the fragments one lane each */
+      RHIP_LAUNCH(k_ord_shade_ibl<S>, std::min(grid * 64, 16384), 256, st, P, need_deriv);
+      break;
+    case RHS_SHADING_SKYBOX:
     case RHS_SHADING_FXAA: RHIP_LAUNCH(k_ord_shade<S RHIP_COMMA kShadingEnv>, grid, 256, st, P, need_deriv); break;
     default: RHIP_LAUNCH(k_ord_shade<S RHIP_COMMA RHS_SHADING_VERTEX_COLOR>, grid, 256, st, P, need_deriv); break;
   }

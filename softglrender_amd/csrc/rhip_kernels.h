@@ -2522,6 +2522,88 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
   }
 }
 
+/* The IBL precompute shaders (irradiance: 252 x 63 terms per fragment;
+ * prefilter: 1024 samples) on listed fragments, one wavefront per fragment:
+ * the lanes compute 64 terms at a time and every lane then adds them in the
+ * shader's order from LDS, so the sum is the sequential one, bit for bit. A
+ * face of 32^2 fragments fills 1024 wavefronts instead of 16. */
+template <int S>
+__global__ __launch_bounds__(256) void k_ord_shade_ibl(DrawParams P, int need_deriv) {
+  __shared__ float4 terms[4][64];
+  const int n = min(P.dcount[3], P.ord_cap);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nwaves = gridDim.x * (blockDim.x >> 6);
+  float4 *L = terms[w];
+  for (int j = blockIdx.x * (blockDim.x >> 6) + w; j < n; j += nwaves) {
+    const uint2 jb = P.jobs[j];
+    if (jb.x == kListVoid) continue;
+    const uint32_t pix = jb.x & 0x0fffffffu;
+    const int x = (int)(pix % (uint32_t)P.fb_w), y = (int)(pix / (uint32_t)P.fb_w);
+    const int id = (int)(jb.y >> 2), inst = (int)(jb.y & 3u);
+    const PrimSetup ps = P.prims[id];
+    const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
+    const float fx = (float)x, fy = (float)y;
+    float4 bs;
+    if (S == 4) {
+      if (!bary(T, fx + 0.5f, fy + 0.5f, bs)) {
+#pragma unroll
+        for (int t = 3; t >= 0; t--) {
+          float4 b;
+          if (bary(T, sampleX(t) + fx, sampleY(t) + fy, b)) bs = b;
+        }
+      }
+    } else {
+      bary(T, fx + 0.5f, fy + 0.5f, bs);
+    }
+    const Frag f = makeFrag<S, kShadingEnv>(P, ps, T, x, y, bs, inst, need_deriv);
+    RGBAf c;
+    if (P.shading == RHS_SHADING_IBL_IRRADIANCE) {
+      const IrrSetup st = irrSetup(f);
+      const int total = P.ibl->nphi * P.ibl->ntheta;
+      f3 irr{0.f, 0.f, 0.f};
+      float nr = 0.f;
+      for (int base = 0; base < total; base += 64) {
+        const int k = base + lane;
+        const f3 t = k < total ? irrTerm(P, st, k) : f3{0.f, 0.f, 0.f};
+        L[lane] = make_float4(t.x, t.y, t.z, 0.f);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int m = min(64, total - base);
+        for (int q = 0; q < m; q++) {
+          const float4 e = L[q];
+          irr = add3(irr, {e.x, e.y, e.z});
+          nr += 1.f;
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
+      c = irrFinish(irr, nr);
+    } else {
+      const PrefSetup st = prefSetup(P, f);
+      f3 acc{0.f, 0.f, 0.f};
+      float tw = 0.f;
+      for (uint32_t base = 0; base < 1024u; base += 64) {
+        L[lane] = prefSample(P, st, base + (uint32_t)lane);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int q = 0; q < 64; q++) {
+          const float4 t = L[q];
+          if (t.w > 0.0f) {
+            acc = {fmaf(t.x, t.w, acc.x), fmaf(t.y, t.w, acc.y), fmaf(t.z, t.w, acc.z)};
+            tw += t.w;
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
+      c = {acc.x / tw, acc.y / tw, acc.z / tw, 1.f};
+    }
+    if (lane == 0)
+      P.ord_col[j] = make_float4(gclamp(c.r, 0.f, 1.f), gclamp(c.g, 0.f, 1.f), gclamp(c.b, 0.f, 1.f),
+                                 gclamp(c.a, 0.f, 1.f));
+  }
+}
+
 /* Each pixel's listed fragments written to its samples in order, blended when
  * blending is on: one workgroup per tile with fragments, one lane per pixel. */
 template <int S>

@@ -853,71 +853,104 @@ __device__ __forceinline__ RGBAf fsSkybox(const DrawParams &P, const Frag &f) {
 
 /* IBLIrradianceSoft.h:74-101: the cosine-weighted hemisphere sum in the
  * reference's loop order; sin and cos of the loop angles come from the host
- * libm (IblTables). */
-__device__ RGBAf fsIrradiance(const DrawParams &P, const Frag &f) {
-  const f3 N = nrm3({f.v[0], f.v[1], f.v[2]});
+ * libm (IblTables). The sum's terms (irrTerm) are independent; only their
+ * addition is ordered, so a caller may compute them in parallel and add them
+ * in order (k_ord_shade_ibl). */
+struct IrrSetup {
+  f3 N, right, up;
+};
+__device__ __forceinline__ IrrSetup irrSetup(const Frag &f) {
+  IrrSetup s;
+  s.N = nrm3({f.v[0], f.v[1], f.v[2]});
   f3 up{0.f, 1.f, 0.f};
-  const f3 right = nrm3(cross3(up, N));
-  up = nrm3(cross3(N, right));
+  s.right = nrm3(cross3(up, s.N));
+  s.up = nrm3(cross3(s.N, s.right));
+  return s;
+}
+/* Term k of the sum: phi index k / ntheta, theta index k % ntheta. */
+__device__ __forceinline__ f3 irrTerm(const DrawParams &P, const IrrSetup &s, int k) {
   const IblTables &T = *P.ibl;
-  const TexDesc *cube = P.tex[RHS_SLOT_CUBE];
-  f3 irr{0.f, 0.f, 0.f};
-  float nr = 0.f;
-  for (int i = 0; i < T.nphi; i++) {
-    const float sp = T.sphi[i], cp = T.cphi[i];
-    for (int j = 0; j < T.ntheta; j++) {
-      const float st = T.stheta[j], ct = T.ctheta[j];
-      const f3 ts{st * cp, st * sp, ct};
-      const f3 sv = add3(add3(scl3(right, ts.x), scl3(up, ts.y)), scl3(N, ts.z));
-      const RGBAf c = texCube(cube, sv, 0.f);
-      irr = add3(irr, {c.r * ct * st, c.g * ct * st, c.b * ct * st});
-      nr += 1.f;
-    }
-  }
+  const int i = k / T.ntheta, j = k - i * T.ntheta;
+  const float sp = T.sphi[i], cp = T.cphi[i], st = T.stheta[j], ct = T.ctheta[j];
+  const f3 ts{st * cp, st * sp, ct};
+  const f3 sv = add3(add3(scl3(s.right, ts.x), scl3(s.up, ts.y)), scl3(s.N, ts.z));
+  const RGBAf c = texCube(P.tex[RHS_SLOT_CUBE], sv, 0.f);
+  return {c.r * ct * st, c.g * ct * st, c.b * ct * st};
+}
+__device__ __forceinline__ RGBAf irrFinish(f3 irr, float nr) {
   irr = scl3(scl3(irr, kPI), 1.0f / nr);
   return {irr.x, irr.y, irr.z, 1.f};
+}
+__device__ RGBAf fsIrradiance(const DrawParams &P, const Frag &f) {
+  const IrrSetup s = irrSetup(f);
+  const int n = P.ibl->nphi * P.ibl->ntheta;
+  f3 irr{0.f, 0.f, 0.f};
+  float nr = 0.f;
+  for (int k = 0; k < n; k++) {
+    irr = add3(irr, irrTerm(P, s, k));
+    nr += 1.f;
+  }
+  return irrFinish(irr, nr);
 }
 
 /* IBLPrefilterSoft.h:88-168: GGX importance sampling over 1024 Hammersley
  * points, each sample read from the source mip level its pdf selects. The
  * reference build contracts (a^2 - 1) Xi.y + 1, 1 - cos^2, the tangent-space
  * sum, 2 (V.H) H - V, the GGX denominator, 1024 pdf + 1e-4 and the colour
- * sum into fused multiply-adds; so does this one, term for term. */
-__device__ RGBAf fsPrefilter(const DrawParams &P, const Frag &f) {
-  const f3 N = nrm3b({f.v[0], f.v[1], f.v[2]});
-  const f3 V = N;
-  const float rough = P.u.aux[1], res = P.u.aux[0];
+ * sum into fused multiply-adds; so does this one, term for term. As for the
+ * irradiance, the samples are independent and only their sum is ordered. */
+struct PrefSetup {
+  f3 N, V, tangent, bitangent;
+  float rough, ra2, saTexel;
+};
+__device__ __forceinline__ PrefSetup prefSetup(const DrawParams &P, const Frag &f) {
+  PrefSetup s;
+  s.N = nrm3b({f.v[0], f.v[1], f.v[2]});
+  s.V = s.N;
+  s.rough = P.u.aux[1];
+  const float res = P.u.aux[0];
+  const float a = s.rough * s.rough;
+  s.ra2 = a * a;
+  const f3 upv = (double)fabsf(s.N.z) < 0.999 ? f3{0.f, 0.f, 1.f} : f3{1.f, 0.f, 0.f};
+  s.tangent = nrm3b(cross3(upv, s.N));
+  s.bitangent = cross3(s.N, s.tangent);
+  s.saTexel = 4.0f * kPI / (6.0f * res * res);
+  return s;
+}
+/* Sample i: the source texel's colour and N.L (the sample counts only when N.L > 0). */
+__device__ __forceinline__ float4 prefSample(const DrawParams &P, const PrefSetup &s, uint32_t i) {
   const IblTables &T = *P.ibl;
-  const TexDesc *cube = P.tex[RHS_SLOT_CUBE];
-  const float a = rough * rough;
-  const f3 upv = (double)fabsf(N.z) < 0.999 ? f3{0.f, 0.f, 1.f} : f3{1.f, 0.f, 0.f};
-  const f3 tangent = nrm3b(cross3(upv, N));
-  const f3 bitangent = cross3(N, tangent);
-  const float saTexel = 4.0f * kPI / (6.0f * res * res);
-  const float ra2 = a * a;
+  const f3 N = s.N, V = s.V, tangent = s.tangent, bitangent = s.bitangent;
+  const float ra2 = s.ra2;
+  const float xiy = (float)((double)__builtin_bitreverse32(i) * 2.3283064365386963e-10);
+  const float cosT = sqrtf((1.0f - xiy) / fmaf(ra2 - 1.0f, xiy, 1.0f));
+  const float sinT = sqrtf(fmaf(-cosT, cosT, 1.0f));
+  const f3 Ht{T.hcos[i] * sinT, T.hsin[i] * sinT, cosT};
+  const f3 H = nrm3b({fmaf(N.x, Ht.z, fmaf(bitangent.x, Ht.y, tangent.x * Ht.x)),
+                      fmaf(N.y, Ht.z, fmaf(bitangent.y, Ht.y, tangent.y * Ht.x)),
+                      fmaf(N.z, Ht.z, fmaf(bitangent.z, Ht.y, tangent.z * Ht.x))});
+  const float vh2 = 2.0f * dot3b(V, H);
+  const f3 L = nrm3b({fmaf(H.x, vh2, -V.x), fmaf(H.y, vh2, -V.y), fmaf(H.z, vh2, -V.z)});
+  const float NdotL = gmax(dot3b(N, L), 0.0f);
+  if (!(NdotL > 0.0f)) return make_float4(0.f, 0.f, 0.f, 0.f);
+  const float NdotH = gmax(dot3b(N, H), 0.0f), HdotV = gmax(dot3b(H, V), 0.0f);
+  const float den = fmaf(NdotH * NdotH, ra2 - 1.0f, 1.0f);
+  const float D = ra2 / (kPI * den * den);
+  const float pdf = D * NdotH / (4.0f * HdotV) + 0.0001f;
+  const float saSample = 1.0f / fmaf(1024.0f, pdf, 0.0001f);
+  const float mip = s.rough == 0.0f ? 0.0f : 0.5f * hlog2f(P.libm, saSample / s.saTexel);
+  const RGBAf c = texCube(P.tex[RHS_SLOT_CUBE], L, mip);
+  return make_float4(c.r, c.g, c.b, NdotL);
+}
+__device__ RGBAf fsPrefilter(const DrawParams &P, const Frag &f) {
+  const PrefSetup s = prefSetup(P, f);
   f3 acc{0.f, 0.f, 0.f};
   float tw = 0.f;
   for (uint32_t i = 0; i < 1024u; i++) {
-    const float xiy = (float)((double)__builtin_bitreverse32(i) * 2.3283064365386963e-10);
-    const float cosT = sqrtf((1.0f - xiy) / fmaf(ra2 - 1.0f, xiy, 1.0f));
-    const float sinT = sqrtf(fmaf(-cosT, cosT, 1.0f));
-    const f3 Ht{T.hcos[i] * sinT, T.hsin[i] * sinT, cosT};
-    const f3 H = nrm3b({fmaf(N.x, Ht.z, fmaf(bitangent.x, Ht.y, tangent.x * Ht.x)),
-                        fmaf(N.y, Ht.z, fmaf(bitangent.y, Ht.y, tangent.y * Ht.x)),
-                        fmaf(N.z, Ht.z, fmaf(bitangent.z, Ht.y, tangent.z * Ht.x))});
-    const float vh2 = 2.0f * dot3b(V, H);
-    const f3 L = nrm3b({fmaf(H.x, vh2, -V.x), fmaf(H.y, vh2, -V.y), fmaf(H.z, vh2, -V.z)});
-    const float NdotL = gmax(dot3b(N, L), 0.0f);
-    if (NdotL > 0.0f) {
-      const float NdotH = gmax(dot3b(N, H), 0.0f), HdotV = gmax(dot3b(H, V), 0.0f);
-      const float den = fmaf(NdotH * NdotH, ra2 - 1.0f, 1.0f);
-      const float D = ra2 / (kPI * den * den);
-      const float pdf = D * NdotH / (4.0f * HdotV) + 0.0001f;
-      const float saSample = 1.0f / fmaf(1024.0f, pdf, 0.0001f);
-      const float mip = rough == 0.0f ? 0.0f : 0.5f * hlog2f(P.libm, saSample / saTexel);
-      const RGBAf c = texCube(cube, L, mip);
-      acc = {fmaf(c.r, NdotL, acc.x), fmaf(c.g, NdotL, acc.y), fmaf(c.b, NdotL, acc.z)};
-      tw += NdotL;
+    const float4 t = prefSample(P, s, i);
+    if (t.w > 0.0f) {
+      acc = {fmaf(t.x, t.w, acc.x), fmaf(t.y, t.w, acc.y), fmaf(t.z, t.w, acc.z)};
+      tw += t.w;
     }
   }
   return {acc.x / tw, acc.y / tw, acc.z / tw, 1.f};
