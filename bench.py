@@ -362,12 +362,12 @@ def run_batched(r, quick=False, reps=5, log=lambda m: None):
     return res
 
 
-def run_ibl(r, reps=5, log=lambda m: None):
-    """The C2 environment's IBL precompute on the GPU (SURVEY's next rows: IBLIrradianceSoft.h:74-101,
-    IBLPrefilterSoft.h:88-168 drawn into cube faces as Environment.cpp:48-103,152-195 does): a 512^2 RGBA8
-    radiance cube with mips -> 32^2 irradiance cube and 128^2 x 5-level prefiltered cube, 36 render passes.
-    Timed from resident resources to the maps written; the CPU reference's time for the same passes on one
-    face (the serial build) is given beside it."""
+def run_ibl(r, reps=5, cpu=True, log=lambda m: None):
+    """The C2 environment's IBL precompute on the GPU (IBLIrradianceSoft.h:74-101, IBLPrefilterSoft.h:88-168
+    drawn into cube faces as Environment.cpp:48-103,152-195 does): a 512^2 RGBA8 radiance cube with mips ->
+    32^2 irradiance cube and 128^2 x 5-level prefiltered cube, 36 render passes. Timed from resident
+    resources to the maps written; with `cpu`, SoftGLRender's RendererSoft as shipped (multithreaded) runs
+    the same 36 passes once on the host beside it (10-20 s)."""
     from softglrender_amd import scenes
     from softglrender_amd.replay import Replayer, parse
     from softglrender_amd import stream as S
@@ -406,10 +406,27 @@ def run_ibl(r, reps=5, log=lambda m: None):
     # fragments: every texel of the 6 irradiance faces and of the prefiltered levels' faces
     frags = 6 * (m["irr_size"] ** 2 + sum(max(1, m["pref_size"] >> lev) ** 2 for lev in range(m["pref_levels"])))
     log(f"[ibl] precompute {ms:.3f} ms ({frags} fragments)")
-    return {"workload": f"{m['src_size']}^2 RGBA8 mipmapped radiance cube -> {m['irr_size']}^2 irradiance, "
-                        f"{m['pref_size']}^2 x {m['pref_levels']} levels prefiltered (36 render passes)",
-            "ms": round(ms, 3), "fragments": frags,
-            "kernels_ms": {k: round(v[0], 4) for k, v in prof.items()}}
+    res = {"workload": f"{m['src_size']}^2 RGBA8 mipmapped radiance cube -> {m['irr_size']}^2 irradiance, "
+                       f"{m['pref_size']}^2 x {m['pref_levels']} levels prefiltered (36 render passes)",
+           "ms": round(ms, 3), "fragments": frags,
+           "kernels_ms": {k: round(v[0], 4) for k, v in prof.items()}}
+    if cpu:
+        try:
+            import oracle
+            if oracle.have_reference(threaded=True):
+                allowed, online = _host_cpus()
+                _, times = oracle.ref_replay(sc.stream.bytes(), sc.stream.output_bytes(), repeats=1,
+                                             threaded=True)
+                res["cpu_baseline"] = {
+                    "value": round(times[0], 1), "unit": "ms", "cores": allowed, "kind": "reference",
+                    "threads": online,
+                    "sample": f"the same 36 passes once, SoftGLRender RendererSoft as shipped ({online}-worker "
+                              f"pool) built from its sources",
+                    "gpu_speedup": round(times[0] / ms, 1)}
+                log(f"[ibl] reference {times[0]:.0f} ms")
+        except Exception as e:  # report, do not hide
+            res["cpu_baseline"] = {"error": repr(e)}
+    return res
 
 
 def run_neural(r, quick=False, log=lambda m: None):
@@ -652,7 +669,7 @@ def main():
     ibl_res = None
     if args.config == "c2" and not args.no_ibl:
         try:
-            ibl_res = run_ibl(r, log=lambda m: print(m, file=sys.stderr, flush=True))
+            ibl_res = run_ibl(r, cpu=not args.no_cpu_baseline, log=lambda m: print(m, file=sys.stderr, flush=True))
         except Exception as e:  # report, do not hide
             ibl_res = {"error": repr(e)}
 
