@@ -429,6 +429,75 @@ def run_ibl(r, reps=5, cpu=True, log=lambda m: None):
     return res
 
 
+def run_c3(r, steps=5, warmup=1, cpu=True, log=lambda m: None):
+    """C3 beside the headline frame: 3840x2160 MSAA-4x, 2,000,000 random small triangles and 2,000 screen-covering
+    ones, vertex colour, LESS (the hierarchical-Z stress scene). ms/frame between synchronisations after `warmup`
+    frames, the kernel breakdown from a second pass with events, and, with `cpu`, the shipped multithreaded
+    RendererSoft on a 240x135 crop of the same triangle density with all 2,000 screen-covering triangles, times
+    256 (a whole frame takes the reference about half an hour)."""
+    from softglrender_amd import scenes
+    from softglrender_amd.replay import Replayer, parse
+    from softglrender_amd import stream as S
+    sc = scenes.scene_c3()
+    rp = Replayer(r, layouts=sc.meta.get("layouts"))
+    setup, frame = [], []
+    in_frame = False
+    for op, p in parse(sc.stream.bytes()):
+        if op == S.OP_FRAME:
+            in_frame = True
+        elif op == S.OP_READ:
+            continue
+        elif in_frame:
+            frame.append((op, p))
+        else:
+            setup.append((op, p))
+    sink = []
+    for op, p in setup:
+        rp.exec(op, p, sink)
+
+    def run_frame():
+        for op, p in frame:
+            rp.exec(op, p, sink)
+
+    for _ in range(warmup):
+        run_frame()
+    r.waitIdle()
+    t = time.perf_counter()
+    for _ in range(steps):
+        run_frame()
+    r.waitIdle()
+    ms = (time.perf_counter() - t) * 1e3 / steps
+    r.profile(True)
+    run_frame()
+    r.waitIdle()
+    prof = r.profileResults()
+    r.profile(False)
+    log(f"[c3] {ms:.3f} ms/frame")
+    res = {"workload": "c3_2160p_msaa4x_2M_random_tris_plus_2k_screen_covering_vertex_colour_less",
+           "ms_per_frame": round(ms, 4), "steps": steps, "warmup": warmup,
+           "triangles": sc.meta.get("triangles"),
+           "kernels_ms_per_frame": {k: round(v[0], 4) for k, v in prof.items()}}
+    if cpu:
+        try:
+            import oracle
+            if oracle.have_reference(threaded=True):
+                allowed, online = _host_cpus()
+                crop = scenes.scene_c3(240, 135, 2_000_000 // 256, 2000, seed=5)
+                med, times = _time_reference(oracle, crop.stream.bytes(), crop.stream.output_bytes(), 3, True)
+                res["cpu_baseline"] = {
+                    "value": round(med * 256, 1), "unit": "ms/frame", "cores": allowed, "kind": "reference",
+                    "threads": online,
+                    "sample": f"one 240x135 crop with {2_000_000 // 256} random triangles and the 2000 "
+                              f"screen-covering ones: median of 3 frames after 1 warm-up "
+                              f"({', '.join(f'{t:.0f}' for t in times)} ms) of SoftGLRender RendererSoft as shipped "
+                              f"({online}-worker pool), times 256 (the area ratio)",
+                    "gpu_speedup": round(med * 256 / ms, 1)}
+                log(f"[c3] reference crop {med:.0f} ms x 256")
+        except Exception as e:  # report, do not hide
+            res["cpu_baseline"] = {"error": repr(e)}
+    return res
+
+
 def run_neural(r, quick=False, log=lambda m: None):
     """C5: the learned PBR shader on the C2 scene (point light off for both shaders). Trains on fragments of
     64 random cameras x 8 random materials, evaluates per material on 32 held-out cameras against the FP64
@@ -455,7 +524,9 @@ def run_neural(r, quick=False, log=lambda m: None):
     t_train = time.time() - t2
     log(f"[neural] {steps} steps in {t_train:.1f}s, loss {losses[0]:.5f} -> {losses[-1]:.6f}")
     table = neural.fallback_table(r, ho[0], ho[1], len(mats))
+    fallback = neural.apply_fallback(r, table)
     e_nn, e_pl, _ = r.neural_evaluate(ho[0], ho[1])
+    r.neural_material(0)
 
     # shading throughput on the C2 view (camera at (0, 0, 2.6), material 0)
     cam = np.array([[0.0, 0.0, 2.6]])
@@ -484,7 +555,8 @@ def run_neural(r, quick=False, log=lambda m: None):
         frame_ms = (time.perf_counter() - t) * 1e3 / reps
         prof = r.profileResults()
         r.profile(False)
-        kerns = ["k_shade_msaa4"] if name == "plain" else ["k_nn_shade_msaa4"]
+        # a material that falls back draws its learned-shader draws with the plain shader
+        kerns = ["k_shade_msaa4"] if name == "plain" or 0 in fallback else ["k_nn_shade_msaa4"]
         times[name] = {"shade_kernel_ms": round(sum(prof[k][0] / prof[k][1] for k in kerns), 4),
                        "frame_ms": round(frame_ms, 4)}
         if name == "neural":
@@ -496,11 +568,12 @@ def run_neural(r, quick=False, log=lambda m: None):
         "scene": "c2 geometry and IBL, image-based PBR (point light off for both shaders)",
         "train_fragments": int(tr[1]), "heldout_fragments": int(ho[1]),
         "train_cameras": n_train, "heldout_cameras": n_held, "materials": len(mats),
-        "steps": steps, "batch": 16384, "train_s": round(t_train, 2), "loss_first_last": [losses[0], losses[-1]],
+        "steps": steps, "batch": 16384, "train_s": round(t_train, 2),
+        "train_us_per_step": round(t_train * 1e6 / max(steps, 1), 2), "loss_first_last": [losses[0], losses[-1]],
         "error_vs_fp64_255": {"network": rnd(e_nn), "plain_gpu_shader": rnd(e_pl)},
         "per_material": [{"material": t["material"], "network_mean": round(t["network"]["mean"], 3),
                           "network_p99": round(t["network"]["p99"], 3), "fallback": t["fallback"]} for t in table],
-        "fallback_materials": [t["material"] for t in table if t["fallback"]],
+        "fallback_materials": fallback,
         "shading": times, "shading_speedup": round(speed, 3),
     }
 
@@ -518,6 +591,7 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=5, help="timed reference frames (after one warm-up)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-views (c4) measurement")
     ap.add_argument("--no-ibl", action="store_true", help="skip the IBL precompute measurement")
+    ap.add_argument("--no-c3", action="store_true", help="skip the C3 frame measured beside the C2 frame")
     ap.add_argument("--batched-quick", action="store_true", help="reduced c4 run (64 views over 16 scenes)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -673,6 +747,13 @@ def main():
         except Exception as e:  # report, do not hide
             ibl_res = {"error": repr(e)}
 
+    c3_res = None
+    if args.config == "c2" and not args.no_c3:
+        try:
+            c3_res = run_c3(r, cpu=not args.no_cpu_baseline, log=lambda m: print(m, file=sys.stderr, flush=True))
+        except Exception as e:  # report, do not hide
+            c3_res = {"error": repr(e)}
+
     batched_res = None
     if args.config == "c2" and not args.no_batched:
         try:
@@ -706,6 +787,7 @@ def main():
         "cpu_baseline": cpu,
         "neural": neural_res,
         "ibl_precompute": ibl_res,
+        "c3": c3_res,
         "batched_views": batched_res,
     }
     print(json.dumps(line), flush=True)

@@ -184,6 +184,8 @@ int rhip_neural_set_weights(rhip_device dev, const float *w, int n);    /* W1 (6
 int rhip_neural_get_weights(rhip_device dev, float *w, int n);
 int rhip_neural_records(rhip_device dev, long long capacity);          /* (re)allocates and empties the record buffer */
 int rhip_neural_capture(rhip_device dev, int enable, int shift, int material); /* 1 in 2^shift fragments of PBR draws */
+int rhip_neural_material(rhip_device dev, int material);               /* material (0..63) of the draws that follow; default 0 */
+int rhip_neural_fallback(rhip_device dev, const int *materials, int n); /* these materials' learned-shader draws use the plain PBR shader */
 int rhip_neural_record_count(rhip_device dev, long long *count);
 int rhip_neural_read_records(rhip_device dev, long long first, long long count, float *out);   /* 24 floats each */
 int rhip_neural_write_records(rhip_device dev, long long first, long long count, const float *in);

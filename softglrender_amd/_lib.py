@@ -121,6 +121,8 @@ SIGNATURES = {
     "rhip_neural_get_weights": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int],
     "rhip_neural_records": [ctypes.c_void_p, ctypes.c_longlong],
     "rhip_neural_capture": [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "rhip_neural_material": [ctypes.c_void_p, ctypes.c_int],
+    "rhip_neural_fallback": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int],
     "rhip_neural_record_count": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)],
     "rhip_neural_read_records": [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_void_p],
     "rhip_neural_write_records": [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_void_p],

@@ -415,11 +415,16 @@ int drawSetup(rhip_device d, int fbo, const int vp[4], const rhip_draw_desc &dd,
   /* a draw that writes no depth: every fragment's tests see the depths the draw started with */
   sh.listed = !sh.lines && !sh.two_phase && !(dt && st.depthTest && st.depthMask);
 
-  if (isEnvShading(pg.shading) && !sh.listed && !sh.lines)
+  /* a learned-shader draw of a material that failed its error bound draws with the plain shader */
+  int shading = pg.shading;
+  if (shading == RHS_SHADING_NEURAL_PBR && d->nn.draw_material >= 0 && d->nn.draw_material < 64 &&
+      ((d->nn.fallback >> d->nn.draw_material) & 1ull))
+    shading = RHS_SHADING_PBR;
+  if (isEnvShading(shading) && !sh.listed && !sh.lines)
     return fail(RHIP_E_UNSUPPORTED,
                 "skybox, IBL and FXAA programs draw without depth writes (the reference's states for them); "
                 "this draw writes depth");
-  if (pg.shading == RHS_SHADING_NEURAL_PBR) {
+  if (shading == RHS_SHADING_NEURAL_PBR) {
     uint32_t enable_light = 0;
     if (dd.uniform_material) memcpy(&enable_light, dd.uniform_material, 4);
     if (!d->nn.ready) return fail(RHIP_E_INVALID, "learned shader: no network (rhip_neural_init or set_weights)");
@@ -430,10 +435,10 @@ int drawSetup(rhip_device d, int fbo, const int vp[4], const rhip_draw_desc &dd,
   }
   decodeUniforms(dd, P.u);
   P.st = st;
-  P.pl_point_size = pg.shading == RHS_SHADING_BASIC ? P.u.point_size : 1.0f; /* BasicSoft.h:68 */
+  P.pl_point_size = shading == RHS_SHADING_BASIC ? P.u.point_size : 1.0f; /* BasicSoft.h:68 */
   if (sh.lines && !(P.pl_point_size >= 0.f && P.pl_point_size <= 1024.f))
     return fail(RHIP_E_UNSUPPORTED, "point size %g (0 to 1024)", (double)P.pl_point_size);
-  P.shading = pg.shading;
+  P.shading = shading;
   P.defines = pg.defines;
   P.vary_groups = varyGroups(P.shading, P.defines, P.u.enable_shadow);
   P.vary_stride = 4 * __builtin_popcount(P.vary_groups);
@@ -445,7 +450,7 @@ int drawSetup(rhip_device d, int fbo, const int vp[4], const rhip_draw_desc &dd,
     if (!t->initialised) return fail(RHIP_E_INVALID, "sampler slot %d: texture %d has no image", s, id);
     P.tex[s] = t->dd;
     /* only the Blinn-Phong and PBR shaders set a derivative LOD on their samplers (PbrSoft.h:138-151) */
-    if (t->samp.filter_min > 1 && s != RHS_SLOT_METAL_ROUGHNESS && s <= RHS_SLOT_AO && !isEnvShading(pg.shading))
+    if (t->samp.filter_min > 1 && s != RHS_SLOT_METAL_ROUGHNESS && s <= RHS_SLOT_AO && !isEnvShading(shading))
       P.need_deriv = 1;
   }
   P.vp_w = (float)vp[2];
@@ -624,7 +629,8 @@ int rhip_device_destroy(rhip_device d) {
   for (void *p : d->libm_bufs) hipFree(p);
   if (d->errors) hipFree(d->errors);
   for (void *p : {(void *)d->nn.params, (void *)d->nn.grad, (void *)d->nn.m1, (void *)d->nn.m2, (void *)d->nn.packed,
-                  (void *)d->nn.scratch, (void *)d->nn.hist, (void *)d->nn.rec_count})
+                  (void *)d->nn.scratch, (void *)d->nn.hist, (void *)d->nn.rec_count, (void *)d->nn.partial,
+                  (void *)d->nn.lossp})
     if (p) hipFree(p);
   hipStreamDestroy(d->stream);
   delete d;
@@ -1234,6 +1240,9 @@ static int nnEnsure(rhip_device d) {
   HIPCHK(hipMalloc(&n.m1, pb));
   HIPCHK(hipMalloc(&n.m2, pb));
   HIPCHK(hipMalloc(&n.packed, sizeof(NnPacked)));
+  HIPCHK(hipMemsetAsync(n.packed, 0, sizeof(NnPacked), d->stream)); /* padding entries stay zero */
+  HIPCHK(hipMalloc(&n.partial, (size_t)kNnMaxParts * kNnParams * sizeof(float)));
+  HIPCHK(hipMalloc(&n.lossp, kNnMaxParts * sizeof(double)));
   HIPCHK(hipMalloc(&n.scratch, 8 * sizeof(double)));
   HIPCHK(hipMalloc(&n.hist, 2 * kNnHistBins * sizeof(unsigned int)));
   HIPCHK(hipMalloc(&n.rec_count, sizeof(unsigned long long)));
@@ -1248,8 +1257,8 @@ static int nnLoad(rhip_device d, const float *w) {
   HIPCHK(hipMemcpyAsync(n.params, w, pb, hipMemcpyHostToDevice, d->stream));
   HIPCHK(hipMemsetAsync(n.m1, 0, pb, d->stream));
   HIPCHK(hipMemsetAsync(n.m2, 0, pb, d->stream));
-  RHIP_LAUNCH(k_nn_adam, 1, 1024, d->stream, n.params, n.grad, n.m1, n.m2, n.packed, 0.f, 0.9f, 0.999f, 1e-8f, 0.f,
-              0);
+  RHIP_LAUNCH(k_nn_update, kNnUpdateBlocks, 256, d->stream, n.params, n.partial, 0, n.lossp, (double *)nullptr,
+              (float *)nullptr, n.m1, n.m2, n.packed, 0.f, 0.9f, 0.999f, 1e-8f, 0.f, 0);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(d->stream));
   n.step = 0;
@@ -1318,6 +1327,27 @@ int rhip_neural_capture(rhip_device d, int enable, int shift, int material) {
   return RHIP_OK;
 }
 
+int rhip_neural_material(rhip_device d, int material) {
+  int rc = checkDev(d);
+  if (rc) return rc;
+  if (material < 0 || material >= 64) return fail(RHIP_E_INVALID, "material %d (0 to 63)", material);
+  d->nn.draw_material = material;
+  return RHIP_OK;
+}
+
+int rhip_neural_fallback(rhip_device d, const int *materials, int n) {
+  int rc = checkDev(d);
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && !materials)) return fail(RHIP_E_INVALID, "fallback list");
+  uint64_t mask = 0;
+  for (int i = 0; i < n; i++) {
+    if (materials[i] < 0 || materials[i] >= 64) return fail(RHIP_E_INVALID, "material %d (0 to 63)", materials[i]);
+    mask |= 1ull << materials[i];
+  }
+  d->nn.fallback = mask;
+  return RHIP_OK;
+}
+
 int rhip_neural_record_count(rhip_device d, long long *count) {
   int rc = checkDev(d);
   if (rc || (rc = nnEnsure(d))) return rc;
@@ -1360,14 +1390,14 @@ int rhip_neural_gradient(rhip_device d, long long first, int count, float *grad,
   int rc = checkDev(d);
   if (rc || (rc = nnRange(d, first, count))) return rc;
   auto &n = d->nn;
-  HIPCHK(hipMemsetAsync(n.grad, 0, kNnParams * sizeof(float), d->stream));
-  HIPCHK(hipMemsetAsync(n.scratch, 0, sizeof(double), d->stream));
-  RHIP_LAUNCH(k_nn_grad, (unsigned)std::min<int>((count + 63) / 64, 1024), 64, d->stream, n.packed,
-              n.records.as<float>(), first, 0ll, count, 0u, n.grad, n.scratch);
+  const int parts = std::min<int>((count + 63) / 64, kNnMaxParts);
+  RHIP_LAUNCH(k_nn_grad, (unsigned)parts, 64, d->stream, n.packed, n.records.as<float>(), first, 0ll, count, 0u,
+              n.partial, n.lossp);
+  RHIP_LAUNCH(k_nn_update, kNnUpdateBlocks, 256, d->stream, n.params, n.partial, parts, n.lossp, n.scratch, n.grad,
+              n.m1, n.m2, n.packed, 0.f, 0.f, 0.f, 0.f, 0.f, 0);
   HIPCHK(hipGetLastError());
   if (grad) HIPCHK(hipMemcpyAsync(grad, n.grad, kNnParams * sizeof(float), hipMemcpyDeviceToHost, d->stream));
   if (loss) HIPCHK(hipMemcpyAsync(loss, n.scratch, sizeof(double), hipMemcpyDeviceToHost, d->stream));
-  HIPCHK(hipMemsetAsync(n.grad, 0, kNnParams * sizeof(float), d->stream));
   HIPCHK(hipStreamSynchronize(d->stream));
   return RHIP_OK;
 }
@@ -1384,16 +1414,17 @@ int rhip_neural_train(rhip_device d, long long first, long long count, int steps
     HIPCHK(hipMalloc(&dl, steps * sizeof(double)));
     HIPCHK(hipMemsetAsync(dl, 0, steps * sizeof(double), d->stream));
   }
-  const unsigned grid = (unsigned)std::min<int>((batch + 63) / 64, 512);
+  const int grid = std::min<int>((batch + 63) / 64, 512);
   for (int i = 0; i < steps; i++) {
     /* cosine schedule from lr_start to lr_end */
     const double t = steps > 1 ? (double)i / (steps - 1) : 0.0;
     const float lr = (float)(lr_end + 0.5 * (lr_start - lr_end) * (1.0 + std::cos(3.14159265358979 * t)));
     const uint32_t sd = seed * 0x9e3779b9u + (uint32_t)(n.step + 1) * 0x85ebca6bu;
-    RHIP_LAUNCH(k_nn_grad, grid, 64, d->stream, n.packed, n.records.as<float>(), first, count, batch, sd, n.grad,
-                dl ? dl + i : n.scratch);
+    RHIP_LAUNCH(k_nn_grad, (unsigned)grid, 64, d->stream, n.packed, n.records.as<float>(), first, count, batch, sd,
+                n.partial, n.lossp);
     n.step++;
-    RHIP_LAUNCH(k_nn_adam, 1, 1024, d->stream, n.params, n.grad, n.m1, n.m2, n.packed, lr, 0.9f, 0.999f, 1e-8f,
+    RHIP_LAUNCH(k_nn_update, kNnUpdateBlocks, 256, d->stream, n.params, n.partial, grid, n.lossp,
+                dl ? dl + i : (double *)nullptr, (float *)nullptr, n.m1, n.m2, n.packed, lr, 0.9f, 0.999f, 1e-8f,
                 1.f / batch, n.step);
   }
   HIPCHK(hipGetLastError());

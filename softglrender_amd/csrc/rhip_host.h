@@ -129,6 +129,8 @@ struct rhip_device_s {
     float *params = nullptr, *grad = nullptr, *m1 = nullptr, *m2 = nullptr;
     rhip::NnPacked *packed = nullptr;
     double *scratch = nullptr; /* losses and evaluation sums */
+    float *partial = nullptr;  /* per-block gradient sums of a step (kNnMaxParts rows) */
+    double *lossp = nullptr;   /* per-block squared errors of a step */
     unsigned int *hist = nullptr;
     int step = 0;
     bool ready = false;
@@ -137,6 +139,8 @@ struct rhip_device_s {
     unsigned long long *rec_count = nullptr;
     bool capturing = false;
     int shift = 0, material = 0;
+    int draw_material = 0; /* material of the draws that follow (rhip_neural_material) */
+    uint64_t fallback = 0; /* materials whose learned-shader draws use the plain shader */
     uint32_t salt = 0;
   } nn;
 };

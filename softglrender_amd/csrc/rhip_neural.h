@@ -30,18 +30,43 @@ constexpr int kNnK1 = 32; /* layer-1 K: 24 features, a constant 1 (bias column),
 constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kNnHid * kNnIn, kOffW2 = kOffB1 + kNnHid,
               kOffB2 = kOffW2 + kNnHid * kNnHid, kOffW3 = kOffB2 + kNnHid, kOffB3 = kOffW3 + kNnOut * kNnHid;
 constexpr int kNnParams = kOffB3 + kNnOut; /* 5955 */
+constexpr int kNnMaxParts = 1024; /* most k_nn_grad blocks in one launch */
+constexpr int kNnUpdateBlocks = (kNnParams + 63) / 64;
 
 /* bf16 operands packed for MFMA (device): W1 with b1 in column 24, W2, W3
- * padded to 16 rows, and the transposes backward needs; biases of layers 2, 3. */
+ * padded to 16 rows, and the transposes backward needs; biases of layers 2, 3.
+ * w2r and w3r are W2 and W3 with their columns in the order nnRegUnit gives,
+ * for the inference kernel that keeps activations in registers. */
 struct NnPacked {
   __bf16 w1[kNnHid * kNnK1];
   __bf16 w2[kNnHid * kNnHid];
   __bf16 w3[16 * kNnHid];
   __bf16 w2t[kNnHid * kNnHid];
   __bf16 w3t[kNnHid * 32];
+  __bf16 w2r[kNnHid * kNnHid];
+  __bf16 w3r[16 * kNnHid];
   float b2[kNnHid];
   float b3[16];
 };
+
+/* The hidden unit behind K position `slot` (0..63) of a layer whose B operand
+ * is built in registers from the previous layer's result tiles. Lane (g, c)
+ * holds units 16m + 4g + v (v < 4) of result tile m; the K block kb of the next
+ * layer takes tiles 2kb and 2kb+1, so its K positions 8g .. 8g+7 are those two
+ * tiles' four units each. The weights' columns are stored in the same order. */
+__host__ __device__ constexpr int nnRegUnit(int slot) {
+  return 32 * (slot >> 5) + 4 * ((slot & 31) >> 3) + ((slot & 7) < 4 ? (slot & 3) : 16 + (slot & 3));
+}
+/* its inverse: the K position of hidden unit u */
+__host__ __device__ constexpr int nnRegSlot(int u) {
+  return 32 * (u >> 5) + 8 * ((u & 15) >> 2) + (u & 3) + ((u & 16) ? 4 : 0);
+}
+constexpr bool nnRegOrderOk() {
+  for (int s = 0; s < 64; s++)
+    if (nnRegSlot(nnRegUnit(s)) != s) return false;
+  return true;
+}
+static_assert(nnRegOrderOk(), "nnRegSlot must invert nnRegUnit");
 
 /* A captured fragment: what the network reads, the plain GPU shader's colour
  * and the FP64 plain-shader colour. */
@@ -107,14 +132,13 @@ __device__ __forceinline__ void recFeatures(const float *r, float *f) {
 constexpr int kXs = kNnK1 + 8, kHs = kNnHid + 8;
 struct NnWaveLds {
   __bf16 x[64 * kXs];
-  __bf16 h[64 * kHs];
   float y[64 * 4];
 };
 /* the block's copy of the packed operands (read by all its wavefronts) */
 struct NnBlockW {
   __bf16 w1[kNnHid * kXs];
-  __bf16 w2[kNnHid * kHs];
-  __bf16 w3[16 * kHs];
+  __bf16 w2r[kNnHid * kHs];
+  __bf16 w3r[16 * kHs];
   float b2[kNnHid];
   float b3[16];
 };
@@ -127,23 +151,72 @@ __device__ __forceinline__ void nnLoadBlockW(const NnPacked &W, NnBlockW &B) {
   }
   for (int i = threadIdx.x; i < kNnHid * kNnHid / 8; i += blockDim.x) {
     const int r = i / (kNnHid / 8), k = (i % (kNnHid / 8)) * 8;
-    *(bf16x8 *)&B.w2[r * kHs + k] = *(const bf16x8 *)&W.w2[r * kNnHid + k];
+    *(bf16x8 *)&B.w2r[r * kHs + k] = *(const bf16x8 *)&W.w2r[r * kNnHid + k];
   }
   for (int i = threadIdx.x; i < 16 * kNnHid / 8; i += blockDim.x) {
     const int r = i / (kNnHid / 8), k = (i % (kNnHid / 8)) * 8;
-    *(bf16x8 *)&B.w3[r * kHs + k] = *(const bf16x8 *)&W.w3[r * kNnHid + k];
+    *(bf16x8 *)&B.w3r[r * kHs + k] = *(const bf16x8 *)&W.w3r[r * kNnHid + k];
   }
   for (int i = threadIdx.x; i < kNnHid; i += blockDim.x) B.b2[i] = W.b2[i];
   for (int i = threadIdx.x; i < 16; i += blockDim.x) B.b3[i] = W.b3[i];
 }
 
-/* four consecutive units of a result tile, ReLU'd, as one 8-byte store */
-__device__ __forceinline__ void storeRelu4(__bf16 *dst, f32x4 d) {
-  typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-  bf16x4 o;
+/* Two result tiles (units 16m + 4g + v of one fragment group), ReLU'd, as the
+ * next layer's B operand in nnRegUnit order. */
+__device__ __forceinline__ bf16x8 reluPack(f32x4 lo, f32x4 hi) {
+  bf16x8 o;
 #pragma unroll
-  for (int v = 0; v < 4; v++) o[v] = (__bf16)fmaxf(d[v], 0.f); /* one v_max_f32 (a NaN would become 0) */
-  *(bf16x4 *)dst = o;
+  for (int v = 0; v < 4; v++) {
+    o[v] = (__bf16)fmaxf(lo[v], 0.f);
+    o[4 + v] = (__bf16)fmaxf(hi[v], 0.f);
+  }
+  return o;
+}
+
+/* The network on the 64 fragments whose input rows are in `X` (row stride kXs):
+ * layer-1 operands from the rows, then the hidden activations in registers
+ * (each layer's result tiles are the next layer's B operand, with the weights'
+ * columns in nnRegUnit order). Returns, in lanes 0..15, channels 0..3 of
+ * fragments 16q + c. `w1`, `w2r`, `w3r` have row strides s1, s2, s2; the same
+ * arithmetic in the same order wherever they live. Every lane must call it. */
+__device__ __forceinline__ void nnLayers(const __bf16 *w1, const __bf16 *w2r, const __bf16 *w3r, int s1, int s2,
+                                         const float *b2, const float *b3, const __bf16 *X, f32x4 *yq) {
+  const int g = (threadIdx.x & 63) >> 4;
+  bf16x8 xb[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) xb[q] = ldOp(X, kXs, 16 * q, 0);
+  /* layer 1: H1^T = W1 . X^T (bias in column 24), packed as layer 2's operands */
+  bf16x8 h1[4][2];
+#pragma unroll
+  for (int kb = 0; kb < 2; kb++) {
+    const bf16x8 a0 = ldOp(w1, s1, 32 * kb, 0), a1 = ldOp(w1, s1, 32 * kb + 16, 0);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const f32x4 z = {0.f, 0.f, 0.f, 0.f};
+      h1[q][kb] = reluPack(mma16(a0, xb[q], z), mma16(a1, xb[q], z));
+    }
+  }
+  /* layer 2: H2^T = W2 . H1^T + b2, packed as layer 3's operands */
+  bf16x8 h2[4][2];
+#pragma unroll
+  for (int kb = 0; kb < 2; kb++) {
+    f32x4 d[2][4];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      const int m = 2 * kb + t;
+      const bf16x8 a0 = ldOp(w2r, s2, 16 * m, 0), a1 = ldOp(w2r, s2, 16 * m, 32);
+      const f32x4 bias = *(const f32x4 *)&b2[16 * m + 4 * g];
+#pragma unroll
+      for (int q = 0; q < 4; q++) d[t][q] = mma16(a1, h1[q][1], mma16(a0, h1[q][0], bias));
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) h2[q][kb] = reluPack(d[0][q], d[1][q]);
+  }
+  /* layer 3: Y^T = W3 . H2^T + b3 */
+  const bf16x8 a0 = ldOp(w3r, s2, 0, 0), a1 = ldOp(w3r, s2, 0, 32);
+  const f32x4 bias = *(const f32x4 *)&b3[4 * g];
+#pragma unroll
+  for (int q = 0; q < 4; q++) yq[q] = mma16(a1, h2[q][1], mma16(a0, h2[q][0], bias));
 }
 
 /* Forward pass of the 64 fragments of a wavefront: lane l's features `f`
@@ -158,50 +231,11 @@ __device__ __forceinline__ void nnForward64(const NnBlockW &W, NnWaveLds &L, con
     *(bf16x8 *)&L.x[l * kXs + k0] = v;
   }
   waveSync();
-  /* layer 1: H1^T = W1 . X^T (bias in column 24) */
-  bf16x8 xb[4];
+  f32x4 yq[4];
+  nnLayers(W.w1, W.w2r, W.w3r, kXs, kHs, W.b2, W.b3, L.x, yq);
+  if (g == 0) {
 #pragma unroll
-  for (int n = 0; n < 4; n++) xb[n] = ldOp(L.x, kXs, 16 * n, 0);
-#pragma unroll
-  for (int m = 0; m < 4; m++) {
-    const bf16x8 a = ldOp(W.w1, kXs, 16 * m, 0);
-#pragma unroll
-    for (int n = 0; n < 4; n++) {
-      f32x4 d = {0.f, 0.f, 0.f, 0.f};
-      d = mma16(a, xb[n], d);
-      storeRelu4(&L.h[(16 * n + c) * kHs + 16 * m + 4 * g], d);
-    }
-  }
-  waveSync();
-  /* layer 2: H2^T = W2 . H1^T + b2, written over H1 once all of it is read */
-  bf16x8 hb[4][2];
-#pragma unroll
-  for (int n = 0; n < 4; n++)
-#pragma unroll
-    for (int kb = 0; kb < 2; kb++) hb[n][kb] = ldOp(L.h, kHs, 16 * n, 32 * kb);
-  waveSync();
-#pragma unroll
-  for (int m = 0; m < 4; m++) {
-    const bf16x8 a0 = ldOp(W.w2, kHs, 16 * m, 0), a1 = ldOp(W.w2, kHs, 16 * m, 32);
-    f32x4 bias;
-#pragma unroll
-    for (int v = 0; v < 4; v++) bias[v] = W.b2[16 * m + 4 * g + v];
-#pragma unroll
-    for (int n = 0; n < 4; n++) storeRelu4(&L.h[(16 * n + c) * kHs + 16 * m + 4 * g],
-                                           mma16(a1, hb[n][1], mma16(a0, hb[n][0], bias)));
-  }
-  waveSync();
-  /* layer 3: Y^T = W3 . H2^T + b3 (rows 0..2) */
-  {
-    const bf16x8 a0 = ldOp(W.w3, kHs, 0, 0), a1 = ldOp(W.w3, kHs, 0, 32);
-    f32x4 bias;
-#pragma unroll
-    for (int v = 0; v < 4; v++) bias[v] = W.b3[4 * g + v];
-#pragma unroll
-    for (int n = 0; n < 4; n++) {
-      const f32x4 d = mma16(a1, ldOp(L.h, kHs, 16 * n, 32), mma16(a0, ldOp(L.h, kHs, 16 * n, 0), bias));
-      if (g == 0) *(f32x4 *)&L.y[(16 * n + c) * 4] = d;
-    }
+    for (int q = 0; q < 4; q++) *(f32x4 *)&L.y[(16 * q + c) * 4] = yq[q];
   }
   waveSync();
   const f32x4 y = *(const f32x4 *)&L.y[l * 4];
@@ -390,32 +424,24 @@ __device__ __forceinline__ void nnJobRow(const DrawParams &P, int j, int need_de
     bary(T, fx + 0.5f, fy + 0.5f, bs);
   }
   const Frag fr = makeFrag<S, RHS_SHADING_NEURAL_PBR>(P, ps, T, px, py, bs, inst, need_deriv);
-  const f2 uv{fr.v[0], fr.v[1]};
-  const TexDesc *ta = P.tex[RHS_SLOT_ALBEDO];
-  const RGBAf alb = (P.defines & RHS_DEF_ALBEDO_MAP)
-                        ? tex2D(ta, uv, lodFor(P, ta, fr))
-                        : RGBAf{P.u.base_color[0], P.u.base_color[1], P.u.base_color[2], P.u.base_color[3]};
-  float metal = 0.f, rough = 1.f;
-  if (P.defines & RHS_DEF_METALROUGHNESS_MAP) {
-    const RGBAf mr = tex2D(P.tex[RHS_SLOT_METAL_ROUGHNESS], uv, 0.f);
-    metal = mr.b;
-    rough = mr.g;
-  }
-  const f3 N = normalFromMap(P, fr, 20, 24);
-  const f3 V = nrm3({fr.v[12], fr.v[13], fr.v[14]});
-  nnFeatures(N, V, rough, metal, alb.r, alb.g, alb.b, f);
-  f[kNnK1 - 1] = alb.a;
+  /* the plain shader's inputs (what the capture recorded for training) */
+  const PbrInputs in = pbrInputs(P, fr);
+  nnFeatures(in.N, in.V, in.rough, in.metallic, in.albedo_rgba.r, in.albedo_rgba.g, in.albedo_rgba.b, f);
+  f[kNnK1 - 1] = in.albedo_rgba.a;
 }
 
 /* Learned shading of the fragments the two-phase raster queued, one kernel:
  * each wavefront takes 64 jobs, every lane writes its job's input row to the
- * wavefront's LDS tile, then the network runs on MFMA from that tile (weights
- * from L2) and each lane writes its fragment's colour to every sample it won. */
+ * wavefront's LDS tile and takes its layer-1 operands from there; the hidden
+ * activations then stay in registers (each layer's result tiles are the next
+ * layer's B operand, with the weights' columns in nnRegUnit order), and the
+ * outputs go back to their fragments' lanes through the tile for coalesced
+ * colour writes to every sample each fragment won. Weights come from L1/L2. */
 template <int S>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_nn_shade(DrawParams P, const NnPacked *W, int need_deriv) {
-  __shared__ __bf16 hbuf[4][64 * kHs]; /* per wavefront: input rows, then hidden activations, then outputs */
+  __shared__ __bf16 xbuf[4][64 * kXs]; /* per wavefront: input rows, then outputs */
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 4, c = l & 15;
-  __bf16 *H = hbuf[wave];
+  __bf16 *X = xbuf[wave];
   const int n = P.dcount[2];
   const int nchunks = (n + 63) / 64;
   unsigned long long nfs = 0, nwr = 0;
@@ -439,50 +465,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         bf16x8 v;
 #pragma unroll
         for (int k = 0; k < 8; k++) v[k] = (__bf16)f[8 * q + k];
-        *(bf16x8 *)&H[l * kHs + 8 * q] = v;
+        *(bf16x8 *)&X[l * kXs + 8 * q] = v;
       }
     }
     waveSync();
-    bf16x8 xb[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) xb[q] = ldOp(H, kHs, 16 * q, 0);
-    waveSync(); /* every lane holds its operands before the tile is overwritten */
-#pragma unroll
-    for (int m = 0; m < 4; m++) {
-      const bf16x8 a = ldOp(W->w1, kNnK1, 16 * m, 0);
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        f32x4 d = {0.f, 0.f, 0.f, 0.f};
-        storeRelu4(&H[(16 * q + c) * kHs + 16 * m + 4 * g], mma16(a, xb[q], d));
-      }
-    }
-    waveSync();
-    bf16x8 hb[4][2];
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-#pragma unroll
-      for (int kb = 0; kb < 2; kb++) hb[q][kb] = ldOp(H, kHs, 16 * q, 32 * kb);
-    waveSync();
-#pragma unroll
-    for (int m = 0; m < 4; m++) {
-      const bf16x8 a0 = ldOp(W->w2, kNnHid, 16 * m, 0), a1 = ldOp(W->w2, kNnHid, 16 * m, 32);
-      const f32x4 bias = *(const f32x4 *)&W->b2[16 * m + 4 * g];
-#pragma unroll
-      for (int q = 0; q < 4; q++)
-        storeRelu4(&H[(16 * q + c) * kHs + 16 * m + 4 * g], mma16(a1, hb[q][1], mma16(a0, hb[q][0], bias)));
-    }
-    waveSync();
-    /* layer 3: lanes 0..15 hold channels 0..3 of fragments 16q + c */
     f32x4 yq[4];
-    {
-      const bf16x8 a0 = ldOp(W->w3, kNnHid, 0, 0), a1 = ldOp(W->w3, kNnHid, 0, 32);
-      const f32x4 bias = *(const f32x4 *)&W->b3[4 * g];
-#pragma unroll
-      for (int q = 0; q < 4; q++) yq[q] = mma16(a1, ldOp(H, kHs, 16 * q, 32), mma16(a0, ldOp(H, kHs, 16 * q, 0), bias));
-    }
-    waveSync();
-    /* outputs to their fragments' lanes through the (now free) LDS tile */
-    float *Y = (float *)H;
+    nnLayers(W->w1, W->w2r, W->w3r, kNnK1, kNnHid, W->b2, W->b3, X, yq);
+    waveSync(); /* every lane has read its operands before the tile is overwritten */
+    /* outputs to their fragments' lanes through the tile */
+    float *Y = (float *)X;
     if (g == 0) {
 #pragma unroll
       for (int q = 0; q < 4; q++) *(f32x4 *)&Y[(16 * q + c) * 4] = yq[q];
@@ -604,10 +595,12 @@ __device__ __forceinline__ bf16x8 onesOp() {
 /* One training step's gradient of mean squared error over `batch` records
  * drawn from [first, first + count) (or the `batch` records starting at
  * `first` when `count` is 0): forward and backward on MFMA, gradients summed
- * per block in LDS and added to `grad` (FP32, unscaled sum). Blocks are one
- * wavefront; each takes chunks of 64 records. */
+ * per block in LDS. Blocks are one wavefront; each takes chunks of 64 records
+ * and writes its sums (FP32, unscaled) to its own row of `partial` and its
+ * squared error to lossp[block]: no atomics, so k_nn_update's fixed-order
+ * sum makes a step reproducible bit for bit. */
 __global__ __launch_bounds__(64) void k_nn_grad(const NnPacked *W, const float *rec, long long first, long long count,
-                                                int batch, uint32_t seed, float *grad, double *loss) {
+                                                int batch, uint32_t seed, float *partial, double *lossp) {
   __shared__ NnTrainLds L;
   const int l = threadIdx.x, g = l >> 4, c = l & 15;
   for (int i = l; i < kNnParams; i += 64) L.g[i] = 0.f;
@@ -808,50 +801,78 @@ __global__ __launch_bounds__(64) void k_nn_grad(const NnPacked *W, const float *
     }
     waveSync();
   }
-  for (int i = l; i < kNnParams; i += 64)
-    if (L.g[i] != 0.f) atomicAdd(&grad[i], L.g[i]);
-  /* loss: sum over the wavefront, one atomic */
+  waveSync();
+  float *dst = partial + (size_t)blockIdx.x * kNnParams;
+  for (int i = l; i < kNnParams; i += 64) dst[i] = L.g[i];
   double t = lsum;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
-  if (l == 0) atomicAdd(loss, t);
+  if (l == 0) lossp[blockIdx.x] = t;
 }
 
-/* Adam on the FP32 parameters with the summed gradient (scaled by 1/batch),
- * then the gradient cleared and the bf16 operands repacked. One block. */
-__global__ __launch_bounds__(1024) void k_nn_adam(float *p, float *grad, float *m1, float *m2, NnPacked *W,
-                                                  float lr, float b1, float b2, float eps, float scale, int step) {
-  const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
-  for (int i = threadIdx.x; i < kNnParams; i += blockDim.x) {
-    if (step > 0) {
-      const float gi = grad[i] * scale;
-      const float a = b1 * m1[i] + (1.f - b1) * gi, b = b2 * m2[i] + (1.f - b2) * gi * gi;
-      m1[i] = a;
-      m2[i] = b;
-      p[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
-    }
-    grad[i] = 0.f;
+/* Parameter i's entries in the packed bf16 operands (the padding entries are
+ * zero from allocation and never written). */
+__device__ __forceinline__ void nnPackParam(NnPacked *W, int i, float x) {
+  const __bf16 h = (__bf16)x;
+  if (i < kOffB1) {
+    W->w1[(i / kNnIn) * kNnK1 + i % kNnIn] = h;
+  } else if (i < kOffW2) {
+    W->w1[(i - kOffB1) * kNnK1 + kNnIn] = h;
+  } else if (i < kOffB2) {
+    const int j = i - kOffW2, u = j / kNnHid, k = j % kNnHid;
+    W->w2[j] = h;
+    W->w2t[k * kNnHid + u] = h;
+    W->w2r[u * kNnHid + nnRegSlot(k)] = h;
+  } else if (i < kOffW3) {
+    W->b2[i - kOffB2] = x;
+  } else if (i < kOffB3) {
+    const int j = i - kOffW3, ch = j / kNnHid, k = j % kNnHid;
+    W->w3[j] = h;
+    W->w3t[k * 32 + ch] = h;
+    W->w3r[ch * kNnHid + nnRegSlot(k)] = h;
+  } else {
+    W->b3[i - kOffB3] = x;
   }
+}
+
+/* The gradient of a step: the `nparts` block sums of k_nn_grad added in a
+ * fixed order (4 interleaved partial sums per parameter, then those 4), and
+ * the loss likewise. With `gout` the gradient is only written there. Otherwise
+ * Adam (step > 0) updates the FP32 parameters with the gradient scaled by
+ * `scale`, and every parameter is repacked into the bf16 operands (step 0:
+ * repack only). 64 parameters per block of 256 threads. */
+__global__ __launch_bounds__(256) void k_nn_update(float *p, const float *partial, int nparts, const double *lossp,
+                                                   double *loss, float *gout, float *m1, float *m2, NnPacked *W,
+                                                   float lr, float b1, float b2, float eps, float scale, int step) {
+  __shared__ float part[4][64];
+  const int t = threadIdx.x & 63, s = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + t;
+  float acc = 0.f;
+  if (i < kNnParams)
+    for (int b = s; b < nparts; b += 4) acc += partial[(size_t)b * kNnParams + i];
+  part[s][t] = acc;
   __syncthreads();
-  for (int i = threadIdx.x; i < kNnHid * kNnK1; i += blockDim.x) {
-    const int u = i / kNnK1, k = i % kNnK1;
-    W->w1[i] = (__bf16)(k < kNnIn ? p[kOffW1 + u * kNnIn + k] : (k == kNnIn ? p[kOffB1 + u] : 0.f));
+  if (s == 0 && i < kNnParams) {
+    const float gs = ((part[0][t] + part[1][t]) + part[2][t]) + part[3][t];
+    if (gout) {
+      gout[i] = gs;
+    } else {
+      if (step > 0) {
+        const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+        const float gi = gs * scale;
+        const float a = b1 * m1[i] + (1.f - b1) * gi, b = b2 * m2[i] + (1.f - b2) * gi * gi;
+        m1[i] = a;
+        m2[i] = b;
+        p[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+      }
+      nnPackParam(W, i, p[i]);
+    }
   }
-  for (int i = threadIdx.x; i < kNnHid * kNnHid; i += blockDim.x) {
-    const int u = i / kNnHid, k = i % kNnHid;
-    W->w2[i] = (__bf16)p[kOffW2 + i];
-    W->w2t[i] = (__bf16)p[kOffW2 + k * kNnHid + u];
+  if (loss && blockIdx.x == 0 && threadIdx.x == 0) {
+    double sum = 0;
+    for (int b = 0; b < nparts; b++) sum += lossp[b];
+    *loss = sum;
   }
-  for (int i = threadIdx.x; i < 16 * kNnHid; i += blockDim.x) {
-    const int ch = i / kNnHid;
-    W->w3[i] = (__bf16)(ch < kNnOut ? p[kOffW3 + i] : 0.f);
-  }
-  for (int i = threadIdx.x; i < kNnHid * 32; i += blockDim.x) {
-    const int u = i / 32, ch = i % 32;
-    W->w3t[i] = (__bf16)(ch < kNnOut ? p[kOffW3 + ch * kNnHid + u] : 0.f);
-  }
-  for (int i = threadIdx.x; i < kNnHid; i += blockDim.x) W->b2[i] = p[kOffB2 + i];
-  for (int i = threadIdx.x; i < 16; i += blockDim.x) W->b3[i] = i < kNnOut ? p[kOffB3 + i] : 0.f;
 }
 
 }  // namespace rhip

@@ -7,7 +7,9 @@
    Adam on FP32 master weights).
 3. Evaluate it on fragments from held-out cameras against the FP64 plain
    shader, per material; a material whose p99 error exceeds the threshold
-   keeps the plain shader (per-material fallback).
+   keeps the plain shader (per-material fallback: `apply_fallback` hands the
+   list to the library, whose learned-shader draws of such a material, set by
+   `renderer.neural_material`, then run the plain shader).
 
 The point light of the C2 material is off for both shaders here: the
 network's inputs (normal, view direction, N.V, roughness, metallic, albedo,
@@ -143,6 +145,14 @@ def capture(renderer, ns: NeuralScene, shift):
     renderer.neural_capture(False)
     renderer.waitIdle()
     return first, renderer.neural_record_count() - first
+
+
+def apply_fallback(renderer, table):
+    """Makes the learned-shader draws of every material the table flags use the plain shader; returns those
+    materials."""
+    bad = [t["material"] for t in table if t["fallback"]]
+    renderer.neural_fallback(bad)
+    return bad
 
 
 def fallback_table(renderer, first, count, n_materials, limit=P99_LIMIT):

@@ -423,6 +423,15 @@ class RendererHIP:
     def neural_capture(self, enable: bool, shift: int = 0, material: int = 0):
         _lib.check(self.lib.rhip_neural_capture(self.dev, int(enable), shift, material))
 
+    def neural_material(self, material: int):
+        """Material (0..63) of the draws that follow, for the per-material fallback."""
+        _lib.check(self.lib.rhip_neural_material(self.dev, material))
+
+    def neural_fallback(self, materials):
+        """Materials whose learned-shader draws use the plain PBR shader instead."""
+        arr = (ctypes.c_int * max(len(materials), 1))(*materials)
+        _lib.check(self.lib.rhip_neural_fallback(self.dev, arr, len(materials)))
+
     def neural_record_count(self) -> int:
         n = ctypes.c_longlong()
         _lib.check(self.lib.rhip_neural_record_count(self.dev, ctypes.byref(n)))

@@ -166,3 +166,59 @@ def test_neural_draw_writes_the_network_output(renderer, small_setup):
           f"mean |learned - plain| {diff:.3f}/255", flush=True)
     assert count > 5000 and bad == 0
     assert diff < 10.0
+
+
+def test_training_is_reproducible(renderer):
+    """Each block writes its gradient sums to its own row and the update kernel adds the rows in a fixed
+    order (no float atomics), so training from the same weights on the same records repeats bit for bit."""
+    rec = _synthetic_records(20_000, 5)
+    renderer.neural_records(len(rec))
+    renderer.neural_write_records(0, rec)
+    runs = []
+    for _ in range(2):
+        renderer.neural_init(3)
+        losses = renderer.neural_train(0, len(rec), steps=50, batch=4096, lr_start=1e-2, lr_end=1e-3, seed=9)
+        runs.append((renderer.neural_get_weights(), np.asarray(losses)))
+    print(f"[gpu] training twice: loss {runs[0][1][0]:.5f} -> {runs[0][1][-1]:.5f}", flush=True)
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+    assert runs[0][1][-1] < runs[0][1][0]
+
+
+def _shot(renderer, mats, normal, m, shading, cams, w=320, h=180):
+    """Material m of `mats` drawn alone (texture ids of list position 0), tagged as material m."""
+    from softglrender_amd.replay import Replayer, parse
+    ns = neural.scene_views(cams, mats[m:m + 1], normal, width=w, height=h, freq=40, shading=shading)
+    ns.scene.stream.read(6, which=1)
+    renderer.neural_material(m)
+    rp = Replayer(renderer, layouts=ns.scene.meta.get("layouts"))
+    reads = []
+    for op, p in parse(ns.scene.stream.bytes()):
+        if op != S.OP_FRAME:
+            rp.exec(op, p, reads)
+    return reads[-1]
+
+
+def test_fallback_materials_draw_with_the_plain_shader(renderer, small_setup):
+    """The per-material fallback acts in program selection: an untrained network misses the p99 bound for
+    every material; a flagged material's learned-shader draws then write exactly the plain shader's samples,
+    while an unflagged material's draws still run the network."""
+    mats, normal = small_setup
+    renderer.neural_records(400_000)
+    first, count = neural.capture(renderer, _small_views(2, 31, mats, normal), 0)
+    renderer.neural_init(5)
+    table = neural.fallback_table(renderer, first, count, len(mats))
+    print(f"[gpu] untrained network p99 per material: {[round(t['network']['p99'], 1) for t in table]}/255",
+          flush=True)
+    assert all(t["fallback"] for t in table)
+    cams = neural.random_cameras(1, 78)
+    plain = [_shot(renderer, mats, normal, m, S.SHADING_PBR, cams) for m in range(2)]
+    try:
+        renderer.neural_fallback([1])
+        learned = [_shot(renderer, mats, normal, m, S.SHADING_NEURAL_PBR, cams) for m in range(2)]
+        assert np.array_equal(learned[1], plain[1])
+        assert not np.array_equal(learned[0], plain[0])
+        assert neural.apply_fallback(renderer, table) == [0, 1]
+        assert np.array_equal(_shot(renderer, mats, normal, 0, S.SHADING_NEURAL_PBR, cams), plain[0])
+    finally:
+        renderer.neural_fallback([])
+        renderer.neural_material(0)
