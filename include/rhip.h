@@ -187,9 +187,10 @@ int rhip_neural_capture(rhip_device dev, int enable, int shift, int material); /
 int rhip_neural_material(rhip_device dev, int material);               /* material (0..63) of the draws that follow; default 0 */
 int rhip_neural_fallback(rhip_device dev, const int *materials, int n); /* these materials' learned-shader draws use the plain PBR shader */
 int rhip_neural_record_count(rhip_device dev, long long *count);
-int rhip_neural_read_records(rhip_device dev, long long first, long long count, float *out);   /* 24 floats each */
+int rhip_neural_read_records(rhip_device dev, long long first, long long count, float *out);   /* 32 floats each */
 int rhip_neural_write_records(rhip_device dev, long long first, long long count, const float *in);
 int rhip_neural_gradient(rhip_device dev, long long first, int count, float *grad, double *loss); /* summed over records */
+/* training repeats bit for bit for the same weights, records and arguments */
 int rhip_neural_train(rhip_device dev, long long first, long long count, int steps, int batch, float lr_start,
                       float lr_end, unsigned int seed, double *losses /* steps entries, may be NULL */);
 int rhip_neural_evaluate(rhip_device dev, long long first, long long count, int material, rhip_nn_error *network,

@@ -34,10 +34,23 @@ OFF_W3 = OFF_B2 + N_HID
 OFF_B3 = OFF_W3 + N_OUT * N_HID
 N_PARAMS = OFF_B3 + N_OUT
 
-# capture record layout (24 floats)
+# capture record layout (32 floats): the network's inputs (N, V, albedo, roughness, metallic), AO, material,
+# FP64 target, plain GPU colour, pixel, sample mask, then the plain shader's exact inputs (N, albedo, roughness,
+# metallic; V is the same) that the FP64 target is computed from
+REC_FLOATS = 32
 REC_N, REC_V, REC_ALBEDO, REC_ROUGH, REC_METAL, REC_AO, REC_MATERIAL, REC_TARGET, REC_PLAIN = 0, 3, 6, 10, 11, 12, 13, \
     14, 17
 REC_PIXEL, REC_MASK = 20, 21
+REC_XN, REC_XALBEDO, REC_XROUGH, REC_XMETAL = 22, 25, 28, 29
+
+
+def exact_from_inputs(rec):
+    """Copies the network-input fields of records into their exact-input fields (records whose inputs are
+    exact, as synthetic ones are)."""
+    rec[:, REC_XN:REC_XN + 3] = rec[:, REC_N:REC_N + 3]
+    rec[:, REC_XALBEDO:REC_XALBEDO + 3] = rec[:, REC_ALBEDO:REC_ALBEDO + 3]
+    rec[:, REC_XROUGH], rec[:, REC_XMETAL] = rec[:, REC_ROUGH], rec[:, REC_METAL]
+    return rec
 
 
 def bf16(x):
@@ -177,9 +190,9 @@ def pbr_ibl_f64(rec, irradiance_faces, prefilter_levels):
     display colour before clamping. Cube sampling: bilinear in a face with clamp-to-edge, linear between
     mip levels floor(lod) and floor(lod) + 1 (lod = 4 roughness), no 8-bit rounding in between."""
     rec = np.asarray(rec, np.float64)
-    N, V = rec[:, REC_N:REC_N + 3], rec[:, REC_V:REC_V + 3]
-    rough, metal, ao = rec[:, REC_ROUGH], rec[:, REC_METAL], rec[:, REC_AO]
-    alb = rec[:, REC_ALBEDO:REC_ALBEDO + 3] ** 2.2
+    N, V = rec[:, REC_XN:REC_XN + 3], rec[:, REC_V:REC_V + 3]
+    rough, metal, ao = rec[:, REC_XROUGH], rec[:, REC_XMETAL], rec[:, REC_AO]
+    alb = rec[:, REC_XALBEDO:REC_XALBEDO + 3] ** 2.2
     F0 = 0.04 * (1 - metal[:, None]) + alb * metal[:, None]
     ndv = np.sum(N * V, 1)
     nv = np.maximum(ndv, 0)

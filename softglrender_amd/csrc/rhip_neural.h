@@ -31,7 +31,7 @@ constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kNnHid * kNnIn, kOffW2 = kOffB1 + kN
               kOffB2 = kOffW2 + kNnHid * kNnHid, kOffW3 = kOffB2 + kNnHid, kOffB3 = kOffW3 + kNnOut * kNnHid;
 constexpr int kNnParams = kOffB3 + kNnOut; /* 5955 */
 constexpr int kNnMaxParts = 1024; /* most k_nn_grad blocks in one launch */
-constexpr int kNnUpdateBlocks = (kNnParams + 63) / 64;
+constexpr int kNnUpdateBlocks = (kNnParams + 15) / 16; /* k_nn_update: 16 parameters per block */
 
 /* bf16 operands packed for MFMA (device): W1 with b1 in column 24, W2, W3
  * padded to 16 rows, and the transposes backward needs; biases of layers 2, 3.
@@ -68,21 +68,26 @@ constexpr bool nnRegOrderOk() {
 }
 static_assert(nnRegOrderOk(), "nnRegSlot must invert nnRegUnit");
 
-/* A captured fragment: what the network reads, the plain GPU shader's colour
- * and the FP64 plain-shader colour. */
-constexpr int kRecFloats = 24;
+/* A captured fragment: what the network reads (the learned shader's inputs,
+ * nnFastInputs), the plain shader's exact inputs, the plain GPU shader's colour
+ * and the FP64 plain-shader colour (from the exact inputs). */
+constexpr int kRecFloats = 32;
 enum {
-  kRecN = 0,       /* 3: shading normal */
-  kRecV = 3,       /* 3: view direction */
-  kRecAlbedo = 6,  /* 4: albedo texel as sampled (sRGB-encoded), alpha */
-  kRecRough = 10,
-  kRecMetal = 11,
+  kRecN = 0,        /* 3: shading normal (network input) */
+  kRecV = 3,        /* 3: view direction (network input and exact) */
+  kRecAlbedo = 6,   /* 4: albedo texel (sRGB-encoded), alpha (network input) */
+  kRecRough = 10,   /* network input */
+  kRecMetal = 11,   /* network input */
   kRecAo = 12,
   kRecMaterial = 13,
-  kRecTarget = 14, /* 3: FP64 plain shader, clamped to [0, 1] */
-  kRecPlain = 17,  /* 3: the GPU plain shader, clamped to [0, 1] */
-  kRecPixel = 20,  /* pixel index y * width + x */
-  kRecMask = 21,   /* samples the fragment covers (bit mask) */
+  kRecTarget = 14,  /* 3: FP64 plain shader on the exact inputs, clamped to [0, 1] */
+  kRecPlain = 17,   /* 3: the GPU plain shader, clamped to [0, 1] */
+  kRecPixel = 20,   /* pixel index y * width + x */
+  kRecMask = 21,    /* samples the fragment covers (bit mask) */
+  kRecXN = 22,      /* 3: the plain shader's shading normal */
+  kRecXAlbedo = 25, /* 3: the plain shader's albedo texel */
+  kRecXRough = 28,
+  kRecXMetal = 29,
 };
 
 __device__ __forceinline__ f32x4 mma16(bf16x8 a, bf16x8 b, f32x4 c) {
@@ -296,15 +301,15 @@ __device__ void cubeF64(const TexDesc &d, double x, double y, double z, double l
 }
 
 /* The image-based PBR shader (PbrSoft.h:225-322, u_enableLight off) in FP64
- * from the fragment's shading inputs; returns the display colour (not clamped). */
+ * from the fragment's exact shading inputs; returns the display colour (not clamped). */
 __device__ void pbrIblF64(const DrawParams &P, const float *r, double *col) {
-  const double N[3] = {r[kRecN], r[kRecN + 1], r[kRecN + 2]};
+  const double N[3] = {r[kRecXN], r[kRecXN + 1], r[kRecXN + 2]};
   const double V[3] = {r[kRecV], r[kRecV + 1], r[kRecV + 2]};
-  const double rough = r[kRecRough], metal = r[kRecMetal], ao = r[kRecAo];
+  const double rough = r[kRecXRough], metal = r[kRecXMetal], ao = r[kRecAo];
   double alb[3], F0[3];
 #pragma unroll
   for (int ch = 0; ch < 3; ch++) {
-    alb[ch] = pow((double)r[kRecAlbedo + ch], 2.2);
+    alb[ch] = pow((double)r[kRecXAlbedo + ch], 2.2);
     F0[ch] = 0.04 * (1 - metal) + alb[ch] * metal;
   }
   const double ndv = N[0] * V[0] + N[1] * V[1] + N[2] * V[2];
@@ -327,6 +332,53 @@ __device__ void pbrIblF64(const DrawParams &P, const float *r, double *col) {
     const double amb = (kD * irr[ch] * alb[ch] + pf[ch] * (F[ch] * bx + by)) * ao;
     col[ch] = pow(amb, 1.0 / 2.2);
   }
+}
+
+/* --------------------------------------------------------------------------- inputs */
+
+/* The learned shader's inputs: the plain shader's (pbrInputs) read at lower
+ * cost. The albedo and normal maps are sampled bilinearly at the one mip level
+ * nearest the LOD (the plain shader blends two levels), the metal-rough map at
+ * the nearest texel of level 0 (the plain shader: bilinear). The capture
+ * records these for training, and the network is judged against the FP64
+ * plain shader evaluated on the exact inputs, which it records beside them. */
+struct NnIn {
+  f3 N, V;
+  float rough, metal;
+  RGBAf albedo;
+};
+__device__ __forceinline__ int nnLevel(const TexDesc &t, float lod) {
+  return t.filter_min >= 2 ? min(max((int)floorf(lod + 0.5f), 0), t.levels - 1) : 0;
+}
+__device__ __forceinline__ NnIn nnFastInputs(const DrawParams &P, const Frag &f) {
+  const float *v = f.v;
+  const Uniforms &u = P.u;
+  NnIn in;
+  const f2 uv{v[0], v[1]};
+  const TexDesc *ta = P.tex[RHS_SLOT_ALBEDO], *tn = P.tex[RHS_SLOT_NORMAL];
+  const bool am = (P.defines & RHS_DEF_ALBEDO_MAP) && ta, nm = (P.defines & RHS_DEF_NORMAL_MAP) && tn;
+  RGBAf nt{0.5f, 0.5f, 1.f, 1.f};
+  if (am && nm && sameShape(*ta, *tn)) {
+    float4 ba, bn;
+    sampleLevelPair(*ta, *tn, 0, nnLevel(*ta, lodFor(P, ta, f)), uv, true, ba, bn);
+    in.albedo = unpack01(ba);
+    nt = unpack01(bn);
+  } else {
+    in.albedo = am ? unpack01(sampleLevel4(*ta, 0, nnLevel(*ta, lodFor(P, ta, f)), uv, true))
+                   : RGBAf{u.base_color[0], u.base_color[1], u.base_color[2], u.base_color[3]};
+    if (nm) nt = unpack01(sampleLevel4(*tn, 0, nnLevel(*tn, lodFor(P, tn, f)), uv, true));
+  }
+  in.N = nm ? normalFromTexel(f, 20, 24, nt) : nrm3({v[4], v[5], v[6]});
+  in.V = nrm3({v[12], v[13], v[14]});
+  in.metal = 0.f;
+  in.rough = 1.f;
+  const TexDesc *tm = P.tex[RHS_SLOT_METAL_ROUGHNESS];
+  if ((P.defines & RHS_DEF_METALROUGHNESS_MAP) && tm) {
+    const RGBAf mr = unpack01(sampleLevel4(*tm, 0, 0, uv, false));
+    in.metal = mr.b;
+    in.rough = mr.g;
+  }
+  return in;
 }
 
 /* --------------------------------------------------------------------------- capture */
@@ -370,16 +422,21 @@ __global__ __launch_bounds__(256) void k_capture(DrawParams P, int need_deriv, f
     const uint32_t smask = jb.x >> 28;
     const Frag fr = makeFrag<S, RHS_SHADING_PBR>(P, ps, T, px, py, bs, inst, need_deriv);
     const PbrInputs in = pbrInputs(P, fr);
+    const NnIn fi = nnFastInputs(P, fr);
     const RGBAf plain = fsPbr(P, fr);
     const unsigned long long k = atomicAdd(count, 1ull);
     if ((long long)k >= cap) continue;
     float r[kRecFloats];
-    r[kRecN] = in.N.x, r[kRecN + 1] = in.N.y, r[kRecN + 2] = in.N.z;
-    r[kRecV] = in.V.x, r[kRecV + 1] = in.V.y, r[kRecV + 2] = in.V.z;
-    r[kRecAlbedo] = in.albedo_rgba.r, r[kRecAlbedo + 1] = in.albedo_rgba.g;
-    r[kRecAlbedo + 2] = in.albedo_rgba.b, r[kRecAlbedo + 3] = in.albedo_rgba.a;
-    r[kRecRough] = in.rough, r[kRecMetal] = in.metallic, r[kRecAo] = in.ao;
+    r[kRecN] = fi.N.x, r[kRecN + 1] = fi.N.y, r[kRecN + 2] = fi.N.z;
+    r[kRecV] = fi.V.x, r[kRecV + 1] = fi.V.y, r[kRecV + 2] = fi.V.z;
+    r[kRecAlbedo] = fi.albedo.r, r[kRecAlbedo + 1] = fi.albedo.g;
+    r[kRecAlbedo + 2] = fi.albedo.b, r[kRecAlbedo + 3] = fi.albedo.a;
+    r[kRecRough] = fi.rough, r[kRecMetal] = fi.metal, r[kRecAo] = in.ao;
     r[kRecMaterial] = (float)material;
+    r[kRecXN] = in.N.x, r[kRecXN + 1] = in.N.y, r[kRecXN + 2] = in.N.z;
+    r[kRecXAlbedo] = in.albedo_rgba.r, r[kRecXAlbedo + 1] = in.albedo_rgba.g, r[kRecXAlbedo + 2] = in.albedo_rgba.b;
+    r[kRecXRough] = in.rough, r[kRecXMetal] = in.metallic;
+    r[30] = r[31] = 0.f;
     double col[3];
     pbrIblF64(P, r, col);
     r[kRecTarget] = (float)fmin(fmax(col[0], 0.0), 1.0);
@@ -390,7 +447,6 @@ __global__ __launch_bounds__(256) void k_capture(DrawParams P, int need_deriv, f
     r[kRecPlain + 2] = gclamp(plain.b, 0.f, 1.f);
     r[kRecPixel] = (float)pix;
     r[kRecMask] = (float)smask;
-    for (int q = kRecMask + 1; q < kRecFloats; q++) r[q] = 0.f;
     float4 *o = (float4 *)(rec + (size_t)k * kRecFloats);
 #pragma unroll
     for (int q = 0; q < kRecFloats / 4; q++) o[q] = make_float4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
@@ -424,10 +480,9 @@ __device__ __forceinline__ void nnJobRow(const DrawParams &P, int j, int need_de
     bary(T, fx + 0.5f, fy + 0.5f, bs);
   }
   const Frag fr = makeFrag<S, RHS_SHADING_NEURAL_PBR>(P, ps, T, px, py, bs, inst, need_deriv);
-  /* the plain shader's inputs (what the capture recorded for training) */
-  const PbrInputs in = pbrInputs(P, fr);
-  nnFeatures(in.N, in.V, in.rough, in.metallic, in.albedo_rgba.r, in.albedo_rgba.g, in.albedo_rgba.b, f);
-  f[kNnK1 - 1] = in.albedo_rgba.a;
+  const NnIn in = nnFastInputs(P, fr);
+  nnFeatures(in.N, in.V, in.rough, in.metal, in.albedo.r, in.albedo.g, in.albedo.b, f);
+  f[kNnK1 - 1] = in.albedo.a;
 }
 
 /* Learned shading of the fragments the two-phase raster queued, one kernel:
@@ -836,24 +891,40 @@ __device__ __forceinline__ void nnPackParam(NnPacked *W, int i, float x) {
 }
 
 /* The gradient of a step: the `nparts` block sums of k_nn_grad added in a
- * fixed order (4 interleaved partial sums per parameter, then those 4), and
- * the loss likewise. With `gout` the gradient is only written there. Otherwise
- * Adam (step > 0) updates the FP32 parameters with the gradient scaled by
- * `scale`, and every parameter is repacked into the bf16 operands (step 0:
- * repack only). 64 parameters per block of 256 threads. */
+ * fixed order (16 interleaved partial sums per parameter, each over batches of
+ * 8 independent loads, then those 16 in order), and the loss by a fixed tree.
+ * With `gout` the gradient is only written there. Otherwise Adam (step > 0)
+ * updates the FP32 parameters with the gradient scaled by `scale`, and every
+ * parameter is repacked into the bf16 operands (step 0: repack only).
+ * kNnUpdateParams parameters per block of 256 threads. */
+constexpr int kNnUpdateSlices = 16, kNnUpdateParams = 256 / kNnUpdateSlices;
+static_assert(kNnUpdateParams == 16 && kNnUpdateBlocks * kNnUpdateParams >= kNnParams, "k_nn_update grid");
 __global__ __launch_bounds__(256) void k_nn_update(float *p, const float *partial, int nparts, const double *lossp,
                                                    double *loss, float *gout, float *m1, float *m2, NnPacked *W,
                                                    float lr, float b1, float b2, float eps, float scale, int step) {
-  __shared__ float part[4][64];
-  const int t = threadIdx.x & 63, s = threadIdx.x >> 6;
-  const int i = blockIdx.x * 64 + t;
+  __shared__ float part[kNnUpdateSlices][kNnUpdateParams];
+  __shared__ double lpart[256];
+  const int t = threadIdx.x % kNnUpdateParams, s = threadIdx.x / kNnUpdateParams;
+  const int i = blockIdx.x * kNnUpdateParams + t;
   float acc = 0.f;
-  if (i < kNnParams)
-    for (int b = s; b < nparts; b += 4) acc += partial[(size_t)b * kNnParams + i];
+  if (i < kNnParams) {
+    for (int b0 = s; b0 < nparts; b0 += 8 * kNnUpdateSlices) {
+      float v[8];
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int b = b0 + j * kNnUpdateSlices;
+        v[j] = b < nparts ? partial[(size_t)b * kNnParams + i] : 0.f;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; j++) acc += v[j];
+    }
+  }
   part[s][t] = acc;
   __syncthreads();
   if (s == 0 && i < kNnParams) {
-    const float gs = ((part[0][t] + part[1][t]) + part[2][t]) + part[3][t];
+    float gs = 0.f;
+#pragma unroll
+    for (int k = 0; k < kNnUpdateSlices; k++) gs += part[k][t];
     if (gout) {
       gout[i] = gs;
     } else {
@@ -868,10 +939,16 @@ __global__ __launch_bounds__(256) void k_nn_update(float *p, const float *partia
       nnPackParam(W, i, p[i]);
     }
   }
-  if (loss && blockIdx.x == 0 && threadIdx.x == 0) {
-    double sum = 0;
-    for (int b = 0; b < nparts; b++) sum += lossp[b];
-    *loss = sum;
+  if (loss && blockIdx.x == 0) {
+    double v = 0;
+    for (int b = threadIdx.x; b < nparts; b += 256) v += lossp[b];
+    lpart[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) lpart[threadIdx.x] += lpart[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) *loss = lpart[0];
   }
 }
 

@@ -403,7 +403,7 @@ class RendererHIP:
 
     # ------------------------------------------------------------- learned PBR shader (no reference counterpart)
 
-    NN_RECORD_FLOATS = 24
+    NN_RECORD_FLOATS = 32
 
     def neural_init(self, seed: int = 1):
         _lib.check(self.lib.rhip_neural_init(self.dev, seed))

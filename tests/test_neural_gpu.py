@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 def _synthetic_records(n, seed):
     rng = np.random.default_rng(seed)
-    rec = np.zeros((n, 24), np.float32)
+    rec = np.zeros((n, nn.REC_FLOATS), np.float32)
     N = rng.normal(size=(n, 3)); N /= np.linalg.norm(N, axis=1, keepdims=True)
     V = rng.normal(size=(n, 3)); V /= np.linalg.norm(V, axis=1, keepdims=True)
     rec[:, nn.REC_N:nn.REC_N + 3], rec[:, nn.REC_V:nn.REC_V + 3] = N, V
@@ -23,7 +23,7 @@ def _synthetic_records(n, seed):
     rec[:, nn.REC_MATERIAL] = rng.integers(0, 3, n)
     rec[:, nn.REC_TARGET:nn.REC_TARGET + 3] = rng.uniform(0, 1, (n, 3))
     rec[:, nn.REC_PLAIN:nn.REC_PLAIN + 3] = rec[:, nn.REC_TARGET:nn.REC_TARGET + 3]
-    return rec
+    return nn.exact_from_inputs(rec)
 
 
 def _random_params(seed):

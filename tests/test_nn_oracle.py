@@ -53,7 +53,7 @@ def test_bf16_rounding_ties_to_even():
 
 
 def test_features_of_axis_vectors():
-    rec = np.zeros((2, 24), np.float32)
+    rec = np.zeros((2, nn.REC_FLOATS), np.float32)
     rec[0, nn.REC_N:nn.REC_N + 3] = (0, 0, 1)
     rec[0, nn.REC_V:nn.REC_V + 3] = (0, 0, 1)
     rec[1, nn.REC_N:nn.REC_N + 3] = (0, 1, 0)
@@ -74,15 +74,19 @@ def test_fp64_shader_closed_forms():
     pref = [[np.full((8 >> k, 8 >> k, 4), (60, 90, 120, 255), np.uint8) for _ in range(6)] for k in range(4)]
     rng = np.random.default_rng(3)
     n = 50
-    rec = np.zeros((n, 24))
+    rec = np.zeros((n, nn.REC_FLOATS))
     N = rng.normal(size=(n, 3)); N /= np.linalg.norm(N, axis=1, keepdims=True)
     V = rng.normal(size=(n, 3)); V /= np.linalg.norm(V, axis=1, keepdims=True)
-    rec[:, 0:3], rec[:, 3:6] = N, V
-    rec[:, 6:9] = rng.uniform(0, 1, (n, 3))
-    rec[:, nn.REC_ROUGH], rec[:, nn.REC_METAL], rec[:, nn.REC_AO] = rng.uniform(0, 1, n), rng.uniform(0, 1, n), 1.0
+    # the shader reads the exact-input fields (and V); the network-input fields get other values
+    rec[:, nn.REC_XN:nn.REC_XN + 3], rec[:, nn.REC_V:nn.REC_V + 3] = N, V
+    rec[:, nn.REC_XALBEDO:nn.REC_XALBEDO + 3] = rng.uniform(0, 1, (n, 3))
+    rec[:, nn.REC_XROUGH], rec[:, nn.REC_XMETAL], rec[:, nn.REC_AO] = rng.uniform(0, 1, n), rng.uniform(0, 1, n), 1.0
+    rec[:, nn.REC_N:nn.REC_N + 3] = -N
+    rec[:, nn.REC_ALBEDO:nn.REC_ALBEDO + 3] = 1 - rec[:, nn.REC_XALBEDO:nn.REC_XALBEDO + 3]
+    rec[:, nn.REC_ROUGH], rec[:, nn.REC_METAL] = 1 - rec[:, nn.REC_XROUGH], 1 - rec[:, nn.REC_XMETAL]
     got = nn.pbr_ibl_f64(rec, irr, pref)
-    alb = rec[:, 6:9] ** 2.2
-    m, r = rec[:, nn.REC_METAL][:, None], rec[:, nn.REC_ROUGH][:, None]
+    alb = rec[:, nn.REC_XALBEDO:nn.REC_XALBEDO + 3] ** 2.2
+    m, r = rec[:, nn.REC_XMETAL][:, None], rec[:, nn.REC_XROUGH][:, None]
     nv = np.maximum(np.sum(N * V, 1), 0)[:, None]
     F0 = 0.04 * (1 - m) + alb * m
     F = F0 + (np.maximum(1 - r, F0) - F0) * (1 - nv) ** 5
