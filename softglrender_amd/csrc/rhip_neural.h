@@ -347,8 +347,16 @@ struct NnIn {
   float rough, metal;
   RGBAf albedo;
 };
-__device__ __forceinline__ int nnLevel(const TexDesc &t, float lod) {
-  return t.filter_min >= 2 ? min(max((int)floorf(lod + 0.5f), 0), t.levels - 1) : 0;
+/* the mip level nearest lodFor's LOD (hardware log2: the level only needs the rounding) */
+__device__ __forceinline__ int nnLevel(const TexDesc &t, const Frag &f) {
+  if (t.filter_min < 2) return 0;
+  const float w = (float)t.width, h = (float)t.height;
+  const float dxx = (f.tc[1].x - f.tc[0].x) * w, dyx = (f.tc[2].x - f.tc[0].x) * w;
+  const float dxy = (f.tc[1].y - f.tc[0].y) * h, dyy = (f.tc[2].y - f.tc[0].y) * h;
+  const float d = fmaxf(fmaf(dxy, dxy, dxx * dxx), fmaf(dyy, dyy, dyx * dyx));
+  const float lod = 0.5f * __builtin_amdgcn_logf(d);
+  /* clamped as a float first: a zero footprint gives -inf, a NaN takes level 0 */
+  return (int)fminf(fmaxf(floorf(lod + 0.5f), 0.f), (float)(t.levels - 1));
 }
 __device__ __forceinline__ NnIn nnFastInputs(const DrawParams &P, const Frag &f) {
   const float *v = f.v;
@@ -360,13 +368,13 @@ __device__ __forceinline__ NnIn nnFastInputs(const DrawParams &P, const Frag &f)
   RGBAf nt{0.5f, 0.5f, 1.f, 1.f};
   if (am && nm && sameShape(*ta, *tn)) {
     float4 ba, bn;
-    sampleLevelPair(*ta, *tn, 0, nnLevel(*ta, lodFor(P, ta, f)), uv, true, ba, bn);
+    sampleLevelPair(*ta, *tn, 0, nnLevel(*ta, f), uv, true, ba, bn);
     in.albedo = unpack01(ba);
     nt = unpack01(bn);
   } else {
-    in.albedo = am ? unpack01(sampleLevel4(*ta, 0, nnLevel(*ta, lodFor(P, ta, f)), uv, true))
+    in.albedo = am ? unpack01(sampleLevel4(*ta, 0, nnLevel(*ta, f), uv, true))
                    : RGBAf{u.base_color[0], u.base_color[1], u.base_color[2], u.base_color[3]};
-    if (nm) nt = unpack01(sampleLevel4(*tn, 0, nnLevel(*tn, lodFor(P, tn, f)), uv, true));
+    if (nm) nt = unpack01(sampleLevel4(*tn, 0, nnLevel(*tn, f), uv, true));
   }
   in.N = nm ? normalFromTexel(f, 20, 24, nt) : nrm3({v[4], v[5], v[6]});
   in.V = nrm3({v[12], v[13], v[14]});
