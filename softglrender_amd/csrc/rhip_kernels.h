@@ -837,43 +837,30 @@ __device__ __forceinline__ int axisQuads(float mn, float mx, int bc, int p, int 
   return n;
 }
 
-/* Position in a pixel of shading candidate p (0 the centre, 1..4 samples
- * 0..3), in eighths: exact, and equal to sampleX / sampleY. */
-__device__ __forceinline__ float candX(int p) { return (float)((0x51734u >> (4 * p)) & 15u) * 0.125f; }
-__device__ __forceinline__ float candY(int p) { return (float)((0x13574u >> (4 * p)) & 15u) * 0.125f; }
-
-/* The shading sample of pixel (fx, fy) as the quad context picks it
- * (RendererSoft.cpp:776-802): the centre if inside, else the first inside
- * sample. Returns false, with the centre's barycentrics, when none is (1x:
- * the centre alone). Candidates sampleSurelyOut rules out skip bary's two
- * divisions, and the exact tests run one per round, so a wavefront pays for
- * its lane with the most candidates left rather than for all five. */
-template <int S>
-__device__ __forceinline__ bool shadingBary(const Tri &T, float fx, float fy, float4 &bc) {
-  if (S == 1) return bary(T, fx + 0.5f, fy + 0.5f, bc);
-  int p = 0;
-  bool in = false;
-  for (;;) {
-    while (p < 5 && sampleSurelyOut(T, candX(p) + fx, candY(p) + fy)) p++;
-    if (p >= 5) break;
-    if (bary(T, candX(p) + fx, candY(p) + fy, bc)) {
-      in = true;
-      break;
-    }
-    p++;
-  }
-  if (!in) bary(T, fx + 0.5f, fy + 0.5f, bc);
-  return in;
-}
-
 /* Shading-sample barycentrics of one pixel of a quad, as the quad context
  * leaves them for interpolation (RendererSoft.cpp:776-820): the centre sample
  * if it is inside, else the first inside sample, perspective-corrected; a pixel
  * with no inside sample keeps the centre's screen-space barycentrics. */
 template <int S>
 __device__ __forceinline__ float4 shadingBc(const Tri &T, float fx, float fy) {
+  const float lx[4] = {0.375f, 0.875f, 0.125f, 0.625f};
+  const float ly[4] = {0.875f, 0.625f, 0.375f, 0.125f};
   float4 bc;
-  const bool in = shadingBary<S>(T, fx, fy, bc);
+  bool in = bary(T, fx + 0.5f, fy + 0.5f, bc);
+  if (S == 4 && !in) {
+    float4 bcs;
+    float4 centre = bc;
+    bool found = false;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      if (!found && bary(T, lx[s] + fx, ly[s] + fy, bcs)) {
+        bc = bcs;
+        found = true;
+      }
+    }
+    in = found;
+    if (!found) bc = centre;
+  }
   if (in) {
     float w = dpps3(bc, T.w);
     float iw = 1.f / w;
@@ -1714,6 +1701,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4, 8
 template <int S, int SHADING>
 __device__ __forceinline__ void shadeJob(const DrawParams &P, int j, int need_deriv, unsigned long long &nfs,
                                          unsigned long long &nwr) {
+  const float lx[4] = {0.375f, 0.875f, 0.125f, 0.625f};
+  const float ly[4] = {0.875f, 0.625f, 0.375f, 0.125f};
   {
     const uint2 jb = P.jobs[j];
     const uint32_t pix = jb.x & 0x0fffffffu, m = jb.x >> 28, w = jb.y;
@@ -1724,7 +1713,17 @@ __device__ __forceinline__ void shadeJob(const DrawParams &P, int j, int need_de
     const float fx = (float)px, fy = (float)py;
     /* the shading sample: the centre if inside, else the first inside sample (RendererSoft.cpp:776-802) */
     float4 bs;
-    shadingBary<S>(T, fx, fy, bs);
+    if (S == 4) {
+      if (!bary(T, fx + 0.5f, fy + 0.5f, bs)) {
+#pragma unroll
+        for (int t = 3; t >= 0; t--) {
+          float4 b;
+          if (bary(T, lx[t] + fx, ly[t] + fy, b)) bs = b;
+        }
+      }
+    } else {
+      bary(T, fx + 0.5f, fy + 0.5f, bs);
+    }
     RGBAf c = shadeAt<S, SHADING, true>(P, ps, T, px, py, bs, inst, need_deriv); /* written as bytes, no blending */
     c = {gclamp(c.r, 0.f, 1.f), gclamp(c.g, 0.f, 1.f), gclamp(c.b, 0.f, 1.f), gclamp(c.a, 0.f, 1.f)};
     const uint32_t packed = pack4(u8t(c.r * 255.f), u8t(c.g * 255.f), u8t(c.b * 255.f), u8t(c.a * 255.f));
@@ -2506,7 +2505,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
     const float fx = (float)x, fy = (float)y;
     float4 bs;
-    shadingBary<S>(T, fx, fy, bs);
+    if (S == 4) {
+      if (!bary(T, fx + 0.5f, fy + 0.5f, bs)) {
+#pragma unroll
+        for (int t = 3; t >= 0; t--) {
+          float4 b;
+          if (bary(T, sampleX(t) + fx, sampleY(t) + fy, b)) bs = b;
+        }
+      }
+    } else {
+      bary(T, fx + 0.5f, fy + 0.5f, bs);
+    }
     const RGBAf c = shadeAt<S, SHADING>(P, ps, T, x, y, bs, inst, need_deriv);
     P.ord_col[j] = make_float4(gclamp(c.r, 0.f, 1.f), gclamp(c.g, 0.f, 1.f), gclamp(c.b, 0.f, 1.f),
                                gclamp(c.a, 0.f, 1.f));
@@ -2535,7 +2544,17 @@ __global__ __launch_bounds__(256) void k_ord_shade_ibl(DrawParams P, int need_de
     const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
     const float fx = (float)x, fy = (float)y;
     float4 bs;
-    shadingBary<S>(T, fx, fy, bs);
+    if (S == 4) {
+      if (!bary(T, fx + 0.5f, fy + 0.5f, bs)) {
+#pragma unroll
+        for (int t = 3; t >= 0; t--) {
+          float4 b;
+          if (bary(T, sampleX(t) + fx, sampleY(t) + fy, b)) bs = b;
+        }
+      }
+    } else {
+      bary(T, fx + 0.5f, fy + 0.5f, bs);
+    }
     const Frag f = makeFrag<S, kShadingEnv>(P, ps, T, x, y, bs, inst, need_deriv);
     RGBAf c;
     if (P.shading == RHS_SHADING_IBL_IRRADIANCE) {

@@ -416,7 +416,17 @@ __global__ __launch_bounds__(256) void k_capture(DrawParams P, int need_deriv, f
     const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
     const float fx = (float)px, fy = (float)py;
     float4 bs;
-    shadingBary<S>(T, fx, fy, bs);
+    if (S == 4) {
+      if (!bary(T, fx + 0.5f, fy + 0.5f, bs)) {
+#pragma unroll
+        for (int t = 3; t >= 0; t--) {
+          float4 b;
+          if (bary(T, sampleX(t) + fx, sampleY(t) + fy, b)) bs = b;
+        }
+      }
+    } else {
+      bary(T, fx + 0.5f, fy + 0.5f, bs);
+    }
     const uint32_t smask = jb.x >> 28;
     const Frag fr = makeFrag<S, RHS_SHADING_PBR>(P, ps, T, px, py, bs, inst, need_deriv);
     const PbrInputs in = pbrInputs(P, fr);
@@ -466,7 +476,17 @@ __device__ __forceinline__ void nnJobRow(const DrawParams &P, int j, int need_de
   const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
   const float fx = (float)px, fy = (float)py;
   float4 bs;
-  shadingBary<S>(T, fx, fy, bs);
+  if (S == 4) {
+    if (!bary(T, fx + 0.5f, fy + 0.5f, bs)) {
+#pragma unroll
+      for (int t = 3; t >= 0; t--) {
+        float4 b;
+        if (bary(T, sampleX(t) + fx, sampleY(t) + fy, b)) bs = b;
+      }
+    }
+  } else {
+    bary(T, fx + 0.5f, fy + 0.5f, bs);
+  }
   const Frag fr = makeFrag<S, RHS_SHADING_NEURAL_PBR>(P, ps, T, px, py, bs, inst, need_deriv);
   const NnIn in = nnFastInputs(P, fr);
   nnFeatures(in.N, in.V, in.rough, in.metal, in.albedo.r, in.albedo.g, in.albedo.b, f);
