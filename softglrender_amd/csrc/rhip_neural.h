@@ -338,7 +338,9 @@ __device__ void pbrIblF64(const DrawParams &P, const float *r, double *col) {
 
 /* The learned shader's inputs: the plain shader's (pbrInputs) read at lower
  * cost. The albedo and normal maps are sampled bilinearly at the one mip level
- * nearest the LOD (the plain shader blends two levels), the metal-rough map at
+ * nearest the LOD (the plain shader blends two levels), the LOD taken from the
+ * triangle's barycentric gradients at the fragment (the plain shader differences
+ * the exact shading samples of its quad's pixels), and the metal-rough map at
  * the nearest texel of level 0 (the plain shader: bilinear). The capture
  * records these for training, and the network is judged against the FP64
  * plain shader evaluated on the exact inputs, which it records beside them. */
@@ -347,18 +349,46 @@ struct NnIn {
   float rough, metal;
   RGBAf albedo;
 };
-/* the mip level nearest lodFor's LOD (hardware log2: the level only needs the rounding) */
-__device__ __forceinline__ int nnLevel(const TexDesc &t, const Frag &f) {
+/* Texture-coordinate change over one pixel in x (dx) and in y (dy) at a
+ * fragment: the perspective-correct coordinate at the fragment's raw
+ * barycentric `b` and at b moved by the barycentric gradients of one pixel. */
+struct NnUvGrad {
+  f2 dx, dy;
+};
+__device__ __forceinline__ NnUvGrad nnUvGrad(const DrawParams &P, const PrimSetup &ps, const Tri &T, float4 b) {
+  if (!(P.vary_groups & 1u)) return {{0.f, 0.f}, {0.f, 0.f}}; /* no texture coordinates stored */
+  /* texture coordinates: slots 0-1, the first stored group */
+  const float2 t0 = *(const float2 *)(P.vary + (size_t)ps.v[0] * P.vary_stride);
+  const float2 t1 = *(const float2 *)(P.vary + (size_t)ps.v[1] * P.vary_stride);
+  const float2 t2 = *(const float2 *)(P.vary + (size_t)ps.v[2] * P.vary_stride);
+  /* bary(): ux = (a1 b2 - a2 b1) / uz, uy = (a2 b0 - a0 b2) / uz with a2 = v0x - x, b2 = v0y - y */
+  const float iuz = 1.f / T.uz;
+  const float uxx = T.b1 * iuz, uyx = -T.b0 * iuz, uxy = -T.a1 * iuz, uyy = T.a0 * iuz;
+  const float W0 = T.w[0] * b.x, W1 = T.w[1] * b.y, W2 = T.w[2] * b.z;
+  const float D = W0 + W1 + W2;
+  const float Nu = W0 * t0.x + W1 * t1.x + W2 * t2.x, Nv = W0 * t0.y + W1 * t1.y + W2 * t2.y;
+  /* weights of the moved barycentric (1 - ux - uy, uy, ux) minus the fragment's */
+  auto step = [&](float dux, float duy) {
+    const float e0 = -(dux + duy) * T.w[0], e1 = duy * T.w[1], e2 = dux * T.w[2];
+    const float iD = 1.f / (D + (e0 + e1 + e2)), iD0 = 1.f / D;
+    return f2{(Nu + e0 * t0.x + e1 * t1.x + e2 * t2.x) * iD - Nu * iD0,
+              (Nv + e0 * t0.y + e1 * t1.y + e2 * t2.y) * iD - Nv * iD0};
+  };
+  return {step(uxx, uyx), step(uxy, uyy)};
+}
+
+/* the mip level nearest the LOD of `g` for texture t (hardware log2: the level only needs the rounding) */
+__device__ __forceinline__ int nnLevel(const TexDesc &t, const NnUvGrad &g) {
   if (t.filter_min < 2) return 0;
   const float w = (float)t.width, h = (float)t.height;
-  const float dxx = (f.tc[1].x - f.tc[0].x) * w, dyx = (f.tc[2].x - f.tc[0].x) * w;
-  const float dxy = (f.tc[1].y - f.tc[0].y) * h, dyy = (f.tc[2].y - f.tc[0].y) * h;
+  const float dxx = g.dx.x * w, dyx = g.dy.x * w;
+  const float dxy = g.dx.y * h, dyy = g.dy.y * h;
   const float d = fmaxf(fmaf(dxy, dxy, dxx * dxx), fmaf(dyy, dyy, dyx * dyx));
   const float lod = 0.5f * __builtin_amdgcn_logf(d);
   /* clamped as a float first: a zero footprint gives -inf, a NaN takes level 0 */
   return (int)fminf(fmaxf(floorf(lod + 0.5f), 0.f), (float)(t.levels - 1));
 }
-__device__ __forceinline__ NnIn nnFastInputs(const DrawParams &P, const Frag &f) {
+__device__ __forceinline__ NnIn nnFastInputs(const DrawParams &P, const Frag &f, const NnUvGrad &g) {
   const float *v = f.v;
   const Uniforms &u = P.u;
   NnIn in;
@@ -368,13 +398,13 @@ __device__ __forceinline__ NnIn nnFastInputs(const DrawParams &P, const Frag &f)
   RGBAf nt{0.5f, 0.5f, 1.f, 1.f};
   if (am && nm && sameShape(*ta, *tn)) {
     float4 ba, bn;
-    sampleLevelPair(*ta, *tn, 0, nnLevel(*ta, f), uv, true, ba, bn);
+    sampleLevelPair(*ta, *tn, 0, nnLevel(*ta, g), uv, true, ba, bn);
     in.albedo = unpack01(ba);
     nt = unpack01(bn);
   } else {
-    in.albedo = am ? unpack01(sampleLevel4(*ta, 0, nnLevel(*ta, f), uv, true))
+    in.albedo = am ? unpack01(sampleLevel4(*ta, 0, nnLevel(*ta, g), uv, true))
                    : RGBAf{u.base_color[0], u.base_color[1], u.base_color[2], u.base_color[3]};
-    if (nm) nt = unpack01(sampleLevel4(*tn, 0, nnLevel(*tn, f), uv, true));
+    if (nm) nt = unpack01(sampleLevel4(*tn, 0, nnLevel(*tn, g), uv, true));
   }
   in.N = nm ? normalFromTexel(f, 20, 24, nt) : nrm3({v[4], v[5], v[6]});
   in.V = nrm3({v[12], v[13], v[14]});
@@ -430,7 +460,7 @@ __global__ __launch_bounds__(256) void k_capture(DrawParams P, int need_deriv, f
     const uint32_t smask = jb.x >> 28;
     const Frag fr = makeFrag<S, RHS_SHADING_PBR>(P, ps, T, px, py, bs, inst, need_deriv);
     const PbrInputs in = pbrInputs(P, fr);
-    const NnIn fi = nnFastInputs(P, fr);
+    const NnIn fi = nnFastInputs(P, fr, nnUvGrad(P, ps, T, bs));
     const RGBAf plain = fsPbr(P, fr);
     const unsigned long long k = atomicAdd(count, 1ull);
     if ((long long)k >= cap) continue;
@@ -487,8 +517,9 @@ __device__ __forceinline__ void nnJobRow(const DrawParams &P, int j, int need_de
   } else {
     bary(T, fx + 0.5f, fy + 0.5f, bs);
   }
-  const Frag fr = makeFrag<S, RHS_SHADING_NEURAL_PBR>(P, ps, T, px, py, bs, inst, need_deriv);
-  const NnIn in = nnFastInputs(P, fr);
+  /* no quad partners: the LOD comes from the triangle's gradients */
+  const Frag fr = makeFrag<S, RHS_SHADING_NEURAL_PBR>(P, ps, T, px, py, bs, inst, 0);
+  const NnIn in = nnFastInputs(P, fr, nnUvGrad(P, ps, T, bs));
   nnFeatures(in.N, in.V, in.rough, in.metal, in.albedo.r, in.albedo.g, in.albedo.b, f);
   f[kNnK1 - 1] = in.albedo.a;
 }
