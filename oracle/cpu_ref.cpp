@@ -71,6 +71,16 @@ static inline V3 normalize3(V3 v) {
   float inv = 1.0f / sqrtf(dot3(v, v)); /* glm::normalize = v * inversesqrt(dot(v, v)) */
   return scale3(v, inv);
 }
+/* glm::dot / glm::normalize as the reference build compiles them in the PBR, skybox, prefilter and FXAA
+ * shaders: fma(z, z', fma(y, y', x x')) */
+static inline float dot3b(V3 a, V3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
+static inline V3 normalize3b(V3 v) { return scale3(v, 1.0f / sqrtf(dot3b(v, v))); }
+/* glm::dot where the reference build rounds each product on its own (it shares them between two dot
+ * products): (x x' + y y') + z z' */
+static inline float dot3s(V3 a, V3 b) {
+  const float px = a.x * b.x, py = a.y * b.y, pz = a.z * b.z;
+  return (px + py) + pz;
+}
 static inline V3 cross3(V3 a, V3 b) {
   return {fmaf(a.y, b.z, -(a.z * b.y)), fmaf(a.z, b.x, -(a.x * b.z)), fmaf(a.x, b.y, -(a.y * b.x))};
 }
@@ -405,9 +415,11 @@ static void runVS(const DrawCtx &c, const float *a, VertexHolder &vh) {
     }
     if (c.defines & RHS_DEF_NORMAL_MAP) {
       V3 tg{a[12], a[13], a[14]};
-      V3 N = normalize3(m3v(u.invT, 4, nrm));
-      V3 T = normalize3(m3v(u.invT, 4, tg));
-      V3 Tn = normalize3(sub3(T, scale3(N, dot3(T, N))));
+      /* as compiled: x-first normalisations, T - dot(T, N) N fused (PbrSoft.h:96-101, BlinnPhongSoft.h:115-120) */
+      V3 N = normalize3b(m3v(u.invT, 4, nrm));
+      V3 T = normalize3b(m3v(u.invT, 4, tg));
+      const float d = dot3b(T, N);
+      V3 Tn = normalize3b({fmaf(-d, N.x, T.x), fmaf(-d, N.y, T.y), fmaf(-d, N.z, T.z)});
       int o = bp ? 24 : 20;
       v[o] = N.x, v[o + 1] = N.y, v[o + 2] = N.z;
       v[o + 4] = Tn.x, v[o + 5] = Tn.y, v[o + 6] = Tn.z;
@@ -496,21 +508,24 @@ static float lodFor(const DrawCtx &c, int slot, const FragIn &f) {
   return 0.f < lod ? lod : 0.f;
 }
 
+/* GetNormalFromMap of the Blinn-Phong shader (BlinnPhongSoft.h:150-163) as the reference build compiles
+ * it: x-first normalisations, T - dot(T, N) N fused, and TBN * tn with x and y in a vector register,
+ * fma(N, tn.z, fma(B, tn.y, T tn.x)), z as a scalar, fma(N.z, tn.z, fma(T.z, tn.x, B.z tn.y)). */
 static V3 normalFromMap(const DrawCtx &c, const FragIn &f, int offN, int offT, float lod) {
   const float *v = f.v[f.self];
   if (c.defines & RHS_DEF_NORMAL_MAP) {
-    V3 N = normalize3({v[offN], v[offN + 1], v[offN + 2]});
-    V3 T = normalize3({v[offT], v[offT + 1], v[offT + 2]});
-    T = normalize3(sub3(T, scale3(N, dot3(T, N))));
+    V3 N = normalize3b({v[offN], v[offN + 1], v[offN + 2]});
+    V3 T = normalize3b({v[offT], v[offT + 1], v[offT + 2]});
+    const float d = dot3b(T, N);
+    T = normalize3b({fmaf(-d, N.x, T.x), fmaf(-d, N.y, T.y), fmaf(-d, N.z, T.z)});
     V3 B = cross3(T, N);
     V4 tn4 = tex2D(c, RHS_SLOT_NORMAL, {v[0], v[1]}, lod);
     V3 tn{tn4.x * 2.0f - 1.0f, tn4.y * 2.0f - 1.0f, tn4.z * 2.0f - 1.0f};
-    /* TBN * tn with TBN = mat3(T, B, N) */
-    V3 r{fmaf(N.x, tn.z, fmaf(T.x, tn.x, B.x * tn.y)), fmaf(N.y, tn.z, fmaf(T.y, tn.x, B.y * tn.y)),
+    V3 r{fmaf(N.x, tn.z, fmaf(B.x, tn.y, T.x * tn.x)), fmaf(N.y, tn.z, fmaf(B.y, tn.y, T.y * tn.x)),
          fmaf(N.z, tn.z, fmaf(T.z, tn.x, B.z * tn.y))};
-    return normalize3(r);
+    return normalize3b(r);
   }
-  return normalize3({v[4], v[5], v[6]});
+  return normalize3b({v[4], v[5], v[6]});
 }
 
 /* BlinnPhongSoft.h:165-189 */
@@ -518,7 +533,7 @@ static float shadowCalc(const DrawCtx &c, V4 sp, V3 n, V3 lightDir) {
   V3 pc{sp.x / sp.w, sp.y / sp.w, sp.z / sp.w};
   float cur = pc.z;
   if (cur < 0.f || cur > 1.f) return 0.0f;
-  float bias = maxf(0.00025f * (1.0f - dot3(n, normalize3(lightDir))), 0.00005f);
+  float bias = maxf(0.00025f * (1.0f - dot3b(n, normalize3b(lightDir))), 0.00005f); /* x-first, as compiled */
   const Texture *t = c.tex[RHS_SLOT_SHADOW];
   float ox = t ? 1.0f / (float)t->w : 0.f, oy = t ? 1.0f / (float)t->h : 0.f;
   float s = 0.f;
@@ -546,19 +561,22 @@ static V4 fsBlinnPhong(const DrawCtx &c, const FragIn &f, bool directional) {
   V3 amb = scale3(mul3(b3, u.ambient), ao);
   V3 dif{0, 0, 0}, spe{0, 0, 0}, emi{0, 0, 0};
   if (u.enableLight) {
+    /* the reference build sums every dot product x first, and computes L + normalize(C) with the
+     * normalisation's multiply fused into the add */
     V3 ldv{v[16], v[17], v[18]};
-    V3 L = normalize3(ldv);
-    float diffuse = maxf(dot3(N, L), 0.0f);
+    V3 L = normalize3b(ldv);
+    float diffuse = maxf(dot3b(N, L), 0.0f);
     if (directional) {
       dif = scale3(mul3(u.lightColor, b3), diffuse);
     } else {
       V3 l = scale3(ldv, 1.0f / 5.f);
-      float att = clampf(1.0f - dot3(l, l), 0.0f, 1.0f);
+      float att = clampf(1.0f - dot3b(l, l), 0.0f, 1.0f);
       dif = scale3(scale3(mul3(u.lightColor, b3), diffuse), att);
     }
-    V3 C = normalize3({v[12], v[13], v[14]});
-    V3 H = normalize3(add3(L, C));
-    float sa = maxf(dot3(N, H), 0.0f);
+    V3 cr{v[12], v[13], v[14]};
+    const float ic = 1.0f / sqrtf(dot3b(cr, cr));
+    V3 H = normalize3b({fmaf(cr.x, ic, L.x), fmaf(cr.y, ic, L.y), fmaf(cr.z, ic, L.z)});
+    float sa = maxf(dot3b(N, H), 0.0f);
     float sp = u.kSpecular * m_powf(sa, 128.f);
     spe = {sp, sp, sp};
     if (u.enableShadow) {
@@ -575,13 +593,33 @@ static V4 fsBlinnPhong(const DrawCtx &c, const FragIn &f, bool directional) {
   return {col.x, col.y, col.z, base.w};
 }
 
+/* GetNormalFromMap of the PBR shader (PbrSoft.h:153-166) as the reference build compiles it: x-first
+ * normalisations and T - dot(T, N) N as one multiply-add per component (the Blinn-Phong shader's copy of
+ * this function compiles differently: normalFromMap). */
+static V3 normalFromMapPbr(const DrawCtx &c, const FragIn &f, float lod) {
+  const float *v = f.v[f.self];
+  if (c.defines & RHS_DEF_NORMAL_MAP) {
+    V3 N = normalize3b({v[20], v[21], v[22]});
+    V3 T = normalize3b({v[24], v[25], v[26]});
+    const float d = dot3b(T, N);
+    T = normalize3b({fmaf(-d, N.x, T.x), fmaf(-d, N.y, T.y), fmaf(-d, N.z, T.z)});
+    V3 B = cross3(T, N);
+    V4 tn4 = tex2D(c, RHS_SLOT_NORMAL, {v[0], v[1]}, lod);
+    V3 tn{tn4.x * 2.0f - 1.0f, tn4.y * 2.0f - 1.0f, tn4.z * 2.0f - 1.0f};
+    V3 r{fmaf(N.x, tn.z, fmaf(T.x, tn.x, B.x * tn.y)), fmaf(N.y, tn.z, fmaf(T.y, tn.x, B.y * tn.y)),
+         fmaf(N.z, tn.z, fmaf(T.z, tn.x, B.z * tn.y))};
+    return normalize3b(r);
+  }
+  return normalize3b({v[4], v[5], v[6]});
+}
+
 /* PbrSoft.h:168-223 */
 static const float kPI = 3.14159265359f;
 static float distGGX(V3 N, V3 H, float r) {
   float a = r * r, a2 = a * a;
-  float nh = maxf(dot3(N, H), 0.0f);
+  float nh = maxf(dot3b(N, H), 0.0f);
   float nh2 = nh * nh;
-  float den = nh2 * (a2 - 1.0f) + 1.0f;
+  float den = fmaf(a2 - 1.0f, nh2, 1.0f); /* the reference build fuses NdotH2 * (a2 - 1) + 1 */
   den = kPI * den * den;
   return a2 / den;
 }
@@ -589,12 +627,14 @@ static float schlickGGX(float nv, float r) {
   float k = ((r + 1.0f) * (r + 1.0f)) / 8.0f;
   return nv / (nv * (1.0f - k) + k);
 }
+/* every multiply-add of EnvBRDFApprox is fused in the reference build */
 static V3 envBRDFApprox(V3 spec, float r, float nv) {
-  float rx = r * -1.f + 1.f, ry = r * -0.0275f + 0.0425f, rz = r * -0.572f + 1.04f, rw = r * 0.022f + -0.04f;
-  float a004 = minf(rx * rx, m_exp2f(-9.28f * nv)) * rx + ry;
-  float ax = -1.04f * a004 + rz, ay = 1.04f * a004 + rw;
+  float rx = fmaf(r, -1.f, 1.f), ry = fmaf(r, -0.0275f, 0.0425f), rz = fmaf(r, -0.572f, 1.04f),
+        rw = fmaf(r, 0.022f, -0.04f);
+  float a004 = fmaf(minf(rx * rx, m_exp2f(-9.28f * nv)), rx, ry);
+  float ax = fmaf(-1.04f, a004, rz), ay = fmaf(1.04f, a004, rw);
   ay *= maxf(0.f, minf(1.f, 50.0f * spec.y));
-  return {spec.x * ax + ay, spec.y * ax + ay, spec.z * ax + ay};
+  return {fmaf(spec.x, ax, ay), fmaf(spec.y, ax, ay), fmaf(spec.z, ax, ay)};
 }
 
 /* PbrSoft.h:225-322 */
@@ -613,30 +653,32 @@ static V4 fsPbr(const DrawCtx &c, const FragIn &f) {
     rough = mr.y;
   }
   float ao = (c.defines & RHS_DEF_AO_MAP) ? tex2D(c, RHS_SLOT_AO, uv, lodFor(c, RHS_SLOT_AO, f)).x : 1.f;
-  V3 N = normalFromMap(c, f, 20, 24, lodFor(c, RHS_SLOT_NORMAL, f));
-  V3 V = normalize3({v[12], v[13], v[14]});
-  /* glm::reflect(-V, N) = I - 2 dot(N, I) N */
+  V3 N = normalFromMapPbr(c, f, lodFor(c, RHS_SLOT_NORMAL, f));
+  V3 V = normalize3b({v[12], v[13], v[14]});
+  /* glm::reflect(-V, N) = I - 2 dot(N, I) N; the reference build rounds the products of dot(N, I) on
+   * their own and reuses them, negated, for dot(N, V) */
   V3 I{-V.x, -V.y, -V.z};
-  float dNI = dot3(N, I);
+  float dNI = dot3s(N, I);
+  const float dNV = -dNI;
   V3 R = sub3(I, scale3(N, dNI * 2.0f));
   V3 F0{mixF(0.04f, albedo.x, metallic), mixF(0.04f, albedo.y, metallic), mixF(0.04f, albedo.z, metallic)};
   V3 Lo{0, 0, 0};
   if (u.enableLight) {
     V3 ldv{v[16], v[17], v[18]};
-    V3 L = normalize3(ldv);
-    V3 H = normalize3(add3(V, L));
+    V3 L = normalize3b(ldv);
+    V3 H = normalize3b(add3(V, L));
     V3 ld = scale3(ldv, 1.0f / 5.f);
-    float att = clampf(1.0f - dot3(ld, ld), 0.0f, 1.0f);
+    float att = clampf(1.0f - dot3b(ld, ld), 0.0f, 1.0f);
     V3 radiance = scale3(u.lightColor, att);
     float NDF = distGGX(N, H, rough);
-    float nv = maxf(dot3(N, V), 0.0f), nl = maxf(dot3(N, L), 0.0f);
+    float nv = maxf(dNV, 0.0f), nl = maxf(dot3b(N, L), 0.0f);
     float G = schlickGGX(nl, rough) * schlickGGX(nv, rough);
-    float ct = maxf(dot3(H, V), 0.0f);
+    float ct = maxf(dot3b(H, V), 0.0f);
     float p5 = m_powf(clampf(1.0f - ct, 0.0f, 1.0f), 5.0f);
-    V3 F{F0.x + (1.0f - F0.x) * p5, F0.y + (1.0f - F0.y) * p5, F0.z + (1.0f - F0.z) * p5};
+    V3 F{fmaf(1.0f - F0.x, p5, F0.x), fmaf(1.0f - F0.y, p5, F0.y), fmaf(1.0f - F0.z, p5, F0.z)};
     float ng = NDF * G;
     V3 num = scale3(F, ng);
-    float den = 4.0f * nv * nl + 0.0001f;
+    float den = fmaf(4.0f * nv, nl, 0.0001f);
     V3 spec{num.x / den, num.y / den, num.z / den};
     V3 kD{(1.0f - F.x) * (1.0f - metallic), (1.0f - F.y) * (1.0f - metallic), (1.0f - F.z) * (1.0f - metallic)};
     V3 t{kD.x * albedo.x / kPI + spec.x, kD.y * albedo.y / kPI + spec.y, kD.z * albedo.z / kPI + spec.z};
@@ -644,18 +686,20 @@ static V4 fsPbr(const DrawCtx &c, const FragIn &f) {
   }
   V3 ambient;
   if (u.enableIBL) {
-    float nv = maxf(dot3(N, V), 0.0f);
+    float nv = maxf(dNV, 0.0f);
     float p5 = m_powf(clampf(1.0f - nv, 0.0f, 1.0f), 5.0f);
     float omr = 1.0f - rough;
-    V3 F{F0.x + (maxf(omr, F0.x) - F0.x) * p5, F0.y + (maxf(omr, F0.y) - F0.y) * p5,
-         F0.z + (maxf(omr, F0.z) - F0.z) * p5};
+    V3 F{fmaf(maxf(omr, F0.x) - F0.x, p5, F0.x), fmaf(maxf(omr, F0.y) - F0.y, p5, F0.y),
+         fmaf(maxf(omr, F0.z) - F0.z, p5, F0.z)};
     V3 kD{(1.0f - F.x) * (1.0f - metallic), (1.0f - F.y) * (1.0f - metallic), (1.0f - F.z) * (1.0f - metallic)};
     V4 irr = texCube(c, RHS_SLOT_IRRADIANCE, N, 0.f);
     V3 diffuse{irr.x * albedo.x, irr.y * albedo.y, irr.z * albedo.z};
     V4 pf = texCube(c, RHS_SLOT_PREFILTER, R, rough * 4.0f);
     V3 brdf = envBRDFApprox(F, rough, nv);
-    V3 spec{pf.x * brdf.x, pf.y * brdf.y, pf.z * brdf.z};
-    ambient = scale3(add3(mul3(kD, diffuse), spec), ao);
+    /* (kD * diffuse + prefiltered * brdf) with the second product fused into the add */
+    V3 pre{fmaf(brdf.x, pf.x, kD.x * diffuse.x), fmaf(brdf.y, pf.y, kD.y * diffuse.y),
+           fmaf(brdf.z, pf.z, kD.z * diffuse.z)};
+    ambient = scale3(pre, ao);
   } else {
     ambient = scale3(mul3(u.ambient, albedo), ao);
   }
@@ -700,10 +744,6 @@ static V4 tex2DOffset(const DrawCtx &c, int slot, V2 uv, float lod, int ox, int 
   return {(float)chan(p, 0) / 255.f, (float)chan(p, 1) / 255.f, (float)chan(p, 2) / 255.f, (float)chan(p, 3) / 255.f};
 }
 
-/* glm::dot / glm::normalize as the reference build compiles them in the skybox, prefilter and FXAA
- * shaders: fma(z, z', fma(y, y', x x')) */
-static inline float dot3b(V3 a, V3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
-static inline V3 normalize3b(V3 v) { return scale3(v, 1.0f / sqrtf(dot3b(v, v))); }
 
 /* SkyboxSoft.h:78-96: the host's atan2f, asin in double as the reference calls it; the reference build fuses
  * `uv * scale + 0.5`. */
@@ -722,20 +762,22 @@ static V4 fsSkybox(const DrawCtx &c, const FragIn &f) {
 /* IBLIrradianceSoft.h:74-101 */
 static V4 fsIrradiance(const DrawCtx &c, const FragIn &f) {
   const float *v = f.v[f.self];
-  V3 N = normalize3({v[0], v[1], v[2]});
+  /* as compiled: x-first normalisations, the sample vector and the sum as multiply-adds */
+  V3 N = normalize3b({v[0], v[1], v[2]});
   V3 irr{0.f, 0.f, 0.f};
   V3 up{0.0f, 1.0f, 0.0f};
-  V3 right = normalize3(cross3(up, N));
-  up = normalize3(cross3(N, right));
+  V3 right = normalize3b(cross3(up, N));
+  up = normalize3b(cross3(N, right));
   const float sampleDelta = 0.025f;
   float nr = 0.0f;
   for (float phi = 0.0f; phi < 2.0f * kPI; phi += sampleDelta) {
     for (float theta = 0.0f; theta < 0.5f * kPI; theta += sampleDelta) {
       V3 ts{m_sinf(theta) * m_cosf(phi), m_sinf(theta) * m_sinf(phi), m_cosf(theta)};
-      V3 sv = add3(add3(scale3(right, ts.x), scale3(up, ts.y)), scale3(N, ts.z));
+      V3 sv{fmaf(N.x, ts.z, fmaf(up.x, ts.y, right.x * ts.x)), fmaf(N.y, ts.z, fmaf(up.y, ts.y, right.y * ts.x)),
+            fmaf(N.z, ts.z, fmaf(up.z, ts.y, right.z * ts.x))};
       V4 t = texCube(c, RHS_SLOT_CUBE, sv, 0.f);
       float ct = m_cosf(theta), st = m_sinf(theta);
-      irr = add3(irr, {t.x * ct * st, t.y * ct * st, t.z * ct * st});
+      irr = {fmaf(t.x * ct, st, irr.x), fmaf(t.y * ct, st, irr.y), fmaf(t.z * ct, st, irr.z)};
       nr++;
     }
   }

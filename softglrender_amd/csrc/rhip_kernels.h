@@ -2564,15 +2564,13 @@ __global__ __launch_bounds__(256) void k_ord_shade_ibl(DrawParams P, int need_de
       float nr = 0.f;
       for (int base = 0; base < total; base += 64) {
         const int k = base + lane;
-        const f3 t = k < total ? irrTerm(P, st, k) : f3{0.f, 0.f, 0.f};
-        L[lane] = make_float4(t.x, t.y, t.z, 0.f);
+        L[lane] = k < total ? irrTerm(P, st, k) : make_float4(0.f, 0.f, 0.f, 0.f);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         const int m = min(64, total - base);
         for (int q = 0; q < m; q++) {
-          const float4 e = L[q];
-          irr = add3(irr, {e.x, e.y, e.z});
+          irr = irrAdd(irr, L[q]);
           nr += 1.f;
         }
         __builtin_amdgcn_wave_barrier();

@@ -406,8 +406,8 @@ __device__ __forceinline__ NnIn nnFastInputs(const DrawParams &P, const Frag &f,
                    : RGBAf{u.base_color[0], u.base_color[1], u.base_color[2], u.base_color[3]};
     if (nm) nt = unpack01(sampleLevel4(*tn, 0, nnLevel(*tn, g), uv, true));
   }
-  in.N = nm ? normalFromTexel(f, 20, 24, nt) : nrm3({v[4], v[5], v[6]});
-  in.V = nrm3({v[12], v[13], v[14]});
+  in.N = nm ? normalFromTexel<true>(f, 20, 24, nt) : nrm3b({v[4], v[5], v[6]});
+  in.V = nrm3b({v[12], v[13], v[14]});
   in.metal = 0.f;
   in.rough = 1.f;
   const TexDesc *tm = P.tex[RHS_SLOT_METAL_ROUGHNESS];

@@ -27,6 +27,16 @@ __device__ __forceinline__ f3 scl3(f3 a, float s) { return {a.x * s, a.y * s, a.
 /* glm::dot(vec3) as compiled by the reference build: fma(z, z', fma(x, x', y*y')) */
 __device__ __forceinline__ float dot3(f3 a, f3 b) { return fmaf(a.z, b.z, fmaf(a.x, b.x, a.y * b.y)); }
 __device__ __forceinline__ f3 nrm3(f3 v) { return scl3(v, 1.0f / sqrtf(dot3(v, v))); }
+/* glm::dot / glm::normalize as the reference build compiles them in the fragment shaders and the
+ * normal-map set-up of the vertex shaders: fma(z, z', fma(y, y', x x')) */
+__device__ __forceinline__ float dot3b(f3 a, f3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
+__device__ __forceinline__ f3 nrm3b(f3 v) { return scl3(v, 1.0f / sqrtf(dot3b(v, v))); }
+/* glm::dot where the reference build rounds each product on its own (products shared by two dot
+ * products): (x x' + y y') + z z' */
+__device__ __forceinline__ float dot3s(f3 a, f3 b) {
+  const float px = a.x * b.x, py = a.y * b.y, pz = a.z * b.z;
+  return (px + py) + pz;
+}
 __device__ __forceinline__ f3 cross3(f3 a, f3 b) {
   return {fmaf(a.y, b.z, -(a.z * b.y)), fmaf(a.z, b.x, -(a.x * b.z)), fmaf(a.x, b.y, -(a.y * b.x))};
 }
@@ -465,9 +475,11 @@ __device__ __forceinline__ float4 runVS(const DrawParams &P, const float *a, flo
     }
     if (P.defines & RHS_DEF_NORMAL_MAP) {
       f3 tg{a[12], a[13], a[14]};
-      f3 N = nrm3(m3v(u.inv_t, 4, nrm));
-      f3 T = nrm3(m3v(u.inv_t, 4, tg));
-      f3 Tn = nrm3(sub3(T, scl3(N, dot3(T, N))));
+      /* as compiled: x-first normalisations, T - dot(T, N) N fused */
+      f3 N = nrm3b(m3v(u.inv_t, 4, nrm));
+      f3 T = nrm3b(m3v(u.inv_t, 4, tg));
+      const float d = dot3b(T, N);
+      f3 Tn = nrm3b({fmaf(-d, N.x, T.x), fmaf(-d, N.y, T.y), fmaf(-d, N.z, T.z)});
       int o = bp ? 24 : 20;
       v[o] = N.x, v[o + 1] = N.y, v[o + 2] = N.z;
       v[o + 4] = Tn.x, v[o + 5] = Tn.y, v[o + 6] = Tn.z;
@@ -546,29 +558,37 @@ __device__ __forceinline__ RGBAf texCube(const TexDesc *t, f3 dir, float lod) {
   return unpack01(textureImpl4(*t, face, uv, lod));
 }
 
-/* The normal from the normal map's texel t4 and the interpolated TBN frame (BlinnPhongSoft.h:152-163,
- * PbrSoft.h:154-166). */
+/* The normal from the normal map's texel t4 and the interpolated TBN frame (BlinnPhongSoft.h:150-163,
+ * PbrSoft.h:153-166) as the reference build compiles each: x-first normalisations and T - dot(T, N) N
+ * fused in both; TBN * tn as fma(N, tn.z, fma(T, tn.x, B tn.y)) per component in the PBR shader, while
+ * the Blinn-Phong shader computes x and y in a vector register as fma(N, tn.z, fma(B, tn.y, T tn.x)). */
+template <bool PBR>
 __device__ __forceinline__ f3 normalFromTexel(const Frag &f, int offN, int offT, RGBAf t4) {
   const float *v = f.v;
-  {
-    f3 N = nrm3({v[offN], v[offN + 1], v[offN + 2]});
-    f3 T = nrm3({v[offT], v[offT + 1], v[offT + 2]});
-    T = nrm3(sub3(T, scl3(N, dot3(T, N))));
-    f3 B = cross3(T, N);
-    f3 tn{t4.r * 2.0f - 1.0f, t4.g * 2.0f - 1.0f, t4.b * 2.0f - 1.0f};
-    f3 r{fmaf(N.x, tn.z, fmaf(T.x, tn.x, B.x * tn.y)), fmaf(N.y, tn.z, fmaf(T.y, tn.x, B.y * tn.y)),
+  f3 N = nrm3b({v[offN], v[offN + 1], v[offN + 2]});
+  f3 T = nrm3b({v[offT], v[offT + 1], v[offT + 2]});
+  const float d = dot3b(T, N);
+  T = nrm3b({fmaf(-d, N.x, T.x), fmaf(-d, N.y, T.y), fmaf(-d, N.z, T.z)});
+  f3 B = cross3(T, N);
+  f3 tn{t4.r * 2.0f - 1.0f, t4.g * 2.0f - 1.0f, t4.b * 2.0f - 1.0f};
+  f3 r;
+  if (PBR)
+    r = {fmaf(N.x, tn.z, fmaf(T.x, tn.x, B.x * tn.y)), fmaf(N.y, tn.z, fmaf(T.y, tn.x, B.y * tn.y)),
          fmaf(N.z, tn.z, fmaf(T.z, tn.x, B.z * tn.y))};
-    return nrm3(r);
-  }
+  else
+    r = {fmaf(N.x, tn.z, fmaf(B.x, tn.y, T.x * tn.x)), fmaf(N.y, tn.z, fmaf(B.y, tn.y, T.y * tn.x)),
+         fmaf(N.z, tn.z, fmaf(T.z, tn.x, B.z * tn.y))};
+  return nrm3b(r);
 }
 
+template <bool PBR>
 __device__ __forceinline__ f3 normalFromMap(const DrawParams &P, const Frag &f, int offN, int offT) {
   const float *v = f.v;
   if (P.defines & RHS_DEF_NORMAL_MAP) {
     const TexDesc *nt = P.tex[RHS_SLOT_NORMAL];
-    return normalFromTexel(f, offN, offT, tex2D(nt, {v[0], v[1]}, lodFor(P, nt, f)));
+    return normalFromTexel<PBR>(f, offN, offT, tex2D(nt, {v[0], v[1]}, lodFor(P, nt, f)));
   }
-  return nrm3({v[4], v[5], v[6]});
+  return nrm3b({v[4], v[5], v[6]});
 }
 
 /* BlinnPhongSoft.h:165-189 */
@@ -576,7 +596,7 @@ __device__ __forceinline__ float shadowCalc(const DrawParams &P, float4 sp, f3 n
   f3 pc{sp.x / sp.w, sp.y / sp.w, sp.z / sp.w};
   float cur = pc.z;
   if (cur < 0.f || cur > 1.f) return 0.0f;
-  float bias = gmax(0.00025f * (1.0f - dot3(n, nrm3(lightDir))), 0.00005f);
+  float bias = gmax(0.00025f * (1.0f - dot3b(n, nrm3b(lightDir))), 0.00005f); /* x-first, as compiled */
   const TexDesc *t = P.tex[RHS_SLOT_SHADOW];
   if (!t) return 0.f;
   float ox = 1.0f / (float)t->width, oy = 1.0f / (float)t->height;
@@ -601,7 +621,7 @@ __device__ __forceinline__ RGBAf fsBlinnPhong(const DrawParams &P, const Frag &f
   RGBAf base = (P.defines & RHS_DEF_ALBEDO_MAP)
                    ? tex2D(ta, uv, lodFor(P, ta, f))
                    : RGBAf{u.base_color[0], u.base_color[1], u.base_color[2], u.base_color[3]};
-  f3 N = normalFromMap(P, f, 24, 28);
+  f3 N = normalFromMap<false>(P, f, 24, 28);
   const TexDesc *tao = P.tex[RHS_SLOT_AO];
   float ao = (P.defines & RHS_DEF_AO_MAP) ? tex2D(tao, uv, lodFor(P, tao, f)).r : 1.f;
   f3 b3{base.r, base.g, base.b};
@@ -609,19 +629,22 @@ __device__ __forceinline__ RGBAf fsBlinnPhong(const DrawParams &P, const Frag &f
   f3 dif{0, 0, 0}, spe{0, 0, 0}, emi{0, 0, 0};
   if (u.enable_light) {
     f3 ldv{v[16], v[17], v[18]};
-    f3 L = nrm3(ldv);
-    float diffuse = gmax(dot3(N, L), 0.0f);
+    /* the reference build sums every dot product x first, and computes L + normalize(C) with the
+     * normalisation's multiply fused into the add */
+    f3 L = nrm3b(ldv);
+    float diffuse = gmax(dot3b(N, L), 0.0f);
     f3 lc{u.light_color[0], u.light_color[1], u.light_color[2]};
     if (directional) {
       dif = scl3(mul3(lc, b3), diffuse);
     } else {
       f3 l = scl3(ldv, 1.0f / 5.f);
-      float att = gclamp(1.0f - dot3(l, l), 0.0f, 1.0f);
+      float att = gclamp(1.0f - dot3b(l, l), 0.0f, 1.0f);
       dif = scl3(scl3(mul3(lc, b3), diffuse), att);
     }
-    f3 C = nrm3({v[12], v[13], v[14]});
-    f3 H = nrm3(add3(L, C));
-    float sa = gmax(dot3(N, H), 0.0f);
+    const f3 cr{v[12], v[13], v[14]};
+    const float ic = 1.0f / sqrtf(dot3b(cr, cr));
+    f3 H = nrm3b({fmaf(cr.x, ic, L.x), fmaf(cr.y, ic, L.y), fmaf(cr.z, ic, L.z)});
+    float sa = gmax(dot3b(N, H), 0.0f);
     float sp = u.k_specular * hpowi<128>(P.libm, sa);
     spe = {sp, sp, sp};
     if (u.enable_shadow) {
@@ -643,9 +666,9 @@ __device__ __forceinline__ RGBAf fsBlinnPhong(const DrawParams &P, const Frag &f
 constexpr float kPI = 3.14159265359f;
 __device__ __forceinline__ float distGGX(f3 N, f3 H, float r) {
   float a = r * r, a2 = a * a;
-  float nh = gmax(dot3(N, H), 0.0f);
+  float nh = gmax(dot3b(N, H), 0.0f);
   float nh2 = nh * nh;
-  float den = nh2 * (a2 - 1.0f) + 1.0f;
+  float den = fmaf(a2 - 1.0f, nh2, 1.0f); /* the reference build fuses NdotH2 * (a2 - 1) + 1 */
   den = kPI * den * den;
   return a2 / den;
 }
@@ -653,12 +676,14 @@ __device__ __forceinline__ float schlickGGX(float nv, float r) {
   float k = ((r + 1.0f) * (r + 1.0f)) / 8.0f;
   return nv / (nv * (1.0f - k) + k);
 }
+/* every multiply-add of EnvBRDFApprox is fused in the reference build */
 __device__ __forceinline__ f3 envBRDFApprox(const DrawParams &P, f3 spec, float r, float nv) {
-  float rx = r * -1.f + 1.f, ry = r * -0.0275f + 0.0425f, rz = r * -0.572f + 1.04f, rw = r * 0.022f + -0.04f;
-  float a004 = gmin(rx * rx, hexp2f(P.libm, -9.28f * nv)) * rx + ry;
-  float ax = -1.04f * a004 + rz, ay = 1.04f * a004 + rw;
+  float rx = fmaf(r, -1.f, 1.f), ry = fmaf(r, -0.0275f, 0.0425f), rz = fmaf(r, -0.572f, 1.04f),
+        rw = fmaf(r, 0.022f, -0.04f);
+  float a004 = fmaf(gmin(rx * rx, hexp2f(P.libm, -9.28f * nv)), rx, ry);
+  float ax = fmaf(-1.04f, a004, rz), ay = fmaf(1.04f, a004, rw);
   ay *= gmax(0.f, gmin(1.f, 50.0f * spec.y));
-  return {spec.x * ax + ay, spec.y * ax + ay, spec.z * ax + ay};
+  return {fmaf(spec.x, ax, ay), fmaf(spec.y, ax, ay), fmaf(spec.z, ax, ay)};
 }
 
 /* powf(x, 2.2) of a channel sampled from an RGBA8 texture: x is k / 255 for a
@@ -692,6 +717,7 @@ struct PbrInputs {
   f3 albedo;
   float metallic, rough, ao;
   f3 N, V, R;
+  float dnv; /* dot(N, V) as the reference build computes it: -dot(N, -V) from the same products */
 };
 
 __device__ __forceinline__ PbrInputs pbrInputs(const DrawParams &P, const Frag &f) {
@@ -729,10 +755,13 @@ __device__ __forceinline__ PbrInputs pbrInputs(const DrawParams &P, const Frag &
   }
   const TexDesc *tao = P.tex[RHS_SLOT_AO];
   in.ao = (P.defines & RHS_DEF_AO_MAP) ? tex2D(tao, uv, lodFor(P, tao, f)).r : 1.f;
-  in.N = pair ? normalFromTexel(f, 20, 24, ntexel) : normalFromMap(P, f, 20, 24);
-  in.V = nrm3({v[12], v[13], v[14]});
+  in.N = pair ? normalFromTexel<true>(f, 20, 24, ntexel) : normalFromMap<true>(P, f, 20, 24);
+  in.V = nrm3b({v[12], v[13], v[14]});
+  /* glm::reflect(-V, N); the reference build rounds the products of dot(N, I) on their own and reuses
+   * them, negated, for dot(N, V) */
   f3 I{-in.V.x, -in.V.y, -in.V.z};
-  float dNI = dot3(in.N, I);
+  float dNI = dot3s(in.N, I);
+  in.dnv = -dNI;
   in.R = sub3(I, scl3(in.N, dNI * 2.0f));
   return in;
 }
@@ -750,20 +779,20 @@ __device__ __forceinline__ RGBAf fsPbr(const DrawParams &P, const Frag &f) {
   f3 Lo{0, 0, 0};
   if (u.enable_light) {
     f3 ldv{v[16], v[17], v[18]};
-    f3 L = nrm3(ldv);
-    f3 H = nrm3(add3(V, L));
+    f3 L = nrm3b(ldv);
+    f3 H = nrm3b(add3(V, L));
     f3 ld = scl3(ldv, 1.0f / 5.f);
-    float att = gclamp(1.0f - dot3(ld, ld), 0.0f, 1.0f);
+    float att = gclamp(1.0f - dot3b(ld, ld), 0.0f, 1.0f);
     f3 radiance = scl3({u.light_color[0], u.light_color[1], u.light_color[2]}, att);
     float NDF = distGGX(N, H, rough);
-    float nv = gmax(dot3(N, V), 0.0f), nl = gmax(dot3(N, L), 0.0f);
+    float nv = gmax(in.dnv, 0.0f), nl = gmax(dot3b(N, L), 0.0f);
     float G = schlickGGX(nl, rough) * schlickGGX(nv, rough);
-    float ct = gmax(dot3(H, V), 0.0f);
+    float ct = gmax(dot3b(H, V), 0.0f);
     float p5 = hpowi<5>(P.libm, gclamp(1.0f - ct, 0.0f, 1.0f));
-    f3 F{F0.x + (1.0f - F0.x) * p5, F0.y + (1.0f - F0.y) * p5, F0.z + (1.0f - F0.z) * p5};
+    f3 F{fmaf(1.0f - F0.x, p5, F0.x), fmaf(1.0f - F0.y, p5, F0.y), fmaf(1.0f - F0.z, p5, F0.z)};
     float ng = NDF * G;
     f3 num = scl3(F, ng);
-    float den = 4.0f * nv * nl + 0.0001f;
+    float den = fmaf(4.0f * nv, nl, 0.0001f);
     f3 spec{num.x / den, num.y / den, num.z / den};
     f3 kD{(1.0f - F.x) * (1.0f - metallic), (1.0f - F.y) * (1.0f - metallic), (1.0f - F.z) * (1.0f - metallic)};
     f3 t{kD.x * albedo.x / kPI + spec.x, kD.y * albedo.y / kPI + spec.y, kD.z * albedo.z / kPI + spec.z};
@@ -771,18 +800,20 @@ __device__ __forceinline__ RGBAf fsPbr(const DrawParams &P, const Frag &f) {
   }
   f3 ambient;
   if (u.enable_ibl) {
-    float nv = gmax(dot3(N, V), 0.0f);
+    float nv = gmax(in.dnv, 0.0f);
     float p5 = hpowi<5>(P.libm, gclamp(1.0f - nv, 0.0f, 1.0f));
     float omr = 1.0f - rough;
-    f3 F{F0.x + (gmax(omr, F0.x) - F0.x) * p5, F0.y + (gmax(omr, F0.y) - F0.y) * p5,
-         F0.z + (gmax(omr, F0.z) - F0.z) * p5};
+    f3 F{fmaf(gmax(omr, F0.x) - F0.x, p5, F0.x), fmaf(gmax(omr, F0.y) - F0.y, p5, F0.y),
+         fmaf(gmax(omr, F0.z) - F0.z, p5, F0.z)};
     f3 kD{(1.0f - F.x) * (1.0f - metallic), (1.0f - F.y) * (1.0f - metallic), (1.0f - F.z) * (1.0f - metallic)};
     RGBAf irr = texCube(P.tex[RHS_SLOT_IRRADIANCE], N, 0.f);
     f3 diffuse{irr.r * albedo.x, irr.g * albedo.y, irr.b * albedo.z};
     RGBAf pf = texCube(P.tex[RHS_SLOT_PREFILTER], in.R, rough * 4.0f);
     f3 brdf = envBRDFApprox(P, F, rough, nv);
-    f3 spec{pf.r * brdf.x, pf.g * brdf.y, pf.b * brdf.z};
-    ambient = scl3(add3(mul3(kD, diffuse), spec), in.ao);
+    /* (kD * diffuse + prefiltered * brdf) with the second product fused into the add */
+    f3 pre{fmaf(brdf.x, pf.r, kD.x * diffuse.x), fmaf(brdf.y, pf.g, kD.y * diffuse.y),
+           fmaf(brdf.z, pf.b, kD.z * diffuse.z)};
+    ambient = scl3(pre, in.ao);
   } else {
     ambient = scl3(mul3({u.ambient[0], u.ambient[1], u.ambient[2]}, albedo), in.ao);
   }
@@ -832,8 +863,6 @@ __device__ RGBAf tex2DOffset(const TexDesc *t, f2 uv, float lod, int ox, int oy)
 /* glm::dot and glm::normalize as the reference build compiles them in the
  * skybox, prefilter and FXAA shaders: fma(z, z', fma(y, y', x x')), x first (the
  * lighting shaders' order, dot3, starts from y). */
-__device__ __forceinline__ float dot3b(f3 a, f3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
-__device__ __forceinline__ f3 nrm3b(f3 v) { return scl3(v, 1.0f / sqrtf(dot3b(v, v))); }
 
 /* SkyboxSoft.h:78-96. The equirectangular lookup calls glibc's atan2f
  * (reproduced step for step, fdAtan2f) and the double-precision asin, rounded
@@ -860,22 +889,30 @@ struct IrrSetup {
   f3 N, right, up;
 };
 __device__ __forceinline__ IrrSetup irrSetup(const Frag &f) {
-  IrrSetup s;
-  s.N = nrm3({f.v[0], f.v[1], f.v[2]});
+  IrrSetup s; /* x-first normalisations, as compiled */
+  s.N = nrm3b({f.v[0], f.v[1], f.v[2]});
   f3 up{0.f, 1.f, 0.f};
-  s.right = nrm3(cross3(up, s.N));
-  s.up = nrm3(cross3(s.N, s.right));
+  s.right = nrm3b(cross3(up, s.N));
+  s.up = nrm3b(cross3(s.N, s.right));
   return s;
 }
-/* Term k of the sum: phi index k / ntheta, theta index k % ntheta. */
-__device__ __forceinline__ f3 irrTerm(const DrawParams &P, const IrrSetup &s, int k) {
+/* Term k of the sum (phi index k / ntheta, theta index k % ntheta): the sample's
+ * colour times cos(theta), and sin(theta). The reference build fuses the last
+ * product into the running sum (irrAdd), and the sample vector into two
+ * multiply-adds. */
+__device__ __forceinline__ float4 irrTerm(const DrawParams &P, const IrrSetup &s, int k) {
   const IblTables &T = *P.ibl;
   const int i = k / T.ntheta, j = k - i * T.ntheta;
   const float sp = T.sphi[i], cp = T.cphi[i], st = T.stheta[j], ct = T.ctheta[j];
   const f3 ts{st * cp, st * sp, ct};
-  const f3 sv = add3(add3(scl3(s.right, ts.x), scl3(s.up, ts.y)), scl3(s.N, ts.z));
+  const f3 sv{fmaf(s.N.x, ts.z, fmaf(s.up.x, ts.y, s.right.x * ts.x)),
+              fmaf(s.N.y, ts.z, fmaf(s.up.y, ts.y, s.right.y * ts.x)),
+              fmaf(s.N.z, ts.z, fmaf(s.up.z, ts.y, s.right.z * ts.x))};
   const RGBAf c = texCube(P.tex[RHS_SLOT_CUBE], sv, 0.f);
-  return {c.r * ct * st, c.g * ct * st, c.b * ct * st};
+  return make_float4(c.r * ct, c.g * ct, c.b * ct, st);
+}
+__device__ __forceinline__ f3 irrAdd(f3 irr, float4 t) {
+  return {fmaf(t.x, t.w, irr.x), fmaf(t.y, t.w, irr.y), fmaf(t.z, t.w, irr.z)};
 }
 __device__ __forceinline__ RGBAf irrFinish(f3 irr, float nr) {
   irr = scl3(scl3(irr, kPI), 1.0f / nr);
@@ -887,7 +924,7 @@ __device__ RGBAf fsIrradiance(const DrawParams &P, const Frag &f) {
   f3 irr{0.f, 0.f, 0.f};
   float nr = 0.f;
   for (int k = 0; k < n; k++) {
-    irr = add3(irr, irrTerm(P, s, k));
+    irr = irrAdd(irr, irrTerm(P, s, k));
     nr += 1.f;
   }
   return irrFinish(irr, nr);

@@ -728,7 +728,7 @@ VARIANTS = ["blend_alpha", "blend_add_one", "blend_minmax", "blend_sub", "depth_
             "side_clip", "filter_nearest", "filter_linear", "filter_nn", "filter_ln", "filter_nl", "wrap_mirror",
             "wrap_clamp_edge", "wrap_border_white", "shadow_blinn", "point_light", "vertex_color",
             "basic_nodepth", "viewport_offset", "msaa_blend", "msaa_clip", "msaa_near", "layout_tiled",
-            "layout_morton", "depth_only_msaa", "ao_emissive"]
+            "layout_morton", "depth_only_msaa", "ao_emissive", "normal_blinn", "normal_blinn_dir"]
 
 
 def scene_variant(name, width=160, height=120, seed=11):
@@ -747,6 +747,12 @@ def scene_variant(name, width=160, height=120, seed=11):
                border=S.BORDER_WHITE if name == "wrap_border_white" else S.BORDER_BLACK,
                usage=S.USAGE_SAMPLER | S.USAGE_UPLOAD)
     st.tex_data(1, [tex])
+    if name.startswith("normal_"):
+        # a tangent-space normal map (Blinn-Phong's normal-mapped path)
+        _, nmap, _ = pbr_textures(64, seed + 1)
+        st.texture(5, 64, 64, mipmaps=True, filter_min=S.FILTER_LINEAR_MIPMAP_LINEAR, filter_mag=S.FILTER_LINEAR,
+                   wrap=S.WRAP_REPEAT, usage=S.USAGE_SAMPLER | S.USAGE_UPLOAD)
+        st.tex_data(5, [nmap])
     if name == "layout_tiled":
         layouts[1] = 1
     if name == "layout_morton":
@@ -776,7 +782,7 @@ def scene_variant(name, width=160, height=120, seed=11):
     model = np.eye(4)
     mvp = proj @ view @ model
     scene_u = S.uniforms_scene((0.2, 0.2, 0.2), eye, (1.0, 1.2, 1.5), (0.9, 0.9, 0.9))
-    if name in ("shadow_blinn", "point_light"):
+    if name in ("shadow_blinn", "point_light", "normal_blinn"):
         scene_u = S.uniforms_scene((0.15, 0.15, 0.15), eye, (1.2, 1.6, 2.0), (1.0, 1.0, 1.0))
     shadow_mvp = None
     if name == "shadow_blinn":
@@ -844,6 +850,11 @@ def scene_variant(name, width=160, height=120, seed=11):
         mat = S.uniforms_material(1, 0, 1 if name == "shadow_blinn" else 0, 1.0, 0.8, (1, 1, 1, 1))
         if name == "shadow_blinn":
             samplers = [(S.SLOT_ALBEDO, 1), (S.SLOT_SHADOW, 4)]
+    elif name in ("normal_blinn", "normal_blinn_dir"):
+        shading = S.SHADING_BLINN_PHONG if name == "normal_blinn" else S.SHADING_BLINN_PHONG_DIR
+        defines = S.DEF_ALBEDO | S.DEF_NORMAL
+        samplers = [(S.SLOT_ALBEDO, 1), (S.SLOT_NORMAL, 5)]
+        mat = S.uniforms_material(1, 0, 0, 1.0, 0.8, (1, 1, 1, 1))
     elif name == "basic_nodepth":
         shading, defines, samplers = S.SHADING_BASIC, 0, []
         rs = S.RenderStates()

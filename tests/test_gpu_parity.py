@@ -1,13 +1,11 @@
 """RendererHIP on the GPU against the CPU checker and the reference's golden
 outputs, through librhip's C-ABI (softglrender_amd.renderer drives it).
 
-The GPU reproduces the host libm's powf/log2f/exp2f (rhip_libm.h), so against
-the checker calling the host libm like the reference every output must be
-identical bit for bit. Against the reference's own outputs (fixtures, and the
-full-size C2 frame rendered live where the reference is built) depth must be
-identical and colour within 1/255, with at most a handful of values off by one
-(contraction choices of the reference's compiler that the checker does not
-replicate)."""
+The GPU reproduces the host libm's powf/log2f/exp2f/atan2f (rhip_libm.h) and
+the multiply-adds the reference's compiler fused in its shaders, so every output
+must be identical bit for bit: against the checker calling the host libm like
+the reference, against the reference's own outputs (fixtures), and against the
+full-size C2 frame rendered live where the reference is built."""
 import os
 
 import numpy as np
@@ -17,7 +15,7 @@ import oracle
 from softglrender_amd import scenes
 from softglrender_amd.replay import replay
 from tests.golden_scenes import GOLDEN
-from tests.util import assert_parity, compare, differing_values, load_golden
+from tests.util import compare, differing_values, load_golden
 
 GOLDEN_DIR = os.path.join(os.path.dirname(__file__), "golden")
 pytestmark = pytest.mark.gpu
@@ -46,8 +44,7 @@ def test_hip_matches_reference_fixture(renderer, name):
     print(f"[gpu] fixture {name}", flush=True)
     got = _hip(renderer, sc)
     want = load_golden(os.path.join(GOLDEN_DIR, f"{name}.npz"))
-    assert_parity(got, want, name)
-    assert differing_values(got, want) <= 1, name
+    assert_identical(got, want, name)
 
 
 @pytest.mark.parametrize("name", sorted(GOLDEN))
@@ -100,8 +97,8 @@ def test_hip_c2_full_size(renderer):
 
 @pytest.mark.skipif(not oracle.have_reference(), reason="the reference RendererSoft (oracle/_ref) is not built here")
 def test_hip_c2_full_size_against_reference(renderer):
-    """The full C2 frame against SoftGLRender's own RendererSoft (serial build) on the same inputs: the
-    north star's bar, depth identical, colour within 1/255, at most one value in a million off by one."""
+    """The full C2 frame against SoftGLRender's own RendererSoft (serial build) on the same inputs: colour,
+    depth, every MSAA sample and the shadow map identical."""
     print("[gpu] c2 full size against the reference", flush=True)
     sc = scenes.scene_c2(read_shadow=True, read_samples=True)
     got = _hip(renderer, sc)
@@ -110,7 +107,8 @@ def test_hip_c2_full_size_against_reference(renderer):
     dmis, cmax, cfrac = compare(got, want, "c2")
     n = differing_values(got, want)
     print(f"[gpu] c2 vs reference: depth mismatches {dmis}, colour max {cmax}, differing values {n}", flush=True)
-    assert dmis == 0 and cmax <= 1 and cfrac <= 1e-6, (dmis, cmax, cfrac)
+    assert dmis == 0 and n == 0, (dmis, cmax, n)
+    assert_identical(got, want, "c2 vs reference")
 
 
 def test_random_triangles_with_screen_covering(renderer):

@@ -21,13 +21,13 @@ def differing_values(got, want):
     return sum(int((g != w).sum()) for g, w in zip(got, want) if g.dtype == np.uint8)
 
 
-def assert_parity(got, want, label="", colour_frac=1e-4):
-    """The parity bar: depth bit-exact, colour within 1/255 per channel on every
-    value and exact on all but `colour_frac` of them."""
-    dmis, cmax, cfrac = compare(got, want, label)
-    assert dmis == 0, f"{label}: {dmis} depth values differ"
-    assert cmax <= 1, f"{label}: colour differs by up to {cmax}"
-    assert cfrac <= colour_frac, f"{label}: {cfrac:.2e} of colour values differ"
+def assert_identical(got, want, label=""):
+    """Every output identical bit for bit (depth, colour, samples)."""
+    assert len(got) == len(want), label
+    for i, (g, w) in enumerate(zip(got, want)):
+        assert g.shape == w.shape, (label, i, g.shape, w.shape)
+        bad = int((g.view(np.uint8) != w.view(np.uint8)).sum())
+        assert bad == 0, f"{label}: output {i} differs in {bad} bytes"
 
 
 def load_golden(path):
