@@ -310,6 +310,36 @@ void launchOrdRest(const DrawParams &P, int grid, int need_deriv, hipStream_t st
 }
 
 /* Listed ordered raster of a draw that writes no depth. */
+
+/* The multisample resolve a draw ends with (RendererSoft.cpp:161-163): launched now, or, for rhip_draw,
+ * left pending until something may read the resolved image (checkDev, drawOne). */
+}  // namespace
+
+namespace rhip {
+int flushResolve(rhip_device d) {
+  if (!d->resolve.on) return RHIP_OK;
+  d->resolve.on = false;
+  Timer tm(d, "k_resolve");
+  RHIP_LAUNCH(k_resolve, d->resolve.grid, 256, d->stream, d->resolve.P);
+  HIPCHK(hipGetLastError());
+  return RHIP_OK;
+}
+}  // namespace rhip
+
+namespace {
+
+static void resolveAfterDraw(rhip_device d, const DrawParams &P, unsigned grid, hipStream_t s) {
+  if (d->defer_resolve && s == d->stream) {
+    d->resolve.on = true;
+    d->resolve.tex = d->fbo[d->cur_fbo].color;
+    d->resolve.grid = grid;
+    d->resolve.P = P;
+    return;
+  }
+  Timer tm(d, "k_resolve");
+  RHIP_LAUNCH(k_resolve, grid, 256, s, P);
+}
+
 template <int S>
 void launchListed(rhip_device d, const DrawParams &P, int need_deriv, hipStream_t st) {
   const size_t npx = (size_t)P.fb_w * P.fb_h;
@@ -329,10 +359,7 @@ void launchListed(rhip_device d, const DrawParams &P, int need_deriv, hipStream_
     Timer tm(d, S == 4 ? "k_ord_rest_msaa4" : "k_ord_rest");
     launchOrdRest<S>(P, std::min(P.ntiles, 512), need_deriv, st);
   }
-  if (S == 4 && P.color) {
-    Timer tm(d, "k_resolve");
-    RHIP_LAUNCH(k_resolve, (unsigned)std::min<size_t>(blocks(npx, 256), 4096), 256, st, P);
-  }
+  if (S == 4 && P.color) resolveAfterDraw(d, P, (unsigned)std::min<size_t>(blocks(npx, 256), 4096), st);
 }
 
 /* Ordered raster (every state the two-phase raster does not take). */
@@ -916,7 +943,7 @@ int rhip_set_viewport(rhip_device d, int x, int y, int w, int h) {
   return RHIP_OK;
 }
 
-int rhip_end_render_pass(rhip_device d) { return d ? RHIP_OK : fail(RHIP_E_INVALID, "null device"); }
+int rhip_end_render_pass(rhip_device d) { return checkDev(d); /* launches a deferred resolve */ }
 
 int rhip_wait_idle(rhip_device d) {
   int rc = checkDev(d);
@@ -1061,10 +1088,8 @@ int drawPointsLines(rhip_device d, DrawParams &P, const DrawShape &sh, bool no_a
       RHIP_LAUNCH(k_pl_apply<1>, gf, 256, s, P, key2, val2, zf, cf, F);
   }
   /* RendererSoft::draw resolves a multisampled colour target after every draw (:161-163) */
-  if (has_colour && P.samples == 4) {
-    Timer tm(d, "k_resolve");
-    RHIP_LAUNCH(k_resolve, (unsigned)std::min<size_t>(blocks((size_t)P.fb_w * P.fb_h, 256), 4096), 256, s, P);
-  }
+  if (has_colour && P.samples == 4)
+    resolveAfterDraw(d, P, (unsigned)std::min<size_t>(blocks((size_t)P.fb_w * P.fb_h, 256), 4096), s);
   HIPCHK(hipGetLastError());
   d->host_stats.draws++;
   return RHIP_OK;
@@ -1074,10 +1099,19 @@ int drawPointsLines(rhip_device d, DrawParams &P, const DrawShape &sh, bool no_a
  * the device's grow-only buffer, or, for no_alloc, in the library's static
  * scratch of kScratchBytes. */
 int drawOne(rhip_device d, const rhip_draw_desc *dd, bool no_alloc) {
-  int rc = checkDev(d);
-  if (rc) return rc;
+  if (!d) return fail(RHIP_E_INVALID, "null device");
+  const hipError_t he = hipSetDevice(d->hip_device);
+  if (he != hipSuccess) return fail(RHIP_E_HIP, "hipSetDevice: %s", hipGetErrorString(he));
+  int rc = RHIP_OK;
   if (!dd) return fail(RHIP_E_INVALID, "null draw");
   if (!d->cur_fbo) return RHIP_OK; /* RendererSoft::draw returns when no framebuffer is bound (:137-139) */
+  /* a deferred resolve waits only for a draw of rhip_draw on the same colour target that does not
+   * sample that target */
+  if (d->resolve.on) {
+    bool keep = d->defer_resolve && d->fbo[d->cur_fbo].color == d->resolve.tex;
+    for (int s = 0; keep && s < RHS_SLOT_COUNT; s++) keep = dd->sampler_tex[s] != d->resolve.tex;
+    if (!keep && (rc = flushResolve(d))) return rc;
+  }
   const int vp[4] = {d->vp_x, d->vp_y, d->vp_w, d->vp_h};
   DrawParams P;
   DrawShape sh;
@@ -1100,8 +1134,7 @@ int drawOne(rhip_device d, const rhip_draw_desc *dd, bool no_alloc) {
   if (sh.T == 0) {
     /* no triangles: the reference still resolves a multisampled colour target (:161-163) */
     if (P.color && P.samples == 4) {
-      Timer tm(d, "k_resolve");
-      RHIP_LAUNCH(k_resolve, (unsigned)std::min<size_t>(blocks((size_t)P.fb_w * P.fb_h, 256), 4096), 256, d->stream, P);
+      resolveAfterDraw(d, P, (unsigned)std::min<size_t>(blocks((size_t)P.fb_w * P.fb_h, 256), 4096), d->stream);
       HIPCHK(hipGetLastError());
     }
     d->host_stats.draws++;
@@ -1157,10 +1190,7 @@ int drawOne(rhip_device d, const rhip_draw_desc *dd, bool no_alloc) {
           RHIP_LAUNCH(k_capture<1>, grid, 256, s, P, need_deriv ? 1 : 0, rec, d->nn.rec_cap, d->nn.rec_count,
                       d->nn.shift, d->nn.salt, d->nn.material);
       }
-      if (samples == 4) {
-        Timer tm(d, "k_resolve");
-        RHIP_LAUNCH(k_resolve, (unsigned)std::min<size_t>(blocks((size_t)fbw * fbh, 256), 4096), 256, s, P);
-      }
+      if (samples == 4) resolveAfterDraw(d, P, (unsigned)std::min<size_t>(blocks((size_t)fbw * fbh, 256), 4096), s);
     }
   } else if (sh.listed) {
     if (samples == 4)
@@ -1185,7 +1215,13 @@ int drawOne(rhip_device d, const rhip_draw_desc *dd, bool no_alloc) {
 
 extern "C" {
 
-int rhip_draw(rhip_device d, const rhip_draw_desc *dd) { return drawOne(d, dd, false); }
+int rhip_draw(rhip_device d, const rhip_draw_desc *dd) {
+  if (!d) return fail(RHIP_E_INVALID, "null device");
+  d->defer_resolve = true;
+  const int rc = drawOne(d, dd, false);
+  d->defer_resolve = false;
+  return rc;
+}
 
 int rhip_draw_noalloc(rhip_device d, const rhip_draw_desc *dd) {
   if (!d) return fail(RHIP_E_INVALID, "null device");

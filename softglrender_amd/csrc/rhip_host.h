@@ -143,15 +143,29 @@ struct rhip_device_s {
     uint64_t fallback = 0; /* materials whose learned-shader draws use the plain shader */
     uint32_t salt = 0;
   } nn;
+  /* The multisample resolve of the last draw on a colour target, not yet launched: a draw's resolve is
+   * overwritten by the next draw's on the same target unless something reads the resolved image in
+   * between, so rhip_draw leaves it here; every other entry point and a draw that reads it or targets
+   * another texture launch it first (flushResolve). */
+  struct PendingResolve {
+    bool on = false;
+    int tex = 0; /* colour attachment (texture id) */
+    unsigned grid = 0;
+    rhip::DrawParams P;
+  } resolve;
+  bool defer_resolve = false; /* set while rhip_draw runs */
 };
 
 namespace rhip {
 
+int flushResolve(rhip_device d);
+
+/* The device made current, and a deferred multisample resolve launched (every entry point but rhip_draw). */
 inline int checkDev(rhip_device d) {
   if (!d) return fail(RHIP_E_INVALID, "null device");
   hipError_t e = hipSetDevice(d->hip_device);
   if (e != hipSuccess) return fail(RHIP_E_HIP, "hipSetDevice: %s", hipGetErrorString(e));
-  return RHIP_OK;
+  return d->resolve.on ? flushResolve(d) : RHIP_OK;
 }
 
 inline Texture *getTex(rhip_device d, int id) {

@@ -731,8 +731,9 @@ VARIANTS = ["blend_alpha", "blend_add_one", "blend_minmax", "blend_sub", "depth_
             "layout_morton", "depth_only_msaa", "ao_emissive", "normal_blinn", "normal_blinn_dir"]
 
 
-def scene_variant(name, width=160, height=120, seed=11):
-    """One small frame exercising one pipeline state (for fixtures and parity tests)."""
+def scene_variant(name, width=160, height=120, seed=11, read_between=False):
+    """One small frame exercising one pipeline state (for fixtures and parity tests). With `read_between`, the
+    colour target (and its resolved image at 4x) is also read between a pass's draws."""
     st = S.Stream()
     msaa = name.startswith("msaa") or name == "depth_only_msaa"
     layouts = {}
@@ -887,6 +888,10 @@ def scene_variant(name, width=160, height=120, seed=11):
         del st2
         return Scene(name, st, width, height, False, dict(layouts={}))
     st.draw(1, shading, defines, rs, model_u, scene_u, mat, samplers=samplers)
+    if read_between and name != "depth_only_msaa":
+        st.read(2, which=0)
+        if msaa:
+            st.read(2, which=1)
     if name.startswith("blend") or name == "msaa_blend":
         # a second, offset sphere blended over the first
         tm = _translate((0.35, 0.2, 0.4))

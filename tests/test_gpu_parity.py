@@ -69,6 +69,16 @@ def test_hip_matches_oracle_new_seeds(renderer, name, seed):
     assert_identical(_hip(renderer, sc), _oracle(sc), name)
 
 
+@pytest.mark.parametrize("name", ["msaa_blend", "blend_alpha", "msaa_clip"])
+def test_resolved_image_read_between_draws(renderer, name):
+    """rhip_draw leaves its multisample resolve pending until something may read the resolved image: reading
+    the target between two draws of one pass must see the first draw's resolve, as RendererSoft resolves after
+    every draw (RendererSoft.cpp:161-163)."""
+    sc = scenes.scene_variant(name, width=120, height=90, seed=3, read_between=True)
+    print(f"[gpu] read between draws: {name}", flush=True)
+    assert_identical(_hip(renderer, sc), _oracle(sc), name)
+
+
 def test_hip_deterministic_and_repeatable(renderer):
     sc = scenes.scene_c2(480, 270, freq=40, tex_size=256, shadow_size=512, read_shadow=True, read_samples=True)
     a = _hip(renderer, sc, repeats=3)
