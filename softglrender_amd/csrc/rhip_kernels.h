@@ -995,6 +995,11 @@ __device__ __forceinline__ uint32_t depthKey(float z) {
   return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
+/* depthKey's inverse (a zero key returns +0) */
+__device__ __forceinline__ float keyDepth(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
 template <int MODE>
 __device__ __forceinline__ unsigned long long visInit(float dep, bool open) {
   /* a NaN in the buffer fails every comparison: no fragment may win there */
@@ -1303,18 +1308,21 @@ __device__ void visTile(const DrawParams &P, int tile) {
   const float lx[4] = {0.375f, 0.875f, 0.125f, 0.625f};
   const float ly[4] = {0.875f, 0.625f, 0.375f, 0.125f};
 
-  if (tb.total() == 0) return;
-  __syncthreads(); /* LDS of a previous tile of this block is no longer read */
-  float dep[PPL][S];
+  float dep[PPL][S]; /* loaded while the bin counts are still on their way */
 #pragma unroll
   for (int q = 0; q < PPL; q++) {
     const int pl = lp + q * NT, px = tx0 + pl % TILE, py = ty0 + pl / TILE;
-    const bool in = px < P.fb_w && py < P.fb_h;
 #pragma unroll
-    for (int s = 0; s < S; s++) {
-      dep[q][s] = in ? P.depth[((size_t)py * P.fb_w + px) * S + s] : 0.f;
-      keys[pl * S + s] = visInit<MODE>(dep[q][s], in);
-    }
+    for (int s = 0; s < S; s++)
+      dep[q][s] = px < P.fb_w && py < P.fb_h ? P.depth[((size_t)py * P.fb_w + px) * S + s] : 0.f;
+  }
+  if (tb.total() == 0) return;
+  __syncthreads(); /* LDS of a previous tile of this block is no longer read */
+#pragma unroll
+  for (int q = 0; q < PPL; q++) {
+    const int pl = lp + q * NT, px = tx0 + pl % TILE, py = ty0 + pl / TILE;
+#pragma unroll
+    for (int s = 0; s < S; s++) keys[pl * S + s] = visInit<MODE>(dep[q][s], px < P.fb_w && py < P.fb_h);
   }
   /* big triangles: the coarse bin's survivors, or all of its and the global bin's entries */
   const int hc = P.hiz_count[coarse];
@@ -1338,17 +1346,43 @@ __device__ void visTile(const DrawParams &P, int tile) {
   if (lp == 0 && hiz) s_bound = MX ? max(s_bound, s_bound2) : min(s_bound, s_bound2);
   __syncthreads();
 
-  /* big triangles that cover the tile tighten the bound */
+  /* One staged triangle per block thread: a big triangle of phase 1a's current
+   * chunk, later one of phase 1b's batches. Phase 1a walks its triangles one at
+   * a time; reading them here instead of from the setup record and then its
+   * vertices saves each step two dependent global loads. */
+  __shared__ float4 s_pr[WAVES][64][6];
+  float4 *const slot = s_pr[lp >> 6][lp & 63];
+  auto stageBig = [&](const PrimSetup &ps, const float4 &f0, const float4 &f1, const float4 &f2, int id) {
+    const Tri T = makeTri(f0, f1, f2);
+    slot[0] = make_float4(ps.min_x, ps.min_y, ps.max_x, ps.max_y);
+    slot[1] = make_float4(__int_as_float(ps.bcx), __int_as_float(ps.bcy), __int_as_float(id), ps.zlo);
+    slot[2] = make_float4(ps.zhi, 0.f, T.v0x, T.v0y);
+    slot[3] = make_float4(T.a0, T.a1, T.b0, T.b1);
+    slot[4] = make_float4(T.uz, T.z[0], T.z[1], T.z[2]);
+    slot[5] = make_float4(__int_as_float(ps.qx0), __int_as_float(ps.qx1), __int_as_float(ps.qy0),
+                          __int_as_float(ps.qy1));
+  };
+  auto bigKey = [&](float zlo, float zhi) { return depthKey(gclamp(MX ? zhi : zlo, 0.f, 1.f)); };
+  auto touches = [&](const PrimSetup &ps) { return ps.qx0 <= X1 && ps.qx1 >= tx0 && ps.qy0 <= Y1 && ps.qy1 >= ty0; };
+
+  /* big triangles that cover the tile tighten the bound; the first chunk's
+   * triangles that touch the tile are staged (id -1 marks the others) */
   for (int i = lp; i < (hiz ? nbig : 0); i += NT) {
     const int id = bigAt(i);
     const PrimSetup ps = P.prims[id];
+    if (!touches(ps)) { /* nor can it cover the tile */
+      if (i < NT) slot[1].z = __int_as_float(-1);
+      continue;
+    }
+    const float4 f0 = P.frag[ps.v[0]], f1 = P.frag[ps.v[1]], f2 = P.frag[ps.v[2]];
     float zmin, zmax;
-    if (coversRect<S>(ps, P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]], tx0, ty0, X1, Y1, zmin, zmax)) {
+    if (coversRect<S>(ps, f0, f1, f2, tx0, ty0, X1, Y1, zmin, zmax)) {
       if (MX)
         atomicMax(&s_bound, depthKey(gclamp(zmin, 0.f, 1.f)));
       else
         atomicMin(&s_bound, depthKey(gclamp(zmax, 0.f, 1.f)));
     }
+    if (i < NT) stageBig(ps, f0, f1, f2, id);
   }
   __syncthreads();
   /* the nearest clipped polygon that covers the tile (no single fan of it may):
@@ -1446,16 +1480,26 @@ __device__ void visTile(const DrawParams &P, int tile) {
   uint32_t wave_lim = MX ? waveMinU(mine) : waveMaxU(mine);
   for (int c0 = 0; c0 < nbig; c0 += NT) {
     if (c0 + lp < nbig) {
-      const int id = bigAt(c0 + lp);
-      const PrimSetup &ps = P.prims[id];
-      if (ps.qx0 <= X1 && ps.qx1 >= tx0 && ps.qy0 <= Y1 && ps.qy1 >= ty0) {
-        if (!zCulled(ps, bound, MX)) {
-          const int k = atomicAdd(&s_n, 1);
-          big[k] = id;
-          big_z[k] = MX ? depthKey(gclamp(ps.zhi, 0.f, 1.f)) : depthKey(gclamp(ps.zlo, 0.f, 1.f));
-        } else {
-          nculled++;
-        }
+      bool touch, take;
+      uint32_t zk;
+      if (hiz && c0 == 0) { /* staged above */
+        const float4 d1 = slot[1];
+        touch = __float_as_int(d1.z) >= 0;
+        zk = touch ? bigKey(d1.w, slot[2].x) : 0u;
+      } else {
+        const int id = bigAt(c0 + lp);
+        const PrimSetup ps = P.prims[id];
+        touch = touches(ps);
+        zk = bigKey(ps.zlo, ps.zhi);
+        if (touch && (MX ? zk >= bound : zk <= bound)) stageBig(ps, P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]], id);
+      }
+      take = touch && (MX ? zk >= bound : zk <= bound); /* zCulled's test */
+      if (take) {
+        const int k = atomicAdd(&s_n, 1);
+        big[k] = lp;
+        big_z[k] = zk;
+      } else if (touch) {
+        nculled++;
       }
     }
     __syncthreads();
@@ -1463,13 +1507,21 @@ __device__ void visTile(const DrawParams &P, int tile) {
     for (int j = 0; j < nb; j++) {
       const uint32_t zk = big_z[j];
       if (hiz && (MX ? zk < wave_lim : zk > wave_lim)) continue; /* wave-uniform */
-      const int id = big[j];
-      const PrimSetup ps = P.prims[id];
-      const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
+      const float4 *d = s_pr[big[j] >> 6][big[j] & 63];
+      const float4 d0 = d[0], d1 = d[1], d2 = d[2], d3 = d[3], d4 = d[4], d5 = d[5];
+      PrimSetup ps;
+      ps.min_x = d0.x, ps.min_y = d0.y, ps.max_x = d0.z, ps.max_y = d0.w;
+      ps.bcx = __float_as_int(d1.x), ps.bcy = __float_as_int(d1.y);
+      Tri T;
+      T.v0x = d2.z, T.v0y = d2.w, T.a0 = d3.x, T.a1 = d3.y, T.b0 = d3.z, T.b1 = d3.w;
+      T.uz = d4.x, T.z[0] = d4.y, T.z[1] = d4.z, T.z[2] = d4.w;
+      const int id = __float_as_int(d1.z);
+      const int qx0 = __float_as_int(d5.x), qx1 = __float_as_int(d5.y);
+      const int qy0 = __float_as_int(d5.z), qy1 = __float_as_int(d5.w);
 #pragma unroll
       for (int q = 0; q < PPL; q++) {
         const int pl = lp + q * NT, px = tx0 + pl % TILE, py = ty0 + pl / TILE;
-        if (px < P.fb_w && py < P.fb_h && px >= ps.qx0 && px <= ps.qx1 && py >= ps.qy0 && py <= ps.qy1)
+        if (px < P.fb_w && py < P.fb_h && px >= qx0 && px <= qx1 && py >= qy0 && py <= qy1)
           visPixel<S, MODE>(ps, T, id, px, py, &keys[pl * S]);
       }
       mine = laneLimit();
@@ -1527,7 +1579,6 @@ __device__ void visTile(const DrawParams &P, int tile) {
    * that the division-free test does not rule out are queued and tested exactly
    * 64 at a time: most pixels of a small triangle's quad range are uncovered. */
   {
-    __shared__ float4 s_pr[WAVES][64][6];
     __shared__ int s_start[WAVES][64];
     __shared__ unsigned long long s_marks[WAVES];
     __shared__ uint32_t s_q[WAVES][128]; /* queued pairs: triangle | x << 8 | y << 16, tile-relative */
@@ -1666,19 +1717,34 @@ __device__ void visTile(const DrawParams &P, int tile) {
       for (int t = 0; t < s; t++) seen = seen || win[q][t] == w;
       if (seen) continue;
       const int id = (int)(w >> 2);
-      const PrimSetup ps = P.prims[id];
-      const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
+      /* A winning key holds the depth the triangle's sample took; it returns
+       * that depth exactly unless it was a zero (depthKey folds -0 into +0) or
+       * a NaN, which take the triangle's interpolation again. */
       uint32_t m = 0;
+      bool again = false;
 #pragma unroll
       for (int t = s; t < S; t++) {
         if (win[q][t] != w) continue;
         m |= 1u << t;
-        float4 bc;
-        if (S == 4)
-          bary(T, lx[t] + (float)px, ly[t] + (float)py, bc);
+        const float z = keyDepth((uint32_t)(keys[pl * S + t] >> 32));
+        if (z > 0.f && z <= 1.f)
+          dep[q][t] = z;
         else
-          bary(T, (float)px + 0.5f, (float)py + 0.5f, bc);
-        dep[q][t] = gclamp(dpps3(bc, T.z), 0.f, 1.f);
+          again = true;
+      }
+      if (again) {
+        const PrimSetup ps = P.prims[id];
+        const Tri T = makeTri(P.frag[ps.v[0]], P.frag[ps.v[1]], P.frag[ps.v[2]]);
+#pragma unroll
+        for (int t = s; t < S; t++) {
+          if (win[q][t] != w) continue;
+          float4 bc;
+          if (S == 4)
+            bary(T, lx[t] + (float)px, ly[t] + (float)py, bc);
+          else
+            bary(T, (float)px + 0.5f, (float)py + 0.5f, bc);
+          dep[q][t] = gclamp(dpps3(bc, T.z), 0.f, 1.f);
+        }
       }
       if (hasC) P.jobs[base++] = make_uint2((uint32_t)pix | (m << 28), w);
     }
