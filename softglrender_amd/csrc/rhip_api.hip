@@ -1149,7 +1149,6 @@ int drawOne(rhip_device d, const rhip_draw_desc *dd, bool no_alloc) {
   { Timer tm(d, "k_vertex"); RHIP_LAUNCH(k_vertex, std::max(blocks(V, 256), 1u), 256, s, P); }
   { Timer tm(d, "k_setup"); RHIP_LAUNCH(k_setup, nblk, 256, s, P); }
   { Timer tm(d, "k_clip"); RHIP_LAUNCH(k_clip, nblk, 256, s, P); }
-  RHIP_LAUNCH(k_scan_fans, 1, 1024, s, P);
   RHIP_LAUNCH(k_clip_fans, nblk, 256, s, P);
   { Timer tm(d, "k_bin_scan"); RHIP_LAUNCH(k_bin_scan, 1, 1024, s, P); }
   { Timer tm(d, "k_bin_emit"); RHIP_LAUNCH(k_bin_emit, std::min(nblk, 8192), 256, s, P); }

@@ -68,7 +68,7 @@ static __global__ __launch_bounds__(1024) void kb_scan_fans(const DrawParams *__
 static __global__ __launch_bounds__(256) void kb_clip_fans(const DrawParams *__restrict__ Pv,
                                                     const uint32_t *__restrict__ tab) {
   const ViewBlock vb = viewBlock(Pv, tab);
-  clipFansBody(*vb.P, vb.local);
+  clipFansBody<false>(*vb.P, vb.local);
 }
 
 /* one block per view */

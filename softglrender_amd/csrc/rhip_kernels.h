@@ -7,9 +7,9 @@
  *   k_setup       assembly, clip test, face culling, bounding box, quad range, bin counts
  *                 (:192-204, :277-299, :1005-1019)
  *   k_clip        Sutherland-Hodgman against the six planes, re-shading new vertices (:489-559, :956-970)
- *   k_scan_fans   where the fan triangles of clipped polygons go
- *   k_clip_fans   those triangles, after all originals in clipped-triangle order (:550-558, :227)
- *   k_bin_scan    start of every bin
+ *   k_clip_fans   the fan triangles of clipped polygons, after all originals in clipped-triangle
+ *                 order (:550-558, :227)
+ *   k_bin_scan    primitive count, start of every bin
  *   k_bin_emit    primitive ids into their bins
  *   k_raster_vis  draws whose result follows from depths and order: visibility, then one
  *                 shading per visible fragment (k_shade)
@@ -655,7 +655,9 @@ __device__ __forceinline__ void clipBody(const DrawParams &P, int b) {
 
 static __global__ __launch_bounds__(256) void k_clip(DrawParams P) { clipBody(P, xcdBlock(blockIdx.x, gridDim.x)); }
 
-/* One block of 1024: exclusive offsets of every block's fan triangles, and the primitive count. */
+/* One block of 1024: exclusive offsets of every block's fan triangles, and the
+ * primitive count (the batched views' stage; a single draw's blocks take their
+ * own offsets in clipFansBody and its totals in fanTotalsBody). */
 __device__ __forceinline__ void scanFansBody(const DrawParams &P) {
   const int total = blockScanArray(P.blk_fans, P.blk_fan_off, P.nblk, nullptr);
   if (threadIdx.x == 0) {
@@ -668,24 +670,38 @@ __device__ __forceinline__ void scanFansBody(const DrawParams &P) {
   }
 }
 
-static __global__ __launch_bounds__(1024) void k_scan_fans(DrawParams P) { scanFansBody(P); }
 
-/* fans appended after all original primitives, in clipped-triangle order (:550-558) */
+/* fans appended after all original primitives, in clipped-triangle order (:550-558).
+ * OWN_OFFSET: a block with fans sums the earlier blocks' fan counts itself
+ * (few blocks clip anything), instead of reading scanFansBody's offsets. */
+template <bool OWN_OFFSET>
 __device__ __forceinline__ void clipFansBody(const DrawParams &P, int b) {
   __shared__ int part[256];
+  __shared__ int s_off;
   if (P.blk_fans[b] == 0) return;
+  int off;
+  if (OWN_OFFSET) {
+    int acc = 0;
+    for (int i = threadIdx.x; i < b; i += blockDim.x) acc += P.blk_fans[i];
+    acc = blockInclusiveScan<256>(acc, part);
+    if (threadIdx.x == 255) s_off = acc;
+    __syncthreads();
+    off = s_off;
+  } else {
+    off = P.blk_fan_off[b];
+  }
   const int t = b * blockDim.x + threadIdx.x;
   const int *poly = P.clip_poly + (size_t)t * (kMaxPoly + 1);
   const int n = t < P.ntris ? poly[kMaxPoly] : 0;
   const int nf = n > 3 ? n - 3 : 0;
   const int inc = blockInclusiveScan<256>(nf, part);
-  const int base = P.ntris + P.blk_fan_off[b] + inc - nf;
+  const int base = P.ntris + off + inc - nf;
   if (n > 3) P.fan_first[t] = base;
   for (int i = 3; i < n; i++) binCount(P, setupPrim(P, base + i - 3, poly[0], poly[i - 1], poly[i], t));
 }
 
 static __global__ __launch_bounds__(256) void k_clip_fans(DrawParams P) {
-  clipFansBody(P, xcdBlock(blockIdx.x, gridDim.x));
+  clipFansBody<true>(P, xcdBlock(blockIdx.x, gridDim.x));
 }
 
 /* --------------------------------------------------------------------------- binning */
@@ -700,7 +716,32 @@ __device__ __forceinline__ void binScanBody(const DrawParams &P) {
   }
 }
 
-static __global__ __launch_bounds__(1024) void k_bin_scan(DrawParams P) { binScanBody(P); }
+/* What scanFansBody writes besides the per-block offsets: the fan total, the
+ * primitive count and the cleared raster counters. */
+__device__ __forceinline__ void fanTotalsBody(const DrawParams &P) {
+  __shared__ int wsum[16];
+  int acc = 0;
+  for (int i = threadIdx.x; i < P.nblk; i += blockDim.x) acc += P.blk_fans[i];
+  const int inc = waveInclusiveScan(acc);
+  if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int total = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) total += wsum[w];
+    P.dcount[0] = total;
+    P.dcount[1] = P.ntris + total;
+    P.dcount[2] = 0;
+    P.dcount[3] = 0;
+    P.dcount[7] = 0;
+    P.dcount[8] = 0;
+  }
+}
+
+/* One block of 1024 after k_clip_fans: the fan totals, then the bin starts. */
+static __global__ __launch_bounds__(1024) void k_bin_scan(DrawParams P) {
+  fanTotalsBody(P);
+  binScanBody(P);
+}
 
 /* Lists every live primitive in its bins (wavefronts walk the ids together). */
 __device__ __forceinline__ void binEmitBody(const DrawParams &P, int blk, int nblocks) {
