@@ -565,9 +565,10 @@ size_t carveArenas(DrawParams &P, const DrawShape &sh, char *base) {
   P.clip_raw = (float *)take(T * kMaxClipVerts * sh.stride);
   P.clip_poly = (int *)take(T * (kMaxPoly + 1) * 4);
   P.fan_first = (int *)take(T * 4);
-  P.blk_clip = (int *)take((size_t)nblk * 3 * 4);
+  P.blk_clip = (int *)take(((size_t)nblk * 3 + fanChunks(nblk)) * 4);
   P.blk_fans = P.blk_clip ? P.blk_clip + nblk : nullptr;
   P.blk_fan_off = P.blk_fans ? P.blk_fans + nblk : nullptr;
+  P.chunk_fans = P.blk_fan_off ? P.blk_fan_off + nblk : nullptr;
   P.dcount = (int *)take(64);
   P.prims = (PrimSetup *)take(pcap * sizeof(PrimSetup));
   P.bin_count = (int *)take(((size_t)P.nbins * 3 + 1) * 4);
@@ -1148,7 +1149,6 @@ int drawOne(rhip_device d, const rhip_draw_desc *dd, bool no_alloc) {
 
   { Timer tm(d, "k_vertex"); RHIP_LAUNCH(k_vertex, std::max(blocks(V, 256), 1u), 256, s, P); }
   { Timer tm(d, "k_setup"); RHIP_LAUNCH(k_setup, nblk, 256, s, P); }
-  { Timer tm(d, "k_clip"); RHIP_LAUNCH(k_clip, nblk, 256, s, P); }
   RHIP_LAUNCH(k_clip_fans, nblk, 256, s, P);
   { Timer tm(d, "k_bin_scan"); RHIP_LAUNCH(k_bin_scan, 1, 1024, s, P); }
   { Timer tm(d, "k_bin_emit"); RHIP_LAUNCH(k_bin_emit, std::min(nblk, 8192), 256, s, P); }

@@ -22,6 +22,8 @@ constexpr int kCounterSlots = 64, kCounterStride = 16;
 constexpr int kVaryStride = 32;   /* varying slots of a vertex shader output (the reference layout) */
 constexpr int kMaxClipVerts = 12; /* new vertices one clipped triangle can create */
 constexpr int kMaxPoly = 9;       /* polygon vertices after six planes */
+/* 64-block chunks of a draw's 256-triangle blocks, whose fan counts are summed together */
+__host__ __device__ inline int fanChunks(int nblk) { return (nblk + 63) >> 6; }
 constexpr int kMaxFans = kMaxPoly - 3; /* extra triangles one clipped triangle can append */
 /* Binning: a primitive whose quads touch at most kBinFine tiles is listed in
  * each of those tiles; a larger one in each kCoarse x kCoarse-tile coarse bin
@@ -176,7 +178,8 @@ struct DrawParams {
   int *fan_first; /* per clipped triangle: id of its second fan triangle (the first keeps its id) */
   int *blk_clip;  /* per 256-triangle block: triangles that need clipping */
   int *blk_fans;  /* per block: fan triangles its clipped triangles append */
-  int *blk_fan_off;
+  int *blk_fan_off; /* per block: its fans' offset (batched views) */
+  int *chunk_fans;  /* per 64 blocks: their fan triangles (a single draw) */
   int nblk;
   int *dcount; /* device scalars: [0] appended fan triangles, [1] primitive count, [2] queued fragments,
                 [3] listed fragments, [7] tiles left to the ordered raster, [8] tiles with listed fragments */

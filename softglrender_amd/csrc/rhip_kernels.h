@@ -113,6 +113,8 @@ __device__ __forceinline__ void vertexBody(const DrawParams &P, int blk) {
 static __global__ __launch_bounds__(256) void k_vertex(DrawParams P) {
   vertexBody(P, blockIdx.x);
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.nbins; i += gridDim.x * blockDim.x) P.bin_count[i] = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < fanChunks(P.nblk); i += gridDim.x * blockDim.x)
+    P.chunk_fans[i] = 0;
 }
 
 /* --------------------------------------------------------------------------- block helpers */
@@ -510,8 +512,9 @@ __device__ __forceinline__ bool validTri(const DrawParams &P, int i0, int i1, in
 }
 
 /* processPolygonAssembly (RendererSoft.cpp:428-437) + the clip test: triangles
- * with a vertex outside the frustum are set up later by k_clip. */
-__device__ __forceinline__ void setupBody(const DrawParams &P, int blk) {
+ * with a vertex outside the frustum are set up after clipping. Returns the
+ * block's count of those (block-uniform). */
+__device__ __forceinline__ int setupBody(const DrawParams &P, int blk) {
   const int t = blk * blockDim.x + threadIdx.x;
   int needs = 0;
   BinRect r = {-1, 0, 0, 0, 0};
@@ -529,9 +532,8 @@ __device__ __forceinline__ void setupBody(const DrawParams &P, int blk) {
   binBlock<false>(P, t, r);
   const int n = __syncthreads_count(needs);
   if (threadIdx.x == 0) P.blk_clip[blk] = n;
+  return n;
 }
-
-static __global__ __launch_bounds__(256) void k_setup(DrawParams P) { setupBody(P, xcdBlock(blockIdx.x, gridDim.x)); }
 
 /* --------------------------------------------------------------------------- clipping */
 
@@ -653,7 +655,27 @@ __device__ __forceinline__ void clipBody(const DrawParams &P, int b) {
   if (threadIdx.x == 0) P.blk_fans[b] = fans;
 }
 
-static __global__ __launch_bounds__(256) void k_clip(DrawParams P) { clipBody(P, xcdBlock(blockIdx.x, gridDim.x)); }
+/* A single draw's setup: the block's triangles set up and binned, then those
+ * that need it clipped by the same block (one launch instead of two); the
+ * block's fan count also goes into its 64-block chunk's sum. */
+static __global__ __launch_bounds__(256) void k_setup(DrawParams P) {
+  __shared__ int fans;
+  const int b = xcdBlock(blockIdx.x, gridDim.x);
+  if (setupBody(P, b) == 0) {
+    if (threadIdx.x == 0) P.blk_fans[b] = 0;
+    return;
+  }
+  if (threadIdx.x == 0) fans = 0;
+  __syncthreads();
+  const int t = b * blockDim.x + threadIdx.x;
+  const int n = t < P.ntris ? clipTriangle(P, t) : 0;
+  if (n) atomicAdd(&fans, n);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    P.blk_fans[b] = fans;
+    if (fans) atomicAdd(&P.chunk_fans[b >> 6], fans);
+  }
+}
 
 /* One block of 1024: exclusive offsets of every block's fan triangles, and the
  * primitive count (the batched views' stage; a single draw's blocks take their
@@ -672,8 +694,9 @@ __device__ __forceinline__ void scanFansBody(const DrawParams &P) {
 
 
 /* fans appended after all original primitives, in clipped-triangle order (:550-558).
- * OWN_OFFSET: a block with fans sums the earlier blocks' fan counts itself
- * (few blocks clip anything), instead of reading scanFansBody's offsets. */
+ * OWN_OFFSET: a block with fans sums the earlier blocks' fan counts itself, from
+ * the sums of whole 64-block chunks and its own chunk's earlier blocks, instead
+ * of reading scanFansBody's offsets. */
 template <bool OWN_OFFSET>
 __device__ __forceinline__ void clipFansBody(const DrawParams &P, int b) {
   __shared__ int part[256];
@@ -681,8 +704,9 @@ __device__ __forceinline__ void clipFansBody(const DrawParams &P, int b) {
   if (P.blk_fans[b] == 0) return;
   int off;
   if (OWN_OFFSET) {
-    int acc = 0;
-    for (int i = threadIdx.x; i < b; i += blockDim.x) acc += P.blk_fans[i];
+    const int c = b >> 6;
+    int acc = threadIdx.x < (b & 63) ? P.blk_fans[(c << 6) + threadIdx.x] : 0;
+    for (int i = threadIdx.x; i < c; i += blockDim.x) acc += P.chunk_fans[i];
     acc = blockInclusiveScan<256>(acc, part);
     if (threadIdx.x == 255) s_off = acc;
     __syncthreads();
@@ -721,7 +745,7 @@ __device__ __forceinline__ void binScanBody(const DrawParams &P) {
 __device__ __forceinline__ void fanTotalsBody(const DrawParams &P) {
   __shared__ int wsum[16];
   int acc = 0;
-  for (int i = threadIdx.x; i < P.nblk; i += blockDim.x) acc += P.blk_fans[i];
+  for (int i = threadIdx.x; i < fanChunks(P.nblk); i += blockDim.x) acc += P.chunk_fans[i];
   const int inc = waveInclusiveScan(acc);
   if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = inc;
   __syncthreads();
