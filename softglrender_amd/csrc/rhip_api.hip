@@ -910,28 +910,36 @@ int rhip_begin_render_pass(rhip_device d, int fbo, const rhip_clear_states *cs) 
   d->cur_fbo = fbo;
   if (!cs) return RHIP_OK;
   const Fbo &f = it->second;
-  /* RendererSoft::beginRenderPass (RendererSoft.cpp:61-90): u8 truncation of clear * 255 */
+  /* RendererSoft::beginRenderPass (RendererSoft.cpp:61-90): u8 truncation of clear * 255; both clears in one launch */
+  uint32_t *cp = nullptr, *dp = nullptr, cv = 0, dv = 0;
+  size_t cn = 0, dn = 0;
+  auto launch = [&]() {
+    const size_t most = std::max(cn, dn) / 4 + 1;
+    RHIP_LAUNCH(k_fill2_u32, std::min(blocks(most, 256), 4096u), 256, d->stream, cp, cn, cv, dp, dn, dv);
+    return hipGetLastError();
+  };
   if (cs->color_flag && f.color) {
     Texture *t = getTex(d, f.color);
     if (!t) return fail(RHIP_E_INVALID, "colour attachment was destroyed");
     auto u8 = [](float v) { return (uint32_t)(uint8_t)(int32_t)(v * 255); };
-    uint32_t px = u8(cs->clear_color[0]) | (u8(cs->clear_color[1]) << 8) | (u8(cs->clear_color[2]) << 16) |
-                  (u8(cs->clear_color[3]) << 24);
+    cv = u8(cs->clear_color[0]) | (u8(cs->clear_color[1]) << 8) | (u8(cs->clear_color[2]) << 16) |
+         (u8(cs->clear_color[3]) << 24);
     const TexLevel &lv = t->hd.lv[f.color_layer][f.color_level];
-    size_t n = (size_t)lv.w * lv.h * (t->hd.msaa ? 4 : 1);
-    RHIP_LAUNCH(k_fill_u32, std::min(blocks(n, 256), 4096u), 256, d->stream, (uint32_t *)(t->data + lv.offset), n, px);
-    HIPCHK(hipGetLastError());
+    cn = (size_t)lv.w * lv.h * (t->hd.msaa ? 4 : 1);
+    cp = (uint32_t *)(t->data + lv.offset);
   }
   if (cs->depth_flag && f.depth) {
     Texture *t = getTex(d, f.depth);
-    if (!t) return fail(RHIP_E_INVALID, "depth attachment was destroyed");
+    if (!t) {
+      if (cp) HIPCHK(launch()); /* the colour clear still happens, as when the clears were separate */
+      return fail(RHIP_E_INVALID, "depth attachment was destroyed");
+    }
     const TexLevel &lv = t->hd.lv[0][0];
-    size_t n = (size_t)lv.w * lv.h * (t->hd.msaa ? 4 : 1);
-    uint32_t bits;
-    memcpy(&bits, &cs->clear_depth, 4);
-    RHIP_LAUNCH(k_fill_u32, std::min(blocks(n, 256), 4096u), 256, d->stream, (uint32_t *)(t->data + lv.offset), n, bits);
-    HIPCHK(hipGetLastError());
+    dn = (size_t)lv.w * lv.h * (t->hd.msaa ? 4 : 1);
+    memcpy(&dv, &cs->clear_depth, 4);
+    dp = (uint32_t *)(t->data + lv.offset);
   }
+  if (cp || dp) HIPCHK(launch());
   return RHIP_OK;
 }
 

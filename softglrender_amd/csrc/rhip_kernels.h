@@ -63,10 +63,23 @@ __device__ __forceinline__ XcdShare xcdShare(int n) {
 
 /* --------------------------------------------------------------------------- clears */
 
-static __global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) p[i] = v;
+/* Words [0, n) of p set to v by thread i of `stride`, 16 bytes a store when p is 16-byte aligned. */
+__device__ __forceinline__ void fillWords(uint32_t *p, size_t n, uint32_t v, size_t i, size_t stride) {
+  if (((uintptr_t)p & 15u) == 0) {
+    const size_t n4 = n >> 2;
+    const uint4 w = make_uint4(v, v, v, v);
+    for (size_t j = i; j < n4; j += stride) reinterpret_cast<uint4 *>(p)[j] = w;
+    for (size_t j = (n4 << 2) + i; j < n; j += stride) p[j] = v;
+  } else {
+    for (size_t j = i; j < n; j += stride) p[j] = v;
+  }
+}
+
+/* A render pass's clears in one launch: colour words [0, n0) of p0 to v0 and depth words [0, n1) of p1 to v1. */
+static __global__ void k_fill2_u32(uint32_t *p0, size_t n0, uint32_t v0, uint32_t *p1, size_t n1, uint32_t v1) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
+  fillWords(p0, n0, v0, i, stride);
+  fillWords(p1, n1, v1, i, stride);
 }
 
 /* --------------------------------------------------------------------------- vertex stage */
