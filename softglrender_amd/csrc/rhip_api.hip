@@ -1434,7 +1434,7 @@ int rhip_neural_gradient(rhip_device d, long long first, int count, float *grad,
   if (rc || (rc = nnRange(d, first, count))) return rc;
   auto &n = d->nn;
   const int parts = std::min<int>((count + 63) / 64, kNnMaxParts);
-  RHIP_LAUNCH(k_nn_grad, (unsigned)parts, 64, d->stream, n.packed, n.records.as<float>(), first, 0ll, count, 0u,
+  RHIP_LAUNCH(k_nn_grad, (unsigned)parts, kNnGradThreads, d->stream, n.packed, n.records.as<float>(), first, 0ll, count, 0u,
               n.partial, n.lossp);
   RHIP_LAUNCH(k_nn_update, kNnUpdateBlocks, 256, d->stream, n.params, n.partial, parts, n.lossp, n.scratch, n.grad,
               n.m1, n.m2, n.packed, 0.f, 0.f, 0.f, 0.f, 0.f, 0);
@@ -1463,7 +1463,7 @@ int rhip_neural_train(rhip_device d, long long first, long long count, int steps
     const double t = steps > 1 ? (double)i / (steps - 1) : 0.0;
     const float lr = (float)(lr_end + 0.5 * (lr_start - lr_end) * (1.0 + std::cos(3.14159265358979 * t)));
     const uint32_t sd = seed * 0x9e3779b9u + (uint32_t)(n.step + 1) * 0x85ebca6bu;
-    RHIP_LAUNCH(k_nn_grad, (unsigned)grid, 64, d->stream, n.packed, n.records.as<float>(), first, count, batch, sd,
+    RHIP_LAUNCH(k_nn_grad, (unsigned)grid, kNnGradThreads, d->stream, n.packed, n.records.as<float>(), first, count, batch, sd,
                 n.partial, n.lossp);
     n.step++;
     RHIP_LAUNCH(k_nn_update, kNnUpdateBlocks, 256, d->stream, n.params, n.partial, grid, n.lossp,

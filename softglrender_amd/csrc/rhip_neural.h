@@ -689,15 +689,22 @@ __device__ __forceinline__ bf16x8 onesOp() {
 /* One training step's gradient of mean squared error over `batch` records
  * drawn from [first, first + count) (or the `batch` records starting at
  * `first` when `count` is 0): forward and backward on MFMA, gradients summed
- * per block in LDS. Blocks are one wavefront; each takes chunks of 64 records
- * and writes its sums (FP32, unscaled) to its own row of `partial` and its
- * squared error to lossp[block]: no atomics, so k_nn_update's fixed-order
- * sum makes a step reproducible bit for bit. */
-__global__ __launch_bounds__(64) void k_nn_grad(const NnPacked *W, const float *rec, long long first, long long count,
-                                                int batch, uint32_t seed, float *partial, double *lossp) {
+ * per block in LDS. A block takes chunks of 64 records with four wavefronts:
+ * wavefront w computes the output tiles of hidden units 16w..16w+15 of every
+ * layer (and fragment tile w where a layer has one row tile), so each gradient
+ * sum has one writer and the same order as with one wavefront; wavefront 0
+ * stages the records and takes their errors. Each block writes its sums (FP32,
+ * unscaled) to its own row of `partial` and its squared error to
+ * lossp[block]: no atomics, so k_nn_update's fixed-order sum makes a step
+ * reproducible bit for bit. */
+constexpr int kNnGradThreads = 256;
+__global__ __launch_bounds__(kNnGradThreads) void k_nn_grad(const NnPacked *W, const float *rec, long long first,
+                                                            long long count, int batch, uint32_t seed,
+                                                            float *partial, double *lossp) {
   __shared__ NnTrainLds L;
-  const int l = threadIdx.x, g = l >> 4, c = l & 15;
-  for (int i = l; i < kNnParams; i += 64) L.g[i] = 0.f;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 4, c = l & 15;
+  const int m = w; /* this wavefront's row tile of the hidden layers */
+  for (int i = threadIdx.x; i < kNnParams; i += kNnGradThreads) L.g[i] = 0.f;
   double lsum = 0;
   const bf16x8 ones = onesOp();
   for (int c0 = blockIdx.x * 64; c0 < batch; c0 += gridDim.x * 64) {
@@ -706,21 +713,22 @@ __global__ __launch_bounds__(64) void k_nn_grad(const NnPacked *W, const float *
     long long ri = first + bi;
     if (count > 0) ri = first + (long long)(hash32((uint32_t)bi * 0x9e3779b9u ^ seed) % (uint32_t)count);
     const float *r = rec + (size_t)(act ? ri : first) * kRecFloats;
-    float f[kNnK1];
-    recFeatures(r, f);
-    if (!act) {
+    if (w == 0) {
+      float f[kNnK1];
+      recFeatures(r, f);
+      if (!act) {
 #pragma unroll
-      for (int k = 0; k < kNnK1; k++) f[k] = 0.f;
-    }
+        for (int k = 0; k < kNnK1; k++) f[k] = 0.f;
+      }
 #pragma unroll
-    for (int k = 0; k < kNnK1; k++) {
-      L.x[l * kNnK1 + k] = (__bf16)f[k];
-      L.xt[k * 64 + l] = (__bf16)f[k];
+      for (int k = 0; k < kNnK1; k++) {
+        L.x[l * kNnK1 + k] = (__bf16)f[k];
+        L.xt[k * 64 + l] = (__bf16)f[k];
+      }
     }
-    waveSync();
+    __syncthreads();
     /* forward: H1, H2 (and transposes), Y */
-#pragma unroll
-    for (int m = 0; m < 4; m++) {
+    {
       const bf16x8 a = ldOp(W->w1, kNnK1, 16 * m, 0);
 #pragma unroll
       for (int n = 0; n < 4; n++) {
@@ -735,9 +743,8 @@ __global__ __launch_bounds__(64) void k_nn_grad(const NnPacked *W, const float *
         }
       }
     }
-    waveSync();
-#pragma unroll
-    for (int m = 0; m < 4; m++) {
+    __syncthreads();
+    {
       const bf16x8 a0 = ldOp(W->w2, kNnHid, 16 * m, 0), a1 = ldOp(W->w2, kNnHid, 16 * m, 32);
       f32x4 bias;
 #pragma unroll
@@ -754,87 +761,81 @@ __global__ __launch_bounds__(64) void k_nn_grad(const NnPacked *W, const float *
         }
       }
     }
-    waveSync();
+    __syncthreads();
     {
       const bf16x8 a0 = ldOp(W->w3, kNnHid, 0, 0), a1 = ldOp(W->w3, kNnHid, 0, 32);
       f32x4 bias;
 #pragma unroll
       for (int v = 0; v < 4; v++) bias[v] = W->b3[4 * g + v];
+      const int n = w, fr = 16 * n + c;
+      const f32x4 d = mma16(a1, ldOp(L.h2, kNnHid, 16 * n, 32), mma16(a0, ldOp(L.h2, kNnHid, 16 * n, 0), bias));
+      if (g == 0) {
 #pragma unroll
-      for (int n = 0; n < 4; n++) {
-        const int fr = 16 * n + c;
-        const f32x4 d = mma16(a1, ldOp(L.h2, kNnHid, 16 * n, 32), mma16(a0, ldOp(L.h2, kNnHid, 16 * n, 0), bias));
-        if (g == 0) {
-#pragma unroll
-          for (int v = 0; v < 4; v++) L.yf[fr * 4 + v] = d[v];
-        }
+        for (int v = 0; v < 4; v++) L.yf[fr * 4 + v] = d[v];
       }
     }
-    waveSync();
-    /* this lane's record: error, dY */
-    float dyv[3];
+    __syncthreads();
+    /* record l's error and dY (wavefront 0) */
+    if (w == 0) {
+      float dyv[3];
 #pragma unroll
-    for (int ch = 0; ch < 3; ch++) {
-      const float y = L.yf[l * 4 + ch];
-      const float e = act ? y - r[kRecTarget + ch] : 0.f;
-      lsum += (double)e * e;
-      dyv[ch] = 2.f * e;
+      for (int ch = 0; ch < 3; ch++) {
+        const float y = L.yf[l * 4 + ch];
+        const float e = act ? y - r[kRecTarget + ch] : 0.f;
+        lsum += (double)e * e;
+        dyv[ch] = 2.f * e;
+      }
+#pragma unroll
+      for (int ch = 0; ch < 32; ch++) L.dy[l * 32 + ch] = (__bf16)(ch < 3 ? dyv[ch] : 0.f);
+#pragma unroll
+      for (int ch = 0; ch < 16; ch++) L.dyt[ch * 64 + l] = (__bf16)(ch < 3 ? dyv[ch] : 0.f);
     }
-    waveSync();
-#pragma unroll
-    for (int ch = 0; ch < 32; ch++) L.dy[l * 32 + ch] = (__bf16)(ch < 3 ? dyv[ch] : 0.f);
-#pragma unroll
-    for (int ch = 0; ch < 16; ch++) L.dyt[ch * 64 + l] = (__bf16)(ch < 3 ? dyv[ch] : 0.f);
-    waveSync();
-    /* dW3 = dY^T . H2, db3 = row sums of dY^T */
+    __syncthreads();
+    /* dW3 = dY^T . H2 (unit tile w), db3 = row sums of dY^T (wavefront 0) */
     {
       const bf16x8 a0 = ldOp(L.dyt, 64, 0, 0), a1 = ldOp(L.dyt, 64, 0, 32);
+      const int n = w;
+      f32x4 d = {0.f, 0.f, 0.f, 0.f};
+      d = mma16(a1, ldOp(L.h2t, 64, 16 * n, 32), mma16(a0, ldOp(L.h2t, 64, 16 * n, 0), d));
 #pragma unroll
-      for (int n = 0; n < 4; n++) {
-        f32x4 d = {0.f, 0.f, 0.f, 0.f};
-        d = mma16(a1, ldOp(L.h2t, 64, 16 * n, 32), mma16(a0, ldOp(L.h2t, 64, 16 * n, 0), d));
+      for (int v = 0; v < 4; v++) {
+        const int ch = 4 * g + v;
+        if (ch < kNnOut) L.g[kOffW3 + ch * kNnHid + 16 * n + c] += d[v];
+      }
+      if (w == 0) {
+        f32x4 s = {0.f, 0.f, 0.f, 0.f};
+        s = mma16(a1, ones, mma16(a0, ones, s));
+        if (c == 0) {
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-          const int ch = 4 * g + v;
-          if (ch < kNnOut) L.g[kOffW3 + ch * kNnHid + 16 * n + c] += d[v];
+          for (int v = 0; v < 4; v++)
+            if (4 * g + v < kNnOut) L.g[kOffB3 + 4 * g + v] += s[v];
         }
       }
-      f32x4 s = {0.f, 0.f, 0.f, 0.f};
-      s = mma16(a1, ones, mma16(a0, ones, s));
-      if (c == 0) {
-#pragma unroll
-        for (int v = 0; v < 4; v++)
-          if (4 * g + v < kNnOut) L.g[kOffB3 + 4 * g + v] += s[v];
-      }
     }
+    __syncthreads(); /* H2^T is read above before it is overwritten below */
     /* dH2^T = W3^T . dY^T, masked by H2 > 0, written over H2 and H2^T */
     {
       bf16x8 bdy[4];
 #pragma unroll
       for (int n = 0; n < 4; n++) bdy[n] = ldOp(L.dy, 32, 16 * n, 0);
-      waveSync();
+      const bf16x8 a = ldOp(W->w3t, 32, 16 * m, 0);
 #pragma unroll
-      for (int m = 0; m < 4; m++) {
-        const bf16x8 a = ldOp(W->w3t, 32, 16 * m, 0);
+      for (int n = 0; n < 4; n++) {
+        f32x4 d = {0.f, 0.f, 0.f, 0.f};
+        d = mma16(a, bdy[n], d);
 #pragma unroll
-        for (int n = 0; n < 4; n++) {
-          f32x4 d = {0.f, 0.f, 0.f, 0.f};
-          d = mma16(a, bdy[n], d);
-#pragma unroll
-          for (int v = 0; v < 4; v++) {
-            const int u = 16 * m + 4 * g + v, fr = 16 * n + c;
-            const bool on = (float)L.h2t[u * 64 + fr] > 0.f;
-            const __bf16 dh = (__bf16)(on ? d[v] : 0.f);
-            L.h2t[u * 64 + fr] = dh;
-            L.h2[fr * kNnHid + u] = dh;
-          }
+        for (int v = 0; v < 4; v++) {
+          const int u = 16 * m + 4 * g + v, fr = 16 * n + c;
+          const bool on = (float)L.h2t[u * 64 + fr] > 0.f;
+          const __bf16 dh = (__bf16)(on ? d[v] : 0.f);
+          L.h2t[u * 64 + fr] = dh;
+          L.h2[fr * kNnHid + u] = dh;
         }
       }
     }
-    waveSync();
+    __syncthreads();
     /* dW2 = dH2^T . H1, db2 = row sums of dH2^T */
-#pragma unroll
-    for (int m = 0; m < 4; m++) {
+    {
       const bf16x8 a0 = ldOp(L.h2t, 64, 16 * m, 0), a1 = ldOp(L.h2t, 64, 16 * m, 32);
 #pragma unroll
       for (int n = 0; n < 4; n++) {
@@ -850,7 +851,7 @@ __global__ __launch_bounds__(64) void k_nn_grad(const NnPacked *W, const float *
         for (int v = 0; v < 4; v++) L.g[kOffB2 + 16 * m + 4 * g + v] += s[v];
       }
     }
-    waveSync();
+    __syncthreads(); /* H1^T is read above before it is overwritten below */
     /* dH1^T = W2^T . dH2^T, masked by H1 > 0, written over H1^T */
     {
       bf16x8 bd[4][2];
@@ -858,26 +859,22 @@ __global__ __launch_bounds__(64) void k_nn_grad(const NnPacked *W, const float *
       for (int n = 0; n < 4; n++)
 #pragma unroll
         for (int kb = 0; kb < 2; kb++) bd[n][kb] = ldOp(L.h2, kNnHid, 16 * n, 32 * kb);
+      const bf16x8 a0 = ldOp(W->w2t, kNnHid, 16 * m, 0), a1 = ldOp(W->w2t, kNnHid, 16 * m, 32);
 #pragma unroll
-      for (int m = 0; m < 4; m++) {
-        const bf16x8 a0 = ldOp(W->w2t, kNnHid, 16 * m, 0), a1 = ldOp(W->w2t, kNnHid, 16 * m, 32);
+      for (int n = 0; n < 4; n++) {
+        f32x4 d = {0.f, 0.f, 0.f, 0.f};
+        d = mma16(a1, bd[n][1], mma16(a0, bd[n][0], d));
 #pragma unroll
-        for (int n = 0; n < 4; n++) {
-          f32x4 d = {0.f, 0.f, 0.f, 0.f};
-          d = mma16(a1, bd[n][1], mma16(a0, bd[n][0], d));
-#pragma unroll
-          for (int v = 0; v < 4; v++) {
-            const int u = 16 * m + 4 * g + v, fr = 16 * n + c;
-            const bool on = (float)L.h1t[u * 64 + fr] > 0.f;
-            L.h1t[u * 64 + fr] = (__bf16)(on ? d[v] : 0.f);
-          }
+        for (int v = 0; v < 4; v++) {
+          const int u = 16 * m + 4 * g + v, fr = 16 * n + c;
+          const bool on = (float)L.h1t[u * 64 + fr] > 0.f;
+          L.h1t[u * 64 + fr] = (__bf16)(on ? d[v] : 0.f);
         }
       }
     }
-    waveSync();
+    __syncthreads();
     /* dW1 (with db1 in column 24) = dH1^T . X */
-#pragma unroll
-    for (int m = 0; m < 4; m++) {
+    {
       const bf16x8 a0 = ldOp(L.h1t, 64, 16 * m, 0), a1 = ldOp(L.h1t, 64, 16 * m, 32);
 #pragma unroll
       for (int n = 0; n < 2; n++) {
@@ -893,15 +890,16 @@ __global__ __launch_bounds__(64) void k_nn_grad(const NnPacked *W, const float *
         }
       }
     }
-    waveSync();
+    __syncthreads(); /* the chunk's tiles are read before the next chunk overwrites them */
   }
-  waveSync();
   float *dst = partial + (size_t)blockIdx.x * kNnParams;
-  for (int i = l; i < kNnParams; i += 64) dst[i] = L.g[i];
-  double t = lsum;
+  for (int i = threadIdx.x; i < kNnParams; i += kNnGradThreads) dst[i] = L.g[i];
+  if (w == 0) {
+    double t = lsum;
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
-  if (l == 0) lossp[blockIdx.x] = t;
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+    if (l == 0) lossp[blockIdx.x] = t;
+  }
 }
 
 /* Parameter i's entries in the packed bf16 operands (the padding entries are
