@@ -334,6 +334,51 @@ def run_batched(r, quick=False, reps=5, log=lambda m: None):
     same = same_targets == len(ref_digest)
     log(f"[bench] batch: {bat_ms:.3f} ms per {nv} views; identical to separate draws: {same} "
         f"({same_targets} of {len(ref_digest)} colour and depth targets)")
+
+    # the same views as k batches run back to back (the tuned configuration): a view completes with its own
+    # batch, so smaller batches trade throughput for per-view completion
+    sub = {}
+    for k in (4, 16):
+        if nv % k:
+            continue
+        per = nv // k
+        parts = []
+        for i in range(k):
+            r.beginBatch()
+            for cmds in views[i * per:(i + 1) * per]:
+                for op, p in cmds:
+                    rp.exec(op, p, sink)
+            bt = r.endBatch()
+            bt.set_config(*best)
+            parts.append(bt)
+        for bt in parts:  # warm-up
+            bt.run()
+        r.waitIdle()
+        ev = _HipEvents(r)
+        a = ev.record()
+        for _ in range(reps):
+            for bt in parts:
+                bt.run()
+        b = ev.record()
+        k_ms = ev.ms(a, b) / reps
+        starts = []
+        for bt in parts:
+            starts.append(ev.record())
+            bt.run()
+        r.waitIdle()
+        comp = []
+        for i, bt in enumerate(parts):
+            comp += list(ev.ms(starts[0], starts[i]) + bt.view_times())
+        ev.close()
+        k_same = sum(x == y for x, y in zip(digest(), ref_digest))
+        for bt in parts:
+            bt.destroy()
+        sub[f"{k}x{per}"] = {"ms_per_256_views" if nv == 256 else "ms_per_views": round(k_ms, 4),
+                             "views_per_s": round(nv / (k_ms * 1e-3), 1),
+                             "completion_ms_p50": _pct(comp, 50), "completion_ms_p99": _pct(comp, 99),
+                             "targets_identical": k_same}
+        log(f"[bench] {k} batches of {per} views: {k_ms:.3f} ms, completion p50 {_pct(comp, 50)} ms "
+            f"p99 {_pct(comp, 99)} ms; {k_same} of {len(ref_digest)} targets identical")
     draws = frame_draws(data)
     algo = sum(raster_algorithmic_bytes(d) for d in draws)
     res = {
@@ -350,6 +395,7 @@ def run_batched(r, quick=False, reps=5, log=lambda m: None):
                            "latency_note": "each draw's own duration (HIP events around its commands); "
                                            "completion = from the first draw's start"},
         "speedup": round(sep_ms / bat_ms, 3),
+        "sub_batches": sub,
         "autotuner": {"best": {"tile": best[0], "waves": best[1]},
                       "ms_per_batch": {f"tile{t}_waves{w}": round(ms, 4) for (t, w), ms in table},
                       "seconds": round(t_tune, 1)},
