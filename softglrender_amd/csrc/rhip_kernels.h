@@ -193,32 +193,31 @@ __device__ __forceinline__ void waveSync() {
 }
 
 /* Exclusive prefix sum of in[0..n) into out[0..n) by one block of 1024 threads,
- * 16,384 entries at a time (a 1080p target's bins in one chunk) with coalesced
- * loads and stores (the next chunk's loads issued before this chunk's scan);
- * also clears clear[0..n) when given. Returns the total (to every thread). */
-constexpr int kScanPer = 16, kScanChunk = kScanPer * 1024;
+ * 8192 entries at a time with coalesced loads and stores (the next chunk's loads
+ * issued before this chunk's scan); also clears clear[0..n) when given. Returns
+ * the total (to every thread). */
 __device__ int blockScanArray(const int *in, int *out, int n, int *clear) {
-  __shared__ int buf[kScanChunk];
+  __shared__ int buf[8192];
   __shared__ int wsum[16];
   __shared__ int s_carry;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   if (t == 0) s_carry = 0;
-  int nxt[kScanPer];
+  int nxt[8];
 #pragma unroll
-  for (int k = 0; k < kScanPer; k++) nxt[k] = k * 1024 + t < n ? in[k * 1024 + t] : 0;
+  for (int k = 0; k < 8; k++) nxt[k] = k * 1024 + t < n ? in[k * 1024 + t] : 0;
   __syncthreads();
-  for (int c0 = 0; c0 < n; c0 += kScanChunk) {
+  for (int c0 = 0; c0 < n; c0 += 8192) {
 #pragma unroll
-    for (int k = 0; k < kScanPer; k++) {
+    for (int k = 0; k < 8; k++) {
       buf[k * 1024 + t] = nxt[k];
-      const int i = c0 + kScanChunk + k * 1024 + t;
+      const int i = c0 + 8192 + k * 1024 + t;
       nxt[k] = i < n ? in[i] : 0;
     }
     __syncthreads();
-    int v[kScanPer], sum = 0;
+    int v[8], sum = 0;
 #pragma unroll
-    for (int k = 0; k < kScanPer; k++) {
-      v[k] = buf[t * kScanPer + k];
+    for (int k = 0; k < 8; k++) {
+      v[k] = buf[t * 8 + k];
       sum += v[k];
     }
     const int inc = waveInclusiveScan(sum);
@@ -227,13 +226,13 @@ __device__ int blockScanArray(const int *in, int *out, int n, int *clear) {
     int run = s_carry + inc - sum;
     for (int w = 0; w < wave; w++) run += wsum[w];
 #pragma unroll
-    for (int k = 0; k < kScanPer; k++) {
-      buf[t * kScanPer + k] = run;
+    for (int k = 0; k < 8; k++) {
+      buf[t * 8 + k] = run;
       run += v[k];
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < kScanPer; k++) {
+    for (int k = 0; k < 8; k++) {
       const int i = c0 + k * 1024 + t;
       if (i < n) {
         out[i] = buf[k * 1024 + t];
